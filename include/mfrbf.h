@@ -1,0 +1,242 @@
+/*
+ * mfrbf.h - C ABI of libmfrbf.so, the MI355X (gfx950) implementation of the
+ * RBF-FD semidiscretization path of MeshfreeTrixi.jl.
+ *
+ * Conventions
+ *  - Every function returns an int status: MF_OK (0) or one of MF_ERR_*;
+ *    mf_last_error() then describes the failure.  Nothing throws.
+ *  - Host pointers are read or written only during the call.
+ *  - State vectors cross the ABI as nvar separate host arrays of n_points
+ *    doubles ("fields", the reference's StructArray components,
+ *    pointcloudsolver/rbfsolver.jl:111-116).  Inside the context they live
+ *    in device memory, node-major (all variables of a node contiguous).
+ *  - Node and stencil indices are 0-based (the reference's are 1-based).
+ *  - One context = one device + one HIP stream.  A context is not
+ *    re-entrant; distinct contexts may be used from different threads.
+ *
+ * Each declaration names the reference interface (file:line, relative to
+ * the MeshfreeTrixi.jl tree) that it replaces.
+ */
+#ifndef MFRBF_H
+#define MFRBF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mf_ctx mf_ctx;
+
+/* status codes */
+#define MF_OK 0
+#define MF_ERR_ARGUMENT 1    /* invalid argument (null pointer, size, enum) */
+#define MF_ERR_HIP 2         /* HIP runtime error */
+#define MF_ERR_STATE 3       /* call out of order (e.g. no stencil yet) */
+#define MF_ERR_SINGULAR 4    /* a stencil's saddle-point system is singular */
+#define MF_ERR_UNSUPPORTED 5 /* combination not provided by this build */
+
+/* equations (Trixi's CompressibleEulerEquations2D / LinearScalarAdvectionEquation2D) */
+#define MF_EQ_ADVECTION_2D 0
+#define MF_EQ_EULER_2D 1
+
+/* boundary condition kinds, PointCloudBCs.jl */
+#define MF_BC_DIRICHLET 0 /* BoundaryConditionDirichlet, :49-63 */
+#define MF_BC_SLIP_WALL 1 /* boundary_condition_slip_wall, :87-106 */
+#define MF_BC_DO_NOTHING 2 /* BoundaryConditionDoNothing, :108-115 */
+
+/* built-in Dirichlet value functions g(x, y, t); parameters, 8 per tag:
+ *  CONST     params[0..nvar)
+ *  PULSE     scalar exp(-((x - a1 t - x0)^2 + (y - a2 t - y0)^2) / w^2),
+ *            params (x0, y0, w, a1, a2)
+ *  WAVE      Euler density wave rho = r0 + A sin(2 pi (x + y - t (v1 + v2)) + phi),
+ *            velocity (v1, v2), pressure p; params (r0, A, v1, v2, p, phi)
+ *  GRADIENT  Euler linear state of the reference's operator tests
+ *  TABLE     values uploaded with mf_set_boundary_values */
+#define MF_FN_CONST 0
+#define MF_FN_PULSE 1
+#define MF_FN_WAVE 2
+#define MF_FN_GRADIENT 3
+#define MF_FN_TABLE 4
+
+/* operators */
+#define MF_OP_FLUX 0            /* Dx, Dy (compute_operators.jl:409-453) */
+#define MF_OP_HYPERVISCOSITY 1  /* Dx^2k + Dy^2k (hyperviscosity.jl:43-44) */
+
+/* viscosity sources (hyperviscosity.jl) */
+#define MF_VISC_NONE 0
+#define MF_VISC_RESIDUAL 1 /* SourceResidualViscosityTominec, :181-244, :382-409 */
+#define MF_VISC_UPWIND 2   /* SourceUpwindViscosityTominec, :148-167, :351-380 */
+
+/* RHS precision */
+#define MF_PREC_F64 0     /* FP64 weights, state and accumulation */
+#define MF_PREC_SPLIT32 1 /* FP32 weights, centred FP64->FP32 differences,
+                             FP32 accumulation; FP64 state */
+
+/* state buffers owned by a context */
+#define MF_BUF_U 0      /* integrator state u */
+#define MF_BUF_DU 1     /* rhs output du */
+#define MF_BUF_AUX0 2   /* caller scratch (e.g. adjoint seed) */
+#define MF_BUF_AUX1 3   /* caller scratch (e.g. adjoint result) */
+
+/* Zhang-Shu limiter variables (Trixi.density, Trixi.pressure) */
+#define MF_VAR_DENSITY 0
+#define MF_VAR_PRESSURE 1
+
+/* mf_ssprk33 flags */
+#define MF_STEP_GRAPH 1   /* capture one step as a hipGraph and replay it */
+#define MF_STEP_LIMITER 2 /* apply the configured stage limiter */
+#define MF_STEP_HISTORY 4 /* HistoryCallback at init and after every step */
+
+/* ------------------------------------------------------------- context */
+/* Creates a context on HIP device `device` with its own stream. */
+int mf_ctx_create(int device, mf_ctx** out);
+int mf_ctx_destroy(mf_ctx* ctx);
+/* Message of the last failed call on ctx ("" if none); valid until the next call. */
+const char* mf_last_error(const mf_ctx* ctx);
+/* Blocks until all work queued on ctx's stream has finished. */
+int mf_synchronize(mf_ctx* ctx);
+
+/* --------------------------------------------------------------- cloud */
+/* Node coordinates.  Replaces the `points` of PointData
+ * (domains/PointCloudDomain/geometry_primatives.jl:297-304, :322-339). */
+int mf_set_points(mf_ctx* ctx, const double* x, const double* y, int64_t n_points);
+
+/* k nearest neighbours of every node (self first), ordered by squared
+ * distance, ties by index.  Replaces knn(KDTree(points), points, nv, true)
+ * and the dx_min/dx_avg statistics of PointData(points, basis)
+ * (geometry_primatives.jl:327-334).  The stencil stays on the device for
+ * mf_compute_operator.  idx_out (n_points*k, row per node) and dist_out
+ * (Euclidean distances, same shape) may be NULL. */
+int mf_knn(mf_ctx* ctx, int k, int32_t* idx_out, double* dist_out, double* dx_min,
+           double* dx_avg);
+
+/* Uploads a stencil computed elsewhere (n_points x k, row per node, self
+ * first); replaces PointData.neighbors (geometry_primatives.jl:299). */
+int mf_set_stencil(mf_ctx* ctx, const int32_t* idx, int64_t n_points, int k);
+
+/* ------------------------------------------------------------- weights */
+/* RBF-FD weights on the current stencil: PHS r^phs_power plus monomials of
+ * total degree <= poly_degree, derivative order deriv_order along x and y.
+ * target MF_OP_FLUX (deriv_order 1) replaces compute_flux_operator(solver,
+ * domain) (pointcloudsolver/compute_operators.jl:409-453); target
+ * MF_OP_HYPERVISCOSITY (deriv_order 2k) replaces compute_flux_operator(
+ * solver, domain, 2k) summed into one operator (compute_operators.jl:549-594,
+ * sources/hyperviscosity.jl:43-44).  Weights stay on the device; wx_out /
+ * wy_out (n_points*k, row per node, may be NULL) receive the x / y operator.
+ * Returns MF_ERR_SINGULAR if some stencil's system is singular. */
+int mf_compute_operator(mf_ctx* ctx, int phs_power, int poly_degree, int deriv_order,
+                        int target, double* wx_out, double* wy_out);
+
+/* Installs host weights for the current stencil (n_points*k each, row per
+ * node).  MF_OP_FLUX: wx = Dx, wy = Dy.  MF_OP_HYPERVISCOSITY: wx = H,
+ * wy must be NULL.  Replaces the cached rbf_differentiation_matrices /
+ * hv_differentiation_matrix (pointcloudsolver/rbfsolver.jl:139,162). */
+int mf_set_operator(mf_ctx* ctx, int target, const double* wx, const double* wy);
+
+/* Batched weights for ragged stencils over the cloud of mf_set_points:
+ * stencil e = idx[ptr[e] .. ptr[e+1]), centre first.  The same per-stencil
+ * computation as compute_operators.jl:422-441, for many stencils of uneven
+ * size in one launch.  ptr/idx are uploaded once; mf_weights_ragged_run
+ * recomputes on the device (timing loop), mf_weights_ragged_get copies
+ * the result (layout like idx). */
+int mf_weights_ragged_set(mf_ctx* ctx, const int64_t* ptr, const int32_t* idx,
+                          int64_t n_stencils);
+int mf_weights_ragged_run(mf_ctx* ctx, int phs_power, int poly_degree, int deriv_order);
+int mf_weights_ragged_get(mf_ctx* ctx, double* wx_out, double* wy_out);
+
+/* ------------------------------------------------------------- physics */
+/* MF_EQ_ADVECTION_2D: p0, p1 = advection velocity; MF_EQ_EULER_2D: p0 = gamma.
+ * Replaces the `equations` argument of Trixi.rhs! (rbfsolver.jl:397). */
+int mf_set_equations(mf_ctx* ctx, int equations, double p0, double p1);
+
+/* Boundary conditions, one tag after another in the order of the
+ * reference's boundary_conditions NamedTuple (rbfsolver.jl:277-286).
+ * Tag g covers entries tag_ptr[g] .. tag_ptr[g+1] of node_idx / normals
+ * (2 per entry); tag_kind[g] is MF_BC_*, tag_fn[g] the MF_FN_* value
+ * function of a Dirichlet tag, tag_params 8 doubles per tag.  Replaces
+ * domain.boundary_tags + boundary_conditions (rbfsolver.jl:288-318). */
+int mf_set_boundary(mf_ctx* ctx, int n_tags, const int32_t* tag_ptr, const int32_t* tag_kind,
+                    const int32_t* tag_fn, const double* tag_params, const int32_t* node_idx,
+                    const double* normals);
+/* Values for MF_FN_TABLE tags: nvar doubles per boundary entry. */
+int mf_set_boundary_values(mf_ctx* ctx, const double* values);
+
+/* Flyer hyperviscosity: du -= gamma * H u with the MF_OP_HYPERVISCOSITY
+ * operator; gamma = c * dx_min^(2k).  Replaces SourceHyperviscosityFlyer
+ * (hyperviscosity.jl:14-64).  gamma = 0 removes the source. */
+int mf_set_source_hyperviscosity(mf_ctx* ctx, double gamma);
+
+/* Residual / upwind viscosity: du -= sum_d D_d^T (eps .* (D_d u)).
+ * Replaces SourceResidualViscosityTominec / SourceUpwindViscosityTominec
+ * (hyperviscosity.jl:181-409).  h = dx_avg.  hv_first = 1 if the
+ * hyperviscosity source comes before this one in the SourceTerms keyword
+ * order (generic_sources.jl:7-18, rbfsolver.jl:388-395). */
+int mf_set_source_viscosity(mf_ctx* ctx, int mode, double c_rv, double c_uw, double h,
+                            int approx_order, int hv_first);
+
+/* --------------------------------------------------------------- state */
+/* Copies nvar host fields into / out of a context buffer (MF_BUF_*). */
+int mf_state_upload(mf_ctx* ctx, int buf, const double* const* fields);
+int mf_state_download(mf_ctx* ctx, int buf, double* const* fields);
+/* Device address of a context buffer (node-major, nvar doubles per node). */
+int mf_state_device_ptr(mf_ctx* ctx, int buf, double** dptr);
+
+/* ----------------------------------------------------------------- rhs! */
+/* du = rhs(u, t) on device buffers; u (MF_BUF_U) is mutated at boundary
+ * nodes exactly like the reference.  Replaces Trixi.rhs!(du, u, t, domain,
+ * equations, ic, bcs, sources, solver, cache), rbfsolver.jl:397-428. */
+int mf_rhs(mf_ctx* ctx, double t, int precision);
+
+/* The same on host fields (upload, rhs!, download of du and u). */
+int mf_rhs_host(mf_ctx* ctx, double t, int precision, double* const* u, double* const* du);
+
+/* Viscosity of the last rhs! with a viscosity source: eps and the choice
+ * code eps_c (0 residual, 1 upwind, 2 eps()), hyperviscosity.jl:331-349. */
+int mf_get_viscosity(mf_ctx* ctx, double* eps, int8_t* eps_c);
+
+/* --------------------------------------------------- step/stage hooks */
+/* HistoryCallback affect! for the viscosity source: shift the solution
+ * history with u = MF_BUF_U at time t and rebuild approx_du
+ * (callbacks_step/history.jl:178-245).  success_iter = integrator.success_iter. */
+int mf_history_update(mf_ctx* ctx, double t, int success_iter);
+int mf_get_approx_du(mf_ctx* ctx, double* const* fields);
+
+/* Stage limiter: Trixi.limiter_zhang_shu!(u, threshold, variable, domain,
+ * ...) on MF_BUF_U (callbacks_stage/positivity_zhang_shu_point2d.jl:22-83). */
+int mf_limiter_zhang_shu(mf_ctx* ctx, double threshold, int variable);
+/* Limiter used by mf_ssprk33 with MF_STEP_LIMITER (up to 2 variables). */
+int mf_set_stage_limiter(mf_ctx* ctx, int n, const double* thresholds, const int32_t* variables);
+
+/* ------------------------------------------------------ time stepping */
+/* n_steps of SSPRK33 (Shu-Osher form, FSAL) from MF_BUF_U at t0 with step
+ * dt; with MF_STEP_HISTORY and a residual viscosity source the
+ * HistoryCallback runs at init (when restart == 0) and after every step.  Replaces solve(ode, SSPRK33(...),
+ * dt, callbacks) on the reference's semidiscretization.  restart = 1
+ * continues from the previous call's FSAL and history state. */
+int mf_ssprk33(mf_ctx* ctx, double t0, double dt, int n_steps, int precision, int flags,
+               int restart);
+
+/* ------------------------------------------------------------- adjoint */
+/* ubar = (d rhs / d u)^T lambda at u = MF_BUF_U (flux divergence, boundary
+ * conditions; no sources) with lambda = MF_BUF_AUX0, result in MF_BUF_AUX1.
+ * No counterpart in the reference (which has no adjoint). */
+int mf_rhs_vjp(mf_ctx* ctx, double t);
+
+/* Gradient of L = sum_i rho_i(T)^2 after n_steps SSPRK33 steps from
+ * u0 = MF_BUF_U with respect to u0 (result in MF_BUF_AUX1, all
+ * variables); checkpoints every `checkpoint_every` steps.  No counterpart
+ * in the reference. */
+int mf_ssprk33_vjp(mf_ctx* ctx, double t0, double dt, int n_steps, int checkpoint_every,
+                   double* loss);
+
+/* ------------------------------------------------------------- timing */
+/* Average device time of one rhs! (ms) over n_calls back-to-back calls on
+ * MF_BUF_U, and of its dominant kernel alone (kernel_ms), measured with
+ * HIP events on the context stream. */
+int mf_time_rhs(mf_ctx* ctx, double t, int precision, int n_calls, double* rhs_ms,
+                double* kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

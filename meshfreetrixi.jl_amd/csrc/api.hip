@@ -1,0 +1,948 @@
+// C ABI of libmfrbf.so (declared in include/mfrbf.h): context and buffer
+// management, uploads/downloads, and the orchestration of the kernels in
+// setup.hip, rhs.hip and adjoint.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <set>
+#include <vector>
+
+#include "mf_device.h"
+
+namespace mf {
+
+int fail(mf_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(mf_ctx* c, hipError_t e, const char* what)
+{
+    if (c) c->err = std::string(what) + ": " + hipGetErrorString(e);
+    return MF_ERR_HIP;
+}
+
+template <class T>
+int dalloc(mf_ctx* c, T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * count);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMalloc");
+    return MF_OK;
+}
+template int dalloc<double>(mf_ctx*, double**, size_t);
+template int dalloc<double2>(mf_ctx*, double2**, size_t);
+template int dalloc<float2>(mf_ctx*, float2**, size_t);
+template int dalloc<float>(mf_ctx*, float**, size_t);
+template int dalloc<int32_t>(mf_ctx*, int32_t**, size_t);
+template int dalloc<int64_t>(mf_ctx*, int64_t**, size_t);
+template int dalloc<int16_t>(mf_ctx*, int16_t**, size_t);
+template int dalloc<int8_t>(mf_ctx*, int8_t**, size_t);
+template int dalloc<uint8_t>(mf_ctx*, uint8_t**, size_t);
+
+void dfree(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+// k-th derivative of r^p along s as sum_t c_t s^a_t r^b_t, built by
+// repeated d/ds (s^a r^b) = a s^(a-1) r^b + b s^(a+1) r^(b-2).
+PhsTable phs_table(int p, int k)
+{
+    PhsTable t{};
+    double cc[16], c2[16];
+    int aa[16], bb[16], a2[16], b2[16];
+    int nt = 1;
+    cc[0] = 1.0;
+    aa[0] = 0;
+    bb[0] = p;
+    for (int it = 0; it < k; ++it) {
+        int n2 = 0;
+        for (int q = 0; q < nt; ++q) {
+            if (aa[q] > 0 && n2 < 16) {
+                c2[n2] = cc[q] * aa[q];
+                a2[n2] = aa[q] - 1;
+                b2[n2] = bb[q];
+                ++n2;
+            }
+            if (bb[q] != 0 && n2 < 16) {
+                c2[n2] = cc[q] * bb[q];
+                a2[n2] = aa[q] + 1;
+                b2[n2] = bb[q] - 2;
+                ++n2;
+            }
+        }
+        nt = 0;
+        for (int q = 0; q < n2; ++q) {
+            int f = -1;
+            for (int r = 0; r < nt; ++r)
+                if (aa[r] == a2[q] && bb[r] == b2[q]) f = r;
+            if (f >= 0) {
+                cc[f] += c2[q];
+            } else {
+                cc[nt] = c2[q];
+                aa[nt] = a2[q];
+                bb[nt] = b2[q];
+                ++nt;
+            }
+        }
+    }
+    t.nt = nt;
+    for (int q = 0; q < nt; ++q) {
+        t.c[q] = cc[q];
+        t.a[q] = aa[q];
+        t.b[q] = bb[q];
+    }
+    return t;
+}
+
+int bc_pass_homog(mf_ctx* c, double* u);
+int bc_pass_state(mf_ctx* c, double* u, double dt_off);
+
+static void free_ops(mf_ctx* c)
+{
+    dfree(c->d_wxy);
+    dfree(c->d_hv);
+    dfree(c->d_wxy32);
+    dfree(c->d_hv32);
+    dfree(c->d_sell_off);
+    dfree(c->d_sell_w);
+    dfree(c->d_trow);
+    dfree(c->d_tw);
+    dfree(c->d_tw32);
+    c->d_wxy = nullptr;
+    c->d_hv = nullptr;
+    c->d_wxy32 = nullptr;
+    c->d_hv32 = nullptr;
+    c->d_sell_off = nullptr;
+    c->d_sell_w = nullptr;
+    c->d_trow = nullptr;
+    c->d_tw = nullptr;
+    c->d_tw32 = nullptr;
+    c->have_flux = c->have_hv = c->have_T = false;
+}
+
+static void free_graph(mf_ctx* c)
+{
+    if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
+    if (c->step_graph) (void)hipGraphDestroy(c->step_graph);
+    c->step_exec = nullptr;
+    c->step_graph = nullptr;
+    c->graph_prec = -1;
+}
+
+static void free_state(mf_ctx* c)
+{
+    for (auto& b : c->d_buf) {
+        dfree(b);
+        b = nullptr;
+    }
+    dfree(c->d_S);
+    dfree(c->d_T);
+    dfree(c->d_G);
+    dfree(c->d_eps);
+    dfree(c->d_epsc);
+    dfree(c->d_approx);
+    dfree(c->d_hist);
+    dfree(c->d_partials);
+    dfree(c->d_lim);
+    dfree(c->d_adj);
+    c->d_S = c->d_T = c->d_G = c->d_eps = c->d_approx = c->d_hist = c->d_partials = c->d_lim = nullptr;
+    c->d_adj = nullptr;
+    c->adj_len = 0;
+    c->d_epsc = nullptr;
+    c->fsal_valid = false;
+    free_graph(c);
+}
+
+static void free_bc(mf_ctx* c)
+{
+    dfree(c->bc.bidx);
+    dfree(c->bc.btag);
+    dfree(c->bc.bnormal);
+    dfree(c->bc.btable);
+    dfree(c->bc.flag);
+    dfree(c->bc.entry);
+    c->bc.bidx = c->bc.btag = c->bc.entry = nullptr;
+    c->bc.bnormal = c->bc.btable = nullptr;
+    c->bc.flag = nullptr;
+    c->have_bc = false;
+}
+
+static void free_stencil(mf_ctx* c)
+{
+    dfree(c->d_idx);
+    dfree(c->d_didx);
+    c->d_idx = nullptr;
+    c->d_didx = nullptr;
+    c->k = 0;
+    c->didx_ok = false;
+    free_ops(c);
+}
+
+// device buffers that depend on N and nvar
+int ensure_state(mf_ctx* c)
+{
+    if (c->N == 0) return fail(c, MF_ERR_STATE, "no point cloud (mf_set_points)");
+    const size_t nv = (size_t)c->N * c->nvar;
+    int rc;
+    for (auto& b : c->d_buf)
+        if (!b && (rc = dalloc(c, &b, nv))) return rc;
+    if (!c->d_S && (rc = dalloc(c, &c->d_S, nv))) return rc;
+    if (!c->d_T && (rc = dalloc(c, &c->d_T, nv))) return rc;
+    if (c->visc_mode != MF_VISC_NONE) {
+        if (!c->d_G && (rc = dalloc(c, &c->d_G, 2 * nv))) return rc;
+        if (!c->d_eps && (rc = dalloc(c, &c->d_eps, c->N))) return rc;
+        if (!c->d_epsc && (rc = dalloc(c, &c->d_epsc, c->N))) return rc;
+        if (!c->d_approx) {
+            if ((rc = dalloc(c, &c->d_approx, nv))) return rc;
+            MF_HIP(c, hipMemsetAsync(c->d_approx, 0, sizeof(double) * nv, c->stream));
+        }
+        if (!c->d_hist && (rc = dalloc(c, &c->d_hist, nv * (size_t)(c->approx_order + 1)))) return rc;
+        if (!c->d_partials) {
+            c->n_partials = (int)std::min<int64_t>(1024, (c->N + 255) / 256);
+            if ((rc = dalloc(c, &c->d_partials, 4 * (size_t)c->n_partials))) return rc;
+        }
+    }
+    if (!c->d_lim && (rc = dalloc(c, &c->d_lim, nv))) return rc;
+    return MF_OK;
+}
+
+int ensure_transpose(mf_ctx* c)
+{
+    if (c->visc_mode == MF_VISC_NONE || c->have_T) return MF_OK;
+    if (!c->have_flux) return fail(c, MF_ERR_STATE, "no flux operator");
+    return launch_build_transpose(c);
+}
+
+static int check_ctx(mf_ctx* c)
+{
+    if (!c) return MF_ERR_ARGUMENT;
+    c->err.clear();
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+    return MF_OK;
+}
+
+static int upload_fields(mf_ctx* c, double* d_dst, const double* const* fields)
+{
+    if (!fields) return fail(c, MF_ERR_ARGUMENT, "null fields");
+    for (int v = 0; v < c->nvar; ++v)
+        if (!fields[v]) return fail(c, MF_ERR_ARGUMENT, "null field pointer");
+    // stage in d_T (SoA), then interleave
+    for (int v = 0; v < c->nvar; ++v)
+        MF_HIP(c, hipMemcpyAsync(c->d_T + (size_t)v * c->N, fields[v], sizeof(double) * c->N,
+                                 hipMemcpyHostToDevice, c->stream));
+    return launch_soa_to_aos(c, c->d_T, d_dst);
+}
+
+static int download_fields(mf_ctx* c, const double* d_src, double* const* fields)
+{
+    if (!fields) return fail(c, MF_ERR_ARGUMENT, "null fields");
+    for (int v = 0; v < c->nvar; ++v)
+        if (!fields[v]) return fail(c, MF_ERR_ARGUMENT, "null field pointer");
+    int rc = launch_aos_to_soa(c, d_src, c->d_T);
+    if (rc) return rc;
+    for (int v = 0; v < c->nvar; ++v)
+        MF_HIP(c, hipMemcpyAsync(fields[v], c->d_T + (size_t)v * c->N, sizeof(double) * c->N,
+                                 hipMemcpyDeviceToHost, c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    return MF_OK;
+}
+
+}  // namespace mf
+
+using namespace mf;
+
+extern "C" {
+
+int mf_ctx_create(int device, mf_ctx** out)
+{
+    if (!out) return MF_ERR_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return MF_ERR_HIP;
+    if (hipSetDevice(device) != hipSuccess) return MF_ERR_HIP;
+    mf_ctx* c = new mf_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return MF_ERR_HIP;
+    }
+    (void)hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, device);
+    if (dalloc(c, &c->d_sc, 1) != MF_OK ||
+        hipMemset(c->d_sc, 0, sizeof(StepScalars)) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return MF_ERR_HIP;
+    }
+    *out = c;
+    return MF_OK;
+}
+
+int mf_ctx_destroy(mf_ctx* c)
+{
+    if (!c) return MF_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_state(c);
+    free_stencil(c);
+    free_bc(c);
+    dfree(c->d_x);
+    dfree(c->d_y);
+    dfree(c->d_sc);
+    dfree(c->d_rptr);
+    dfree(c->d_ridx);
+    dfree(c->d_rwx);
+    dfree(c->d_rwy);
+    dfree(c->d_status);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MF_OK;
+}
+
+const char* mf_last_error(const mf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mf_synchronize(mf_ctx* c)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    return MF_OK;
+}
+
+int mf_set_points(mf_ctx* c, const double* x, const double* y, int64_t n)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!x || !y || n < 2) return fail(c, MF_ERR_ARGUMENT, "mf_set_points: need >= 2 points");
+    if (n > (int64_t)INT32_MAX) return fail(c, MF_ERR_ARGUMENT, "mf_set_points: too many points");
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    if (n != c->N) {
+        free_state(c);
+        free_stencil(c);
+        free_bc(c);
+        dfree(c->d_x);
+        dfree(c->d_y);
+        c->d_x = c->d_y = nullptr;
+        c->N = n;
+        if ((rc = dalloc(c, &c->d_x, n)) || (rc = dalloc(c, &c->d_y, n))) return rc;
+    } else {
+        free_graph(c);
+    }
+    MF_HIP(c, hipMemcpyAsync(c->d_x, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    MF_HIP(c, hipMemcpyAsync(c->d_y, y, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    return MF_OK;
+}
+
+static int set_k(mf_ctx* c, int k)
+{
+    free_stencil(c);
+    free_graph(c);
+    int rc;
+    c->k = k;
+    if ((rc = dalloc(c, &c->d_idx, (size_t)k * c->N)) || (rc = dalloc(c, &c->d_didx, (size_t)k * c->N)))
+        return rc;
+    return MF_OK;
+}
+
+int mf_knn(mf_ctx* c, int k, int32_t* idx_out, double* dist_out, double* dx_min, double* dx_avg)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->N == 0) return fail(c, MF_ERR_STATE, "mf_knn: no points");
+    if (k < 2 || k > 64 || k > c->N) return fail(c, MF_ERR_ARGUMENT, "mf_knn: need 2 <= k <= min(64, N)");
+    if ((rc = set_k(c, k))) return rc;
+    double* d2 = nullptr;
+    if ((rc = dalloc(c, &d2, (size_t)k * c->N))) return rc;
+    if ((rc = launch_knn(c, k, d2))) return rc;
+    if ((rc = launch_build_didx(c))) return rc;
+    double mn = 0, av = 0;
+    if ((rc = launch_dx_stats(c, d2, &mn, &av))) return rc;
+    if (dx_min) *dx_min = mn;
+    if (dx_avg) *dx_avg = av;
+    if (idx_out) {
+        int32_t* rows = nullptr;
+        if ((rc = dalloc(c, &rows, (size_t)k * c->N))) return rc;
+        if ((rc = launch_ell_to_rows(c, c->d_idx, rows, k))) return rc;
+        MF_HIP(c, hipMemcpyAsync(idx_out, rows, sizeof(int32_t) * k * c->N, hipMemcpyDeviceToHost, c->stream));
+        MF_HIP(c, hipStreamSynchronize(c->stream));
+        dfree(rows);
+    }
+    if (dist_out) {
+        double* rows = nullptr;
+        if ((rc = dalloc(c, &rows, (size_t)k * c->N))) return rc;
+        if ((rc = launch_ell_to_rows_d(c, d2, rows, k))) return rc;
+        MF_HIP(c, hipMemcpyAsync(dist_out, rows, sizeof(double) * k * c->N, hipMemcpyDeviceToHost, c->stream));
+        MF_HIP(c, hipStreamSynchronize(c->stream));
+        dfree(rows);
+        for (int64_t q = 0; q < (int64_t)k * c->N; ++q) dist_out[q] = std::sqrt(dist_out[q]);
+    }
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(d2);
+    return MF_OK;
+}
+
+int mf_set_stencil(mf_ctx* c, const int32_t* idx, int64_t n_points, int k)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!idx || n_points != c->N || k < 1 || k > 64) return fail(c, MF_ERR_ARGUMENT, "mf_set_stencil: bad size");
+    for (int64_t q = 0; q < n_points * k; ++q)
+        if (idx[q] < 0 || idx[q] >= n_points) return fail(c, MF_ERR_ARGUMENT, "mf_set_stencil: index out of range");
+    if ((rc = set_k(c, k))) return rc;
+    int32_t* rows = nullptr;
+    if ((rc = dalloc(c, &rows, (size_t)k * c->N))) return rc;
+    MF_HIP(c, hipMemcpyAsync(rows, idx, sizeof(int32_t) * k * c->N, hipMemcpyHostToDevice, c->stream));
+    if ((rc = launch_rows_to_ell(c, rows, c->d_idx, k))) return rc;
+    if ((rc = launch_build_didx(c))) return rc;
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(rows);
+    return MF_OK;
+}
+
+static int alloc_op(mf_ctx* c, int target)
+{
+    const size_t n = (size_t)c->N * c->k;
+    int rc;
+    if (target == MF_OP_FLUX) {
+        if (!c->d_wxy && (rc = dalloc(c, &c->d_wxy, n))) return rc;
+        if (!c->d_wxy32 && (rc = dalloc(c, &c->d_wxy32, n))) return rc;
+    } else {
+        if (!c->d_hv && (rc = dalloc(c, &c->d_hv, n))) return rc;
+        if (!c->d_hv32 && (rc = dalloc(c, &c->d_hv32, n))) return rc;
+    }
+    return MF_OK;
+}
+
+static void op_installed(mf_ctx* c, int target)
+{
+    free_graph(c);
+    if (target == MF_OP_FLUX) {
+        c->have_flux = true;
+        c->have_T = false;
+    } else {
+        c->have_hv = true;
+    }
+}
+
+int mf_compute_operator(mf_ctx* c, int phs_power, int poly_degree, int deriv_order, int target,
+                        double* wx_out, double* wy_out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->k == 0) return fail(c, MF_ERR_STATE, "mf_compute_operator: no stencil");
+    if (target != MF_OP_FLUX && target != MF_OP_HYPERVISCOSITY)
+        return fail(c, MF_ERR_ARGUMENT, "mf_compute_operator: bad target");
+    if (phs_power < 1 || poly_degree < 0 || deriv_order < 1 || deriv_order > 8)
+        return fail(c, MF_ERR_ARGUMENT, "mf_compute_operator: bad basis");
+    const int np = (poly_degree + 1) * (poly_degree + 2) / 2;
+    if (c->k + np > kMaxStencilM) return fail(c, MF_ERR_UNSUPPORTED, "mf_compute_operator: stencil + monomials > 64");
+    if (np > c->k) return fail(c, MF_ERR_ARGUMENT, "mf_compute_operator: fewer stencil nodes than monomials");
+    const size_t n = (size_t)c->N * c->k;
+    double *wx = nullptr, *wy = nullptr;
+    if ((rc = dalloc(c, &wx, n)) || (rc = dalloc(c, &wy, n))) return rc;
+    if (!c->d_status && (rc = dalloc(c, &c->d_status, 1))) return rc;
+    MF_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
+    if ((rc = launch_weights(c, nullptr, c->d_idx, c->k, c->N, c->k, phs_power, poly_degree, deriv_order,
+                             wx, wy, c->d_status)))
+        return rc;
+    if ((rc = alloc_op(c, target))) return rc;
+    if ((rc = launch_pack_ops(c, target, wx, wy))) return rc;
+    int st = 0;
+    MF_HIP(c, hipMemcpyAsync(&st, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    for (int w = 0; w < 2; ++w) {
+        double* out = w == 0 ? wx_out : wy_out;
+        if (!out) continue;
+        double* rows = nullptr;
+        if ((rc = dalloc(c, &rows, n))) return rc;
+        if ((rc = launch_ell_to_rows_d(c, w == 0 ? wx : wy, rows, c->k))) return rc;
+        MF_HIP(c, hipMemcpyAsync(out, rows, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        MF_HIP(c, hipStreamSynchronize(c->stream));
+        dfree(rows);
+    }
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(wx);
+    dfree(wy);
+    op_installed(c, target);
+    if (st) return fail(c, MF_ERR_SINGULAR, "mf_compute_operator: singular stencil system");
+    return MF_OK;
+}
+
+int mf_set_operator(mf_ctx* c, int target, const double* wx, const double* wy)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->k == 0) return fail(c, MF_ERR_STATE, "mf_set_operator: no stencil");
+    if (!wx || (target == MF_OP_FLUX && !wy) || (target == MF_OP_HYPERVISCOSITY && wy) ||
+        (target != MF_OP_FLUX && target != MF_OP_HYPERVISCOSITY))
+        return fail(c, MF_ERR_ARGUMENT, "mf_set_operator: bad arguments");
+    const size_t n = (size_t)c->N * c->k;
+    double *rx = nullptr, *ex = nullptr, *ey = nullptr;
+    if ((rc = dalloc(c, &rx, n)) || (rc = dalloc(c, &ex, n))) return rc;
+    if (wy && (rc = dalloc(c, &ey, n))) return rc;
+    MF_HIP(c, hipMemcpyAsync(rx, wx, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    if ((rc = launch_rows_to_ell_d(c, rx, ex, c->k))) return rc;
+    if (wy) {
+        MF_HIP(c, hipStreamSynchronize(c->stream));
+        MF_HIP(c, hipMemcpyAsync(rx, wy, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        if ((rc = launch_rows_to_ell_d(c, rx, ey, c->k))) return rc;
+    }
+    if ((rc = alloc_op(c, target))) return rc;
+    if ((rc = launch_pack_ops(c, target, ex, ey))) return rc;
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(rx);
+    dfree(ex);
+    dfree(ey);
+    op_installed(c, target);
+    return MF_OK;
+}
+
+int mf_weights_ragged_set(mf_ctx* c, const int64_t* ptr, const int32_t* idx, int64_t ns)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!ptr || !idx || ns < 1) return fail(c, MF_ERR_ARGUMENT, "mf_weights_ragged_set: bad arguments");
+    if (c->N == 0) return fail(c, MF_ERR_STATE, "mf_weights_ragged_set: no points");
+    int maxn = 0;
+    for (int64_t e = 0; e < ns; ++e) {
+        const int64_t n = ptr[e + 1] - ptr[e];
+        if (n < 1 || n > 64) return fail(c, MF_ERR_ARGUMENT, "mf_weights_ragged_set: stencil size out of range");
+        maxn = std::max<int>(maxn, (int)n);
+    }
+    const int64_t nnz = ptr[ns] - ptr[0];
+    if (ptr[0] != 0) return fail(c, MF_ERR_ARGUMENT, "mf_weights_ragged_set: ptr[0] must be 0");
+    for (int64_t q = 0; q < nnz; ++q)
+        if (idx[q] < 0 || idx[q] >= c->N) return fail(c, MF_ERR_ARGUMENT, "mf_weights_ragged_set: index out of range");
+    dfree(c->d_rptr);
+    dfree(c->d_ridx);
+    dfree(c->d_rwx);
+    dfree(c->d_rwy);
+    c->d_rptr = nullptr;
+    c->d_ridx = nullptr;
+    c->d_rwx = c->d_rwy = nullptr;
+    if ((rc = dalloc(c, &c->d_rptr, ns + 1)) || (rc = dalloc(c, &c->d_ridx, nnz)) ||
+        (rc = dalloc(c, &c->d_rwx, nnz)) || (rc = dalloc(c, &c->d_rwy, nnz)))
+        return rc;
+    if (!c->d_status && (rc = dalloc(c, &c->d_status, 1))) return rc;
+    MF_HIP(c, hipMemcpyAsync(c->d_rptr, ptr, sizeof(int64_t) * (ns + 1), hipMemcpyHostToDevice, c->stream));
+    MF_HIP(c, hipMemcpyAsync(c->d_ridx, idx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    c->rn = ns;
+    c->rnnz = nnz;
+    c->rmaxn = maxn;
+    return MF_OK;
+}
+
+int mf_weights_ragged_run(mf_ctx* c, int phs_power, int poly_degree, int deriv_order)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->rn == 0) return fail(c, MF_ERR_STATE, "mf_weights_ragged_run: no stencils");
+    if (phs_power < 1 || poly_degree < 0 || deriv_order < 1 || deriv_order > 8)
+        return fail(c, MF_ERR_ARGUMENT, "mf_weights_ragged_run: bad basis");
+    MF_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
+    return launch_weights(c, c->d_rptr, c->d_ridx, 0, c->rn, c->rmaxn, phs_power, poly_degree, deriv_order,
+                          c->d_rwx, c->d_rwy, c->d_status);
+}
+
+int mf_weights_ragged_get(mf_ctx* c, double* wx, double* wy)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->rn == 0) return fail(c, MF_ERR_STATE, "mf_weights_ragged_get: no stencils");
+    int st = 0;
+    MF_HIP(c, hipMemcpyAsync(&st, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (wx) MF_HIP(c, hipMemcpyAsync(wx, c->d_rwx, sizeof(double) * c->rnnz, hipMemcpyDeviceToHost, c->stream));
+    if (wy) MF_HIP(c, hipMemcpyAsync(wy, c->d_rwy, sizeof(double) * c->rnnz, hipMemcpyDeviceToHost, c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    if (st) return fail(c, MF_ERR_SINGULAR, "ragged weights: singular stencil system");
+    return MF_OK;
+}
+
+int mf_set_equations(mf_ctx* c, int equations, double p0, double p1)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    int nv;
+    if (equations == MF_EQ_ADVECTION_2D) {
+        nv = 1;
+    } else if (equations == MF_EQ_EULER_2D) {
+        nv = 4;
+        if (!(p0 > 1.0)) return fail(c, MF_ERR_ARGUMENT, "mf_set_equations: gamma must exceed 1");
+    } else {
+        return fail(c, MF_ERR_ARGUMENT, "mf_set_equations: unknown equations");
+    }
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    if (nv != c->nvar) free_state(c);
+    free_graph(c);
+    c->eq = equations;
+    c->nvar = nv;
+    c->p0 = p0;
+    c->p1 = p1;
+    return MF_OK;
+}
+
+int mf_set_boundary(mf_ctx* c, int n_tags, const int32_t* tag_ptr, const int32_t* tag_kind,
+                    const int32_t* tag_fn, const double* tag_params, const int32_t* node_idx,
+                    const double* normals)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->N == 0) return fail(c, MF_ERR_STATE, "mf_set_boundary: no points");
+    if (n_tags < 0 || n_tags > kMaxTags) return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: too many tags");
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    free_bc(c);
+    free_graph(c);
+    BcDev& b = c->bc;
+    b.ntags = n_tags;
+    b.nb = 0;
+    b.any_slip = 0;
+    b.multi = 0;
+    if (n_tags == 0) return MF_OK;
+    if (!tag_ptr || !tag_kind || !node_idx) return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: null arrays");
+    if (tag_ptr[0] != 0) return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: tag_ptr[0] must be 0");
+    const int nb = tag_ptr[n_tags];
+    std::vector<int32_t> btag(std::max(nb, 1));
+    std::vector<uint8_t> flag(c->N, 0);
+    std::vector<int32_t> entry(c->N, -1);
+    for (int g = 0; g < n_tags; ++g) {
+        if (tag_ptr[g + 1] < tag_ptr[g]) return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: tag_ptr not monotone");
+        const int kind = tag_kind[g];
+        if (kind < MF_BC_DIRICHLET || kind > MF_BC_DO_NOTHING)
+            return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: unknown boundary kind");
+        if (kind == MF_BC_SLIP_WALL && c->nvar != 4)
+            return fail(c, MF_ERR_UNSUPPORTED, "mf_set_boundary: slip walls need the Euler equations");
+        const int fn = tag_fn ? tag_fn[g] : MF_FN_CONST;
+        if (kind == MF_BC_DIRICHLET && (fn < MF_FN_CONST || fn > MF_FN_TABLE))
+            return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: unknown value function");
+        if (kind == MF_BC_DIRICHLET && (fn == MF_FN_WAVE || fn == MF_FN_GRADIENT) && c->nvar != 4)
+            return fail(c, MF_ERR_UNSUPPORTED, "mf_set_boundary: Euler value function on a scalar equation");
+        b.tag_kind[g] = kind;
+        b.tag_fn[g] = fn;
+        for (int q = 0; q < 8; ++q) b.tag_params[8 * g + q] = tag_params ? tag_params[8 * g + q] : 0.0;
+        b.tag_ptr[g] = tag_ptr[g];
+        if (kind == MF_BC_SLIP_WALL) b.any_slip = 1;
+        for (int e = tag_ptr[g]; e < tag_ptr[g + 1]; ++e) {
+            const int32_t node = node_idx[e];
+            if (node < 0 || node >= c->N) return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: node index out of range");
+            btag[e] = g;
+            if (entry[node] >= 0) b.multi = 1;
+            entry[node] = e;
+            if (kind == MF_BC_DIRICHLET) flag[node] |= 1u;
+            if (kind == MF_BC_SLIP_WALL) flag[node] |= 2u;
+        }
+    }
+    b.tag_ptr[n_tags] = nb;
+    b.nb = nb;
+    if ((rc = dalloc(c, &b.bidx, nb)) || (rc = dalloc(c, &b.btag, nb)) || (rc = dalloc(c, &b.bnormal, 2 * (size_t)nb)) ||
+        (rc = dalloc(c, &b.btable, (size_t)nb * 4)) || (rc = dalloc(c, &b.flag, c->N)) || (rc = dalloc(c, &b.entry, c->N)))
+        return rc;
+    std::vector<double> nrm(2 * (size_t)std::max(nb, 1), 0.0);
+    if (normals) std::copy(normals, normals + 2 * (size_t)nb, nrm.begin());
+    MF_HIP(c, hipMemcpyAsync(b.bidx, node_idx, sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream));
+    MF_HIP(c, hipMemcpyAsync(b.btag, btag.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream));
+    MF_HIP(c, hipMemcpyAsync(b.bnormal, nrm.data(), sizeof(double) * 2 * nb, hipMemcpyHostToDevice, c->stream));
+    MF_HIP(c, hipMemsetAsync(b.btable, 0, sizeof(double) * nb * 4, c->stream));
+    MF_HIP(c, hipMemcpyAsync(b.flag, flag.data(), c->N, hipMemcpyHostToDevice, c->stream));
+    MF_HIP(c, hipMemcpyAsync(b.entry, entry.data(), sizeof(int32_t) * c->N, hipMemcpyHostToDevice, c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    c->have_bc = true;
+    return MF_OK;
+}
+
+int mf_set_boundary_values(mf_ctx* c, const double* values)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->have_bc) return fail(c, MF_ERR_STATE, "mf_set_boundary_values: no boundary");
+    if (!values) return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary_values: null values");
+    MF_HIP(c, hipMemcpyAsync(c->bc.btable, values, sizeof(double) * c->bc.nb * c->nvar, hipMemcpyHostToDevice,
+                             c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    return MF_OK;
+}
+
+int mf_set_source_hyperviscosity(mf_ctx* c, double gamma)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (gamma != 0.0 && !c->have_hv) return fail(c, MF_ERR_STATE, "hyperviscosity source without its operator");
+    free_graph(c);
+    c->hv_gamma = gamma;
+    return MF_OK;
+}
+
+int mf_set_source_viscosity(mf_ctx* c, int mode, double c_rv, double c_uw, double h, int approx_order,
+                            int hv_first)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (mode < MF_VISC_NONE || mode > MF_VISC_UPWIND) return fail(c, MF_ERR_ARGUMENT, "unknown viscosity mode");
+    if (mode != MF_VISC_NONE && c->eq != MF_EQ_EULER_2D)
+        return fail(c, MF_ERR_UNSUPPORTED, "viscosity sources need the Euler equations");
+    if (approx_order < 0 || approx_order + 1 > kMaxHist) return fail(c, MF_ERR_ARGUMENT, "approx_order out of range");
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    free_graph(c);
+    dfree(c->d_hist);
+    c->d_hist = nullptr;
+    c->visc_mode = mode;
+    c->c_rv = c_rv;
+    c->c_uw = c_uw;
+    c->h_visc = h;
+    c->approx_order = approx_order;
+    c->hv_first = hv_first ? 1 : 0;
+    return MF_OK;
+}
+
+int mf_state_upload(mf_ctx* c, int buf, const double* const* fields)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (buf < 0 || buf > 3) return fail(c, MF_ERR_ARGUMENT, "bad buffer");
+    if ((rc = ensure_state(c))) return rc;
+    if ((rc = upload_fields(c, c->d_buf[buf], fields))) return rc;
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    if (buf == MF_BUF_U) c->fsal_valid = false;
+    return MF_OK;
+}
+
+int mf_state_download(mf_ctx* c, int buf, double* const* fields)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (buf < 0 || buf > 3) return fail(c, MF_ERR_ARGUMENT, "bad buffer");
+    if ((rc = ensure_state(c))) return rc;
+    return download_fields(c, c->d_buf[buf], fields);
+}
+
+int mf_state_device_ptr(mf_ctx* c, int buf, double** dptr)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (buf < 0 || buf > 3 || !dptr) return fail(c, MF_ERR_ARGUMENT, "bad buffer");
+    if ((rc = ensure_state(c))) return rc;
+    *dptr = c->d_buf[buf];
+    return MF_OK;
+}
+
+static int prepare_rhs(mf_ctx* c)
+{
+    int rc;
+    if ((rc = ensure_state(c))) return rc;
+    if ((rc = ensure_transpose(c))) return rc;
+    if (c->hv_gamma != 0.0 && !c->have_hv) return fail(c, MF_ERR_STATE, "hyperviscosity operator missing");
+    return MF_OK;
+}
+
+int mf_rhs(mf_ctx* c, double t, int precision)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = prepare_rhs(c))) return rc;
+    if ((rc = run_set_time(c, t, -1))) return rc;
+    StageOut none;
+    if ((rc = run_rhs(c, precision, c->d_buf[MF_BUF_U], c->d_buf[MF_BUF_DU], 0.0, none))) return rc;
+    c->fsal_valid = false;
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    return MF_OK;
+}
+
+int mf_rhs_host(mf_ctx* c, double t, int precision, double* const* u, double* const* du)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if ((rc = prepare_rhs(c))) return rc;
+    if ((rc = upload_fields(c, c->d_buf[MF_BUF_U], u))) return rc;
+    if ((rc = mf_rhs(c, t, precision))) return rc;
+    if ((rc = download_fields(c, c->d_buf[MF_BUF_DU], du))) return rc;
+    return download_fields(c, c->d_buf[MF_BUF_U], u);
+}
+
+int mf_get_viscosity(mf_ctx* c, double* eps, int8_t* eps_c)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->visc_mode == MF_VISC_NONE || !c->d_eps) return fail(c, MF_ERR_STATE, "no viscosity source evaluated");
+    if (eps) MF_HIP(c, hipMemcpyAsync(eps, c->d_eps, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream));
+    if (eps_c) MF_HIP(c, hipMemcpyAsync(eps_c, c->d_epsc, c->N, hipMemcpyDeviceToHost, c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    return MF_OK;
+}
+
+int mf_history_update(mf_ctx* c, double t, int success_iter)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->visc_mode != MF_VISC_RESIDUAL) return MF_OK;  // modify_cache! fallback: no-op
+    if (success_iter < 0) return fail(c, MF_ERR_ARGUMENT, "success_iter < 0");
+    if ((rc = ensure_state(c))) return rc;
+    if ((rc = run_set_time(c, t, success_iter))) return rc;
+    if ((rc = run_history(c, c->d_buf[MF_BUF_U], success_iter == 0 ? 1 : 0))) return rc;
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    c->fsal_valid = false;
+    return MF_OK;
+}
+
+int mf_get_approx_du(mf_ctx* c, double* const* fields)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->d_approx) return fail(c, MF_ERR_STATE, "no residual viscosity history");
+    return download_fields(c, c->d_approx, fields);
+}
+
+int mf_limiter_zhang_shu(mf_ctx* c, double threshold, int variable)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->k == 0) return fail(c, MF_ERR_STATE, "limiter: no stencil");
+    if (variable != MF_VAR_DENSITY && variable != MF_VAR_PRESSURE) return fail(c, MF_ERR_ARGUMENT, "limiter: bad variable");
+    if (variable == MF_VAR_PRESSURE && c->nvar != 4) return fail(c, MF_ERR_UNSUPPORTED, "limiter: pressure needs Euler");
+    if ((rc = ensure_state(c))) return rc;
+    if ((rc = run_limiter(c, c->d_buf[MF_BUF_U], threshold, variable))) return rc;
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    return MF_OK;
+}
+
+int mf_set_stage_limiter(mf_ctx* c, int n, const double* thresholds, const int32_t* variables)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (n < 0 || n > 2 || (n > 0 && (!thresholds || !variables))) return fail(c, MF_ERR_ARGUMENT, "stage limiter: bad arguments");
+    for (int q = 0; q < n; ++q)
+        if (variables[q] != MF_VAR_DENSITY && variables[q] != MF_VAR_PRESSURE)
+            return fail(c, MF_ERR_ARGUMENT, "stage limiter: bad variable");
+    free_graph(c);
+    c->lim_n = n;
+    for (int q = 0; q < n; ++q) {
+        c->lim_thr[q] = thresholds[q];
+        c->lim_var[q] = variables[q];
+    }
+    return MF_OK;
+}
+
+// One SSPRK33 step on U with the FSAL stage S = U + dt f(U) already formed.
+static int ssprk_step(mf_ctx* c, double dt, int prec, bool limiter, bool history)
+{
+    int rc;
+    auto limit = [&](double* u) -> int {
+        if (!limiter) return MF_OK;
+        for (int q = 0; q < c->lim_n; ++q) {
+            int r2 = run_limiter(c, u, c->lim_thr[q], c->lim_var[q]);
+            if (r2) return r2;
+        }
+        return MF_OK;
+    };
+    double* U = c->d_buf[MF_BUF_U];
+    if ((rc = limit(c->d_S))) return rc;
+    StageOut s2;
+    s2.kind = ST_TWO;
+    s2.dt = dt;
+    s2.uprev = U;
+    s2.out = c->d_T;
+    if ((rc = run_rhs(c, prec, c->d_S, c->d_buf[MF_BUF_DU], dt, s2))) return rc;
+    if ((rc = limit(c->d_T))) return rc;
+    StageOut s3;
+    s3.kind = ST_THREE;
+    s3.dt = dt;
+    s3.uprev = U;
+    s3.out = U;
+    if ((rc = run_rhs(c, prec, c->d_T, c->d_buf[MF_BUF_DU], dt / 2.0, s3))) return rc;
+    if ((rc = limit(U))) return rc;
+    StageOut s1;
+    s1.kind = ST_FSAL;
+    s1.dt = dt;
+    s1.out = c->d_S;
+    if ((rc = run_rhs(c, prec, U, c->d_buf[MF_BUF_DU], dt, s1))) return rc;
+    if ((rc = run_advance_time(c, dt))) return rc;
+    if (history && (rc = run_history(c, U, 0))) return rc;
+    return MF_OK;
+}
+
+int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int flags, int restart)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (n_steps < 0 || !(dt > 0.0)) return fail(c, MF_ERR_ARGUMENT, "mf_ssprk33: bad step");
+    if ((rc = prepare_rhs(c))) return rc;
+    const bool limiter = (flags & MF_STEP_LIMITER) && c->lim_n > 0;
+    const bool history = (flags & MF_STEP_HISTORY) && c->visc_mode == MF_VISC_RESIDUAL;
+    if (restart) {
+        if (!c->fsal_valid || c->graph_dt != dt)
+            return fail(c, MF_ERR_STATE, "mf_ssprk33: restart needs the previous call's state and the same dt");
+    } else {
+        if ((rc = run_set_time(c, t0, 0))) return rc;
+        if (history && (rc = run_history(c, c->d_buf[MF_BUF_U], 1))) return rc;
+        StageOut s1;
+        s1.kind = ST_FSAL;
+        s1.dt = dt;
+        s1.out = c->d_S;
+        if ((rc = run_rhs(c, precision, c->d_buf[MF_BUF_U], c->d_buf[MF_BUF_DU], 0.0, s1))) return rc;
+        c->host_success_iter = 0;
+    }
+    c->graph_dt = dt;
+    if (flags & MF_STEP_GRAPH) {
+        if (!c->step_exec || c->graph_prec != precision || c->graph_flags != flags || c->graph_dt != dt) {
+            free_graph(c);
+            MF_HIP(c, hipStreamSynchronize(c->stream));
+            MF_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            rc = ssprk_step(c, dt, precision, limiter, history);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+            c->step_graph = g;
+            MF_HIP(c, hipGraphInstantiate(&c->step_exec, g, nullptr, nullptr, 0));
+            c->graph_prec = precision;
+            c->graph_flags = flags;
+            c->graph_dt = dt;
+        }
+        for (int n = 0; n < n_steps; ++n) MF_HIP(c, hipGraphLaunch(c->step_exec, c->stream));
+    } else {
+        for (int n = 0; n < n_steps; ++n)
+            if ((rc = ssprk_step(c, dt, precision, limiter, history))) return rc;
+    }
+    c->host_success_iter += n_steps;
+    c->fsal_valid = true;
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    return MF_OK;
+}
+
+int mf_time_rhs(mf_ctx* c, double t, int precision, int n_calls, double* rhs_ms, double* kernel_ms)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (n_calls < 1) return fail(c, MF_ERR_ARGUMENT, "mf_time_rhs: n_calls < 1");
+    if ((rc = prepare_rhs(c))) return rc;
+    if ((rc = run_set_time(c, t, -1))) return rc;
+    StageOut none;
+    hipEvent_t e0, e1;
+    MF_HIP(c, hipEventCreate(&e0));
+    MF_HIP(c, hipEventCreate(&e1));
+    // warm
+    if ((rc = run_rhs(c, precision, c->d_buf[MF_BUF_U], c->d_buf[MF_BUF_DU], 0.0, none))) return rc;
+    MF_HIP(c, hipEventRecord(e0, c->stream));
+    for (int q = 0; q < n_calls; ++q)
+        if ((rc = run_rhs(c, precision, c->d_buf[MF_BUF_U], c->d_buf[MF_BUF_DU], 0.0, none))) return rc;
+    MF_HIP(c, hipEventRecord(e1, c->stream));
+    MF_HIP(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    MF_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+    if (rhs_ms) *rhs_ms = ms / n_calls;
+    if (kernel_ms) *kernel_ms = -1.0;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->fsal_valid = false;
+    return MF_OK;
+}
+
+}  // extern "C"

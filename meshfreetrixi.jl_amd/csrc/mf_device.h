@@ -1,0 +1,119 @@
+// Device-side helpers shared by the HIP sources: node-major state access,
+// the physical fluxes, boundary value functions and launch index mapping.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mf_internal.h"
+
+namespace mf {
+
+constexpr int kBlock = 256;
+
+// Logical node of this thread under the XCD-aware block remap (grid size a
+// multiple of 8; workgroup b runs on XCD b % 8).
+__device__ __forceinline__ int64_t remapped_node()
+{
+    const unsigned b = blockIdx.x, per = gridDim.x >> 3;
+    const unsigned lb = (b & 7u) * per + (b >> 3);
+    return (int64_t)lb * blockDim.x + threadIdx.x;
+}
+
+template <int V>
+__device__ __forceinline__ void load_state(const double* __restrict__ u, int64_t j, double* s)
+{
+    if constexpr (V == 4) {
+        const double4 q = reinterpret_cast<const double4*>(u)[j];
+        s[0] = q.x;
+        s[1] = q.y;
+        s[2] = q.z;
+        s[3] = q.w;
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v) s[v] = u[j * V + v];
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void store_state(double* __restrict__ u, int64_t j, const double* s)
+{
+    if constexpr (V == 4) {
+        reinterpret_cast<double4*>(u)[j] = make_double4(s[0], s[1], s[2], s[3]);
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v) u[j * V + v] = s[v];
+    }
+}
+
+struct EqParams {
+    double a1, a2, gamma;
+};
+
+// Physical fluxes in x and y: Trixi's flux(u, 1|2, equations) for the
+// linear advection and compressible Euler equations (the latter evaluated
+// with fused multiply-adds like Trixi's @muladd code).
+template <int V>
+__device__ __forceinline__ void flux_xy(const EqParams& e, const double* u, double* f1,
+                                        double* f2)
+{
+    if constexpr (V == 1) {
+        f1[0] = e.a1 * u[0];
+        f2[0] = e.a2 * u[0];
+    } else {
+        const double rho = u[0], m1 = u[1], m2 = u[2], E = u[3];
+        const double v1 = m1 / rho, v2 = m2 / rho;
+        const double p = (e.gamma - 1.0) * fma(-0.5, fma(m1, v1, m2 * v2), E);
+        const double ep = E + p;
+        f1[0] = m1;
+        f1[1] = fma(m1, v1, p);
+        f1[2] = m1 * v2;
+        f1[3] = ep * v1;
+        f2[0] = m2;
+        f2[1] = m2 * v1;
+        f2[2] = fma(m2, v2, p);
+        f2[3] = ep * v2;
+    }
+}
+
+// Boundary/initial value functions (see MF_FN_* in mfrbf.h).
+template <int V>
+__device__ __forceinline__ void bc_value(int fn, const double* q, double gamma, double x,
+                                         double y, double t, double* out)
+{
+    if (fn == MF_FN_CONST) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) out[v] = q[v];
+    } else if (fn == MF_FN_PULSE) {
+        const double dx = (x - q[3] * t) - q[0], dy = (y - q[4] * t) - q[1];
+        out[0] = exp(-(dx * dx + dy * dy) / (q[2] * q[2]));
+#pragma unroll
+        for (int v = 1; v < V; ++v) out[v] = 0.0;
+    } else if (fn == MF_FN_WAVE) {
+        if constexpr (V == 4) {
+            const double arg = (x + y) - t * (q[2] + q[3]);
+            const double rho = q[0] + q[1] * sin(2.0 * M_PI * arg + q[5]);
+            out[0] = rho;
+            out[1] = rho * q[2];
+            out[2] = rho * q[3];
+            out[3] = q[4] / (gamma - 1.0) + 0.5 * rho * (q[2] * q[2] + q[3] * q[3]);
+        }
+    } else if (fn == MF_FN_GRADIENT) {
+        if constexpr (V == 4) {
+            out[0] = 1.4 + 0.1 * x + 0.1 * y;
+            out[1] = 4.1 + 0.1 * x + 0.1 * y;
+            out[2] = 0.0 + 0.1 * x + 0.1 * y;
+            out[3] = 8.8 + 0.1 * x + 0.1 * y;
+        }
+    }
+}
+
+// Julia's max(x, y) for floats: NaN wins, otherwise the larger value.
+__device__ __forceinline__ double jmax(double x, double y)
+{
+    if (isnan(x)) return x;
+    if (isnan(y)) return y;
+    return (y > x || (signbit(y) < signbit(x))) ? y : x;
+}
+
+}  // namespace mf

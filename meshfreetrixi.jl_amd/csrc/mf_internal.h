@@ -1,0 +1,200 @@
+// Internal declarations shared by the libmfrbf sources (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mfrbf.h"
+
+namespace mf {
+
+constexpr int kMaxVar = 4;
+constexpr int kMaxTags = 64;
+constexpr int kMaxHist = 8;
+constexpr int kSellC = 64;  // slice height of the transposed operator
+constexpr int kMaxStencilM = 64;  // saddle systems up to 64 x 64 (one wavefront)
+
+// Per-step scalars kept in device memory so that captured graphs read the
+// current time, history weights and success_iter at replay.
+struct StepScalars {
+    double t;                    // integrator time t_n
+    double time_hist[kMaxHist];  // most recent first
+    double tw[kMaxHist];         // time-derivative weights
+    double norms[2 * kMaxVar];   // mean(u), ninf of |u - mean| (residual viscosity)
+    int success_iter;
+    int hist_head;  // ring slot of the most recent snapshot
+    int q;          // number of history points in use
+    int ticket;     // last-block detection of the reductions
+    int ticket2;
+    int hist_si;  // success_iter as last seen by the history callback
+    int pad[2];
+};
+
+// Term table of the k-th derivative of r^p along one coordinate s:
+// sum_t c_t s^a_t r^b_t.
+struct PhsTable {
+    int nt;
+    int a[16];
+    int b[16];
+    double c[16];
+};
+PhsTable phs_table(int p, int k);
+
+struct BcDev {
+    int ntags;
+    int nb;
+    int any_slip;
+    int multi;  // some node belongs to more than one tag
+    int32_t* bidx = nullptr;     // nb
+    int32_t* btag = nullptr;     // nb
+    double* bnormal = nullptr;   // 2 nb
+    double* btable = nullptr;    // nvar nb
+    uint8_t* flag = nullptr;     // N: bit0 Dirichlet, bit1 slip
+    int32_t* entry = nullptr;    // N: boundary entry of the node, -1 if none
+    int tag_kind[kMaxTags];
+    int tag_fn[kMaxTags];
+    int tag_ptr[kMaxTags + 1];
+    double tag_params[kMaxTags * 8];
+};
+
+}  // namespace mf
+
+struct mf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int cu_count = 0;
+
+    // cloud
+    int64_t N = 0;
+    double* d_x = nullptr;
+    double* d_y = nullptr;
+
+    // stencil (ELL, slot-major: entry s of node i at s*N + i)
+    int k = 0;
+    int32_t* d_idx = nullptr;
+    int16_t* d_didx = nullptr;  // idx - i as int16 (split precision), if it fits
+    bool didx_ok = false;
+
+    // operators
+    double2* d_wxy = nullptr;  // (Dx, Dy)
+    double* d_hv = nullptr;    // Dx^2k + Dy^2k
+    float2* d_wxy32 = nullptr;
+    float* d_hv32 = nullptr;
+    bool have_flux = false, have_hv = false;
+    // transposed flux operator, SELL-64
+    int64_t n_slices = 0;
+    int64_t* d_sell_off = nullptr;  // n_slices + 1
+    int32_t* d_sell_w = nullptr;    // n_slices
+    int32_t* d_trow = nullptr;
+    double2* d_tw = nullptr;
+    float2* d_tw32 = nullptr;
+    bool have_T = false;
+
+    // physics
+    int eq = MF_EQ_ADVECTION_2D;
+    int nvar = 1;
+    double p0 = 1.0, p1 = 1.0;  // advection velocity or (gamma, -)
+    mf::BcDev bc;
+    bool have_bc = false;
+    double hv_gamma = 0.0;
+    int visc_mode = MF_VISC_NONE;
+    double c_rv = 1.0, c_uw = 1.0, h_visc = 0.0;
+    int approx_order = 0;
+    int hv_first = 1;
+
+    // state
+    double* d_buf[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* d_S = nullptr;
+    double* d_T = nullptr;
+    double* d_G = nullptr;
+    double* d_eps = nullptr;
+    int8_t* d_epsc = nullptr;
+    double* d_approx = nullptr;
+    double* d_hist = nullptr;
+    double* d_partials = nullptr;
+    int n_partials = 0;
+    mf::StepScalars* d_sc = nullptr;
+    int host_success_iter = 0;
+    bool fsal_valid = false;
+
+    // stage limiter
+    int lim_n = 0;
+    double lim_thr[2] = {0, 0};
+    int lim_var[2] = {0, 0};
+    double* d_lim = nullptr;  // 2 nvar N scratch
+
+    // graph of one SSPRK33 step
+    hipGraphExec_t step_exec = nullptr;
+    hipGraph_t step_graph = nullptr;
+    double graph_dt = 0.0;
+    int graph_prec = -1, graph_flags = -1;
+
+    // ragged weights batch
+    int64_t rn = 0, rnnz = 0;
+    int rmaxn = 0;
+    int64_t* d_rptr = nullptr;
+    int32_t* d_ridx = nullptr;
+    double* d_rwx = nullptr;
+    double* d_rwy = nullptr;
+    int* d_status = nullptr;
+
+    // adjoint scratch
+    double* d_adj = nullptr;
+    int64_t adj_len = 0;
+};
+
+namespace mf {
+
+// error helpers
+int fail(mf_ctx* c, int code, const std::string& msg);
+int hip_fail(mf_ctx* c, hipError_t e, const char* what);
+#define MF_HIP(c, expr)                                   \
+    do {                                                  \
+        hipError_t _e = (expr);                           \
+        if (_e != hipSuccess) return mf::hip_fail((c), _e, #expr); \
+    } while (0)
+
+template <class T>
+int dalloc(mf_ctx* c, T** p, size_t count);
+void dfree(void* p);
+
+// XCD-aware launch helper: blocks are dealt round-robin to the 8 XCDs; the
+// remap gives every XCD one contiguous range of logical blocks.
+inline unsigned xcd_grid(int64_t nblocks) { return (unsigned)(((nblocks + 7) / 8) * 8); }
+
+// kernels / launchers (defined in the .hip files)
+int launch_knn(mf_ctx* c, int k, double* d_dist2);
+int launch_weights(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k,
+                   int64_t n_stencils, int maxn, int p, int deg, int kd, double* d_wx,
+                   double* d_wy, int* d_status);
+int launch_pack_ops(mf_ctx* c, int target, const double* d_wx, const double* d_wy);
+int launch_build_transpose(mf_ctx* c);
+int launch_dx_stats(mf_ctx* c, const double* d_dist2, double* dx_min, double* dx_avg);
+int launch_ell_to_rows(mf_ctx* c, const int32_t* d_ell, int32_t* d_rows, int k);
+int launch_ell_to_rows_d(mf_ctx* c, const double* d_ell, double* d_rows, int k);
+int launch_rows_to_ell(mf_ctx* c, const int32_t* d_rows, int32_t* d_ell, int k);
+int launch_rows_to_ell_d(mf_ctx* c, const double* d_rows, double* d_ell, int k);
+int launch_soa_to_aos(mf_ctx* c, const double* d_soa, double* d_aos);
+int launch_aos_to_soa(mf_ctx* c, const double* d_aos, double* d_soa);
+int launch_build_didx(mf_ctx* c);
+
+// rhs
+enum StageKind { ST_NONE = 0, ST_FSAL = 1, ST_TWO = 2, ST_THREE = 3 };
+struct StageOut {
+    int kind = ST_NONE;
+    double dt = 0.0;
+    const double* uprev = nullptr;
+    double* out = nullptr;
+};
+int run_rhs(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const StageOut& st);
+int run_set_time(mf_ctx* c, double t, int success_iter);
+int run_history(mf_ctx* c, const double* d_u, int init);
+int run_advance_time(mf_ctx* c, double dt);
+int run_limiter(mf_ctx* c, double* d_u, double thr, int var);
+int run_axpby(mf_ctx* c, int64_t n, double a, const double* x, double b, double* y);
+int run_fill(mf_ctx* c, double* p, int64_t n, double v);
+
+}  // namespace mf

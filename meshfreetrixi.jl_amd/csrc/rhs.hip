@@ -1,0 +1,992 @@
+// rhs! on the GPU: strong boundary conditions, the fused flux-divergence /
+// hyperviscosity / residual-viscosity stencil kernels, SSPRK33 stage
+// updates, the history callback and the Zhang-Shu stage limiter.
+//
+// Reference: Trixi.rhs!, pointcloudsolver/rbfsolver.jl:397-428 (and the
+// functions it calls, cited at each kernel).
+//
+// Layout: node-major state (V doubles per node), ELL stencils slot-major
+// (entry s of node i at s*N + i), operator weights (Dx, Dy) interleaved.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "mf_device.h"
+
+namespace mf {
+
+// ----------------------------------------------------------- arguments
+struct RhsArgs {
+    int64_t N;
+    int k;
+    const int32_t* __restrict__ idx;
+    const int16_t* __restrict__ didx;
+    const double2* __restrict__ wxy;
+    const double* __restrict__ hv;
+    const float2* __restrict__ wxy32;
+    const float* __restrict__ hv32;
+    double hv_gamma;
+    EqParams eq;
+    const double* __restrict__ u;
+    double* __restrict__ du;
+    const uint8_t* __restrict__ flag;
+    const int32_t* __restrict__ entry;
+    const double* __restrict__ bnormal;
+    // viscosity
+    int visc;
+    int hv_first;
+    double c_rv, c_uw, h;
+    const double* __restrict__ approx;
+    const StepScalars* __restrict__ sc;
+    double* __restrict__ G;
+    double* __restrict__ eps;
+    int8_t* __restrict__ epsc;
+    // transposed operator
+    const int64_t* __restrict__ sell_off;
+    const int32_t* __restrict__ sell_w;
+    const int32_t* __restrict__ trow;
+    const double2* __restrict__ tw;
+    const float2* __restrict__ tw32;
+    // stage output
+    int st_kind;
+    double dt;
+    const double* __restrict__ uprev;
+    double* __restrict__ out;
+};
+
+// Slip-wall projection of the momentum (PointCloudBCs.jl:15-21, :87-106).
+__device__ __forceinline__ void slip_project(const double* nrm2, double* s)
+{
+    const double n1 = nrm2[0], n2 = nrm2[1];
+    const double nn = sqrt(n1 * n1 + n2 * n2);
+    const double e1 = n1 / nn, e2 = n2 / nn;
+    const double m1 = s[1], m2 = s[2];
+    const double dot = m1 * e1 + m2 * e2;
+    s[1] = fma(-dot, e1, m1);
+    s[2] = fma(-dot, e2, m2);
+}
+
+// Second boundary pass on du (rbfsolver.jl:422-425): Dirichlet nodes get
+// du = 0, slip nodes du momentum = 0.
+template <int V>
+__device__ __forceinline__ void mask_du(const RhsArgs& a, int64_t i, double* du)
+{
+    if (!a.flag) return;
+    const uint8_t f = a.flag[i];
+    if (f & 1u) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) du[v] = 0.0;
+    }
+    if constexpr (V == 4) {
+        if (f & 2u) {
+            du[1] = 0.0;
+            du[2] = 0.0;
+        }
+    }
+}
+
+// Writes du, or the next SSPRK33 stage built from k = du at this node:
+//   FSAL : out = u + dt k
+//   TWO  : out = (3 uprev + u + dt k) / 4
+//   THREE: out = (uprev + 2 u + 2 dt k) / 3
+// u is this node's (boundary-imposed) input state; slip nodes see the
+// second boundary pass's projection first, as the reference's u does.
+template <int V>
+__device__ __forceinline__ void emit(const RhsArgs& a, int64_t i, const double* du)
+{
+    if (a.st_kind == ST_NONE) {
+        store_state<V>(a.du, i, du);
+        return;
+    }
+    double ui[V], o[V];
+    load_state<V>(a.u, i, ui);
+    if constexpr (V == 4) {
+        if (a.flag && (a.flag[i] & 2u)) slip_project(a.bnormal + 2 * a.entry[i], ui);
+    }
+    if (a.st_kind == ST_FSAL) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[v] = fma(a.dt, du[v], ui[v]);
+    } else {
+        double up[V];
+        load_state<V>(a.uprev, i, up);
+        if (a.st_kind == ST_TWO) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[v] = fma(3.0, up[v], fma(a.dt, du[v], ui[v])) / 4.0;
+        } else {
+            const double dt2 = 2.0 * a.dt;
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[v] = fma(2.0, ui[v], fma(dt2, du[v], up[v])) / 3.0;
+        }
+    }
+    store_state<V>(a.out, i, o);
+}
+
+// Residual/upwind viscosity at node i (hyperviscosity.jl:246-349): eps_uw
+// from the local state, eps_rv from |approx_du - du| scaled by the global
+// norms, then the choice of update_visc!.
+__device__ __forceinline__ double node_viscosity(const RhsArgs& a, int64_t i, const double* ui,
+                                                 const double* dui, int8_t* code)
+{
+    const double gam = a.eq.gamma;
+    const double rho = ui[0], m1 = ui[1], m2 = ui[2], E = ui[3];
+    const double v1 = m1 / rho, v2 = m2 / rho;
+    double p = (gam - 1.0) * fma(-0.5, fma(m1, v1, m2 * v2), E);
+    const double speed = sqrt(v1 * v1 + v2 * v2);
+    double sound = 0.0;
+    if (!(p < 0.0 || rho < 0.0)) sound = sqrt(gam * p / rho);
+    const double eps_uw = a.c_uw * 0.5 * a.h * (speed + sound);
+    if (a.visc == MF_VISC_UPWIND) {
+        *code = 1;
+        return eps_uw;
+    }
+    double ad[4];
+    load_state<4>(a.approx, i, ad);
+    const double* ninf = a.sc->norms + 4;
+    double mr = 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const double r = fabs(ad[v] - dui[v]) / ninf[v];
+        mr = (v == 0) ? r : jmax(mr, r);
+    }
+    const double eps_rv = 0.5 * a.c_rv * (a.h * a.h) * mr;
+    if (isnan(eps_rv) || isinf(eps_rv) || a.sc->hist_si == 0) {
+        if (isnan(eps_uw) || isinf(eps_uw)) {
+            *code = 2;
+            return DBL_EPSILON;
+        }
+        *code = 1;
+        return eps_uw;
+    }
+    *code = eps_rv < eps_uw ? 0 : 1;
+    return eps_rv < eps_uw ? eps_rv : eps_uw;
+}
+
+// ------------------------------------------------------------ FP64 kernel
+// du_i = -sum_j Dx_ij F1(u_j) - sum_j Dy_ij F2(u_j)     (calc_fluxes!, :247-265)
+//        - gamma sum_j H_ij u_j                         (Flyer HV, hyperviscosity.jl:52-64)
+// With a viscosity source this is phase A: it also stores
+// G_i = eps_i (Dx u, Dy u)_i for the transposed pass (hyperviscosity.jl:401-407).
+template <int V, int NS, bool HV, bool RV>
+__global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
+{
+    const int64_t i = remapped_node();
+    if (i >= a.N) return;
+    const int64_t N = a.N;
+    const int K = NS ? NS : a.k;
+    double ax[V], ay[V], gx[V], gy[V], hh[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) ax[v] = ay[v] = gx[v] = gy[v] = hh[v] = 0.0;
+#pragma unroll 3
+    for (int s = 0; s < K; ++s) {
+        const int64_t q = (int64_t)s * N + i;
+        const int32_t j = a.idx[q];
+        const double2 w = a.wxy[q];
+        double uj[V], f1[V], f2[V];
+        load_state<V>(a.u, j, uj);
+        flux_xy<V>(a.eq, uj, f1, f2);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            ax[v] = fma(w.x, f1[v], ax[v]);
+            ay[v] = fma(w.y, f2[v], ay[v]);
+        }
+        if constexpr (RV) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                gx[v] = fma(w.x, uj[v], gx[v]);
+                gy[v] = fma(w.y, uj[v], gy[v]);
+            }
+        }
+        if constexpr (HV) {
+            const double hw = a.hv[q];
+#pragma unroll
+            for (int v = 0; v < V; ++v) hh[v] = fma(hw, uj[v], hh[v]);
+        }
+    }
+    double du[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) du[v] = -ax[v] - ay[v];
+    if constexpr (HV) {
+        if (!RV || a.hv_first) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) du[v] = fma(-a.hv_gamma, hh[v], du[v]);
+        }
+    }
+    if constexpr (RV) {
+        double ui[V];
+        load_state<V>(a.u, i, ui);
+        int8_t code = 1;
+        const double e = node_viscosity(a, i, ui, du, &code);
+        a.eps[i] = e;
+        a.epsc[i] = code;
+        double g[2 * V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            g[v] = e * gx[v];
+            g[V + v] = e * gy[v];
+        }
+        store_state<V>(a.G, 2 * i, g);
+        store_state<V>(a.G, 2 * i + 1, g + V);
+        if constexpr (HV) {
+            if (!a.hv_first) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) du[v] = fma(-a.hv_gamma, hh[v], du[v]);
+            }
+        }
+        store_state<V>(a.du, i, du);
+        return;
+    }
+    mask_du<V>(a, i, du);
+    emit<V>(a, i, du);
+}
+
+// ------------------------------------------------- split-FP32 kernel
+// Same operator; the stencil sums run in FP32 on centred differences
+// formed in FP64: sum_j w_ij F(u_j) ~= sum_j w_ij (F(u_j) - F(u_i)),
+// exact up to F(u_i) sum_j w_ij, which vanishes with the weights' row sums.
+// Weights FP32, neighbour indices as int16 offsets when the cloud allows.
+template <int V, int NS, bool HV, bool RV, bool D16>
+__global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
+{
+    const int64_t i = remapped_node();
+    if (i >= a.N) return;
+    const int64_t N = a.N;
+    const int K = NS ? NS : a.k;
+    double ui[V], fi1[V], fi2[V];
+    load_state<V>(a.u, i, ui);
+    flux_xy<V>(a.eq, ui, fi1, fi2);
+    float ax[V], ay[V], gx[V], gy[V], hh[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) ax[v] = ay[v] = gx[v] = gy[v] = hh[v] = 0.0f;
+#pragma unroll 3
+    for (int s = 0; s < K; ++s) {
+        const int64_t q = (int64_t)s * N + i;
+        int64_t j;
+        if constexpr (D16)
+            j = i + (int64_t)a.didx[q];
+        else
+            j = a.idx[q];
+        const float2 w = a.wxy32[q];
+        double uj[V], f1[V], f2[V];
+        load_state<V>(a.u, j, uj);
+        flux_xy<V>(a.eq, uj, f1, f2);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            ax[v] = fmaf(w.x, (float)(f1[v] - fi1[v]), ax[v]);
+            ay[v] = fmaf(w.y, (float)(f2[v] - fi2[v]), ay[v]);
+        }
+        if constexpr (RV || HV) {
+            float dd[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) dd[v] = (float)(uj[v] - ui[v]);
+            if constexpr (RV) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    gx[v] = fmaf(w.x, dd[v], gx[v]);
+                    gy[v] = fmaf(w.y, dd[v], gy[v]);
+                }
+            }
+            if constexpr (HV) {
+                const float hw = a.hv32[q];
+#pragma unroll
+                for (int v = 0; v < V; ++v) hh[v] = fmaf(hw, dd[v], hh[v]);
+            }
+        }
+    }
+    double du[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) du[v] = -(double)ax[v] - (double)ay[v];
+    if constexpr (HV) {
+        if (!RV || a.hv_first) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) du[v] = fma(-a.hv_gamma, (double)hh[v], du[v]);
+        }
+    }
+    if constexpr (RV) {
+        int8_t code = 1;
+        const double e = node_viscosity(a, i, ui, du, &code);
+        a.eps[i] = e;
+        a.epsc[i] = code;
+        double g[2 * V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            g[v] = e * (double)gx[v];
+            g[V + v] = e * (double)gy[v];
+        }
+        store_state<V>(a.G, 2 * i, g);
+        store_state<V>(a.G, 2 * i + 1, g + V);
+        if constexpr (HV) {
+            if (!a.hv_first) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) du[v] = fma(-a.hv_gamma, (double)hh[v], du[v]);
+            }
+        }
+        store_state<V>(a.du, i, du);
+        return;
+    }
+    mask_du<V>(a, i, du);
+    emit<V>(a, i, du);
+}
+
+// ------------------------------------------ phase B: transposed gather
+// du_i -= sum_r Dx_ri Gx_r ; du_i -= sum_r Dy_ri Gy_r   (the D_d' products of
+// hyperviscosity.jl:405), over the sorted SELL-64 transpose, then the
+// second boundary pass and the output.
+template <int V, bool S32>
+__global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
+{
+    const int64_t i = remapped_node();
+    if (i >= a.N) return;
+    const int64_t slice = i >> 6;
+    const int lane = (int)(i & 63);
+    const int64_t off = a.sell_off[slice];
+    const int W = a.sell_w[slice];
+    double tx[V], ty[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) tx[v] = ty[v] = 0.0;
+#pragma unroll 4
+    for (int e = 0; e < W; ++e) {
+        const int64_t q = off + (int64_t)e * kSellC + lane;
+        const int32_t r = a.trow[q];
+        double2 w;
+        if constexpr (S32) {
+            const float2 w32 = a.tw32[q];
+            w = make_double2((double)w32.x, (double)w32.y);
+        } else {
+            w = a.tw[q];
+        }
+        double g0[V], g1[V];
+        load_state<V>(a.G, 2 * (int64_t)r, g0);
+        load_state<V>(a.G, 2 * (int64_t)r + 1, g1);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            tx[v] = fma(w.x, g0[v], tx[v]);
+            ty[v] = fma(w.y, g1[v], ty[v]);
+        }
+    }
+    double du[V];
+    load_state<V>(a.du, i, du);
+#pragma unroll
+    for (int v = 0; v < V; ++v) du[v] = (du[v] - tx[v]) - ty[v];
+    mask_du<V>(a, i, du);
+    emit<V>(a, i, du);
+}
+
+// ----------------------------------------- boundary pass on the state
+// First boundary pass (rbfsolver.jl:405-408, PointCloudBCs.jl:49-106) over
+// entries [e0, e1): Dirichlet u_b <- g(x_b, t), slip u_b <- projection.
+// With homog = 1 it is the adjoint of that map: Dirichlet rows zeroed,
+// slip momentum projected.
+struct BcArgs {
+    int e0, e1;
+    const int32_t* bidx;
+    const int32_t* btag;
+    const double* bnormal;
+    const double* btable;
+    const double* px;
+    const double* py;
+    double gamma;
+    const StepScalars* sc;
+    double dt_off;
+    int homog;
+    int tag_kind[kMaxTags];
+    int tag_fn[kMaxTags];
+    double tag_params[kMaxTags * 8];
+};
+
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_bc_state(const BcArgs b, double* u)
+{
+    const int e = b.e0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= b.e1) return;
+    const int32_t node = b.bidx[e];
+    const int tag = b.btag[e];
+    const int kind = b.tag_kind[tag];
+    double s[V];
+    load_state<V>(u, node, s);
+    if (kind == MF_BC_DIRICHLET) {
+        if (b.homog) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) s[v] = 0.0;
+        } else if (b.tag_fn[tag] == MF_FN_TABLE) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) s[v] = b.btable[(int64_t)e * V + v];
+        } else {
+            const double t = b.sc->t + b.dt_off;
+            bc_value<V>(b.tag_fn[tag], b.tag_params + 8 * tag, b.gamma, b.px[node], b.py[node], t,
+                        s);
+        }
+    } else if (kind == MF_BC_SLIP_WALL) {
+        if constexpr (V == 4) slip_project(b.bnormal + 2 * e, s);
+    } else {
+        return;
+    }
+    store_state<V>(u, node, s);
+}
+
+// ------------------------------------ residual-viscosity reductions
+// mean(u) = sum(u) / (V N)  (ode_mean, auxiliary/mpi.jl:40-52) and the
+// lexicographic maximum of |u_i - mean| (ode_maximum, mpi.jl:71-82) with
+// zeros replaced by eps() (hyperviscosity.jl:305-311).  Each block reduces
+// a contiguous node range in a fixed order; the last block to finish
+// combines the partials.
+__device__ __forceinline__ bool lex_less(const double* a, const double* b)
+{
+    // Julia isless over vectors: first differing component decides, NaN largest
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const bool an = isnan(a[v]), bn = isnan(b[v]);
+        if (an && bn) continue;
+        if (bn) return true;
+        if (an) return false;
+        if (a[v] < b[v]) return true;
+        if (a[v] > b[v]) return false;
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rv_sum(int64_t N, int64_t chunk, const double* u,
+                                                   double* partials, StepScalars* sc)
+{
+    __shared__ double red[kBlock][4];
+    __shared__ int last;
+    const int64_t lo = (int64_t)blockIdx.x * chunk;
+    const int64_t hi = min(N, lo + chunk);
+    double acc[4] = {0, 0, 0, 0};
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        double s[4];
+        load_state<4>(u, i, s);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[v] += s[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = acc[v];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) red[threadIdx.x][v] += red[threadIdx.x + w][v];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) partials[4 * blockIdx.x + v] = red[0][v];
+        __threadfence();
+        last = (atomicAdd(&sc->ticket, 1) == (int)gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double tot[4] = {0, 0, 0, 0};
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) tot[v] += partials[4 * b + v];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = tot[v];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) red[threadIdx.x][v] += red[threadIdx.x + w][v];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sc->norms[v] = red[0][v] / (double)(4 * N);
+        sc->ticket = 0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rv_max(int64_t N, int64_t chunk, const double* u,
+                                                   double* partials, StepScalars* sc)
+{
+    __shared__ double red[kBlock][4];
+    __shared__ int last;
+    const int64_t lo = (int64_t)blockIdx.x * chunk;
+    const int64_t hi = min(N, lo + chunk);
+    double mean[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) mean[v] = sc->norms[v];
+    double best[4] = {-1.0, -1.0, -1.0, -1.0};  // below every |.|
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        double s[4], c[4];
+        load_state<4>(u, i, s);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) c[v] = fabs(s[v] - mean[v]);
+        if (lex_less(best, c))
+#pragma unroll
+            for (int v = 0; v < 4; ++v) best[v] = c[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = best[v];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w && lex_less(red[threadIdx.x], red[threadIdx.x + w]))
+#pragma unroll
+            for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = red[threadIdx.x + w][v];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) partials[4 * blockIdx.x + v] = red[0][v];
+        __threadfence();
+        last = (atomicAdd(&sc->ticket2, 1) == (int)gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double b2[4] = {-1.0, -1.0, -1.0, -1.0};
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+        double c[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) c[v] = partials[4 * b + v];
+        if (lex_less(b2, c))
+#pragma unroll
+            for (int v = 0; v < 4; ++v) b2[v] = c[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = b2[v];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w && lex_less(red[threadIdx.x], red[threadIdx.x + w]))
+#pragma unroll
+            for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = red[threadIdx.x + w][v];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sc->norms[4 + v] = (red[0][v] == 0.0) ? DBL_EPSILON : red[0][v];
+        sc->ticket2 = 0;
+    }
+}
+
+// ------------------------------------------------------ small kernels
+__global__ void k_set_time(StepScalars* sc, double t, int success_iter, int set_si)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        sc->t = t;
+        if (set_si) sc->success_iter = success_iter;
+    }
+}
+
+__global__ void k_advance_time(StepScalars* sc, double dt)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        sc->t = sc->t + dt;
+        sc->success_iter = sc->success_iter + 1;
+    }
+}
+
+// HistoryCallback (history.jl:207-268).  Single thread: shift the time
+// history, count the points in use and solve the transposed Vandermonde
+// system for the derivative weights (partial-pivoting LU, as `A' \ b`).
+__global__ void k_history_weights(StepScalars* sc, int P, int init)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (init) {
+        for (int i = 0; i < kMaxHist; ++i) sc->time_hist[i] = 0.0;
+        sc->hist_head = 0;
+    } else {
+        sc->hist_head = (sc->hist_head + P - 1) % P;
+    }
+    for (int i = P - 1; i >= 1; --i) sc->time_hist[i] = sc->time_hist[i - 1];
+    sc->time_hist[0] = sc->t;
+    const int si = sc->success_iter;
+    sc->hist_si = si;
+    const int q = (si + 1 < P) ? si + 1 : P;
+    sc->q = (si > 0) ? q : 0;
+    if (si <= 0) return;
+    double A[kMaxHist * kMaxHist], b[kMaxHist], ts[kMaxHist];
+    double mx = 0.0;
+    for (int i = 0; i < q; ++i) mx = fmax(mx, fabs(sc->time_hist[i]));
+    const double scale = 1.0 / mx;
+    for (int i = 0; i < q; ++i) ts[i] = sc->time_hist[i] * scale;
+    const double te = ts[0];
+    for (int kk = 0; kk < q; ++kk) {
+        for (int i = 0; i < q; ++i) {
+            // ts^kk, nearly correctly rounded (double-double)
+            double hi = 1.0, lo = 0.0;
+            for (int r = 0; r < kk; ++r) {
+                const double ph = hi * ts[i];
+                const double pl = fma(hi, ts[i], -ph) + lo * ts[i];
+                hi = ph + pl;
+                lo = pl - (hi - ph);
+            }
+            A[kk + i * q] = hi;
+        }
+        double hi = 1.0, lo = 0.0;
+        for (int r = 0; r + 1 < kk; ++r) {
+            const double ph = hi * te;
+            const double pl = fma(hi, te, -ph) + lo * te;
+            hi = ph + pl;
+            lo = pl - (hi - ph);
+        }
+        b[kk] = (kk == 0) ? 0.0 : (double)kk * hi;
+    }
+    int piv[kMaxHist];
+    for (int j = 0; j < q; ++j) {
+        int p = j;
+        double pm = fabs(A[j + j * q]);
+        for (int i = j + 1; i < q; ++i)
+            if (fabs(A[i + j * q]) > pm) {
+                pm = fabs(A[i + j * q]);
+                p = i;
+            }
+        piv[j] = p;
+        if (p != j)
+            for (int c2 = 0; c2 < q; ++c2) {
+                const double tmp = A[j + c2 * q];
+                A[j + c2 * q] = A[p + c2 * q];
+                A[p + c2 * q] = tmp;
+            }
+        const double r = 1.0 / A[j + j * q];
+        for (int i = j + 1; i < q; ++i) A[i + j * q] = A[i + j * q] * r;
+        for (int c2 = j + 1; c2 < q; ++c2)
+            for (int i = j + 1; i < q; ++i)
+                A[i + c2 * q] = __dsub_rn(A[i + c2 * q], __dmul_rn(A[i + j * q], A[j + c2 * q]));
+    }
+    for (int j = 0; j < q; ++j)
+        if (piv[j] != j) {
+            const double tmp = b[j];
+            b[j] = b[piv[j]];
+            b[piv[j]] = tmp;
+        }
+    for (int j = 0; j < q; ++j)
+        for (int i = j + 1; i < q; ++i) b[i] = __dsub_rn(b[i], __dmul_rn(A[i + j * q], b[j]));
+    for (int j = q - 1; j >= 0; --j) {
+        b[j] = b[j] / A[j + j * q];
+        for (int i = 0; i < j; ++i) b[i] = __dsub_rn(b[i], __dmul_rn(A[i + j * q], b[j]));
+    }
+    for (int i = 0; i < q; ++i) sc->tw[i] = scale * b[i];
+}
+
+// Snapshot u into the ring slot of the most recent entry and rebuild
+// approx_du = sum_i w_i u^(i) (history.jl:221-245).
+__global__ __launch_bounds__(kBlock) void k_history_apply(int64_t NV, int P, const double* u,
+                                                          double* hist, double* approx,
+                                                          const StepScalars* sc)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= NV) return;
+    const int head = sc->hist_head;
+    const double val = u[q];
+    hist[(int64_t)head * NV + q] = val;
+    double acc = 0.0;
+    const int nq = sc->q;
+    for (int r = 0; r < nq; ++r) {
+        const int slot = (head + r) % P;
+        const double h = (r == 0) ? val : hist[(int64_t)slot * NV + q];
+        acc = fma(sc->tw[r], h, acc);
+    }
+    approx[q] = acc;
+}
+
+// Zhang-Shu limiter (positivity_zhang_shu_point2d.jl:22-83), Jacobi style:
+// pass 1 computes the blended state of every node needing it, pass 2
+// writes the non-zero ones back.
+template <int V>
+__device__ __forceinline__ double limiter_variable(int var, double gamma, const double* s)
+{
+    if (var == MF_VAR_DENSITY || V == 1) return s[0];
+    return (gamma - 1.0) * (s[3] - 0.5 * (s[1] * s[1] + s[2] * s[2]) / s[0]);
+}
+
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_limiter_blend(int64_t N, int K, const int32_t* idx,
+                                                          const double* u, double* loc, double thr,
+                                                          int var, double gamma)
+{
+    const int64_t i = remapped_node();
+    if (i >= N) return;
+    double vmin = INFINITY;
+    for (int s = 0; s < K; ++s) {
+        double uj[V];
+        load_state<V>(u, idx[(int64_t)s * N + i], uj);
+        const double val = limiter_variable<V>(var, gamma, uj);
+        vmin = (val < vmin) ? val : vmin;
+    }
+    double o[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) o[v] = 0.0;
+    if (vmin < thr) {
+        double um[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) um[v] = 0.0;
+        for (int s = 0; s < K; ++s) {
+            double uj[V];
+            load_state<V>(u, idx[(int64_t)s * N + i], uj);
+#pragma unroll
+            for (int v = 0; v < V; ++v) um[v] = um[v] + uj[v];
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) um[v] = um[v] / (double)K;
+        const double vmean = limiter_variable<V>(var, gamma, um);
+        const double theta = (vmean - thr) / (vmean - vmin);
+        double ui[V];
+        load_state<V>(u, i, ui);
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[v] = fma(theta, ui[v], (1.0 - theta) * um[v]);
+    }
+    store_state<V>(loc, i, o);
+}
+
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_limiter_commit(int64_t N, const double* loc, double* u)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double o[V];
+    load_state<V>(loc, i, o);
+    bool nz = false;
+#pragma unroll
+    for (int v = 0; v < V; ++v) nz |= (o[v] != 0.0);
+    if (nz) store_state<V>(u, i, o);
+}
+
+__global__ void k_axpby(int64_t n, double a, const double* x, double b, double* y)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) y[q] = fma(a, x[q], b * y[q]);
+}
+
+__global__ void k_fill(double* p, int64_t n, double v)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = v;
+}
+
+// ============================================================ launchers
+static RhsArgs make_args(mf_ctx* c, double* u, double* du)
+{
+    RhsArgs a{};
+    a.N = c->N;
+    a.k = c->k;
+    a.idx = c->d_idx;
+    a.didx = c->d_didx;
+    a.wxy = c->d_wxy;
+    a.hv = c->d_hv;
+    a.wxy32 = c->d_wxy32;
+    a.hv32 = c->d_hv32;
+    a.hv_gamma = c->hv_gamma;
+    a.eq.a1 = c->p0;
+    a.eq.a2 = c->p1;
+    a.eq.gamma = c->p0;
+    if (c->eq == MF_EQ_EULER_2D) a.eq.a1 = a.eq.a2 = 0.0;
+    a.u = u;
+    a.du = du;
+    a.flag = c->have_bc ? c->bc.flag : nullptr;
+    a.entry = c->have_bc ? c->bc.entry : nullptr;
+    a.bnormal = c->have_bc ? c->bc.bnormal : nullptr;
+    a.visc = c->visc_mode;
+    a.hv_first = c->hv_first;
+    a.c_rv = c->c_rv;
+    a.c_uw = c->c_uw;
+    a.h = c->h_visc;
+    a.approx = c->d_approx;
+    a.sc = c->d_sc;
+    a.G = c->d_G;
+    a.eps = c->d_eps;
+    a.epsc = c->d_epsc;
+    a.sell_off = c->d_sell_off;
+    a.sell_w = c->d_sell_w;
+    a.trow = c->d_trow;
+    a.tw = c->d_tw;
+    a.tw32 = c->d_tw32;
+    return a;
+}
+
+static BcArgs make_bc(mf_ctx* c, double dt_off, int homog)
+{
+    BcArgs b{};
+    b.bidx = c->bc.bidx;
+    b.btag = c->bc.btag;
+    b.bnormal = c->bc.bnormal;
+    b.btable = c->bc.btable;
+    b.px = c->d_x;
+    b.py = c->d_y;
+    b.gamma = c->p0;
+    b.sc = c->d_sc;
+    b.dt_off = dt_off;
+    b.homog = homog;
+    for (int g = 0; g < c->bc.ntags; ++g) {
+        b.tag_kind[g] = c->bc.tag_kind[g];
+        b.tag_fn[g] = c->bc.tag_fn[g];
+        for (int q = 0; q < 8; ++q) b.tag_params[8 * g + q] = c->bc.tag_params[8 * g + q];
+    }
+    return b;
+}
+
+// One boundary pass on a state buffer; tags run one after another when a
+// node belongs to several (the reference applies them in tag order).
+static int bc_state_pass(mf_ctx* c, double* u, double dt_off, int homog, bool slip_only)
+{
+    if (!c->have_bc || c->bc.nb == 0) return MF_OK;
+    if (slip_only && !c->bc.any_slip) return MF_OK;
+    BcArgs b = make_bc(c, dt_off, homog);
+    if (slip_only)
+        for (int g = 0; g < c->bc.ntags; ++g)
+            if (b.tag_kind[g] != MF_BC_SLIP_WALL) b.tag_kind[g] = MF_BC_DO_NOTHING;
+    auto launch = [&](int e0, int e1) {
+        if (e1 <= e0) return;
+        b.e0 = e0;
+        b.e1 = e1;
+        const unsigned g = (unsigned)((e1 - e0 + kBlock - 1) / kBlock);
+        if (c->nvar == 4)
+            hipLaunchKernelGGL(k_bc_state<4>, dim3(g), dim3(kBlock), 0, c->stream, b, u);
+        else
+            hipLaunchKernelGGL(k_bc_state<1>, dim3(g), dim3(kBlock), 0, c->stream, b, u);
+    };
+    if (c->bc.multi) {
+        for (int g = 0; g < c->bc.ntags; ++g) launch(c->bc.tag_ptr[g], c->bc.tag_ptr[g + 1]);
+    } else {
+        launch(0, c->bc.nb);
+    }
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+template <int V, int NS>
+static void launch_flux(mf_ctx* c, int prec, const RhsArgs& a, bool hv, bool rv, unsigned grid)
+{
+    const dim3 G(grid), B(kBlock);
+    hipStream_t s = c->stream;
+    if (prec == MF_PREC_F64) {
+        if (hv && rv) hipLaunchKernelGGL((k_rhs_f64<V, NS, true, true>), G, B, 0, s, a);
+        else if (hv) hipLaunchKernelGGL((k_rhs_f64<V, NS, true, false>), G, B, 0, s, a);
+        else if (rv) hipLaunchKernelGGL((k_rhs_f64<V, NS, false, true>), G, B, 0, s, a);
+        else hipLaunchKernelGGL((k_rhs_f64<V, NS, false, false>), G, B, 0, s, a);
+    } else if (c->didx_ok) {
+        if (hv && rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, true, true>), G, B, 0, s, a);
+        else if (hv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, false, true>), G, B, 0, s, a);
+        else if (rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, false, true, true>), G, B, 0, s, a);
+        else hipLaunchKernelGGL((k_rhs_s32<V, NS, false, false, true>), G, B, 0, s, a);
+    } else {
+        if (hv && rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, true, false>), G, B, 0, s, a);
+        else if (hv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, false, false>), G, B, 0, s, a);
+        else if (rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, false, true, false>), G, B, 0, s, a);
+        else hipLaunchKernelGGL((k_rhs_s32<V, NS, false, false, false>), G, B, 0, s, a);
+    }
+}
+
+template <int V>
+static void launch_flux_v(mf_ctx* c, int prec, const RhsArgs& a, bool hv, bool rv, unsigned grid)
+{
+    switch (c->k) {
+        case 15: launch_flux<V, 15>(c, prec, a, hv, rv, grid); break;
+        case 20: launch_flux<V, 20>(c, prec, a, hv, rv, grid); break;
+        case 30: launch_flux<V, 30>(c, prec, a, hv, rv, grid); break;
+        default: launch_flux<V, 0>(c, prec, a, hv, rv, grid); break;
+    }
+}
+
+int run_rhs(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const StageOut& st)
+{
+    if (!c->have_flux) return fail(c, MF_ERR_STATE, "rhs!: no flux operator (mf_compute_operator/mf_set_operator)");
+    if (prec != MF_PREC_F64 && prec != MF_PREC_SPLIT32)
+        return fail(c, MF_ERR_ARGUMENT, "rhs!: unknown precision");
+    const bool hv = c->have_hv && c->hv_gamma != 0.0;
+    const bool rv = c->visc_mode != MF_VISC_NONE;
+    if (rv && c->nvar != 4) return fail(c, MF_ERR_UNSUPPORTED, "viscosity sources need the Euler equations");
+    if (rv && !c->have_T) return fail(c, MF_ERR_STATE, "viscosity source without transposed operator");
+    // first boundary pass on u
+    int rc = bc_state_pass(c, d_u, dt_off, 0, false);
+    if (rc) return rc;
+    RhsArgs a = make_args(c, d_u, d_du);
+    a.st_kind = st.kind;
+    a.dt = st.dt;
+    a.uprev = st.uprev;
+    a.out = st.out;
+    const int64_t nblk = (c->N + kBlock - 1) / kBlock;
+    const unsigned grid = xcd_grid(nblk);
+    if (rv) {
+        if (c->visc_mode == MF_VISC_RESIDUAL) {
+            const int64_t chunk = ((c->N + c->n_partials - 1) / c->n_partials + kBlock - 1) / kBlock * kBlock;
+            const unsigned nb = (unsigned)((c->N + chunk - 1) / chunk);
+            hipLaunchKernelGGL(k_rv_sum, dim3(nb), dim3(kBlock), 0, c->stream, c->N, chunk, d_u,
+                               c->d_partials, c->d_sc);
+            hipLaunchKernelGGL(k_rv_max, dim3(nb), dim3(kBlock), 0, c->stream, c->N, chunk, d_u,
+                               c->d_partials, c->d_sc);
+        }
+        RhsArgs pa = a;
+        pa.st_kind = ST_NONE;
+        launch_flux_v<4>(c, prec, pa, hv, true, grid);
+        if (prec == MF_PREC_F64)
+            hipLaunchKernelGGL((k_rhs_transpose<4, false>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((k_rhs_transpose<4, true>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+    } else if (c->nvar == 4) {
+        launch_flux_v<4>(c, prec, a, hv, false, grid);
+    } else {
+        launch_flux_v<1>(c, prec, a, hv, false, grid);
+    }
+    MF_HIP(c, hipGetLastError());
+    // second boundary pass on u: only the slip projection changes anything
+    return bc_state_pass(c, d_u, dt_off, 0, true);
+}
+
+int run_set_time(mf_ctx* c, double t, int success_iter)
+{
+    hipLaunchKernelGGL(k_set_time, dim3(1), dim3(1), 0, c->stream, c->d_sc, t, success_iter,
+                       success_iter >= 0 ? 1 : 0);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+int run_advance_time(mf_ctx* c, double dt)
+{
+    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, c->stream, c->d_sc, dt);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+int run_history(mf_ctx* c, const double* d_u, int init)
+{
+    const int P = c->approx_order + 1;
+    hipLaunchKernelGGL(k_history_weights, dim3(1), dim3(1), 0, c->stream, c->d_sc, P, init);
+    const int64_t NV = c->N * c->nvar;
+    hipLaunchKernelGGL(k_history_apply, dim3((unsigned)((NV + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       c->stream, NV, P, d_u, c->d_hist, c->d_approx, c->d_sc);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+int run_limiter(mf_ctx* c, double* d_u, double thr, int var)
+{
+    const int64_t nblk = (c->N + kBlock - 1) / kBlock;
+    const unsigned grid = xcd_grid(nblk);
+    const double gam = c->p0;
+    if (c->nvar == 4) {
+        hipLaunchKernelGGL(k_limiter_blend<4>, dim3(grid), dim3(kBlock), 0, c->stream, c->N, c->k,
+                           c->d_idx, d_u, c->d_lim, thr, var, gam);
+        hipLaunchKernelGGL(k_limiter_commit<4>, dim3((unsigned)nblk), dim3(kBlock), 0, c->stream,
+                           c->N, c->d_lim, d_u);
+    } else {
+        hipLaunchKernelGGL(k_limiter_blend<1>, dim3(grid), dim3(kBlock), 0, c->stream, c->N, c->k,
+                           c->d_idx, d_u, c->d_lim, thr, var, gam);
+        hipLaunchKernelGGL(k_limiter_commit<1>, dim3((unsigned)nblk), dim3(kBlock), 0, c->stream,
+                           c->N, c->d_lim, d_u);
+    }
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+int run_axpby(mf_ctx* c, int64_t n, double a, const double* x, double b, double* y)
+{
+    hipLaunchKernelGGL(k_axpby, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       c->stream, n, a, x, b, y);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+int run_fill(mf_ctx* c, double* p, int64_t n, double v)
+{
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       c->stream, p, n, v);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+// exported for the adjoint source
+int bc_pass_homog(mf_ctx* c, double* u) { return bc_state_pass(c, u, 0.0, 1, false); }
+int bc_pass_state(mf_ctx* c, double* u, double dt_off) { return bc_state_pass(c, u, dt_off, 0, false); }
+
+}  // namespace mf

@@ -1,0 +1,960 @@
+// Setup kernels: kNN stencils, RBF-FD weights, operator packing and the
+// transposed operator used by the viscosity sources.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <hipcub/hipcub.hpp>
+
+#include "mf_device.h"
+
+namespace mf {
+
+// ================================================================= kNN
+// Reference: PointData(points, basis), geometry_primatives.jl:322-339
+// (NearestNeighbors knn(KDTree(points), points, nv, true)).  Uniform grid
+// binning; one thread per query keeps its k best (d2, index) pairs in LDS
+// ordered by squared distance, ties by index; rings of cells are searched
+// until the k-th candidate is closer than every unvisited cell.
+// d2 = (xi - xj)^2 + (yi - yj)^2 without fused multiply-add, as the
+// reference's distance evaluation.
+
+__global__ void k_bbox(int64_t N, const double* x, const double* y, double* out)
+{
+    __shared__ double r[4][1024];
+    double a = INFINITY, b = -INFINITY, c = INFINITY, d = -INFINITY;
+    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
+        a = fmin(a, x[i]);
+        b = fmax(b, x[i]);
+        c = fmin(c, y[i]);
+        d = fmax(d, y[i]);
+    }
+    r[0][threadIdx.x] = a;
+    r[1][threadIdx.x] = b;
+    r[2][threadIdx.x] = c;
+    r[3][threadIdx.x] = d;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            r[0][threadIdx.x] = fmin(r[0][threadIdx.x], r[0][threadIdx.x + w]);
+            r[1][threadIdx.x] = fmax(r[1][threadIdx.x], r[1][threadIdx.x + w]);
+            r[2][threadIdx.x] = fmin(r[2][threadIdx.x], r[2][threadIdx.x + w]);
+            r[3][threadIdx.x] = fmax(r[3][threadIdx.x], r[3][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; ++q) out[q] = r[q][0];
+}
+
+struct Grid {
+    double x0, y0, cs;
+    int64_t ncx, ncy;
+};
+
+__device__ __forceinline__ int64_t cell_of(const Grid& g, double px, double py, int64_t* cx,
+                                           int64_t* cy)
+{
+    int64_t ix = (int64_t)floor((px - g.x0) / g.cs), iy = (int64_t)floor((py - g.y0) / g.cs);
+    ix = ix < 0 ? 0 : (ix >= g.ncx ? g.ncx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy >= g.ncy ? g.ncy - 1 : iy);
+    *cx = ix;
+    *cy = iy;
+    return iy * g.ncx + ix;
+}
+
+__global__ void k_cell_count(int64_t N, const double* x, const double* y, Grid g, int32_t* cell,
+                             int32_t* count)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int64_t cx, cy;
+    const int64_t c = cell_of(g, x[i], y[i], &cx, &cy);
+    cell[i] = (int32_t)c;
+    atomicAdd(&count[c], 1);
+}
+
+__global__ void k_cell_scatter(int64_t N, const int32_t* cell, const int32_t* start,
+                               int32_t* cursor, int32_t* order)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int32_t c = cell[i];
+    const int32_t pos = start[c] + atomicAdd(&cursor[c], 1);
+    order[pos] = (int32_t)i;
+}
+
+// Sort each cell's members by index (cells hold a handful of points) and
+// gather their coordinates in cell order.
+__global__ void k_cell_finish(int64_t ncell, const int32_t* start, const int32_t* count,
+                              int32_t* order, const double* x, const double* y, double2* xys)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int32_t s = start[c], n = count[c];
+    for (int a = 1; a < n; ++a) {
+        const int32_t v = order[s + a];
+        int b = a - 1;
+        while (b >= 0 && order[s + b] > v) {
+            order[s + b + 1] = order[s + b];
+            --b;
+        }
+        order[s + b + 1] = v;
+    }
+    for (int a = 0; a < n; ++a) {
+        const int32_t j = order[s + a];
+        xys[s + a] = make_double2(x[j], y[j]);
+    }
+}
+
+__device__ __forceinline__ bool knn_before(double da, int32_t ia, double db, int32_t ib)
+{
+    return (da < db) || (da == db && ia < ib);
+}
+
+struct KnnArgs {
+    int64_t N;
+    int k;
+    Grid g;
+    const int32_t* start;
+    const int32_t* count;
+    const int32_t* order;
+    const double2* xys;
+    int32_t* out_idx;   // slot-major
+    double* out_d2;     // slot-major
+};
+
+__global__ __launch_bounds__(64) void k_knn_query(const KnnArgs a)
+{
+#pragma clang fp contract(off)
+    extern __shared__ char smem[];
+    const int lane = threadIdx.x;
+    double* bd = reinterpret_cast<double*>(smem);
+    int32_t* bi = reinterpret_cast<int32_t*>(smem + sizeof(double) * 64 * a.k);
+    const int64_t p = (int64_t)blockIdx.x * 64 + lane;  // position in cell order
+    if (p >= a.N) return;
+    const int32_t q = a.order[p];
+    const double2 qp = a.xys[p];
+    const double qx = qp.x, qy = qp.y;
+    int64_t cx, cy;
+    cell_of(a.g, qx, qy, &cx, &cy);
+    const int K = a.k;
+    int cnt = 0;
+    for (int64_t r = 0;; ++r) {
+        const int64_t ax = cx - r, bx = cx + r, ay = cy - r, by = cy + r;
+        for (int64_t jy = ay; jy <= by; ++jy) {
+            if (jy < 0 || jy >= a.g.ncy) continue;
+            for (int64_t jx = ax; jx <= bx; ++jx) {
+                if (jx < 0 || jx >= a.g.ncx) continue;
+                if (r > 0 && jy != ay && jy != by && jx != ax && jx != bx) continue;
+                const int64_t c = jy * a.g.ncx + jx;
+                const int32_t s0 = a.start[c], s1 = s0 + a.count[c];
+                for (int32_t pp = s0; pp < s1; ++pp) {
+                    const double2 cp = a.xys[pp];
+                    const double dx = qx - cp.x, dy = qy - cp.y;
+                    const double d2 = dx * dx + dy * dy;
+                    const int32_t j = a.order[pp];
+                    if (cnt == K && !knn_before(d2, j, bd[(K - 1) * 64 + lane], bi[(K - 1) * 64 + lane]))
+                        continue;
+                    int pos = (cnt < K) ? cnt : K - 1;
+                    while (pos > 0 && knn_before(d2, j, bd[(pos - 1) * 64 + lane], bi[(pos - 1) * 64 + lane])) {
+                        bd[pos * 64 + lane] = bd[(pos - 1) * 64 + lane];
+                        bi[pos * 64 + lane] = bi[(pos - 1) * 64 + lane];
+                        --pos;
+                    }
+                    bd[pos * 64 + lane] = d2;
+                    bi[pos * 64 + lane] = j;
+                    if (cnt < K) ++cnt;
+                }
+            }
+        }
+        const bool all = (ax <= 0 && bx >= a.g.ncx - 1 && ay <= 0 && by >= a.g.ncy - 1);
+        if (all) break;
+        if (cnt == K) {
+            double dm = INFINITY;
+            if (ax > 0) dm = fmin(dm, qx - (a.g.x0 + (double)ax * a.g.cs));
+            if (bx < a.g.ncx - 1) dm = fmin(dm, (a.g.x0 + (double)(bx + 1) * a.g.cs) - qx);
+            if (ay > 0) dm = fmin(dm, qy - (a.g.y0 + (double)ay * a.g.cs));
+            if (by < a.g.ncy - 1) dm = fmin(dm, (a.g.y0 + (double)(by + 1) * a.g.cs) - qy);
+            dm *= (1.0 - 1e-9);
+            if (dm > 0 && bd[(K - 1) * 64 + lane] < dm * dm) break;
+        }
+    }
+    for (int s = 0; s < K; ++s) {
+        a.out_idx[(int64_t)s * a.N + q] = bi[s * 64 + lane];
+        if (a.out_d2) a.out_d2[(int64_t)s * a.N + q] = bd[s * 64 + lane];
+    }
+}
+
+int launch_knn(mf_ctx* c, int k, double* d_d2)
+{
+    const int64_t N = c->N;
+    double* d_box = nullptr;
+    int rc = dalloc(c, &d_box, 4);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, c->stream, N, c->d_x, c->d_y, d_box);
+    double box[4];
+    MF_HIP(c, hipMemcpyAsync(box, d_box, sizeof box, hipMemcpyDeviceToHost, c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(d_box);
+    Grid g;
+    g.x0 = box[0];
+    g.y0 = box[2];
+    double W = box[1] - box[0], H = box[3] - box[2];
+    if (W <= 0) W = 1e-300;
+    if (H <= 0) H = 1e-300;
+    g.cs = sqrt(2.0 * W * H / (double)N);
+    if (!(g.cs > 0)) g.cs = fmax(W, H);
+    g.ncx = (int64_t)floor(W / g.cs) + 1;
+    g.ncy = (int64_t)floor(H / g.cs) + 1;
+    const int64_t ncell = g.ncx * g.ncy;
+    if (ncell > (int64_t)1 << 30) return fail(c, MF_ERR_ARGUMENT, "kNN: degenerate point cloud extent");
+    int32_t *cell = nullptr, *count = nullptr, *start = nullptr, *cursor = nullptr, *order = nullptr;
+    double2* xys = nullptr;
+    if ((rc = dalloc(c, &cell, N)) || (rc = dalloc(c, &count, ncell)) || (rc = dalloc(c, &start, ncell)) ||
+        (rc = dalloc(c, &cursor, ncell)) || (rc = dalloc(c, &order, N)) || (rc = dalloc(c, &xys, N)))
+        return rc;
+    MF_HIP(c, hipMemsetAsync(count, 0, sizeof(int32_t) * ncell, c->stream));
+    MF_HIP(c, hipMemsetAsync(cursor, 0, sizeof(int32_t) * ncell, c->stream));
+    const unsigned gN = (unsigned)((N + 255) / 256), gC = (unsigned)((ncell + 255) / 256);
+    hipLaunchKernelGGL(k_cell_count, dim3(gN), dim3(256), 0, c->stream, N, c->d_x, c->d_y, g, cell, count);
+    size_t tmp_bytes = 0;
+    MF_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, count, start, (int)ncell, c->stream));
+    void* tmp = nullptr;
+    MF_HIP(c, hipMalloc(&tmp, tmp_bytes + 16));
+    MF_HIP(c, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, count, start, (int)ncell, c->stream));
+    hipLaunchKernelGGL(k_cell_scatter, dim3(gN), dim3(256), 0, c->stream, N, cell, start, cursor, order);
+    hipLaunchKernelGGL(k_cell_finish, dim3(gC), dim3(256), 0, c->stream, ncell, start, count, order,
+                       c->d_x, c->d_y, xys);
+    KnnArgs a;
+    a.N = N;
+    a.k = k;
+    a.g = g;
+    a.start = start;
+    a.count = count;
+    a.order = order;
+    a.xys = xys;
+    a.out_idx = c->d_idx;
+    a.out_d2 = d_d2;
+    const size_t lds = (size_t)64 * k * (sizeof(double) + sizeof(int32_t));
+    hipLaunchKernelGGL(k_knn_query, dim3((unsigned)((N + 63) / 64)), dim3(64), lds, c->stream, a);
+    MF_HIP(c, hipGetLastError());
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(tmp);
+    dfree(cell);
+    dfree(count);
+    dfree(start);
+    dfree(cursor);
+    dfree(order);
+    dfree(xys);
+    return MF_OK;
+}
+
+// dx_min = min, dx_avg = mean of the nearest-neighbour distance
+// (geometry_primatives.jl:332-334).  One block, fixed reduction order.
+__global__ void k_dx_stats(int64_t N, const double* d2nn, double* out)
+{
+    __shared__ double smin[1024], ssum[1024];
+    double mn = INFINITY, sm = 0.0;
+    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
+        const double d = sqrt(d2nn[i]);
+        mn = fmin(mn, d);
+        sm += d;
+    }
+    smin[threadIdx.x] = mn;
+    ssum[threadIdx.x] = sm;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+            ssum[threadIdx.x] += ssum[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = smin[0];
+        out[1] = ssum[0] / (double)N;
+    }
+}
+
+int launch_dx_stats(mf_ctx* c, const double* d_d2, double* dx_min, double* dx_avg)
+{
+    double* d_out = nullptr;
+    int rc = dalloc(c, &d_out, 2);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_dx_stats, dim3(1), dim3(1024), 0, c->stream, c->N, d_d2 + c->N, d_out);
+    double h[2];
+    MF_HIP(c, hipMemcpyAsync(h, d_out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(d_out);
+    *dx_min = h[0];
+    *dx_avg = h[1];
+    return MF_OK;
+}
+
+// ============================================================= weights
+// Reference: compute_flux_operator, compute_operators.jl:409-453 and
+// :549-594.  One wavefront per stencil.  The m x m saddle matrix
+// [R P; P' 0] lives in LDS (column-major, leading dimension ld); lane i
+// owns row i while assembling.  The solve restates LAPACK DSYTF2/DSYTRS
+// (UPLO='U', Bunch-Kaufman pivoting), which Julia's Symmetric(M) \ rhs
+// (:435) calls; the operation order matches that algorithm so the pivot
+// sequence is the reference's.
+struct WeightsArgs {
+    const double* x;
+    const double* y;
+    const int64_t* ptr;
+    const int32_t* ridx;
+    const int32_t* ell;
+    int ell_k;
+    int64_t N;
+    int p, np, kd, ld;
+    double kfact;
+    PhsTable tab;
+    double* wx;
+    double* wy;
+    int* status;
+};
+
+__device__ __forceinline__ double ipow(double v, int e)
+{
+    double r = 1.0;
+    for (int i = 0; i < e; ++i) r = r * v;
+    return r;
+}
+
+__device__ __forceinline__ void mono_exp(int t, int* ax, int* ay)
+{
+    int d = 0;
+    while (t >= d + 1) {
+        t -= d + 1;
+        ++d;
+    }
+    *ax = d - t;
+    *ay = t;
+}
+
+__device__ double phs_eval(const PhsTable& tb, double s, double x, double y)
+{
+#pragma clang fp contract(off)
+    const double r = sqrt(x * x + y * y);
+    double val = 0.0;
+    for (int t = 0; t < tb.nt; ++t) {
+        if (tb.c[t] == 0.0) continue;
+        double term = tb.c[t] * ipow(s, tb.a[t]);
+        if (tb.b[t] >= 0)
+            term = term * ipow(r, tb.b[t]);
+        else
+            term = term / ipow(r, -tb.b[t]);
+        val = val + term;
+    }
+    return val;
+}
+
+// s^k nearly correctly rounded (double-double product).
+__device__ double pow_k(double s, int k)
+{
+    double hi = 1.0, lo = 0.0;
+    for (int r = 0; r < k; ++r) {
+        const double ph = hi * s;
+        const double pl = fma(hi, s, -ph) + lo * s;
+        hi = ph + pl;
+        lo = pl - (hi - ph);
+    }
+    return hi;
+}
+
+__device__ __forceinline__ void argmax_wave(double& v, int& idx)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const double v2 = __shfl_xor(v, o);
+        const int i2 = __shfl_xor(idx, o);
+        if (v2 > v || (v2 == v && i2 < idx)) {
+            v = v2;
+            idx = i2;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_weights(const WeightsArgs a)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double sm[];
+    const int ld = a.ld;
+    double* A = sm;
+    double* B = A + (size_t)ld * ld;
+    double* X = B + 2 * ld;
+    int* ipiv = reinterpret_cast<int*>(X + 2 * ld);
+    const int64_t e = blockIdx.x;
+    const int lane = threadIdx.x;
+    int64_t off;
+    int n;
+    if (a.ptr) {
+        off = a.ptr[e];
+        n = (int)(a.ptr[e + 1] - off);
+    } else {
+        off = e;
+        n = a.ell_k;
+    }
+    auto sidx = [&](int j) -> int32_t { return a.ptr ? a.ridx[off + j] : a.ell[(int64_t)j * a.N + e]; };
+    const int m = n + a.np;
+#define AA(i, j) A[((i)-1) + (size_t)((j)-1) * ld]
+#define BB(i, c) B[((i)-1) + (size_t)(c) * ld]
+    // shift_stencil (:225-246): centre on node 0, scale each dimension
+    const int32_t c0 = sidx(0);
+    double px = 0.0, py = 0.0;
+    if (lane < n) {
+        const int32_t q = sidx(lane);
+        px = a.x[q] - a.x[c0];
+        py = a.y[q] - a.y[c0];
+    }
+    double mx = fabs(px), my = fabs(py);
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, o));
+        my = fmax(my, __shfl_xor(my, o));
+    }
+    const double sx = 1.0 / mx, sy = 1.0 / my;
+    if (lane < n) {
+        X[2 * lane] = px * sx;
+        X[2 * lane + 1] = py * sy;
+    }
+    __syncthreads();
+    // interpolation_block [R P; P' 0] and right-hand sides [L_x, L_y]
+    if (lane < m) {
+        const int i = lane;
+        if (i < n) {
+            const double xi = X[2 * i], yi = X[2 * i + 1];
+            for (int j = 0; j < n; ++j) {
+                const double dx = xi - X[2 * j], dy = yi - X[2 * j + 1];
+                A[i + (size_t)j * ld] = ipow(sqrt(dx * dx + dy * dy), a.p);
+            }
+            for (int t = 0; t < a.np; ++t) {
+                int ax, ay;
+                mono_exp(t, &ax, &ay);
+                A[i + (size_t)(n + t) * ld] = ipow(xi, ax) * ipow(yi, ay);
+            }
+            double qx = -xi, qy = -yi;
+            if (i == 0) qx = qy = DBL_EPSILON;
+            B[i] = phs_eval(a.tab, qx, qx, qy);
+            B[i + ld] = phs_eval(a.tab, qy, qx, qy);
+        } else {
+            const int t = i - n;
+            int ax, ay;
+            mono_exp(t, &ax, &ay);
+            for (int j = 0; j < n; ++j) A[i + (size_t)j * ld] = ipow(X[2 * j], ax) * ipow(X[2 * j + 1], ay);
+            for (int j = n; j < m; ++j) A[i + (size_t)j * ld] = 0.0;
+            B[i] = (ax == a.kd && ay == 0) ? a.kfact : 0.0;
+            B[i + ld] = (ay == a.kd && ax == 0) ? a.kfact : 0.0;
+        }
+    }
+    __syncthreads();
+    // ---- DSYTF2, UPLO = 'U'
+    const double alpha = (1.0 + sqrt(17.0)) / 8.0;
+    int info = 0;
+    int k = m;
+    double* W = X;  // reused: 2 x ld scratch for the 2x2-pivot multipliers
+    while (k >= 1) {
+        int kstep = 1;
+        const double absakk = fabs(AA(k, k));
+        double colmax = 0.0;
+        int imax = 0;
+        if (k > 1) {
+            double v = (lane < k - 1) ? fabs(AA(lane + 1, k)) : -1.0;
+            int id = (lane < k - 1) ? lane + 1 : 0x7fffffff;
+            argmax_wave(v, id);
+            colmax = v;
+            imax = id;
+        }
+        int kp;
+        if (fmax(absakk, colmax) == 0.0 || isnan(absakk)) {
+            if (info == 0) info = k;
+            kp = k;
+        } else {
+            if (absakk >= alpha * colmax) {
+                kp = k;
+            } else {
+                // largest off-diagonal magnitude in row/column imax of the leading k x k block
+                const int jj = lane + 1;
+                double v = -1.0;
+                if (jj > imax && jj <= k) v = fabs(AA(imax, jj));
+                else if (jj < imax) v = fabs(AA(jj, imax));
+                for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+                const double rowmax = v;
+                if (absakk >= alpha * colmax * (colmax / rowmax)) {
+                    kp = k;
+                } else if (fabs(AA(imax, imax)) >= alpha * rowmax) {
+                    kp = imax;
+                } else {
+                    kp = imax;
+                    kstep = 2;
+                }
+            }
+            const int kk = k - kstep + 1;
+            if (kp != kk) {
+                const int ii = lane + 1;
+                if (ii <= kp - 1) {
+                    const double t = AA(ii, kk);
+                    AA(ii, kk) = AA(ii, kp);
+                    AA(ii, kp) = t;
+                }
+                if (ii >= kp + 1 && ii <= kk - 1) {
+                    const double t = AA(ii, kk);
+                    AA(ii, kk) = AA(kp, ii);
+                    AA(kp, ii) = t;
+                }
+                if (lane == 0) {
+                    const double t = AA(kk, kk);
+                    AA(kk, kk) = AA(kp, kp);
+                    AA(kp, kp) = t;
+                    if (kstep == 2) {
+                        const double t2 = AA(k - 1, k);
+                        AA(k - 1, k) = AA(kp, k);
+                        AA(kp, k) = t2;
+                    }
+                }
+            }
+            __syncthreads();
+            if (kstep == 1) {
+                const double r1 = 1.0 / AA(k, k);
+                const int i = lane + 1;
+                if (i <= k - 1) {
+                    const double xi = AA(i, k);
+                    for (int j = i; j <= k - 1; ++j) {
+                        const double xj = AA(j, k);
+                        if (xj != 0.0) {
+                            const double temp = -r1 * xj;
+                            AA(i, j) = AA(i, j) + xi * temp;
+                        }
+                    }
+                }
+                __syncthreads();
+                if (i <= k - 1) AA(i, k) = r1 * AA(i, k);
+            } else if (k > 2) {
+                double d12 = AA(k - 1, k);
+                const double d22 = AA(k - 1, k - 1) / d12;
+                const double d11 = AA(k, k) / d12;
+                const double t = 1.0 / (d11 * d22 - 1.0);
+                d12 = t / d12;
+                const int j = lane + 1;
+                if (j <= k - 2) {
+                    W[j - 1] = d12 * (d11 * AA(j, k - 1) - AA(j, k));   // wkm1
+                    W[ld + j - 1] = d12 * (d22 * AA(j, k) - AA(j, k - 1));  // wk
+                }
+                __syncthreads();
+                const int i = lane + 1;
+                if (i <= k - 2) {
+                    const double aik = AA(i, k), aik1 = AA(i, k - 1);
+                    for (int jc = i; jc <= k - 2; ++jc)
+                        AA(i, jc) = AA(i, jc) - aik * W[ld + jc - 1] - aik1 * W[jc - 1];
+                }
+                __syncthreads();
+                if (j <= k - 2) {
+                    AA(j, k) = W[ld + j - 1];
+                    AA(j, k - 1) = W[j - 1];
+                }
+            }
+        }
+        if (lane == 0) {
+            if (kstep == 1) {
+                ipiv[k - 1] = kp;
+            } else {
+                ipiv[k - 1] = -kp;
+                ipiv[k - 2] = -kp;
+            }
+        }
+        __syncthreads();
+        k -= kstep;
+    }
+    if (info && lane == 0) atomicOr(a.status, 1);
+    // ---- DSYTRS, UPLO = 'U', two right-hand sides
+    k = m;
+    while (k >= 1) {
+        if (ipiv[k - 1] > 0) {
+            const int kp = ipiv[k - 1];
+            if (kp != k && lane < 2) {
+                const double t = BB(k, lane);
+                BB(k, lane) = BB(kp, lane);
+                BB(kp, lane) = t;
+            }
+            __syncthreads();
+            const int i = lane + 1;
+            const double b0 = BB(k, 0), b1 = BB(k, 1);
+            if (i <= k - 1) {
+                if (b0 != 0.0) BB(i, 0) = BB(i, 0) + AA(i, k) * (-1.0 * b0);
+                if (b1 != 0.0) BB(i, 1) = BB(i, 1) + AA(i, k) * (-1.0 * b1);
+            }
+            __syncthreads();
+            if (lane < 2) {
+                const double r = 1.0 / AA(k, k);
+                BB(k, lane) = r * BB(k, lane);
+            }
+            __syncthreads();
+            k -= 1;
+        } else {
+            const int kp = -ipiv[k - 1];
+            if (kp != k - 1 && lane < 2) {
+                const double t = BB(k - 1, lane);
+                BB(k - 1, lane) = BB(kp, lane);
+                BB(kp, lane) = t;
+            }
+            __syncthreads();
+            const int i = lane + 1;
+            const double b0 = BB(k, 0), b1 = BB(k, 1);
+            const double c0v = BB(k - 1, 0), c1v = BB(k - 1, 1);
+            if (i <= k - 2) {
+                if (b0 != 0.0) BB(i, 0) = BB(i, 0) + AA(i, k) * (-1.0 * b0);
+                if (b1 != 0.0) BB(i, 1) = BB(i, 1) + AA(i, k) * (-1.0 * b1);
+                if (c0v != 0.0) BB(i, 0) = BB(i, 0) + AA(i, k - 1) * (-1.0 * c0v);
+                if (c1v != 0.0) BB(i, 1) = BB(i, 1) + AA(i, k - 1) * (-1.0 * c1v);
+            }
+            __syncthreads();
+            if (lane < 2) {
+                const double akm1k = AA(k - 1, k);
+                const double akm1 = AA(k - 1, k - 1) / akm1k;
+                const double ak = AA(k, k) / akm1k;
+                const double denom = akm1 * ak - 1.0;
+                const double bkm1 = BB(k - 1, lane) / akm1k;
+                const double bk = BB(k, lane) / akm1k;
+                BB(k - 1, lane) = (ak * bkm1 - bk) / denom;
+                BB(k, lane) = (akm1 * bk - bkm1) / denom;
+            }
+            __syncthreads();
+            k -= 2;
+        }
+    }
+    k = 1;
+    while (k <= m) {
+        const bool one = ipiv[k - 1] > 0;
+        if (lane < 2) {
+            double temp = 0.0;
+            for (int i = 1; i <= k - 1; ++i) temp = temp + BB(i, lane) * AA(i, k);
+            const double nk = BB(k, lane) + -1.0 * temp;
+            double nk1 = 0.0;
+            if (!one) {
+                double temp2 = 0.0;
+                for (int i = 1; i <= k - 1; ++i) temp2 = temp2 + BB(i, lane) * AA(i, k + 1);
+                nk1 = BB(k + 1, lane) + -1.0 * temp2;
+            }
+            BB(k, lane) = nk;
+            if (!one) BB(k + 1, lane) = nk1;
+            const int kp = one ? ipiv[k - 1] : -ipiv[k - 1];
+            if (kp != k) {
+                const double t = BB(k, lane);
+                BB(k, lane) = BB(kp, lane);
+                BB(kp, lane) = t;
+            }
+        }
+        __syncthreads();
+        k += one ? 1 : 2;
+    }
+#undef AA
+#undef BB
+    // scale back (:438-439, :579-580)
+    if (lane < n) {
+        const double fx = pow_k(sx, a.kd), fy = pow_k(sy, a.kd);
+        const int64_t o = a.ptr ? off + lane : (int64_t)lane * a.N + e;
+        a.wx[o] = fx * B[lane];
+        a.wy[o] = fy * B[lane + ld];
+    }
+}
+
+int launch_weights(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k, int64_t n_stencils,
+                   int maxn, int p, int deg, int kd, double* d_wx, double* d_wy, int* d_status)
+{
+    WeightsArgs a;
+    a.x = c->d_x;
+    a.y = c->d_y;
+    a.ptr = d_ptr;
+    a.ridx = d_ptr ? d_idx : nullptr;
+    a.ell = d_ptr ? nullptr : d_idx;
+    a.ell_k = ell_k;
+    a.N = c->N;
+    a.p = p;
+    a.np = (deg + 1) * (deg + 2) / 2;
+    a.kd = kd;
+    const int mmax = maxn + a.np;
+    if (mmax > kMaxStencilM)
+        return fail(c, MF_ERR_UNSUPPORTED, "weights: stencil + polynomial size exceeds 64");
+    a.ld = mmax | 1;
+    double kf = 1.0;
+    for (int i = 2; i <= kd; ++i) kf *= i;
+    a.kfact = kf;
+    a.tab = phs_table(p, kd);
+    a.wx = d_wx;
+    a.wy = d_wy;
+    a.status = d_status;
+    const size_t lds = sizeof(double) * ((size_t)a.ld * a.ld + 4 * a.ld) + sizeof(int) * a.ld;
+    if (n_stencils > 0x7fffffff) return fail(c, MF_ERR_ARGUMENT, "weights: too many stencils");
+    hipLaunchKernelGGL(k_weights, dim3((unsigned)n_stencils), dim3(64), lds, c->stream, a);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+// ---------------------------------------------------- operator packing
+__global__ void k_pack_flux(int64_t n, const double* wx, const double* wy, double2* wxy, float2* w32)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    wxy[q] = make_double2(wx[q], wy[q]);
+    w32[q] = make_float2((float)wx[q], (float)wy[q]);
+}
+
+// H = Dx^2k + Dy^2k, elementwise (hyperviscosity.jl:44)
+__global__ void k_pack_hv(int64_t n, const double* wx, const double* wy, double* hv, float* h32)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double h = wy ? wx[q] + wy[q] : wx[q];
+    hv[q] = h;
+    h32[q] = (float)h;
+}
+
+int launch_pack_ops(mf_ctx* c, int target, const double* d_wx, const double* d_wy)
+{
+    const int64_t n = c->N * c->k;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (target == MF_OP_FLUX)
+        hipLaunchKernelGGL(k_pack_flux, dim3(g), dim3(256), 0, c->stream, n, d_wx, d_wy, c->d_wxy, c->d_wxy32);
+    else
+        hipLaunchKernelGGL(k_pack_hv, dim3(g), dim3(256), 0, c->stream, n, d_wx, d_wy, c->d_hv, c->d_hv32);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+// ------------------------------------------------- transposed operator
+// Column i of D lists the rows r whose stencil contains i; stored as
+// SELL-64 (slices of 64 consecutive columns, padded to the slice's
+// largest count, entry e of column i at off[slice] + e*64 + i%64), each
+// column sorted by row like the reference's CSC (compute_operators.jl:443-452).
+__global__ void k_t_count(int64_t N, int K, const int32_t* idx, int32_t* deg)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N * K) return;
+    atomicAdd(&deg[idx[q]], 1);
+}
+
+__global__ void k_t_width(int64_t N, int64_t ns, const int32_t* deg, int32_t* w, int64_t* wsize)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    int mx = 0;
+    for (int l = 0; l < kSellC; ++l) {
+        const int64_t i = s * kSellC + l;
+        if (i < N) mx = max(mx, deg[i]);
+    }
+    w[s] = mx;
+    wsize[s] = (int64_t)mx * kSellC;
+}
+
+__global__ void k_t_fill(int64_t N, int K, const int32_t* idx, const double2* wxy, const int64_t* off,
+                         int32_t* cursor, int32_t* trow, double2* tw)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N * K) return;
+    const int64_t r = q % N;
+    const int32_t j = idx[q];
+    const int pos = atomicAdd(&cursor[j], 1);
+    const int64_t t = off[j / kSellC] + (int64_t)pos * kSellC + (j % kSellC);
+    trow[t] = (int32_t)r;
+    tw[t] = wxy[q];
+}
+
+__global__ void k_t_sort(int64_t N, const int32_t* deg, const int32_t* w, const int64_t* off,
+                         int32_t* trow, double2* tw, float2* tw32)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int64_t base = off[i / kSellC] + (i % kSellC);
+    const int n = deg[i];
+    for (int a2 = 1; a2 < n; ++a2) {
+        const int32_t rv = trow[base + (int64_t)a2 * kSellC];
+        const double2 wv = tw[base + (int64_t)a2 * kSellC];
+        int b = a2 - 1;
+        while (b >= 0 && trow[base + (int64_t)b * kSellC] > rv) {
+            trow[base + (int64_t)(b + 1) * kSellC] = trow[base + (int64_t)b * kSellC];
+            tw[base + (int64_t)(b + 1) * kSellC] = tw[base + (int64_t)b * kSellC];
+            --b;
+        }
+        trow[base + (int64_t)(b + 1) * kSellC] = rv;
+        tw[base + (int64_t)(b + 1) * kSellC] = wv;
+    }
+    const int W = w[i / kSellC];
+    for (int a2 = n; a2 < W; ++a2) {
+        trow[base + (int64_t)a2 * kSellC] = (int32_t)i;
+        tw[base + (int64_t)a2 * kSellC] = make_double2(0.0, 0.0);
+    }
+    for (int a2 = 0; a2 < W; ++a2) {
+        const double2 v = tw[base + (int64_t)a2 * kSellC];
+        tw32[base + (int64_t)a2 * kSellC] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+int launch_build_transpose(mf_ctx* c)
+{
+    const int64_t N = c->N, K = c->k;
+    const int64_t ns = (N + kSellC - 1) / kSellC;
+    int32_t *deg = nullptr, *cursor = nullptr;
+    int64_t* wsize = nullptr;
+    int rc;
+    dfree(c->d_sell_off);
+    dfree(c->d_sell_w);
+    dfree(c->d_trow);
+    dfree(c->d_tw);
+    dfree(c->d_tw32);
+    c->d_sell_off = nullptr;
+    c->d_sell_w = nullptr;
+    c->d_trow = nullptr;
+    c->d_tw = nullptr;
+    c->d_tw32 = nullptr;
+    if ((rc = dalloc(c, &deg, N)) || (rc = dalloc(c, &cursor, N)) || (rc = dalloc(c, &wsize, ns + 1)) ||
+        (rc = dalloc(c, &c->d_sell_off, ns + 1)) || (rc = dalloc(c, &c->d_sell_w, ns)))
+        return rc;
+    MF_HIP(c, hipMemsetAsync(deg, 0, sizeof(int32_t) * N, c->stream));
+    MF_HIP(c, hipMemsetAsync(cursor, 0, sizeof(int32_t) * N, c->stream));
+    MF_HIP(c, hipMemsetAsync(wsize, 0, sizeof(int64_t) * (ns + 1), c->stream));
+    const unsigned gq = (unsigned)((N * K + 255) / 256), gs = (unsigned)((ns + 255) / 256);
+    hipLaunchKernelGGL(k_t_count, dim3(gq), dim3(256), 0, c->stream, N, (int)K, c->d_idx, deg);
+    hipLaunchKernelGGL(k_t_width, dim3(gs), dim3(256), 0, c->stream, N, ns, deg, c->d_sell_w, wsize);
+    size_t tb = 0;
+    MF_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, wsize, c->d_sell_off, (int)(ns + 1), c->stream));
+    void* tmp = nullptr;
+    MF_HIP(c, hipMalloc(&tmp, tb + 16));
+    MF_HIP(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, wsize, c->d_sell_off, (int)(ns + 1), c->stream));
+    int64_t total = 0;
+    MF_HIP(c, hipMemcpyAsync(&total, c->d_sell_off + ns, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(tmp);
+    if ((rc = dalloc(c, &c->d_trow, total)) || (rc = dalloc(c, &c->d_tw, total)) ||
+        (rc = dalloc(c, &c->d_tw32, total)))
+        return rc;
+    hipLaunchKernelGGL(k_t_fill, dim3(gq), dim3(256), 0, c->stream, N, (int)K, c->d_idx, c->d_wxy,
+                       c->d_sell_off, cursor, c->d_trow, c->d_tw);
+    hipLaunchKernelGGL(k_t_sort, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c->stream, N, deg,
+                       c->d_sell_w, c->d_sell_off, c->d_trow, c->d_tw, c->d_tw32);
+    MF_HIP(c, hipGetLastError());
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(deg);
+    dfree(cursor);
+    dfree(wsize);
+    c->n_slices = ns;
+    c->have_T = true;
+    return MF_OK;
+}
+
+// int16 neighbour offsets j - i, when every one fits
+__global__ void k_didx(int64_t N, int K, const int32_t* idx, int16_t* didx, int* bad)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N * K) return;
+    const int64_t i = q % N;
+    const int64_t dlt = (int64_t)idx[q] - i;
+    if (dlt < -32768 || dlt > 32767) {
+        atomicOr(bad, 1);
+        didx[q] = 0;
+    } else {
+        didx[q] = (int16_t)dlt;
+    }
+}
+
+int launch_build_didx(mf_ctx* c)
+{
+    const int64_t n = c->N * c->k;
+    int* bad = nullptr;
+    int rc = dalloc(c, &bad, 1);
+    if (rc) return rc;
+    MF_HIP(c, hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_didx, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->N, c->k,
+                       c->d_idx, c->d_didx, bad);
+    int hb = 0;
+    MF_HIP(c, hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(bad);
+    c->didx_ok = (hb == 0);
+    return MF_OK;
+}
+
+// ------------------------------------------------- layout conversions
+template <class T>
+__global__ void k_ell_to_rows(int64_t N, int K, const T* ell, T* rows)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N * K) return;
+    const int64_t i = q / K, s = q % K;
+    rows[q] = ell[s * N + i];
+}
+
+template <class T>
+__global__ void k_rows_to_ell(int64_t N, int K, const T* rows, T* ell)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N * K) return;
+    const int64_t i = q / K, s = q % K;
+    ell[s * N + i] = rows[q];
+}
+
+int launch_ell_to_rows(mf_ctx* c, const int32_t* d_ell, int32_t* d_rows, int k)
+{
+    const int64_t n = c->N * k;
+    hipLaunchKernelGGL(k_ell_to_rows<int32_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       c->N, k, d_ell, d_rows);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+int launch_ell_to_rows_d(mf_ctx* c, const double* d_ell, double* d_rows, int k)
+{
+    const int64_t n = c->N * k;
+    hipLaunchKernelGGL(k_ell_to_rows<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       c->N, k, d_ell, d_rows);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+int launch_rows_to_ell(mf_ctx* c, const int32_t* d_rows, int32_t* d_ell, int k)
+{
+    const int64_t n = c->N * k;
+    hipLaunchKernelGGL(k_rows_to_ell<int32_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       c->N, k, d_rows, d_ell);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+int launch_rows_to_ell_d(mf_ctx* c, const double* d_rows, double* d_ell, int k)
+{
+    const int64_t n = c->N * k;
+    hipLaunchKernelGGL(k_rows_to_ell<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       c->N, k, d_rows, d_ell);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+__global__ void k_soa_to_aos(int64_t N, int V, const double* soa, double* aos)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N * V) return;
+    const int64_t i = q / V, v = q % V;
+    aos[q] = soa[v * N + i];
+}
+
+__global__ void k_aos_to_soa(int64_t N, int V, const double* aos, double* soa)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N * V) return;
+    const int64_t i = q / V, v = q % V;
+    soa[v * N + i] = aos[q];
+}
+
+int launch_soa_to_aos(mf_ctx* c, const double* d_soa, double* d_aos)
+{
+    const int64_t n = c->N * c->nvar;
+    hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->N,
+                       c->nvar, d_soa, d_aos);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+int launch_aos_to_soa(mf_ctx* c, const double* d_aos, double* d_soa)
+{
+    const int64_t n = c->N * c->nvar;
+    hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->N,
+                       c->nvar, d_aos, d_soa);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+}  // namespace mf

@@ -1,0 +1,79 @@
+"""Source terms: SourceTerms and the dissipation sources.
+
+Mirrors sources/generic_sources.jl:7-18 and sources/hyperviscosity.jl.
+The operators and per-call work run on the GPU; these objects carry the
+parameters and are installed into a semidiscretization's context.
+"""
+from collections import OrderedDict
+
+from . import _lib as L
+
+
+class SourceTerms:
+    """SourceTerms(; kwargs...): sources applied in keyword order
+    (generic_sources.jl:7-18, rbfsolver.jl:388-395)."""
+
+    def __init__(self, **sources):
+        self.sources = OrderedDict(sources)
+
+    def __iter__(self):
+        return iter(self.sources.values())
+
+    def __len__(self):
+        return len(self.sources)
+
+    def keys(self):
+        return self.sources.keys()
+
+    def values(self):
+        return self.sources.values()
+
+    def items(self):
+        return self.sources.items()
+
+
+class SourceHyperviscosityFlyer:
+    """du -= gamma * (Dx^2k + Dy^2k) u with gamma = c * dx_min^(2k)
+    (hyperviscosity.jl:14-64)."""
+
+    def __init__(self, solver, equations, domain, k=2, c=1.0):
+        self.k = int(k)
+        self.c = float(c)
+        self.gamma = self.c * domain.pd.dx_min ** (2 * self.k)
+
+
+class SourceResidualViscosityTominec:
+    """Residual viscosity: eps = min(eps_rv, eps_uw), du -= sum_d D_d^T(eps D_d u)
+    (hyperviscosity.jl:181-244, :382-409)."""
+
+    mode = L.MF_VISC_RESIDUAL
+
+    def __init__(self, solver, equations, domain, c_rv=1.0, c_uw=1.0, polydeg=4):
+        self.c_rv, self.c_uw, self.polydeg = float(c_rv), float(c_uw), int(polydeg)
+        self.h = domain.pd.dx_avg
+
+
+class SourceUpwindViscosityTominec:
+    """Upwind viscosity: eps = eps_uw (hyperviscosity.jl:148-167, :351-380)."""
+
+    mode = L.MF_VISC_UPWIND
+
+    def __init__(self, solver, equations, domain, c=1.0, c_uw=1.0, polydeg=4):
+        self.c_rv, self.c_uw, self.polydeg = float(c), float(c_uw), int(polydeg)
+        self.h = domain.pd.dx_avg
+
+
+class SourceHyperviscosityTominec:
+    """lap' * lap hyperviscosity (hyperviscosity.jl:80-134).  Its operator
+    has the two-ring pattern of a sparse product, outside the fixed-width
+    stencil layout of this build."""
+
+    def __init__(self, *a, **kw):
+        raise NotImplementedError("SourceHyperviscosityTominec is not provided by the HIP engine")
+
+
+class SourceIGR:
+    """Information geometric regularization (sources/IGR.jl): not provided."""
+
+    def __init__(self, *a, **kw):
+        raise NotImplementedError("SourceIGR is not provided by the HIP engine")

@@ -1,0 +1,199 @@
+"""GPU parity: libmfrbf (through the C ABI) against the CPU oracle on the same
+seeded inputs.  Tolerances: integer outputs bit-exact; FP64 rhs! 1e-12
+norm-wise relative (max|d| / max|ref|); split-FP32 rhs! 1e-6."""
+import numpy as np
+import pytest
+
+import cases
+import oracle as O
+from conftest import rel_inf
+from mfrbf import _lib as L
+from mfrbf import configs
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-6
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+# ----------------------------------------------------------------- kNN
+@pytest.mark.parametrize("side,k,seed", [(40, 15, 20261202), (48, 30, 20261203), (33, 50, 5)])
+def test_knn_bit_exact_vs_brute(mf, side, k, seed):
+    _say("knn brute side=%d k=%d" % (side, k))
+    x, y, _, _, _ = O.cloud_square(side, seed)
+    ctx = L.Context(0)
+    ctx.set_points(x, y)
+    idx, dist, dmin, davg = ctx.knn(k, want_dist=True)
+    ib, d2 = O.knn_brute(x, y, k)
+    assert np.array_equal(idx, ib)
+    assert np.array_equal(dist, np.sqrt(d2))
+    omin, oavg = O.dx_stats(d2[:, 1])
+    assert dmin == omin
+    assert abs(davg - oavg) <= 2e-15 * oavg  # mean: tree vs pairwise summation order
+
+
+def test_knn_bit_exact_large(mf):
+    _say("knn grid side=256 k=30")
+    x, y, _, _, _ = O.cloud_square(256, 20261205)
+    ctx = L.Context(0)
+    ctx.set_points(x, y)
+    idx, _, _, _ = ctx.knn(30)
+    ig, _ = O.knn_grid(x, y, 30)
+    assert np.array_equal(idx, ig)
+
+
+# ------------------------------------------------------------- weights
+@pytest.mark.parametrize("p,deg,k,kd", [(3, 2, 15, 1), (5, 3, 30, 1), (5, 3, 30, 4), (3, 3, 20, 1)])
+def test_weights_vs_oracle(mf, p, deg, k, kd):
+    _say("weights p=%d deg=%d k=%d kd=%d" % (p, deg, k, kd))
+    x, y, _, _, _ = O.cloud_square(40, 20261202)
+    N = len(x)
+    idx, _ = O.knn_grid(x, y, k)
+    ctx = L.Context(0)
+    ctx.set_points(x, y)
+    ctx.set_stencil(idx)
+    wx, wy = ctx.compute_operator(p, deg, kd, L.MF_OP_FLUX, want=True, k=k)
+    ox, oy, rc = O.weights(x, y, O.ell_ptr(N, k), idx, p, deg, kd)
+    assert rc == 0
+    ex = rel_inf(wx.ravel(), ox)
+    ey = rel_inf(wy.ravel(), oy)
+    _say("  weights rel diff %.2e %.2e  bit-equal fraction %.3f" %
+         (ex, ey, np.mean(wx.ravel() == ox)))
+    assert ex < 1e-10 and ey < 1e-10
+
+
+def test_weights_ragged_vs_oracle(mf):
+    _say("ragged weights")
+    x, y, _, _, _ = O.cloud_square(48, 20261204)
+    N = len(x)
+    sizes = configs.ragged_sizes(N, 20261204, p_boundary=0.2)
+    big, _ = O.knn_grid(x, y, 50)
+    ptr = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    idx = np.concatenate([big[e, :sizes[e]] for e in range(N)]).astype(np.int32)
+    ctx = L.Context(0)
+    ctx.set_points(x, y)
+    ctx.weights_ragged_set(ptr, idx)
+    ctx.weights_ragged_run(5, 3, 1)
+    wx, wy = ctx.weights_ragged_get()
+    ox, oy, rc = O.weights(x, y, ptr, idx, 5, 3, 1)
+    assert rc == 0
+    assert rel_inf(wx, ox) < 1e-9 and rel_inf(wy, oy) < 1e-9
+
+
+# ---------------------------------------------------------------- rhs!
+def _rhs_pair(c, prec=L.MF_PREC_F64, t=0.0):
+    u = c.u0.copy()
+    du = np.zeros_like(u)
+    c.ctx.rhs_host(t, u, du, prec)
+    uo = c.u0.copy()
+    duo, eps, epsc = c.prob.rhs(uo, t)
+    return u, du, uo, duo, eps, epsc
+
+
+def test_rhs_advection_f64(mf):
+    _say("rhs advection C1")
+    c = cases.build(configs.c1())
+    u, du, uo, duo, _, _ = _rhs_pair(c, t=0.05)
+    assert rel_inf(du, duo) < TOL64
+    assert rel_inf(u, uo) < 1e-15
+
+
+def test_rhs_euler_f64(mf):
+    _say("rhs euler 64x64 k=30")
+    c = cases.build(configs.c2(), side=64)
+    u, du, uo, duo, _, _ = _rhs_pair(c, t=0.01)
+    assert rel_inf(du, duo) < TOL64
+    assert rel_inf(u, uo) < 1e-15
+
+
+def test_rhs_euler_hv_f64(mf):
+    _say("rhs euler + flyer hv")
+    c = cases.build(configs.c2(), side=64, hv=(2, 1e-4))
+    u, du, uo, duo, _, _ = _rhs_pair(c)
+    assert rel_inf(du, duo) < TOL64
+
+
+@pytest.mark.parametrize("visc", ["upwind", "residual"])
+def test_rhs_euler_viscosity_f64(mf, visc):
+    _say("rhs euler + %s viscosity" % visc)
+    c = cases.build(configs.c2(), side=64, visc=visc)
+    if visc == "residual":
+        # a history so that eps_rv is active: approx_du from two snapshots
+        u1 = c.u0 * 1.0001
+        th = np.zeros(4)
+        sh = np.zeros(4 * c.u0.size)
+        ad = np.zeros(c.u0.size)
+        O.history_update(3, 0, 0.0, c.u0.ravel().copy(), th, sh, ad)
+        O.history_update(3, 1, 0.001, u1.ravel().copy(), th, sh, ad)
+        c.prob.approx_du[:] = ad
+        c.prob.cfg.success_iter = 1
+        c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+        c.ctx.history_update(0.0, 0)
+        c.ctx.upload(L.MF_BUF_U, np.ascontiguousarray(u1))
+        c.ctx.history_update(0.001, 1)
+        assert rel_inf(c.ctx.approx_du(), ad.reshape(4, -1)) < 1e-12
+    u, du, uo, duo, eps, epsc = _rhs_pair(c, t=0.001)
+    geps, gcode = c.ctx.viscosity()
+    assert rel_inf(geps, eps) < 1e-12
+    flips = int(np.sum(gcode != epsc))
+    _say("  eps code flips %d" % flips)
+    assert flips == 0
+    assert rel_inf(du, duo) < TOL64
+
+
+def test_rhs_split32_error(mf):
+    _say("rhs euler split32")
+    c = cases.build(configs.c2(), side=96)
+    u, du, uo, duo, _, _ = _rhs_pair(c, prec=L.MF_PREC_SPLIT32)
+    err = rel_inf(du, duo)
+    _say("  split32 rel err %.2e" % err)
+    assert err < TOL32
+
+
+def test_rhs_gpu_weights_vs_oracle(mf):
+    _say("rhs euler with GPU weights")
+    c = cases.build(configs.c2(), side=64, weights="gpu")
+    u, du, uo, duo, _, _ = _rhs_pair(c)
+    assert rel_inf(du, duo) < 1e-10
+
+
+# ------------------------------------------------------------- SSPRK3
+def test_ssprk33_advection(mf):
+    _say("ssprk33 advection 20 steps")
+    c = cases.build(configs.c1())
+    n = 20
+    c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+    c.ctx.ssprk33(0.0, c.dt, n)
+    ug = c.ctx.download(L.MF_BUF_U)
+    uo = c.u0.copy()
+    c.prob.ssprk33(uo, 0.0, c.dt, n)
+    assert rel_inf(ug, uo) < 1e-12
+
+
+def test_ssprk33_euler_rv_history(mf):
+    _say("ssprk33 euler + residual viscosity 6 steps, graph")
+    c = cases.build(configs.c2(), side=48, visc="residual")
+    n = 6
+    c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+    c.ctx.ssprk33(0.0, c.dt, n, flags=L.MF_STEP_HISTORY | L.MF_STEP_GRAPH)
+    ug = c.ctx.download(L.MF_BUF_U)
+    uo = c.u0.copy()
+    c.prob.ssprk33(uo, 0.0, c.dt, n, approx_order=3)
+    assert rel_inf(ug, uo) < 1e-12
+
+
+# ------------------------------------------------------------- adjoint
+def test_rhs_vjp_vs_oracle(mf):
+    _say("vjp euler")
+    c = cases.build(configs.c2(), side=48)
+    lam = np.random.default_rng(1).standard_normal(c.u0.shape)
+    c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+    c.ctx.upload(L.MF_BUF_AUX0, lam)
+    c.ctx.rhs_vjp(0.0)
+    ub = c.ctx.download(L.MF_BUF_AUX1)
+    ubo = c.prob.vjp(c.u0.copy(), lam, 0.0)
+    assert rel_inf(ub, ubo) < 1e-12

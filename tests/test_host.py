@@ -1,0 +1,85 @@
+"""Host-side logic and the C-ABI library (no GPU needed)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+from mfrbf import _lib as L
+
+
+def _header_symbols():
+    src = open(os.path.join(ROOT, "include", "mfrbf.h")).read()
+    return sorted(set(re.findall(r"\b(mf_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_header_symbol():
+    import ctypes
+
+    lib = ctypes.CDLL(L.LIB_PATH)
+    syms = _header_symbols()
+    assert len(syms) >= 30
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert sorted(syms) == L.exported_symbols()
+
+
+def test_header_constants_match_binding():
+    src = open(os.path.join(ROOT, "include", "mfrbf.h")).read()
+    for name, val in re.findall(r"#define (MF_[A-Z0-9_]+) (\d+)", src):
+        assert getattr(L, name) == int(val), name
+
+
+def test_context_fails_loudly_without_device():
+    from conftest import gpu_available
+
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(L.MfError):
+        L.Context(0)
+
+
+def test_basis_stencil_size_rule():
+    from mfrbf import PointCloudBasis, Point2D, PolyharmonicSpline, RBF, setproperties
+
+    assert PointCloudBasis(Point2D(), 2).nv == 15
+    assert PointCloudBasis(Point2D(), 3, approximation_type=RBF(PolyharmonicSpline(5))).nv == 20
+    assert PointCloudBasis(Point2D(), 4).nv == 30
+    b = setproperties(PointCloudBasis(Point2D(), 3, approximation_type=RBF(PolyharmonicSpline(5))), nv=30)
+    assert b.nv == 30 and b.phs_power == 5 and b.npoly == 10
+
+
+def test_medusa_reader_and_empty_group_quirk(tmp_path):
+    from mfrbf import read_medusa_file
+
+    pos, interior, bidx, bnorm, normals, ranges, types = read_medusa_file(
+        os.path.join(GOLDEN, "cyl_0_05", "cyl_0_05"))
+    assert [len(g) for g in bidx] == [40, 40, 60, 60, 26]
+    assert all(np.allclose(np.linalg.norm(n, axis=1), 1.0) for n in bnorm)
+    # group -2 empty: dropped from the index lists, kept (empty) in the normals
+    base = str(tmp_path / "c")
+    np.savetxt(base + "_positions.txt", [[0, 0], [1, 0], [0, 1], [1, 1]], delimiter=" , ")
+    np.savetxt(base + "_types.txt", [-1, -3, -3, 0], fmt="%d")
+    np.savetxt(base + "_boundary.txt", [0, 1, 2], fmt="%d")
+    np.savetxt(base + "_interior.txt", [3], fmt="%d")
+    np.savetxt(base + "_normals.txt", [[0, -1], [1, 0], [0, 1]], delimiter=" , ")
+    pos, interior, bidx, bnorm, *_ = read_medusa_file(base)
+    assert len(bidx) == 2 and len(bnorm) == 3 and len(bnorm[1]) == 0
+
+
+def test_configs_sizes():
+    from mfrbf import configs
+
+    sides, phases = configs.c5_sizes()
+    assert len(sides) == 64 and sides.min() >= 128 and sides.max() <= 512
+    n = configs.c3_stencils(side=256)
+    frac = np.mean(n > 30)
+    assert 0.06 < frac < 0.10 and n.min() == 30 and set(np.unique(n[n > 30])) <= set(range(40, 51))
+
+
+def test_gaussian_noise_moments():
+    from mfrbf.clouds import gaussian_noise
+
+    g = gaussian_noise(1, 200000, 1e-3)
+    assert abs(g.mean()) < 1e-5 and abs(g.std() - 1e-3) < 1e-5

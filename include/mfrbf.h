@@ -71,6 +71,11 @@ typedef struct mf_ctx mf_ctx;
 #define MF_PREC_F64 0     /* FP64 weights, state and accumulation */
 #define MF_PREC_SPLIT32 1 /* FP32 weights, centred FP64->FP32 differences,
                              FP32 accumulation; FP64 state */
+#define MF_PREC_F64_REF 2 /* FP64 in the reference's operation order: per
+                             node, terms in ascending column order, products
+                             rounded before each add, x terms then y terms
+                             then sources (the order of SparseArrays' CSC
+                             mul!), reproducible bit for bit */
 
 /* state buffers owned by a context */
 #define MF_BUF_U 0      /* integrator state u */
@@ -231,8 +236,11 @@ int mf_ssprk33_vjp(mf_ctx* ctx, double t0, double dt, int n_steps, int checkpoin
 
 /* ------------------------------------------------------------- timing */
 /* Average device time of one rhs! (ms) over n_calls back-to-back calls on
- * MF_BUF_U, and of its dominant kernel alone (kernel_ms), measured with
- * HIP events on the context stream. */
+ * MF_BUF_U, measured with HIP events on the context stream.  kernel_ms
+ * (4 doubles, may be NULL) receives the average time of each part run
+ * alone: [0] the stencil kernel (flux divergence, sources, phase A of the
+ * viscosity), [1] the transposed gather of the viscosity, [2] the two
+ * global reductions of the residual viscosity, [3] the boundary passes. */
 int mf_time_rhs(mf_ctx* ctx, double t, int precision, int n_calls, double* rhs_ms,
                 double* kernel_ms);
 

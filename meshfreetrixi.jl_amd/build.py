@@ -16,6 +16,9 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 CFLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # multiply-adds fuse only where the source says fma(): the reference-order
+    # kernels and the boundary/viscosity formulas round like the reference
+    "-ffp-contract=off",
     # keep the compiler from emitting packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",

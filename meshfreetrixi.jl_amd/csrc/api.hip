@@ -123,6 +123,10 @@ static void free_ops(mf_ctx* c)
     c->d_trow = nullptr;
     c->d_tw = nullptr;
     c->d_tw32 = nullptr;
+    dfree(c->d_wx_s);
+    dfree(c->d_wy_s);
+    dfree(c->d_hv_s);
+    c->d_wx_s = c->d_wy_s = c->d_hv_s = nullptr;
     c->have_flux = c->have_hv = c->have_T = false;
 }
 
@@ -177,8 +181,12 @@ static void free_stencil(mf_ctx* c)
 {
     dfree(c->d_idx);
     dfree(c->d_didx);
+    dfree(c->d_idx_s);
+    dfree(c->d_perm_s);
     c->d_idx = nullptr;
     c->d_didx = nullptr;
+    c->d_idx_s = nullptr;
+    c->d_perm_s = nullptr;
     c->k = 0;
     c->didx_ok = false;
     free_ops(c);
@@ -204,8 +212,9 @@ int ensure_state(mf_ctx* c)
         }
         if (!c->d_hist && (rc = dalloc(c, &c->d_hist, nv * (size_t)(c->approx_order + 1)))) return rc;
         if (!c->d_partials) {
-            c->n_partials = (int)std::min<int64_t>(1024, (c->N + 255) / 256);
-            if ((rc = dalloc(c, &c->d_partials, 4 * (size_t)c->n_partials))) return rc;
+            // 256 blocks for the two reduction passes: sums, then maxima
+            c->n_partials = (int)std::min<int64_t>(256, (c->N + 255) / 256);
+            if ((rc = dalloc(c, &c->d_partials, 8 * (size_t)c->n_partials))) return rc;
         }
     }
     if (!c->d_lim && (rc = dalloc(c, &c->d_lim, nv))) return rc;
@@ -346,9 +355,18 @@ static int set_k(mf_ctx* c, int k)
     free_graph(c);
     int rc;
     c->k = k;
-    if ((rc = dalloc(c, &c->d_idx, (size_t)k * c->N)) || (rc = dalloc(c, &c->d_didx, (size_t)k * c->N)))
+    if ((rc = dalloc(c, &c->d_idx, (size_t)k * c->N)) || (rc = dalloc(c, &c->d_didx, (size_t)k * c->N)) ||
+        (rc = dalloc(c, &c->d_idx_s, (size_t)k * c->N)) || (rc = dalloc(c, &c->d_perm_s, (size_t)k * c->N)))
         return rc;
     return MF_OK;
+}
+
+// derived stencil layouts once d_idx is filled
+static int stencil_ready(mf_ctx* c)
+{
+    int rc;
+    if ((rc = launch_build_didx(c))) return rc;
+    return launch_sort_stencil(c);
 }
 
 int mf_knn(mf_ctx* c, int k, int32_t* idx_out, double* dist_out, double* dx_min, double* dx_avg)
@@ -361,7 +379,7 @@ int mf_knn(mf_ctx* c, int k, int32_t* idx_out, double* dist_out, double* dx_min,
     double* d2 = nullptr;
     if ((rc = dalloc(c, &d2, (size_t)k * c->N))) return rc;
     if ((rc = launch_knn(c, k, d2))) return rc;
-    if ((rc = launch_build_didx(c))) return rc;
+    if ((rc = stencil_ready(c))) return rc;
     double mn = 0, av = 0;
     if ((rc = launch_dx_stats(c, d2, &mn, &av))) return rc;
     if (dx_min) *dx_min = mn;
@@ -400,7 +418,7 @@ int mf_set_stencil(mf_ctx* c, const int32_t* idx, int64_t n_points, int k)
     if ((rc = dalloc(c, &rows, (size_t)k * c->N))) return rc;
     MF_HIP(c, hipMemcpyAsync(rows, idx, sizeof(int32_t) * k * c->N, hipMemcpyHostToDevice, c->stream));
     if ((rc = launch_rows_to_ell(c, rows, c->d_idx, k))) return rc;
-    if ((rc = launch_build_didx(c))) return rc;
+    if ((rc = stencil_ready(c))) return rc;
     MF_HIP(c, hipStreamSynchronize(c->stream));
     dfree(rows);
     return MF_OK;
@@ -412,9 +430,12 @@ static int alloc_op(mf_ctx* c, int target)
     int rc;
     if (target == MF_OP_FLUX) {
         if (!c->d_wxy && (rc = dalloc(c, &c->d_wxy, n))) return rc;
+        if (!c->d_wx_s && (rc = dalloc(c, &c->d_wx_s, n))) return rc;
+        if (!c->d_wy_s && (rc = dalloc(c, &c->d_wy_s, n))) return rc;
         if (!c->d_wxy32 && (rc = dalloc(c, &c->d_wxy32, n))) return rc;
     } else {
         if (!c->d_hv && (rc = dalloc(c, &c->d_hv, n))) return rc;
+        if (!c->d_hv_s && (rc = dalloc(c, &c->d_hv_s, n))) return rc;
         if (!c->d_hv32 && (rc = dalloc(c, &c->d_hv32, n))) return rc;
     }
     return MF_OK;
@@ -454,6 +475,7 @@ int mf_compute_operator(mf_ctx* c, int phs_power, int poly_degree, int deriv_ord
         return rc;
     if ((rc = alloc_op(c, target))) return rc;
     if ((rc = launch_pack_ops(c, target, wx, wy))) return rc;
+    if ((rc = launch_pack_sorted(c, target, wx, wy))) return rc;
     int st = 0;
     MF_HIP(c, hipMemcpyAsync(&st, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     for (int w = 0; w < 2; ++w) {
@@ -495,6 +517,7 @@ int mf_set_operator(mf_ctx* c, int target, const double* wx, const double* wy)
     }
     if ((rc = alloc_op(c, target))) return rc;
     if ((rc = launch_pack_ops(c, target, ex, ey))) return rc;
+    if ((rc = launch_pack_sorted(c, target, ex, ey))) return rc;
     MF_HIP(c, hipStreamSynchronize(c->stream));
     dfree(rx);
     dfree(ex);
@@ -925,22 +948,51 @@ int mf_time_rhs(mf_ctx* c, double t, int precision, int n_calls, double* rhs_ms,
     if ((rc = prepare_rhs(c))) return rc;
     if ((rc = run_set_time(c, t, -1))) return rc;
     StageOut none;
-    hipEvent_t e0, e1;
-    MF_HIP(c, hipEventCreate(&e0));
-    MF_HIP(c, hipEventCreate(&e1));
-    // warm
+    // warm, eager
     if ((rc = run_rhs(c, precision, c->d_buf[MF_BUF_U], c->d_buf[MF_BUF_DU], 0.0, none))) return rc;
-    MF_HIP(c, hipEventRecord(e0, c->stream));
-    for (int q = 0; q < n_calls; ++q)
-        if ((rc = run_rhs(c, precision, c->d_buf[MF_BUF_U], c->d_buf[MF_BUF_DU], 0.0, none))) return rc;
-    MF_HIP(c, hipEventRecord(e1, c->stream));
-    MF_HIP(c, hipEventSynchronize(e1));
-    float ms = 0.f;
-    MF_HIP(c, hipEventElapsedTime(&ms, e0, e1));
-    if (rhs_ms) *rhs_ms = ms / n_calls;
-    if (kernel_ms) *kernel_ms = -1.0;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    // chains of `reps` back-to-back calls captured in one graph, replayed
+    const int reps = std::min(n_calls, 20);
+    const int launches = (n_calls + reps - 1) / reps;
+    auto time_parts = [&](int parts, double* out_ms) -> int {
+        MF_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int r2 = MF_OK;
+        for (int q = 0; q < reps && r2 == MF_OK; ++q)
+            r2 = run_rhs_parts(c, precision, c->d_buf[MF_BUF_U], c->d_buf[MF_BUF_DU], 0.0, none, parts);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (r2 != MF_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return r2;
+        }
+        if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+        hipGraphExec_t ex = nullptr;
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
+        hipEvent_t e0, e1;
+        MF_HIP(c, hipEventCreate(&e0));
+        MF_HIP(c, hipEventCreate(&e1));
+        MF_HIP(c, hipGraphLaunch(ex, c->stream));  // warm
+        MF_HIP(c, hipEventRecord(e0, c->stream));
+        for (int l = 0; l < launches; ++l) MF_HIP(c, hipGraphLaunch(ex, c->stream));
+        MF_HIP(c, hipEventRecord(e1, c->stream));
+        MF_HIP(c, hipEventSynchronize(e1));
+        float ms = 0.f;
+        MF_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+        *out_ms = ms / ((double)reps * launches);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipGraphExecDestroy(ex);
+        return MF_OK;
+    };
+    double full = 0.0;
+    if ((rc = time_parts(RHS_ALL, &full))) return rc;
+    if (rhs_ms) *rhs_ms = full;
+    // the kernels of one rhs! one part at a time (same inputs, same outputs)
+    const int parts[4] = {RHS_FLUX, RHS_TRANSPOSE, RHS_REDUCE, RHS_BC};
+    for (int q = 0; q < 4 && kernel_ms; ++q)
+        if ((rc = time_parts(parts[q], &kernel_ms[q]))) return rc;
     c->fsal_valid = false;
     return MF_OK;
 }

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cstdint>
 
 #include "mf_internal.h"
@@ -114,6 +115,80 @@ __device__ __forceinline__ double jmax(double x, double y)
     if (isnan(x)) return x;
     if (isnan(y)) return y;
     return (y > x || (signbit(y) < signbit(x))) ? y : x;
+}
+
+// Julia isless over 4-vectors: the first differing component decides,
+// NaN above every number.
+__device__ __forceinline__ bool lex_less(const double* a, const double* b)
+{
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const bool an = isnan(a[v]), bn = isnan(b[v]);
+        if (an && bn) continue;
+        if (bn) return true;
+        if (an) return false;
+        if (a[v] < b[v]) return true;
+        if (a[v] > b[v]) return false;
+    }
+    return false;
+}
+
+// Fixed-order block reductions over 4-vectors held in LDS (kBlock entries).
+__device__ __forceinline__ void block_sum4(double (*red)[4])
+{
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) red[threadIdx.x][v] += red[threadIdx.x + w][v];
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void block_lexmax4(double (*red)[4])
+{
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w && lex_less(red[threadIdx.x], red[threadIdx.x + w]))
+#pragma unroll
+            for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = red[threadIdx.x + w][v];
+        __syncthreads();
+    }
+}
+
+// mean(u) from the per-block partial sums (every caller gets the same bits).
+__device__ __forceinline__ void mean_from_partials(const double* psum, int nparts, int64_t N,
+                                                   double (*red)[4], double* mean)
+{
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += kBlock)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[threadIdx.x][v] += psum[4 * b + v];
+    block_sum4(red);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) mean[v] = red[0][v] / (double)(4 * N);
+    __syncthreads();
+}
+
+// ninf from the per-block partial lexicographic maxima, zeros -> eps().
+__device__ __forceinline__ void ninf_from_partials(const double* pmax, int nparts, double (*red)[4],
+                                                   double* ninf)
+{
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = -1.0;
+    for (int b = threadIdx.x; b < nparts; b += kBlock) {
+        double c[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) c[v] = pmax[4 * b + v];
+        if (lex_less(red[threadIdx.x], c))
+#pragma unroll
+            for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = c[v];
+    }
+    block_lexmax4(red);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ninf[v] = (red[0][v] == 0.0) ? DBL_EPSILON : red[0][v];
+    __syncthreads();
 }
 
 }  // namespace mf

@@ -77,6 +77,13 @@ struct mf_ctx {
     int32_t* d_idx = nullptr;
     int16_t* d_didx = nullptr;  // idx - i as int16 (split precision), if it fits
     bool didx_ok = false;
+    // the same stencil with each node's entries sorted by column (the
+    // reference's CSC order) and the slot permutation that sorts it
+    int32_t* d_idx_s = nullptr;
+    uint8_t* d_perm_s = nullptr;
+    double* d_wx_s = nullptr;
+    double* d_wy_s = nullptr;
+    double* d_hv_s = nullptr;
 
     // operators
     double2* d_wxy = nullptr;  // (Dx, Dy)
@@ -180,6 +187,8 @@ int launch_rows_to_ell_d(mf_ctx* c, const double* d_rows, double* d_ell, int k);
 int launch_soa_to_aos(mf_ctx* c, const double* d_soa, double* d_aos);
 int launch_aos_to_soa(mf_ctx* c, const double* d_aos, double* d_soa);
 int launch_build_didx(mf_ctx* c);
+int launch_sort_stencil(mf_ctx* c);
+int launch_pack_sorted(mf_ctx* c, int target, const double* d_wx, const double* d_wy);
 
 // rhs
 enum StageKind { ST_NONE = 0, ST_FSAL = 1, ST_TWO = 2, ST_THREE = 3 };
@@ -190,6 +199,10 @@ struct StageOut {
     double* out = nullptr;
 };
 int run_rhs(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const StageOut& st);
+// parts of one rhs! (for timing single kernels)
+enum RhsPart { RHS_BC = 1, RHS_REDUCE = 2, RHS_FLUX = 4, RHS_TRANSPOSE = 8, RHS_ALL = 15 };
+int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const StageOut& st,
+                  int parts);
 int run_set_time(mf_ctx* c, double t, int success_iter);
 int run_history(mf_ctx* c, const double* d_u, int init);
 int run_advance_time(mf_ctx* c, double dt);

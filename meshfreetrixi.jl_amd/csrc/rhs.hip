@@ -38,9 +38,17 @@ struct RhsArgs {
     double c_rv, c_uw, h;
     const double* __restrict__ approx;
     const StepScalars* __restrict__ sc;
+    const double* __restrict__ pmax;  // per-block lexicographic maxima of the max pass
+    int nparts;
+    double* norms_out;  // ninf published here (inspection)
     double* __restrict__ G;
     double* __restrict__ eps;
     int8_t* __restrict__ epsc;
+    // column-sorted operators (reference operation order)
+    const int32_t* __restrict__ idx_s;
+    const double* __restrict__ wx_s;
+    const double* __restrict__ wy_s;
+    const double* __restrict__ hv_s;
     // transposed operator
     const int64_t* __restrict__ sell_off;
     const int32_t* __restrict__ sell_w;
@@ -125,7 +133,7 @@ __device__ __forceinline__ void emit(const RhsArgs& a, int64_t i, const double* 
 // from the local state, eps_rv from |approx_du - du| scaled by the global
 // norms, then the choice of update_visc!.
 __device__ __forceinline__ double node_viscosity(const RhsArgs& a, int64_t i, const double* ui,
-                                                 const double* dui, int8_t* code)
+                                                 const double* dui, const double* ninf, int8_t* code)
 {
     const double gam = a.eq.gamma;
     const double rho = ui[0], m1 = ui[1], m2 = ui[2], E = ui[3];
@@ -141,7 +149,6 @@ __device__ __forceinline__ double node_viscosity(const RhsArgs& a, int64_t i, co
     }
     double ad[4];
     load_state<4>(a.approx, i, ad);
-    const double* ninf = a.sc->norms + 4;
     double mr = 0.0;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -166,9 +173,24 @@ __device__ __forceinline__ double node_viscosity(const RhsArgs& a, int64_t i, co
 //        - gamma sum_j H_ij u_j                         (Flyer HV, hyperviscosity.jl:52-64)
 // With a viscosity source this is phase A: it also stores
 // G_i = eps_i (Dx u, Dy u)_i for the transposed pass (hyperviscosity.jl:401-407).
+// Residual-viscosity norms: every block reduces the max pass's partials
+// (same bits everywhere); block 0 publishes them.
+__device__ __forceinline__ void block_norms(const RhsArgs& a, double* ninf)
+{
+    __shared__ double red[kBlock][4];
+    ninf_from_partials(a.pmax, a.nparts, red, ninf);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.norms_out)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) a.norms_out[4 + v] = ninf[v];
+}
+
 template <int V, int NS, bool HV, bool RV>
 __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
 {
+    double ninf[4] = {1.0, 1.0, 1.0, 1.0};
+    if constexpr (RV) {
+        if (a.visc == MF_VISC_RESIDUAL) block_norms(a, ninf);
+    }
     const int64_t i = remapped_node();
     if (i >= a.N) return;
     const int64_t N = a.N;
@@ -176,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
     double ax[V], ay[V], gx[V], gy[V], hh[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) ax[v] = ay[v] = gx[v] = gy[v] = hh[v] = 0.0;
-#pragma unroll 3
+#pragma unroll 6
     for (int s = 0; s < K; ++s) {
         const int64_t q = (int64_t)s * N + i;
         const int32_t j = a.idx[q];
@@ -215,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
         double ui[V];
         load_state<V>(a.u, i, ui);
         int8_t code = 1;
-        const double e = node_viscosity(a, i, ui, du, &code);
+        const double e = node_viscosity(a, i, ui, du, ninf, &code);
         a.eps[i] = e;
         a.epsc[i] = code;
         double g[2 * V];
@@ -239,6 +261,93 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
     emit<V>(a, i, du);
 }
 
+// --------------------------------------- FP64 in the reference's order
+// The same operator evaluated in the operation order of the reference's
+// SparseArrays products (pointcloudsolver/rbfsolver.jl:21-27, :247-265;
+// hyperviscosity.jl:61-63, :401-407): for each node, the x-flux terms in
+// ascending column order, du = du - w*F with the product rounded first,
+// then the y-flux terms onto that result, then (Flyer) the terms
+// du = du + h*(u_j*(-gamma)); D_d u accumulated the same way.  Bitwise
+// reproducible against a CSC implementation with the same weights.
+template <int V, int NS, bool HV, bool RV>
+__global__ __launch_bounds__(kBlock) void k_rhs_ref(const RhsArgs a)
+{
+    double ninf[4] = {1.0, 1.0, 1.0, 1.0};
+    if constexpr (RV) {
+        if (a.visc == MF_VISC_RESIDUAL) block_norms(a, ninf);
+    }
+    const int64_t i = remapped_node();
+    if (i >= a.N) return;
+    const int64_t N = a.N;
+    const int K = NS ? NS : a.k;
+    double du[V], gx[V], gy[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) du[v] = gx[v] = gy[v] = 0.0;
+#pragma unroll 6
+    for (int s = 0; s < K; ++s) {
+        const int64_t q = (int64_t)s * N + i;
+        const int32_t j = a.idx_s[q];
+        const double w = a.wx_s[q];
+        double uj[V], f1[V], f2[V];
+        load_state<V>(a.u, j, uj);
+        flux_xy<V>(a.eq, uj, f1, f2);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            du[v] = du[v] - w * f1[v];
+            if constexpr (RV) gx[v] = gx[v] + w * uj[v];
+        }
+    }
+#pragma unroll 6
+    for (int s = 0; s < K; ++s) {
+        const int64_t q = (int64_t)s * N + i;
+        const int32_t j = a.idx_s[q];
+        const double w = a.wy_s[q];
+        double uj[V], f1[V], f2[V];
+        load_state<V>(a.u, j, uj);
+        flux_xy<V>(a.eq, uj, f1, f2);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            du[v] = du[v] - w * f2[v];
+            if constexpr (RV) gy[v] = gy[v] + w * uj[v];
+        }
+    }
+    if constexpr (HV) {
+        if (!RV || a.hv_first) {
+            const double ng = -a.hv_gamma;
+#pragma unroll 6
+            for (int s = 0; s < K; ++s) {
+                const int64_t q = (int64_t)s * N + i;
+                const int32_t j = a.idx_s[q];
+                const double h = a.hv_s[q];
+                double uj[V];
+                load_state<V>(a.u, j, uj);
+#pragma unroll
+                for (int v = 0; v < V; ++v) du[v] = du[v] + h * (uj[v] * ng);
+            }
+        }
+    }
+    if constexpr (RV) {
+        double ui[V];
+        load_state<V>(a.u, i, ui);
+        int8_t code = 1;
+        const double e = node_viscosity(a, i, ui, du, ninf, &code);
+        a.eps[i] = e;
+        a.epsc[i] = code;
+        double g[2 * V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            g[v] = e * gx[v];
+            g[V + v] = e * gy[v];
+        }
+        store_state<V>(a.G, 2 * i, g);
+        store_state<V>(a.G, 2 * i + 1, g + V);
+        store_state<V>(a.du, i, du);
+        return;
+    }
+    mask_du<V>(a, i, du);
+    emit<V>(a, i, du);
+}
+
 // ------------------------------------------------- split-FP32 kernel
 // Same operator; the stencil sums run in FP32 on centred differences
 // formed in FP64: sum_j w_ij F(u_j) ~= sum_j w_ij (F(u_j) - F(u_i)),
@@ -247,6 +356,10 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
 template <int V, int NS, bool HV, bool RV, bool D16>
 __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
 {
+    double ninf[4] = {1.0, 1.0, 1.0, 1.0};
+    if constexpr (RV) {
+        if (a.visc == MF_VISC_RESIDUAL) block_norms(a, ninf);
+    }
     const int64_t i = remapped_node();
     if (i >= a.N) return;
     const int64_t N = a.N;
@@ -257,7 +370,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
     float ax[V], ay[V], gx[V], gy[V], hh[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) ax[v] = ay[v] = gx[v] = gy[v] = hh[v] = 0.0f;
-#pragma unroll 3
+#pragma unroll 6
     for (int s = 0; s < K; ++s) {
         const int64_t q = (int64_t)s * N + i;
         int64_t j;
@@ -303,7 +416,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
     }
     if constexpr (RV) {
         int8_t code = 1;
-        const double e = node_viscosity(a, i, ui, du, &code);
+        const double e = node_viscosity(a, i, ui, du, ninf, &code);
         a.eps[i] = e;
         a.epsc[i] = code;
         double g[2 * V];
@@ -331,7 +444,10 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
 // du_i -= sum_r Dx_ri Gx_r ; du_i -= sum_r Dy_ri Gy_r   (the D_d' products of
 // hyperviscosity.jl:405), over the sorted SELL-64 transpose, then the
 // second boundary pass and the output.
-template <int V, bool S32>
+// TM: 0 FP64 (fused multiply-add), 1 FP32 weights, 2 FP64 in the reference's
+// order (tmp = tmp + w*G over ascending rows, then du - tmp; a Flyer source
+// that follows the viscosity in keyword order is added last).
+template <int V, int TM>
 __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
 {
     const int64_t i = remapped_node();
@@ -343,12 +459,12 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
     double tx[V], ty[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) tx[v] = ty[v] = 0.0;
-#pragma unroll 4
+#pragma unroll 8
     for (int e = 0; e < W; ++e) {
         const int64_t q = off + (int64_t)e * kSellC + lane;
         const int32_t r = a.trow[q];
         double2 w;
-        if constexpr (S32) {
+        if constexpr (TM == 1) {
             const float2 w32 = a.tw32[q];
             w = make_double2((double)w32.x, (double)w32.y);
         } else {
@@ -359,14 +475,32 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
         load_state<V>(a.G, 2 * (int64_t)r + 1, g1);
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            tx[v] = fma(w.x, g0[v], tx[v]);
-            ty[v] = fma(w.y, g1[v], ty[v]);
+            if constexpr (TM == 2) {
+                tx[v] = tx[v] + w.x * g0[v];
+                ty[v] = ty[v] + w.y * g1[v];
+            } else {
+                tx[v] = fma(w.x, g0[v], tx[v]);
+                ty[v] = fma(w.y, g1[v], ty[v]);
+            }
         }
     }
     double du[V];
     load_state<V>(a.du, i, du);
 #pragma unroll
     for (int v = 0; v < V; ++v) du[v] = (du[v] - tx[v]) - ty[v];
+    if constexpr (TM == 2) {
+        if (a.hv_s && a.hv_gamma != 0.0 && !a.hv_first) {
+            const double ng = -a.hv_gamma;
+            for (int s = 0; s < a.k; ++s) {
+                const int64_t q = (int64_t)s * a.N + i;
+                double uj[V];
+                load_state<V>(a.u, a.idx_s[q], uj);
+                const double h = a.hv_s[q];
+#pragma unroll
+                for (int v = 0; v < V; ++v) du[v] = du[v] + h * (uj[v] * ng);
+            }
+        }
+    }
     mask_du<V>(a, i, du);
     emit<V>(a, i, du);
 }
@@ -384,6 +518,8 @@ struct BcArgs {
     const double* btable;
     const double* px;
     const double* py;
+    const uint8_t* flag;   // per node (fused use)
+    const int32_t* entry;  // per node (fused use)
     double gamma;
     const StepScalars* sc;
     double dt_off;
@@ -393,16 +529,13 @@ struct BcArgs {
     double tag_params[kMaxTags * 8];
 };
 
+// The first-pass condition of boundary entry e on the state s of its node;
+// false if the entry leaves the state unchanged.
 template <int V>
-__global__ __launch_bounds__(kBlock) void k_bc_state(const BcArgs b, double* u)
+__device__ __forceinline__ bool bc_entry(const BcArgs& b, int e, int32_t node, double* s)
 {
-    const int e = b.e0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= b.e1) return;
-    const int32_t node = b.bidx[e];
     const int tag = b.btag[e];
     const int kind = b.tag_kind[tag];
-    double s[V];
-    load_state<V>(u, node, s);
     if (kind == MF_BC_DIRICHLET) {
         if (b.homog) {
 #pragma unroll
@@ -412,15 +545,28 @@ __global__ __launch_bounds__(kBlock) void k_bc_state(const BcArgs b, double* u)
             for (int v = 0; v < V; ++v) s[v] = b.btable[(int64_t)e * V + v];
         } else {
             const double t = b.sc->t + b.dt_off;
-            bc_value<V>(b.tag_fn[tag], b.tag_params + 8 * tag, b.gamma, b.px[node], b.py[node], t,
-                        s);
+            bc_value<V>(b.tag_fn[tag], b.tag_params + 8 * tag, b.gamma, b.px[node], b.py[node], t, s);
         }
-    } else if (kind == MF_BC_SLIP_WALL) {
-        if constexpr (V == 4) slip_project(b.bnormal + 2 * e, s);
-    } else {
-        return;
+        return true;
     }
-    store_state<V>(u, node, s);
+    if (kind == MF_BC_SLIP_WALL) {
+        if constexpr (V == 4) {
+            slip_project(b.bnormal + 2 * e, s);
+            return true;
+        }
+    }
+    return false;
+}
+
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_bc_state(const BcArgs b, double* u)
+{
+    const int e = b.e0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= b.e1) return;
+    const int32_t node = b.bidx[e];
+    double s[V];
+    load_state<V>(u, node, s);
+    if (bc_entry<V>(b, e, node, s)) store_state<V>(u, node, s);
 }
 
 // ------------------------------------ residual-viscosity reductions
@@ -429,85 +575,50 @@ __global__ __launch_bounds__(kBlock) void k_bc_state(const BcArgs b, double* u)
 // zeros replaced by eps() (hyperviscosity.jl:305-311).  Each block reduces
 // a contiguous node range in a fixed order; the last block to finish
 // combines the partials.
-__device__ __forceinline__ bool lex_less(const double* a, const double* b)
-{
-    // Julia isless over vectors: first differing component decides, NaN largest
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const bool an = isnan(a[v]), bn = isnan(b[v]);
-        if (an && bn) continue;
-        if (bn) return true;
-        if (an) return false;
-        if (a[v] < b[v]) return true;
-        if (a[v] > b[v]) return false;
-    }
-    return false;
-}
-
-__global__ __launch_bounds__(kBlock) void k_rv_sum(int64_t N, int64_t chunk, const double* u,
-                                                   double* partials, StepScalars* sc)
+// Sum pass: block b reduces nodes [b*chunk, (b+1)*chunk) in a fixed order.
+// With bc.flag set it also applies the first boundary pass to the nodes it
+// visits (each node is touched by exactly one thread here).
+__global__ __launch_bounds__(kBlock) void k_rv_sum(int64_t N, int64_t chunk, double* u, double* psum,
+                                                   const BcArgs bc)
 {
     __shared__ double red[kBlock][4];
-    __shared__ int last;
     const int64_t lo = (int64_t)blockIdx.x * chunk;
     const int64_t hi = min(N, lo + chunk);
     double acc[4] = {0, 0, 0, 0};
-    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
         double s[4];
         load_state<4>(u, i, s);
+        if (bc.flag && bc.flag[i]) {
+            if (bc_entry<4>(bc, bc.entry[i], (int32_t)i, s)) store_state<4>(u, i, s);
+        }
 #pragma unroll
         for (int v = 0; v < 4; ++v) acc[v] += s[v];
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = acc[v];
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
+    block_sum4(red);
+    if (threadIdx.x == 0)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) red[threadIdx.x][v] += red[threadIdx.x + w][v];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) partials[4 * blockIdx.x + v] = red[0][v];
-        __threadfence();
-        last = (atomicAdd(&sc->ticket, 1) == (int)gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double tot[4] = {0, 0, 0, 0};
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) tot[v] += partials[4 * b + v];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = tot[v];
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) red[threadIdx.x][v] += red[threadIdx.x + w][v];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) sc->norms[v] = red[0][v] / (double)(4 * N);
-        sc->ticket = 0;
-    }
+        for (int v = 0; v < 4; ++v) psum[4 * blockIdx.x + v] = red[0][v];
 }
 
+// Max pass: mean from the sum pass's partials (recomputed identically in
+// every block; block 0 publishes it), then per-block lexicographic maxima
+// of |u_i - mean|.
 __global__ __launch_bounds__(kBlock) void k_rv_max(int64_t N, int64_t chunk, const double* u,
-                                                   double* partials, StepScalars* sc)
+                                                   const double* psum, int nparts, double* pmax,
+                                                   StepScalars* sc)
 {
     __shared__ double red[kBlock][4];
-    __shared__ int last;
+    double mean[4];
+    mean_from_partials(psum, nparts, N, red, mean);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sc->norms[v] = mean[v];
     const int64_t lo = (int64_t)blockIdx.x * chunk;
     const int64_t hi = min(N, lo + chunk);
-    double mean[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) mean[v] = sc->norms[v];
     double best[4] = {-1.0, -1.0, -1.0, -1.0};  // below every |.|
-    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
         double s[4], c[4];
         load_state<4>(u, i, s);
 #pragma unroll
@@ -518,45 +629,10 @@ __global__ __launch_bounds__(kBlock) void k_rv_max(int64_t N, int64_t chunk, con
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = best[v];
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w && lex_less(red[threadIdx.x], red[threadIdx.x + w]))
+    block_lexmax4(red);
+    if (threadIdx.x == 0)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = red[threadIdx.x + w][v];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) partials[4 * blockIdx.x + v] = red[0][v];
-        __threadfence();
-        last = (atomicAdd(&sc->ticket2, 1) == (int)gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double b2[4] = {-1.0, -1.0, -1.0, -1.0};
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-        double c[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) c[v] = partials[4 * b + v];
-        if (lex_less(b2, c))
-#pragma unroll
-            for (int v = 0; v < 4; ++v) b2[v] = c[v];
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = b2[v];
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w && lex_less(red[threadIdx.x], red[threadIdx.x + w]))
-#pragma unroll
-            for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = red[threadIdx.x + w][v];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) sc->norms[4 + v] = (red[0][v] == 0.0) ? DBL_EPSILON : red[0][v];
-        sc->ticket2 = 0;
-    }
+        for (int v = 0; v < 4; ++v) pmax[4 * blockIdx.x + v] = red[0][v];
 }
 
 // ------------------------------------------------------ small kernels
@@ -783,6 +859,13 @@ static RhsArgs make_args(mf_ctx* c, double* u, double* du)
     a.h = c->h_visc;
     a.approx = c->d_approx;
     a.sc = c->d_sc;
+    a.idx_s = c->d_idx_s;
+    a.wx_s = c->d_wx_s;
+    a.wy_s = c->d_wy_s;
+    a.hv_s = c->d_hv_s;
+    a.pmax = c->d_partials ? c->d_partials + 4 * c->n_partials : nullptr;
+    a.nparts = c->n_partials;
+    a.norms_out = c->d_sc ? c->d_sc->norms : nullptr;
     a.G = c->d_G;
     a.eps = c->d_eps;
     a.epsc = c->d_epsc;
@@ -805,6 +888,8 @@ static BcArgs make_bc(mf_ctx* c, double dt_off, int homog)
     b.py = c->d_y;
     b.gamma = c->p0;
     b.sc = c->d_sc;
+    b.flag = c->bc.flag;
+    b.entry = c->bc.entry;
     b.dt_off = dt_off;
     b.homog = homog;
     for (int g = 0; g < c->bc.ntags; ++g) {
@@ -854,6 +939,11 @@ static void launch_flux(mf_ctx* c, int prec, const RhsArgs& a, bool hv, bool rv,
         else if (hv) hipLaunchKernelGGL((k_rhs_f64<V, NS, true, false>), G, B, 0, s, a);
         else if (rv) hipLaunchKernelGGL((k_rhs_f64<V, NS, false, true>), G, B, 0, s, a);
         else hipLaunchKernelGGL((k_rhs_f64<V, NS, false, false>), G, B, 0, s, a);
+    } else if (prec == MF_PREC_F64_REF) {
+        if (hv && rv) hipLaunchKernelGGL((k_rhs_ref<V, NS, true, true>), G, B, 0, s, a);
+        else if (hv) hipLaunchKernelGGL((k_rhs_ref<V, NS, true, false>), G, B, 0, s, a);
+        else if (rv) hipLaunchKernelGGL((k_rhs_ref<V, NS, false, true>), G, B, 0, s, a);
+        else hipLaunchKernelGGL((k_rhs_ref<V, NS, false, false>), G, B, 0, s, a);
     } else if (c->didx_ok) {
         if (hv && rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, true, true>), G, B, 0, s, a);
         else if (hv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, false, true>), G, B, 0, s, a);
@@ -880,15 +970,23 @@ static void launch_flux_v(mf_ctx* c, int prec, const RhsArgs& a, bool hv, bool r
 
 int run_rhs(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const StageOut& st)
 {
+    return run_rhs_parts(c, prec, d_u, d_du, dt_off, st, RHS_ALL);
+}
+
+int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const StageOut& st,
+                  int parts)
+{
     if (!c->have_flux) return fail(c, MF_ERR_STATE, "rhs!: no flux operator (mf_compute_operator/mf_set_operator)");
-    if (prec != MF_PREC_F64 && prec != MF_PREC_SPLIT32)
+    if (prec != MF_PREC_F64 && prec != MF_PREC_SPLIT32 && prec != MF_PREC_F64_REF)
         return fail(c, MF_ERR_ARGUMENT, "rhs!: unknown precision");
     const bool hv = c->have_hv && c->hv_gamma != 0.0;
     const bool rv = c->visc_mode != MF_VISC_NONE;
     if (rv && c->nvar != 4) return fail(c, MF_ERR_UNSUPPORTED, "viscosity sources need the Euler equations");
     if (rv && !c->have_T) return fail(c, MF_ERR_STATE, "viscosity source without transposed operator");
     // first boundary pass on u
-    int rc = bc_state_pass(c, d_u, dt_off, 0, false);
+    // with the residual viscosity the first boundary pass rides on the sum pass
+    const bool fused_bc = rv && c->visc_mode == MF_VISC_RESIDUAL && c->have_bc && !c->bc.multi;
+    int rc = ((parts & RHS_BC) && !fused_bc) ? bc_state_pass(c, d_u, dt_off, 0, false) : MF_OK;
     if (rc) return rc;
     RhsArgs a = make_args(c, d_u, d_du);
     a.st_kind = st.kind;
@@ -898,29 +996,37 @@ int run_rhs(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const
     const int64_t nblk = (c->N + kBlock - 1) / kBlock;
     const unsigned grid = xcd_grid(nblk);
     if (rv) {
-        if (c->visc_mode == MF_VISC_RESIDUAL) {
-            const int64_t chunk = ((c->N + c->n_partials - 1) / c->n_partials + kBlock - 1) / kBlock * kBlock;
-            const unsigned nb = (unsigned)((c->N + chunk - 1) / chunk);
-            hipLaunchKernelGGL(k_rv_sum, dim3(nb), dim3(kBlock), 0, c->stream, c->N, chunk, d_u,
-                               c->d_partials, c->d_sc);
-            hipLaunchKernelGGL(k_rv_max, dim3(nb), dim3(kBlock), 0, c->stream, c->N, chunk, d_u,
-                               c->d_partials, c->d_sc);
+        if (c->visc_mode == MF_VISC_RESIDUAL && (parts & RHS_REDUCE)) {
+            const int nparts = c->n_partials;
+            const int64_t chunk = (c->N + nparts - 1) / nparts;
+            BcArgs bc = make_bc(c, dt_off, 0);
+            if (!((parts & RHS_BC) && fused_bc)) bc.flag = nullptr;
+            hipLaunchKernelGGL(k_rv_sum, dim3(nparts), dim3(kBlock), 0, c->stream, c->N, chunk, d_u,
+                               c->d_partials, bc);
+            hipLaunchKernelGGL(k_rv_max, dim3(nparts), dim3(kBlock), 0, c->stream, c->N, chunk,
+                               (const double*)d_u, (const double*)c->d_partials, nparts,
+                               c->d_partials + 4 * nparts, c->d_sc);
         }
         RhsArgs pa = a;
         pa.st_kind = ST_NONE;
-        launch_flux_v<4>(c, prec, pa, hv, true, grid);
-        if (prec == MF_PREC_F64)
-            hipLaunchKernelGGL((k_rhs_transpose<4, false>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+        if (parts & RHS_FLUX) launch_flux_v<4>(c, prec, pa, hv, true, grid);
+        if (parts & RHS_TRANSPOSE) {
+            if (prec == MF_PREC_F64)
+                hipLaunchKernelGGL((k_rhs_transpose<4, 0>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+            else if (prec == MF_PREC_SPLIT32)
+                hipLaunchKernelGGL((k_rhs_transpose<4, 1>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+            else
+                hipLaunchKernelGGL((k_rhs_transpose<4, 2>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+        }
+    } else if (parts & RHS_FLUX) {
+        if (c->nvar == 4)
+            launch_flux_v<4>(c, prec, a, hv, false, grid);
         else
-            hipLaunchKernelGGL((k_rhs_transpose<4, true>), dim3(grid), dim3(kBlock), 0, c->stream, a);
-    } else if (c->nvar == 4) {
-        launch_flux_v<4>(c, prec, a, hv, false, grid);
-    } else {
-        launch_flux_v<1>(c, prec, a, hv, false, grid);
+            launch_flux_v<1>(c, prec, a, hv, false, grid);
     }
     MF_HIP(c, hipGetLastError());
     // second boundary pass on u: only the slip projection changes anything
-    return bc_state_pass(c, d_u, dt_off, 0, true);
+    return (parts & RHS_BC) ? bc_state_pass(c, d_u, dt_off, 0, true) : MF_OK;
 }
 
 int run_set_time(mf_ctx* c, double t, int success_iter)

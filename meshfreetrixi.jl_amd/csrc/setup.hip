@@ -872,6 +872,82 @@ int launch_build_didx(mf_ctx* c)
     return MF_OK;
 }
 
+// ------------------------------------------ column-sorted stencil copy
+// Sorted slot order of every node's stencil (ascending column index), the
+// order in which the reference's CSC products accumulate a row.
+__global__ void k_sort_stencil(int64_t N, int K, const int32_t* idx, int32_t* idx_s, uint8_t* perm)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int32_t key[kMaxStencilM];
+    uint8_t pos[kMaxStencilM];
+    for (int s = 0; s < K; ++s) {
+        key[s] = idx[(int64_t)s * N + i];
+        pos[s] = (uint8_t)s;
+    }
+    for (int a2 = 1; a2 < K; ++a2) {
+        const int32_t kv = key[a2];
+        const uint8_t pv = pos[a2];
+        int b = a2 - 1;
+        while (b >= 0 && key[b] > kv) {
+            key[b + 1] = key[b];
+            pos[b + 1] = pos[b];
+            --b;
+        }
+        key[b + 1] = kv;
+        pos[b + 1] = pv;
+    }
+    for (int s = 0; s < K; ++s) {
+        idx_s[(int64_t)s * N + i] = key[s];
+        perm[(int64_t)s * N + i] = pos[s];
+    }
+}
+
+__global__ void k_permute_slots(int64_t N, int K, const uint8_t* perm, const double* src, double* dst)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N * K) return;
+    const int64_t i = q % N;
+    dst[q] = src[(int64_t)perm[q] * N + i];
+}
+
+__global__ void k_permute_sum(int64_t N, int K, const uint8_t* perm, const double* a, const double* b,
+                              double* dst)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N * K) return;
+    const int64_t i = q % N;
+    const int64_t src = (int64_t)perm[q] * N + i;
+    dst[q] = b ? a[src] + b[src] : a[src];
+}
+
+int launch_sort_stencil(mf_ctx* c)
+{
+    const int64_t N = c->N;
+    hipLaunchKernelGGL(k_sort_stencil, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c->stream, N, c->k,
+                       c->d_idx, c->d_idx_s, c->d_perm_s);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+// Column-sorted copies of the operators: Dx, Dy separately; H = Dx^2k + Dy^2k.
+int launch_pack_sorted(mf_ctx* c, int target, const double* d_wx, const double* d_wy)
+{
+    const int64_t n = c->N * c->k;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (target == MF_OP_FLUX) {
+        hipLaunchKernelGGL(k_permute_slots, dim3(g), dim3(256), 0, c->stream, c->N, c->k, c->d_perm_s, d_wx,
+                           c->d_wx_s);
+        hipLaunchKernelGGL(k_permute_slots, dim3(g), dim3(256), 0, c->stream, c->N, c->k, c->d_perm_s, d_wy,
+                           c->d_wy_s);
+    } else {
+        hipLaunchKernelGGL(k_permute_sum, dim3(g), dim3(256), 0, c->stream, c->N, c->k, c->d_perm_s, d_wx, d_wy,
+                           c->d_hv_s);
+    }
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
 // ------------------------------------------------- layout conversions
 template <class T>
 __global__ void k_ell_to_rows(int64_t N, int K, const T* ell, T* rows)

@@ -19,7 +19,7 @@ MF_BC_DIRICHLET, MF_BC_SLIP_WALL, MF_BC_DO_NOTHING = 0, 1, 2
 MF_FN_CONST, MF_FN_PULSE, MF_FN_WAVE, MF_FN_GRADIENT, MF_FN_TABLE = 0, 1, 2, 3, 4
 MF_OP_FLUX, MF_OP_HYPERVISCOSITY = 0, 1
 MF_VISC_NONE, MF_VISC_RESIDUAL, MF_VISC_UPWIND = 0, 1, 2
-MF_PREC_F64, MF_PREC_SPLIT32 = 0, 1
+MF_PREC_F64, MF_PREC_SPLIT32, MF_PREC_F64_REF = 0, 1, 2
 MF_BUF_U, MF_BUF_DU, MF_BUF_AUX0, MF_BUF_AUX1 = 0, 1, 2, 3
 MF_VAR_DENSITY, MF_VAR_PRESSURE = 0, 1
 MF_STEP_GRAPH, MF_STEP_LIMITER, MF_STEP_HISTORY = 1, 2, 4
@@ -137,6 +137,9 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def synchronize(self):
+        self._chk(self._L.mf_synchronize(self.h))
 
     def _chk(self, rc):
         if rc != MF_OK:
@@ -270,6 +273,8 @@ class Context:
         return loss.value
 
     def time_rhs(self, t, precision, n_calls):
-        a, b = C.c_double(), C.c_double()
-        self._chk(self._L.mf_time_rhs(self.h, t, precision, n_calls, C.byref(a), C.byref(b)))
-        return a.value, b.value
+        """(ms per rhs!, [ms of stencil kernel, transposed gather, reductions, boundary])."""
+        a = C.c_double()
+        parts = np.zeros(4)
+        self._chk(self._L.mf_time_rhs(self.h, t, precision, n_calls, C.byref(a), dptr(parts)))
+        return a.value, parts

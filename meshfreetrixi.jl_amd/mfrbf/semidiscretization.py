@@ -18,7 +18,7 @@ from .equations import CompressibleEulerEquations2D, LinearScalarAdvectionEquati
 from .sources import (SourceHyperviscosityFlyer, SourceResidualViscosityTominec,
                       SourceTerms, SourceUpwindViscosityTominec)
 
-PRECISIONS = {"f64": L.MF_PREC_F64, "split32": L.MF_PREC_SPLIT32}
+PRECISIONS = {"f64": L.MF_PREC_F64, "split32": L.MF_PREC_SPLIT32, "f64_ref": L.MF_PREC_F64_REF}
 
 
 def _precision(p):

@@ -197,3 +197,62 @@ def test_rhs_vjp_vs_oracle(mf):
     ub = c.ctx.download(L.MF_BUF_AUX1)
     ubo = c.prob.vjp(c.u0.copy(), lam, 0.0)
     assert rel_inf(ub, ubo) < 1e-12
+
+
+# ------------------------------------------------ reference operation order
+@pytest.mark.parametrize("cfg,bc", [("c1", ("nothing", "nothing")), ("c2", ("nothing", "nothing")),
+                                    ("c2", ("dirichlet", "nothing"))])
+def test_rhs_reference_order_bitwise(mf, cfg, bc):
+    _say("rhs f64_ref %s %s" % (cfg, bc))
+    spec = getattr(configs, cfg)()
+    c = cases.build(spec, side=48, bc=bc)
+    u, du, uo, duo, _, _ = _rhs_pair(c, prec=L.MF_PREC_F64_REF, t=0.02)
+    if bc[0] == "nothing":
+        assert np.array_equal(du, duo)  # same operation order, same bits
+    else:
+        # device sin/exp of the boundary values may differ in the last bit
+        assert rel_inf(du, duo) < 1e-14
+
+
+def test_rhs_reference_order_hv_and_rv(mf):
+    _say("rhs f64_ref euler + hv + upwind viscosity")
+    c = cases.build(configs.c2(), side=40, bc=("nothing", "nothing"), hv=(2, 1e-4), visc="upwind")
+    u, du, uo, duo, eps, _ = _rhs_pair(c, prec=L.MF_PREC_F64_REF)
+    geps, _ = c.ctx.viscosity()
+    assert np.array_equal(geps, eps)
+    assert rel_inf(du, duo) < 1e-15
+
+
+def test_rhs_full_size_c2(mf):
+    """262,144 nodes (the headline configuration): reference-order FP64
+    within 1e-12 of the oracle, fused-multiply-add FP64 and split-FP32 within
+    their stated tolerances."""
+    _say("rhs full size 512x512 (kNN + weights on the GPU, oracle on the CPU)")
+    from mfrbf.clouds import cloud_square
+    spec = configs.c2()
+    x, y, tag, _, _ = cloud_square(512, spec.seed)
+    N = len(x)
+    ctx = L.Context(0)
+    ctx.set_equations(L.MF_EQ_EULER_2D, 1.4)
+    ctx.set_points(x, y)
+    idx, _, dmin, davg = ctx.knn(30)
+    wx, wy = ctx.compute_operator(5, 3, 1, L.MF_OP_FLUX, want=True, k=30)
+    inflow = np.nonzero(tag == 1)[0].astype(np.int32)
+    prm = spec.ic.params()
+    ctx.set_boundary([0, len(inflow)], [L.MF_BC_DIRICHLET], [L.MF_FN_WAVE], prm, inflow,
+                     np.zeros((len(inflow), 2)))
+    from mfrbf.equations import CompressibleEulerEquations2D
+    u0 = np.ascontiguousarray(spec.ic(np.c_[x, y], 0.0, CompressibleEulerEquations2D(1.4)))
+    prob = O.Problem(x, y, O.ell_ptr(N, 30), idx, wx, wy, O.EQ_EULER, 4,
+                     boundary=[(O.BC_DIRICHLET, inflow, None, O.FN_WAVE, prm, None)])
+    duo, _, _ = prob.rhs(u0.copy(), 0.0)
+    errs = {}
+    for name, prec in (("f64_ref", L.MF_PREC_F64_REF), ("f64", L.MF_PREC_F64), ("split32", L.MF_PREC_SPLIT32)):
+        u = u0.copy()
+        du = np.zeros_like(u)
+        ctx.rhs_host(0.0, u, du, prec)
+        errs[name] = rel_inf(du, duo)
+    _say("  full-size rel err vs oracle: %s" % errs)
+    assert errs["f64_ref"] < 1e-12
+    assert errs["f64"] < 5e-12
+    assert errs["split32"] < TOL32

@@ -182,7 +182,7 @@ def main():
         sys.exit(2)
     import mfrbf  # noqa: F401
     from mfrbf import _lib as L
-    from mfrbf import configs
+    from mfrbf import configs, tune
 
     spec = configs.c2(args.side)
     log("setup: %d x %d cloud, k=%d" % (args.side, args.side, spec.k))
@@ -196,6 +196,11 @@ def main():
                                                           args.warmup))
     ctx.time_rhs(t_now, L.MF_PREC_F64, max(1, args.warmup))
     ctx.synchronize()
+    log("autotune: load-batch sizes of the stencil and transposed kernels")
+    tuning = tune.autotune(ctx, t_now, N, k, L.MF_EQ_EULER_2D, L.MF_VISC_RESIDUAL,
+                           precisions=(L.MF_PREC_F64, L.MF_PREC_SPLIT32, L.MF_PREC_F64_REF),
+                           use_cache=False)
+    log("tuning: " + json.dumps({p: (v["flux"], v["transpose"]) for p, v in tuning.items()}))
     log("timing %d rhs! (FP64)" % args.steps)
     w0 = time.perf_counter()
     rhs_ms, parts = ctx.time_rhs(t_now, L.MF_PREC_F64, args.steps)
@@ -249,6 +254,7 @@ def main():
                     "kernels_ms": {"stencil": r_parts[0], "transpose": r_parts[1]},
                     "rhs_frac": rbytes / (r_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "setup_ms": {"knn": 1e3 * info["t_knn"], "weights": 1e3 * info["t_weights"]},
+        "tuning": tuning,
     }
     if not args.no_extras:
         log("adjoint: 100 SSPRK33 steps, 65,536 nodes")

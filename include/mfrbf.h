@@ -234,6 +234,16 @@ int mf_rhs_vjp(mf_ctx* ctx, double t);
 int mf_ssprk33_vjp(mf_ctx* ctx, double t0, double dt, int n_steps, int checkpoint_every,
                    double* loss);
 
+/* ------------------------------------------------------------- tuning */
+#define MF_TUNE_FLUX_BATCH 0      /* stencil entries loaded per batch: 3, 5, 6, 10 (0 = default) */
+#define MF_TUNE_TRANSPOSE_BATCH 1 /* transposed entries loaded per batch: 2, 4, 8 (0 = default) */
+/* Selects a compiled kernel variant for one precision (MF_PREC_*, or -1 for
+ * all).  A batch size that does not divide the stencil size falls back to
+ * the default.  Results do not depend on it (the per-node operation order is
+ * the same for every batch size).  No counterpart in the reference (the CPU
+ * path has no launch shapes); set by the autotuner of the Python package. */
+int mf_set_tuning(mf_ctx* ctx, int key, int precision, int value);
+
 /* ------------------------------------------------------------- timing */
 /* Average device time of one rhs! (ms) over n_calls back-to-back calls on
  * MF_BUF_U, measured with HIP events on the context stream.  kernel_ms

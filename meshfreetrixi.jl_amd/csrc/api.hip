@@ -940,6 +940,28 @@ int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int 
     return MF_OK;
 }
 
+int mf_set_tuning(mf_ctx* c, int key, int precision, int value)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (precision < -1 || precision > MF_PREC_F64_REF)
+        return fail(c, MF_ERR_ARGUMENT, "mf_set_tuning: unknown precision");
+    const int p0 = precision < 0 ? 0 : precision, p1 = precision < 0 ? MF_PREC_F64_REF : precision;
+    if (key == MF_TUNE_FLUX_BATCH) {
+        if (value != 0 && value != 3 && value != 5 && value != 6 && value != 10)
+            return fail(c, MF_ERR_ARGUMENT, "mf_set_tuning: flux batch must be 0, 3, 5, 6 or 10");
+        for (int p = p0; p <= p1; ++p) c->tune_flux_batch[p] = value;
+    } else if (key == MF_TUNE_TRANSPOSE_BATCH) {
+        if (value != 0 && value != 2 && value != 4 && value != 8)
+            return fail(c, MF_ERR_ARGUMENT, "mf_set_tuning: transpose batch must be 0, 2, 4 or 8");
+        for (int p = p0; p <= p1; ++p) c->tune_transpose_batch[p] = value;
+    } else {
+        return fail(c, MF_ERR_ARGUMENT, "mf_set_tuning: unknown key");
+    }
+    free_graph(c);
+    return MF_OK;
+}
+
 int mf_time_rhs(mf_ctx* c, double t, int precision, int n_calls, double* rhs_ms, double* kernel_ms)
 {
     int rc = check_ctx(c);

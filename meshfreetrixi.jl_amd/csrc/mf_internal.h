@@ -151,6 +151,11 @@ struct mf_ctx {
     // adjoint scratch
     double* d_adj = nullptr;
     int64_t adj_len = 0;
+
+    // launch tuning (mf_set_tuning)
+    // per precision (MF_PREC_*); 0 = the default
+    int tune_flux_batch[3] = {0, 0, 0};       // stencil entries per load batch
+    int tune_transpose_batch[3] = {0, 0, 0};  // transposed entries per load batch
 };
 
 namespace mf {

@@ -184,7 +184,11 @@ __device__ __forceinline__ void block_norms(const RhsArgs& a, double* ninf)
         for (int v = 0; v < 4; ++v) a.norms_out[4 + v] = ninf[v];
 }
 
-template <int V, int NS, bool HV, bool RV>
+// Batches of BT stencil entries: every load of a batch (indices, weights,
+// then the gathered states) is issued before any of it is used, so BT
+// gathers are in flight per thread.  Entries past K read the node itself
+// with weight 0.
+template <int V, int NS, bool HV, bool RV, int BT>
 __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
 {
     double ninf[4] = {1.0, 1.0, 1.0, 1.0};
@@ -198,30 +202,48 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
     double ax[V], ay[V], gx[V], gy[V], hh[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) ax[v] = ay[v] = gx[v] = gy[v] = hh[v] = 0.0;
-#pragma unroll 6
-    for (int s = 0; s < K; ++s) {
-        const int64_t q = (int64_t)s * N + i;
-        const int32_t j = a.idx[q];
-        const double2 w = a.wxy[q];
-        double uj[V], f1[V], f2[V];
-        load_state<V>(a.u, j, uj);
-        flux_xy<V>(a.eq, uj, f1, f2);
+#pragma unroll 1
+    for (int s0 = 0; s0 < K; s0 += BT) {
+        int32_t j[BT];
+        double2 w[BT];
+        double hw[BT];
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-            ax[v] = fma(w.x, f1[v], ax[v]);
-            ay[v] = fma(w.y, f2[v], ay[v]);
-        }
-        if constexpr (RV) {
-#pragma unroll
-            for (int v = 0; v < V; ++v) {
-                gx[v] = fma(w.x, uj[v], gx[v]);
-                gy[v] = fma(w.y, uj[v], gy[v]);
+        for (int b = 0; b < BT; ++b) {
+            const int s = s0 + b;
+            if ((NS && NS % BT == 0) || s < K) {
+                const int64_t q = (int64_t)s * N + i;
+                j[b] = a.idx[q];
+                w[b] = a.wxy[q];
+                if constexpr (HV) hw[b] = a.hv[q];
+            } else {
+                j[b] = (int32_t)i;
+                w[b] = make_double2(0.0, 0.0);
+                hw[b] = 0.0;
             }
         }
-        if constexpr (HV) {
-            const double hw = a.hv[q];
+        double uj[BT][V];
 #pragma unroll
-            for (int v = 0; v < V; ++v) hh[v] = fma(hw, uj[v], hh[v]);
+        for (int b = 0; b < BT; ++b) load_state<V>(a.u, j[b], uj[b]);
+#pragma unroll
+        for (int b = 0; b < BT; ++b) {
+            double f1[V], f2[V];
+            flux_xy<V>(a.eq, uj[b], f1, f2);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                ax[v] = fma(w[b].x, f1[v], ax[v]);
+                ay[v] = fma(w[b].y, f2[v], ay[v]);
+            }
+            if constexpr (RV) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    gx[v] = fma(w[b].x, uj[b][v], gx[v]);
+                    gy[v] = fma(w[b].y, uj[b][v], gy[v]);
+                }
+            }
+            if constexpr (HV) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) hh[v] = fma(hw[b], uj[b][v], hh[v]);
+            }
         }
     }
     double du[V];
@@ -269,7 +291,50 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
 // then the y-flux terms onto that result, then (Flyer) the terms
 // du = du + h*(u_j*(-gamma)); D_d u accumulated the same way.  Bitwise
 // reproducible against a CSC implementation with the same weights.
-template <int V, int NS, bool HV, bool RV>
+// One pass over the column-sorted stencil in batches of BT entries (loads
+// first, then the adds in entry order).  PASS 0: x flux (and D_x u),
+// 1: y flux (and D_y u), 2: Flyer term.
+template <int V, int NS, bool RV, int BT, int PASS>
+__device__ __forceinline__ void ref_pass(const RhsArgs& a, int64_t i, int K, double* du, double* g)
+{
+    const int64_t N = a.N;
+    const double* __restrict__ wsrc = PASS == 0 ? a.wx_s : (PASS == 1 ? a.wy_s : a.hv_s);
+    const double ng = -a.hv_gamma;
+#pragma unroll 1
+    for (int s0 = 0; s0 < K; s0 += BT) {
+        int32_t j[BT];
+        double w[BT];
+#pragma unroll
+        for (int b = 0; b < BT; ++b) {
+            const int s = s0 + b;
+            const bool in = (NS && NS % BT == 0) || s < K;
+            const int64_t q = (int64_t)(in ? s : s0) * N + i;
+            j[b] = a.idx_s[q];
+            w[b] = wsrc[q];
+        }
+        double uj[BT][V];
+#pragma unroll
+        for (int b = 0; b < BT; ++b) load_state<V>(a.u, j[b], uj[b]);
+#pragma unroll
+        for (int b = 0; b < BT; ++b) {
+            if (!((NS && NS % BT == 0) || s0 + b < K)) continue;
+            if constexpr (PASS == 2) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) du[v] = du[v] + w[b] * (uj[b][v] * ng);
+            } else {
+                double f1[V], f2[V];
+                flux_xy<V>(a.eq, uj[b], f1, f2);
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    du[v] = du[v] - w[b] * (PASS == 0 ? f1[v] : f2[v]);
+                    if constexpr (RV) g[v] = g[v] + w[b] * uj[b][v];
+                }
+            }
+        }
+    }
+}
+
+template <int V, int NS, bool HV, bool RV, int BT>
 __global__ __launch_bounds__(kBlock) void k_rhs_ref(const RhsArgs a)
 {
     double ninf[4] = {1.0, 1.0, 1.0, 1.0};
@@ -278,53 +343,14 @@ __global__ __launch_bounds__(kBlock) void k_rhs_ref(const RhsArgs a)
     }
     const int64_t i = remapped_node();
     if (i >= a.N) return;
-    const int64_t N = a.N;
     const int K = NS ? NS : a.k;
     double du[V], gx[V], gy[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) du[v] = gx[v] = gy[v] = 0.0;
-#pragma unroll 6
-    for (int s = 0; s < K; ++s) {
-        const int64_t q = (int64_t)s * N + i;
-        const int32_t j = a.idx_s[q];
-        const double w = a.wx_s[q];
-        double uj[V], f1[V], f2[V];
-        load_state<V>(a.u, j, uj);
-        flux_xy<V>(a.eq, uj, f1, f2);
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-            du[v] = du[v] - w * f1[v];
-            if constexpr (RV) gx[v] = gx[v] + w * uj[v];
-        }
-    }
-#pragma unroll 6
-    for (int s = 0; s < K; ++s) {
-        const int64_t q = (int64_t)s * N + i;
-        const int32_t j = a.idx_s[q];
-        const double w = a.wy_s[q];
-        double uj[V], f1[V], f2[V];
-        load_state<V>(a.u, j, uj);
-        flux_xy<V>(a.eq, uj, f1, f2);
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-            du[v] = du[v] - w * f2[v];
-            if constexpr (RV) gy[v] = gy[v] + w * uj[v];
-        }
-    }
+    ref_pass<V, NS, RV, BT, 0>(a, i, K, du, gx);
+    ref_pass<V, NS, RV, BT, 1>(a, i, K, du, gy);
     if constexpr (HV) {
-        if (!RV || a.hv_first) {
-            const double ng = -a.hv_gamma;
-#pragma unroll 6
-            for (int s = 0; s < K; ++s) {
-                const int64_t q = (int64_t)s * N + i;
-                const int32_t j = a.idx_s[q];
-                const double h = a.hv_s[q];
-                double uj[V];
-                load_state<V>(a.u, j, uj);
-#pragma unroll
-                for (int v = 0; v < V; ++v) du[v] = du[v] + h * (uj[v] * ng);
-            }
-        }
+        if (!RV || a.hv_first) ref_pass<V, NS, false, BT, 2>(a, i, K, du, nullptr);
     }
     if constexpr (RV) {
         double ui[V];
@@ -353,7 +379,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_ref(const RhsArgs a)
 // formed in FP64: sum_j w_ij F(u_j) ~= sum_j w_ij (F(u_j) - F(u_i)),
 // exact up to F(u_i) sum_j w_ij, which vanishes with the weights' row sums.
 // Weights FP32, neighbour indices as int16 offsets when the cloud allows.
-template <int V, int NS, bool HV, bool RV, bool D16>
+template <int V, int NS, bool HV, bool RV, bool D16, int BT>
 __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
 {
     double ninf[4] = {1.0, 1.0, 1.0, 1.0};
@@ -370,17 +396,36 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
     float ax[V], ay[V], gx[V], gy[V], hh[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) ax[v] = ay[v] = gx[v] = gy[v] = hh[v] = 0.0f;
-#pragma unroll 6
-    for (int s = 0; s < K; ++s) {
-        const int64_t q = (int64_t)s * N + i;
-        int64_t j;
-        if constexpr (D16)
-            j = i + (int64_t)a.didx[q];
-        else
-            j = a.idx[q];
-        const float2 w = a.wxy32[q];
-        double uj[V], f1[V], f2[V];
-        load_state<V>(a.u, j, uj);
+#pragma unroll 1
+    for (int s0 = 0; s0 < K; s0 += BT) {
+        int64_t jj[BT];
+        float2 ww[BT];
+        float hw32[BT];
+#pragma unroll
+        for (int b = 0; b < BT; ++b) {
+            const int s = s0 + b;
+            if ((NS && NS % BT == 0) || s < K) {
+                const int64_t q = (int64_t)s * N + i;
+                if constexpr (D16)
+                    jj[b] = i + (int64_t)a.didx[q];
+                else
+                    jj[b] = a.idx[q];
+                ww[b] = a.wxy32[q];
+                if constexpr (HV) hw32[b] = a.hv32[q];
+            } else {
+                jj[b] = i;
+                ww[b] = make_float2(0.0f, 0.0f);
+                hw32[b] = 0.0f;
+            }
+        }
+        double ub[BT][V];
+#pragma unroll
+        for (int b = 0; b < BT; ++b) load_state<V>(a.u, jj[b], ub[b]);
+#pragma unroll
+        for (int b = 0; b < BT; ++b) {
+        const float2 w = ww[b];
+        const double* uj = ub[b];
+        double f1[V], f2[V];
         flux_xy<V>(a.eq, uj, f1, f2);
 #pragma unroll
         for (int v = 0; v < V; ++v) {
@@ -399,10 +444,11 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
                 }
             }
             if constexpr (HV) {
-                const float hw = a.hv32[q];
+                const float hw = hw32[b];
 #pragma unroll
                 for (int v = 0; v < V; ++v) hh[v] = fmaf(hw, dd[v], hh[v]);
             }
+        }
         }
     }
     double du[V];
@@ -447,7 +493,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
 // TM: 0 FP64 (fused multiply-add), 1 FP32 weights, 2 FP64 in the reference's
 // order (tmp = tmp + w*G over ascending rows, then du - tmp; a Flyer source
 // that follows the viscosity in keyword order is added last).
-template <int V, int TM>
+template <int V, int TM, int BT>
 __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
 {
     const int64_t i = remapped_node();
@@ -459,28 +505,40 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
     double tx[V], ty[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) tx[v] = ty[v] = 0.0;
-#pragma unroll 8
-    for (int e = 0; e < W; ++e) {
-        const int64_t q = off + (int64_t)e * kSellC + lane;
-        const int32_t r = a.trow[q];
-        double2 w;
-        if constexpr (TM == 1) {
-            const float2 w32 = a.tw32[q];
-            w = make_double2((double)w32.x, (double)w32.y);
-        } else {
-            w = a.tw[q];
-        }
-        double g0[V], g1[V];
-        load_state<V>(a.G, 2 * (int64_t)r, g0);
-        load_state<V>(a.G, 2 * (int64_t)r + 1, g1);
+#pragma unroll 1
+    for (int e0 = 0; e0 < W; e0 += BT) {
+        int32_t r[BT];
+        double2 w[BT];
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-            if constexpr (TM == 2) {
-                tx[v] = tx[v] + w.x * g0[v];
-                ty[v] = ty[v] + w.y * g1[v];
+        for (int b = 0; b < BT; ++b) {
+            const int e = (e0 + b < W) ? e0 + b : e0;
+            const int64_t q = off + (int64_t)e * kSellC + lane;
+            r[b] = a.trow[q];
+            if constexpr (TM == 1) {
+                const float2 w32 = a.tw32[q];
+                w[b] = make_double2((double)w32.x, (double)w32.y);
             } else {
-                tx[v] = fma(w.x, g0[v], tx[v]);
-                ty[v] = fma(w.y, g1[v], ty[v]);
+                w[b] = a.tw[q];
+            }
+        }
+        double g0[BT][V], g1[BT][V];
+#pragma unroll
+        for (int b = 0; b < BT; ++b) {
+            load_state<V>(a.G, 2 * (int64_t)r[b], g0[b]);
+            load_state<V>(a.G, 2 * (int64_t)r[b] + 1, g1[b]);
+        }
+#pragma unroll
+        for (int b = 0; b < BT; ++b) {
+            if (e0 + b >= W) continue;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                if constexpr (TM == 2) {
+                    tx[v] = tx[v] + w[b].x * g0[b][v];
+                    ty[v] = ty[v] + w[b].y * g1[b][v];
+                } else {
+                    tx[v] = fma(w[b].x, g0[b][v], tx[v]);
+                    ty[v] = fma(w[b].y, g1[b][v], ty[v]);
+                }
             }
         }
     }
@@ -929,42 +987,67 @@ static int bc_state_pass(mf_ctx* c, double* u, double dt_off, int homog, bool sl
     return MF_OK;
 }
 
-template <int V, int NS>
-static void launch_flux(mf_ctx* c, int prec, const RhsArgs& a, bool hv, bool rv, unsigned grid)
+template <int V, int NS, int BT>
+static void launch_flux_bt(mf_ctx* c, int prec, const RhsArgs& a, bool hv, bool rv, unsigned grid)
 {
     const dim3 G(grid), B(kBlock);
     hipStream_t s = c->stream;
     if (prec == MF_PREC_F64) {
-        if (hv && rv) hipLaunchKernelGGL((k_rhs_f64<V, NS, true, true>), G, B, 0, s, a);
-        else if (hv) hipLaunchKernelGGL((k_rhs_f64<V, NS, true, false>), G, B, 0, s, a);
-        else if (rv) hipLaunchKernelGGL((k_rhs_f64<V, NS, false, true>), G, B, 0, s, a);
-        else hipLaunchKernelGGL((k_rhs_f64<V, NS, false, false>), G, B, 0, s, a);
+        if (hv && rv) hipLaunchKernelGGL((k_rhs_f64<V, NS, true, true, BT>), G, B, 0, s, a);
+        else if (hv) hipLaunchKernelGGL((k_rhs_f64<V, NS, true, false, BT>), G, B, 0, s, a);
+        else if (rv) hipLaunchKernelGGL((k_rhs_f64<V, NS, false, true, BT>), G, B, 0, s, a);
+        else hipLaunchKernelGGL((k_rhs_f64<V, NS, false, false, BT>), G, B, 0, s, a);
     } else if (prec == MF_PREC_F64_REF) {
-        if (hv && rv) hipLaunchKernelGGL((k_rhs_ref<V, NS, true, true>), G, B, 0, s, a);
-        else if (hv) hipLaunchKernelGGL((k_rhs_ref<V, NS, true, false>), G, B, 0, s, a);
-        else if (rv) hipLaunchKernelGGL((k_rhs_ref<V, NS, false, true>), G, B, 0, s, a);
-        else hipLaunchKernelGGL((k_rhs_ref<V, NS, false, false>), G, B, 0, s, a);
+        if (hv && rv) hipLaunchKernelGGL((k_rhs_ref<V, NS, true, true, BT>), G, B, 0, s, a);
+        else if (hv) hipLaunchKernelGGL((k_rhs_ref<V, NS, true, false, BT>), G, B, 0, s, a);
+        else if (rv) hipLaunchKernelGGL((k_rhs_ref<V, NS, false, true, BT>), G, B, 0, s, a);
+        else hipLaunchKernelGGL((k_rhs_ref<V, NS, false, false, BT>), G, B, 0, s, a);
     } else if (c->didx_ok) {
-        if (hv && rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, true, true>), G, B, 0, s, a);
-        else if (hv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, false, true>), G, B, 0, s, a);
-        else if (rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, false, true, true>), G, B, 0, s, a);
-        else hipLaunchKernelGGL((k_rhs_s32<V, NS, false, false, true>), G, B, 0, s, a);
+        if (hv && rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, true, true, BT>), G, B, 0, s, a);
+        else if (hv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, false, true, BT>), G, B, 0, s, a);
+        else if (rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, false, true, true, BT>), G, B, 0, s, a);
+        else hipLaunchKernelGGL((k_rhs_s32<V, NS, false, false, true, BT>), G, B, 0, s, a);
     } else {
-        if (hv && rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, true, false>), G, B, 0, s, a);
-        else if (hv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, false, false>), G, B, 0, s, a);
-        else if (rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, false, true, false>), G, B, 0, s, a);
-        else hipLaunchKernelGGL((k_rhs_s32<V, NS, false, false, false>), G, B, 0, s, a);
+        if (hv && rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, true, false, BT>), G, B, 0, s, a);
+        else if (hv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, false, false, BT>), G, B, 0, s, a);
+        else if (rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, false, true, false, BT>), G, B, 0, s, a);
+        else hipLaunchKernelGGL((k_rhs_s32<V, NS, false, false, false, BT>), G, B, 0, s, a);
     }
 }
 
+// Stencil batch sizes compiled per stencil size: the tuning value picks
+// one (it must divide the stencil size; 0 = the default, the first).
 template <int V>
 static void launch_flux_v(mf_ctx* c, int prec, const RhsArgs& a, bool hv, bool rv, unsigned grid)
 {
+    const int bt = c->tune_flux_batch[prec];
     switch (c->k) {
-        case 15: launch_flux<V, 15>(c, prec, a, hv, rv, grid); break;
-        case 20: launch_flux<V, 20>(c, prec, a, hv, rv, grid); break;
-        case 30: launch_flux<V, 30>(c, prec, a, hv, rv, grid); break;
-        default: launch_flux<V, 0>(c, prec, a, hv, rv, grid); break;
+        case 15:
+            if (bt == 3) launch_flux_bt<V, 15, 3>(c, prec, a, hv, rv, grid);
+            else launch_flux_bt<V, 15, 5>(c, prec, a, hv, rv, grid);
+            break;
+        case 20:
+            if (bt == 10) launch_flux_bt<V, 20, 10>(c, prec, a, hv, rv, grid);
+            else launch_flux_bt<V, 20, 5>(c, prec, a, hv, rv, grid);
+            break;
+        case 30:
+            if (bt == 3) launch_flux_bt<V, 30, 3>(c, prec, a, hv, rv, grid);
+            else if (bt == 6) launch_flux_bt<V, 30, 6>(c, prec, a, hv, rv, grid);
+            else if (bt == 10) launch_flux_bt<V, 30, 10>(c, prec, a, hv, rv, grid);
+            else launch_flux_bt<V, 30, 5>(c, prec, a, hv, rv, grid);
+            break;
+        default: launch_flux_bt<V, 0, 4>(c, prec, a, hv, rv, grid); break;
+    }
+}
+
+template <int TM>
+static void launch_transpose(mf_ctx* c, const RhsArgs& a, unsigned grid)
+{
+    const dim3 G(grid), B(kBlock);
+    switch (c->tune_transpose_batch[TM]) {
+        case 2: hipLaunchKernelGGL((k_rhs_transpose<4, TM, 2>), G, B, 0, c->stream, a); break;
+        case 8: hipLaunchKernelGGL((k_rhs_transpose<4, TM, 8>), G, B, 0, c->stream, a); break;
+        default: hipLaunchKernelGGL((k_rhs_transpose<4, TM, 4>), G, B, 0, c->stream, a); break;
     }
 }
 
@@ -1012,11 +1095,11 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
         if (parts & RHS_FLUX) launch_flux_v<4>(c, prec, pa, hv, true, grid);
         if (parts & RHS_TRANSPOSE) {
             if (prec == MF_PREC_F64)
-                hipLaunchKernelGGL((k_rhs_transpose<4, 0>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+                launch_transpose<0>(c, a, grid);
             else if (prec == MF_PREC_SPLIT32)
-                hipLaunchKernelGGL((k_rhs_transpose<4, 1>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+                launch_transpose<1>(c, a, grid);
             else
-                hipLaunchKernelGGL((k_rhs_transpose<4, 2>), dim3(grid), dim3(kBlock), 0, c->stream, a);
+                launch_transpose<2>(c, a, grid);
         }
     } else if (parts & RHS_FLUX) {
         if (c->nvar == 4)

@@ -256,3 +256,61 @@ def test_rhs_full_size_c2(mf):
     assert errs["f64_ref"] < 1e-12
     assert errs["f64"] < 5e-12
     assert errs["split32"] < TOL32
+
+
+# ------------------------------------------------------- launch variants
+def _rhs_residual_case(spec, side, hv=None):
+    c = cases.build(spec, side=side, visc="residual", hv=hv)
+    u1 = c.u0 * 1.0001
+    c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+    c.ctx.history_update(0.0, 0)
+    c.ctx.upload(L.MF_BUF_U, np.ascontiguousarray(u1))
+    c.ctx.history_update(0.001, 1)
+    return c
+
+
+@pytest.mark.parametrize("prec", [L.MF_PREC_F64, L.MF_PREC_SPLIT32, L.MF_PREC_F64_REF])
+def test_rhs_batch_variants_bitwise(mf, prec):
+    """Every load-batch variant of the stencil and transposed kernels keeps
+    the per-node operation order, so all give the same bits."""
+    _say("rhs batch variants prec %d" % prec)
+    c = _rhs_residual_case(configs.c2(), 48, hv=(2, 1e-4))
+    outs = {}
+    for fb in (3, 5, 6, 10):
+        for tb in (2, 4, 8):
+            c.ctx.set_tuning(L.MF_TUNE_FLUX_BATCH, fb, prec)
+            c.ctx.set_tuning(L.MF_TUNE_TRANSPOSE_BATCH, tb, prec)
+            u = c.u0.copy()
+            du = np.zeros_like(u)
+            c.ctx.rhs_host(0.001, u, du, prec)
+            outs[(fb, tb)] = du
+    ref = outs[(3, 2)]
+    for key, du in outs.items():
+        assert np.array_equal(du, ref), key
+
+
+def test_set_tuning_rejects_bad_values(mf):
+    c = cases.build(configs.c1(), side=16)
+    with pytest.raises(L.MfError):
+        c.ctx.set_tuning(L.MF_TUNE_FLUX_BATCH, 7)
+    with pytest.raises(L.MfError):
+        c.ctx.set_tuning(L.MF_TUNE_TRANSPOSE_BATCH, 3)
+    with pytest.raises(L.MfError):
+        c.ctx.set_tuning(9, 4)
+
+
+@pytest.mark.parametrize("k", [13, 25])
+def test_rhs_generic_stencil_size(mf, k):
+    """Stencil sizes without a compiled specialisation take the generic
+    kernels, whose last load batch is padded (weight 0 on the node itself)."""
+    _say("rhs generic stencil size k=%d" % k)
+    spec = configs.c2()
+    spec.k = k
+    spec.p, spec.deg = 3, 2
+    c = _rhs_residual_case(spec, 40)
+    ad = c.ctx.approx_du()
+    c.prob.approx_du[:] = ad.ravel()
+    c.prob.cfg.success_iter = 1
+    for prec, tol in ((L.MF_PREC_F64, TOL64), (L.MF_PREC_F64_REF, TOL64), (L.MF_PREC_SPLIT32, TOL32)):
+        u, du, uo, duo, _, _ = _rhs_pair(c, prec=prec, t=0.001)
+        assert rel_inf(du, duo) < tol, prec

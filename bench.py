@@ -86,6 +86,27 @@ def kernel_bytes(N, k, V=4):
     return a, b
 
 
+def measured_traffic(kernel, tuning, side):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (FETCH_SIZE
+    scaled by the copy-kernel calibration recorded beside it), or None."""
+    path = os.path.join(ROOT, "profiles", "rhs_euler_rv_262144_hbm_traffic.json")
+    if side != 512 or not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        tab = json.load(f)
+    if kernel == "transpose":
+        want = "k_rhs_transpose<4, 0, %d>" % (tuning["f64"]["transpose"] or 4)
+        family = "k_rhs_transpose<4, 0,"
+    else:
+        want = "k_rhs_f64<4, 30, false, true, %d>" % (tuning["f64"]["flux"] or 5)
+        family = "k_rhs_f64<4, 30, false, true,"
+    names = [n for n in tab["kernels"] if want in n] or [n for n in tab["kernels"] if family in n]
+    if not names:
+        return None, None
+    return tab["kernels"][names[0]]["bytes"], names[0]
+
+
 def cpu_baseline(L, ctx, info, spec, t_now, budget_s):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -213,6 +234,7 @@ def main():
     kms, kbytes = kernels[dom]
     achieved = kbytes / (kms * 1e-3) / 1e9
     rbytes = rhs_bytes(N, k)
+    traffic, traffic_kernel = measured_traffic(dom, tuning, args.side)
     log("rhs %.4f ms (%.3e node-evals/s); stencil %.4f ms, transpose %.4f ms, reductions %.4f ms, bc %.4f ms"
         % (rhs_ms, value, parts[0], parts[1], parts[2], parts[3]))
     # split-FP32 and the reference-order FP64
@@ -242,7 +264,8 @@ def main():
                                "PHS r^5 + degree-3 stencils, FP64" % (N, args.side, args.side),
                    "nodes": N, "stencil": k, "precision": "f64", "timed_wall_s": wall},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                     "traffic_kernel": traffic_kernel,
                      "kernel_ms": kms, "kernel_bytes": kbytes,
                      "rhs_frac": rbytes / (rhs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "rhs_bytes": rbytes},
         "kernels_ms": {"stencil": parts[0], "transpose": parts[1], "reductions": parts[2],

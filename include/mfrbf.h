@@ -215,11 +215,12 @@ int mf_set_stage_limiter(mf_ctx* ctx, int n, const double* thresholds, const int
 /* ------------------------------------------------------ time stepping */
 /* n_steps of SSPRK33 (Shu-Osher form, FSAL) from MF_BUF_U at t0 with step
  * dt; with MF_STEP_HISTORY and a residual viscosity source the
- * HistoryCallback runs at init (when restart == 0) and after every step.  Replaces solve(ode, SSPRK33(...),
- * dt, callbacks) on the reference's semidiscretization.  restart = 1
- * continues from the previous call's FSAL and history state. */
+ * HistoryCallback runs at init (when resume == 0) and after every step.
+ * Replaces solve(ode, SSPRK33(...), dt, callbacks) on the reference's
+ * semidiscretization.  resume = 1 continues from the previous call's FSAL
+ * and history state (same dt; MF_BUF_U as that call left it). */
 int mf_ssprk33(mf_ctx* ctx, double t0, double dt, int n_steps, int precision, int flags,
-               int restart);
+               int resume);
 
 /* ------------------------------------------------------------- adjoint */
 /* ubar = (d rhs / d u)^T lambda at u = MF_BUF_U (flux divergence, boundary
@@ -246,11 +247,14 @@ int mf_set_tuning(mf_ctx* ctx, int key, int precision, int value);
 
 /* ------------------------------------------------------------- timing */
 /* Average device time of one rhs! (ms) over n_calls back-to-back calls on
- * MF_BUF_U, measured with HIP events on the context stream.  kernel_ms
- * (4 doubles, may be NULL) receives the average time of each part run
- * alone: [0] the stencil kernel (flux divergence, sources, phase A of the
+ * MF_BUF_U, measured with HIP events on the context stream (chains of the
+ * captured call).  kernel_ms (4 doubles, may be NULL) receives each part's
+ * average time inside n_calls further whole calls (events recorded between
+ * the kernels, so each part sees the caches as a full call leaves them):
+ * [0] the stencil kernel (flux divergence, sources, phase A of the
  * viscosity), [1] the transposed gather of the viscosity, [2] the two
- * global reductions of the residual viscosity, [3] the boundary passes. */
+ * global reductions of the residual viscosity (with the first boundary pass
+ * when it rides on them), [3] the boundary passes. */
 int mf_time_rhs(mf_ctx* ctx, double t, int precision, int n_calls, double* rhs_ms,
                 double* kernel_ms);
 

@@ -888,7 +888,7 @@ static int ssprk_step(mf_ctx* c, double dt, int prec, bool limiter, bool history
     return MF_OK;
 }
 
-int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int flags, int restart)
+int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int flags, int resume)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -896,9 +896,9 @@ int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int 
     if ((rc = prepare_rhs(c))) return rc;
     const bool limiter = (flags & MF_STEP_LIMITER) && c->lim_n > 0;
     const bool history = (flags & MF_STEP_HISTORY) && c->visc_mode == MF_VISC_RESIDUAL;
-    if (restart) {
+    if (resume) {
         if (!c->fsal_valid || c->graph_dt != dt)
-            return fail(c, MF_ERR_STATE, "mf_ssprk33: restart needs the previous call's state and the same dt");
+            return fail(c, MF_ERR_STATE, "mf_ssprk33: resume needs the previous call's state and the same dt");
     } else {
         if ((rc = run_set_time(c, t0, 0))) return rc;
         if (history && (rc = run_history(c, c->d_buf[MF_BUF_U], 1))) return rc;
@@ -1011,10 +1011,43 @@ int mf_time_rhs(mf_ctx* c, double t, int precision, int n_calls, double* rhs_ms,
     double full = 0.0;
     if ((rc = time_parts(RHS_ALL, &full))) return rc;
     if (rhs_ms) *rhs_ms = full;
-    // the kernels of one rhs! one part at a time (same inputs, same outputs)
-    const int parts[4] = {RHS_FLUX, RHS_TRANSPOSE, RHS_REDUCE, RHS_BC};
-    for (int q = 0; q < 4 && kernel_ms; ++q)
-        if ((rc = time_parts(parts[q], &kernel_ms[q]))) return rc;
+    // each kernel's share of whole rhs! calls: events between the kernels
+    // of n_calls eager calls (so every kernel sees the caches as the full
+    // call leaves them), in batches of up to 32 calls
+    if (kernel_ms) {
+        constexpr int kB = 32, kE = 6;
+        std::vector<hipEvent_t> ev(kB * kE, nullptr);
+        for (auto& e : ev) MF_HIP(c, hipEventCreate(&e));
+        double acc[4] = {0, 0, 0, 0};
+        int done = 0;
+        while (done < n_calls && rc == MF_OK) {
+            const int nb = std::min(kB, n_calls - done);
+            for (int q = 0; q < nb && rc == MF_OK; ++q) {
+                c->probe = &ev[q * kE];
+                rc = run_rhs(c, precision, c->d_buf[MF_BUF_U], c->d_buf[MF_BUF_DU], 0.0, none);
+            }
+            c->probe = nullptr;
+            if (rc) break;
+            hipError_t e = hipEventSynchronize(ev[(nb - 1) * kE + kE - 1]);
+            if (e != hipSuccess) { rc = hip_fail(c, e, "hipEventSynchronize"); break; }
+            for (int q = 0; q < nb; ++q) {
+                float m[kE - 1];
+                for (int s = 0; s + 1 < kE; ++s) {
+                    e = hipEventElapsedTime(&m[s], ev[q * kE + s], ev[q * kE + s + 1]);
+                    if (e != hipSuccess) m[s] = 0.f;
+                }
+                acc[0] += m[2];         // stencil kernel
+                acc[1] += m[3];         // transposed gather
+                acc[2] += m[1];         // reductions
+                acc[3] += m[0] + m[4];  // boundary passes
+            }
+            done += nb;
+        }
+        c->probe = nullptr;
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        if (rc) return rc;
+        for (int q = 0; q < 4; ++q) kernel_ms[q] = acc[q] / n_calls;
+    }
     c->fsal_valid = false;
     return MF_OK;
 }

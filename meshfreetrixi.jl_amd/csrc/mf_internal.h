@@ -156,6 +156,11 @@ struct mf_ctx {
     // per precision (MF_PREC_*); 0 = the default
     int tune_flux_batch[3] = {0, 0, 0};       // stencil entries per load batch
     int tune_transpose_batch[3] = {0, 0, 0};  // transposed entries per load batch
+
+    // when set, run_rhs_parts records these 6 events at its part boundaries
+    // (start, after boundary pass 1, reductions, stencil kernel, transposed
+    // gather, boundary pass 2)
+    hipEvent_t* probe = nullptr;
 };
 
 namespace mf {

@@ -1056,6 +1056,11 @@ int run_rhs(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const
     return run_rhs_parts(c, prec, d_u, d_du, dt_off, st, RHS_ALL);
 }
 
+static inline void probe(mf_ctx* c, int slot)
+{
+    if (c->probe) (void)hipEventRecord(c->probe[slot], c->stream);
+}
+
 int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const StageOut& st,
                   int parts)
 {
@@ -1069,8 +1074,10 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
     // first boundary pass on u
     // with the residual viscosity the first boundary pass rides on the sum pass
     const bool fused_bc = rv && c->visc_mode == MF_VISC_RESIDUAL && c->have_bc && !c->bc.multi;
+    probe(c, 0);
     int rc = ((parts & RHS_BC) && !fused_bc) ? bc_state_pass(c, d_u, dt_off, 0, false) : MF_OK;
     if (rc) return rc;
+    probe(c, 1);
     RhsArgs a = make_args(c, d_u, d_du);
     a.st_kind = st.kind;
     a.dt = st.dt;
@@ -1090,9 +1097,11 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
                                (const double*)d_u, (const double*)c->d_partials, nparts,
                                c->d_partials + 4 * nparts, c->d_sc);
         }
+        probe(c, 2);
         RhsArgs pa = a;
         pa.st_kind = ST_NONE;
         if (parts & RHS_FLUX) launch_flux_v<4>(c, prec, pa, hv, true, grid);
+        probe(c, 3);
         if (parts & RHS_TRANSPOSE) {
             if (prec == MF_PREC_F64)
                 launch_transpose<0>(c, a, grid);
@@ -1101,15 +1110,23 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
             else
                 launch_transpose<2>(c, a, grid);
         }
-    } else if (parts & RHS_FLUX) {
-        if (c->nvar == 4)
-            launch_flux_v<4>(c, prec, a, hv, false, grid);
-        else
-            launch_flux_v<1>(c, prec, a, hv, false, grid);
+        probe(c, 4);
+    } else {
+        probe(c, 2);
+        if (parts & RHS_FLUX) {
+            if (c->nvar == 4)
+                launch_flux_v<4>(c, prec, a, hv, false, grid);
+            else
+                launch_flux_v<1>(c, prec, a, hv, false, grid);
+        }
+        probe(c, 3);
+        probe(c, 4);
     }
     MF_HIP(c, hipGetLastError());
     // second boundary pass on u: only the slip projection changes anything
-    return (parts & RHS_BC) ? bc_state_pass(c, d_u, dt_off, 0, true) : MF_OK;
+    rc = (parts & RHS_BC) ? bc_state_pass(c, d_u, dt_off, 0, true) : MF_OK;
+    probe(c, 5);
+    return rc;
 }
 
 int run_set_time(mf_ctx* c, double t, int success_iter)

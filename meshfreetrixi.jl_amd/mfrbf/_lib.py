@@ -263,8 +263,8 @@ class Context:
         va = np.ascontiguousarray(variables, np.int32)
         self._chk(self._L.mf_set_stage_limiter(self.h, len(th), dptr(th), i32ptr(va)))
 
-    def ssprk33(self, t0, dt, nsteps, precision=MF_PREC_F64, flags=0, restart=False):
-        self._chk(self._L.mf_ssprk33(self.h, t0, dt, nsteps, precision, flags, 1 if restart else 0))
+    def ssprk33(self, t0, dt, nsteps, precision=MF_PREC_F64, flags=0, resume=False):
+        self._chk(self._L.mf_ssprk33(self.h, t0, dt, nsteps, precision, flags, 1 if resume else 0))
 
     def rhs_vjp(self, t):
         self._chk(self._L.mf_rhs_vjp(self.h, t))

@@ -1,0 +1,200 @@
+"""GPU workflows through the host mirror of the reference API and the C ABI,
+against the CPU oracle: Medusa fixture with slip walls, the Zhang-Shu
+limiter (standalone and as the stage limiter), full configuration runs,
+graph replay against eager stepping, and the adjoint against finite
+differences."""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import oracle as O
+from conftest import GOLDEN, rel_inf
+from mfrbf import _lib as L
+from mfrbf import configs
+
+import mfrbf
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+CYL = os.path.join(GOLDEN, "cyl_0_05", "cyl_0_05")
+CYL_TAGS = {"inlet": 1, "outlet": 2, "bottom": 3, "top": 4, "cylinder": 5}
+
+
+def _cyl_semi(visc=None, host_dirichlet=False):
+    basis = mfrbf.PointCloudBasis(mfrbf.Point2D(), 3, approximation_type=mfrbf.RBF(mfrbf.PolyharmonicSpline(5)))
+    solver = mfrbf.PointCloudSolver(basis)
+    domain = mfrbf.PointCloudDomain(solver, CYL, CYL_TAGS)
+    eqs = mfrbf.CompressibleEulerEquations2D(1.4)
+    grad = mfrbf.LinearGradient()
+    g = (lambda x, t, e: grad(x, t, e)) if host_dirichlet else grad
+    bcs = {"inlet": mfrbf.BoundaryConditionDirichlet(g), "outlet": mfrbf.BoundaryConditionDoNothing(),
+           "bottom": mfrbf.boundary_condition_slip_wall, "top": mfrbf.boundary_condition_slip_wall,
+           "cylinder": mfrbf.boundary_condition_slip_wall}
+    src = None
+    if visc == "upwind":
+        src = mfrbf.SourceTerms(uv=mfrbf.SourceUpwindViscosityTominec(solver, eqs, domain, c=1.0, c_uw=1.0))
+    semi = mfrbf.SemidiscretizationHyperbolic(domain, eqs, grad, solver, boundary_conditions=bcs,
+                                              source_terms=src)
+    return semi, domain
+
+
+def _cyl_oracle(domain, visc=None):
+    pd = domain.pd
+    x, y = pd.points[:, 0].copy(), pd.points[:, 1].copy()
+    N, k = pd.num_points, pd.num_neighbors
+    idx, _ = O.knn_brute(x, y, k)
+    assert np.array_equal(idx, pd.neighbors)  # GPU kNN of the host API is exact
+    wx, wy, rc = O.weights(x, y, O.ell_ptr(N, k), idx, 5, 3, 1)
+    assert rc == 0
+    tags = domain.boundary_tags
+    boundary = [(O.BC_DIRICHLET, tags["inlet"].idx, tags["inlet"].normals, O.FN_GRADIENT, None, None),
+                (O.BC_NOTHING, tags["outlet"].idx, tags["outlet"].normals, None, None, None)]
+    for name in ("bottom", "top", "cylinder"):
+        boundary.append((O.BC_SLIP, tags[name].idx, tags[name].normals, None, None, None))
+    rv = O.RV_UPWIND if visc == "upwind" else O.RV_NONE
+    return O.Problem(x, y, O.ell_ptr(N, k), idx, wx, wy, O.EQ_EULER, 4, boundary=boundary, rv_mode=rv,
+                     c_rv=1.0, c_uw=1.0, dx_avg=pd.dx_avg)
+
+
+@pytest.mark.parametrize("visc,host_dirichlet", [(None, False), ("upwind", False), (None, True)])
+def test_host_api_cyl_slip_walls(mf, visc, host_dirichlet):
+    """Trixi.rhs! on the reference's Medusa cloud (2,154 nodes, five boundary
+    groups): Dirichlet inlet, do-nothing outlet, three slip walls."""
+    _say("host API cyl_0_05 visc=%s host_dirichlet=%s" % (visc, host_dirichlet))
+    semi, domain = _cyl_semi(visc, host_dirichlet)
+    prob = _cyl_oracle(domain, visc)
+    ode = mfrbf.semidiscretize(semi, (0.0, 0.1))
+    u = ode.u0.copy()
+    rng = np.random.default_rng(7)
+    u[1:3] += 0.05 * rng.standard_normal(u[1:3].shape)  # momentum not tangent at the walls
+    uo = u.copy()
+    du = np.zeros_like(u)
+    mfrbf.rhs_(du, u, semi, 0.02)
+    duo, _, _ = prob.rhs(uo, 0.02)
+    assert rel_inf(u, uo) < 1e-15  # boundary passes mutate u identically
+    assert rel_inf(du, duo) < 1e-12
+    for name in ("bottom", "top", "cylinder"):
+        b = domain.boundary_tags[name]
+        nrm = b.normals / np.linalg.norm(b.normals, axis=1)[:, None]
+        assert np.abs((u[1:3, b.idx].T * nrm).sum(1)).max() < 1e-14
+        assert np.all(du[1:3, b.idx] == 0.0)
+    assert np.all(du[:, domain.boundary_tags["inlet"].idx] == 0.0)
+
+
+def test_limiter_vs_oracle(mf):
+    """Trixi.limiter_zhang_shu! (positivity_zhang_shu_point2d.jl:22-83) on
+    density and then pressure, with nodes pushed below the thresholds."""
+    _say("zhang-shu limiter")
+    c = cases.build(configs.c2(), side=40)
+    u = c.u0.copy()
+    rng = np.random.default_rng(3)
+    bad = rng.choice(c.N, 60, replace=False)
+    u[0, bad[:30]] = 0.05
+    u[3, bad[30:]] = 0.2 * u[3, bad[30:]]  # low pressure
+    ptr = O.ell_ptr(c.N, c.k)
+    for thr, var in ((0.5, L.MF_VAR_DENSITY), (0.5, L.MF_VAR_PRESSURE)):
+        ug = u.copy()
+        c.ctx.upload(L.MF_BUF_U, ug)
+        c.ctx.limiter_zhang_shu(thr, var)
+        ug = c.ctx.download(L.MF_BUF_U)
+        uo = O.limiter_zhang_shu(u.copy(), ptr, c.idx, thr, var)
+        changed = np.any(uo != u, axis=0)
+        assert changed.sum() > 0
+        assert rel_inf(ug, uo) < 1e-14
+
+
+def test_ssprk33_stage_limiter_vs_oracle(mf):
+    _say("ssprk33 with stage limiter")
+    c = cases.build(configs.c2(), side=40)
+    u0 = c.u0.copy()
+    u0[0, ::97] = 0.3
+    lim = [(0.6, O.VAR_DENSITY), (0.5, O.VAR_PRESSURE)]
+    c.ctx.set_stage_limiter([t for t, _ in lim], [L.MF_VAR_DENSITY, L.MF_VAR_PRESSURE])
+    c.ctx.upload(L.MF_BUF_U, u0.copy())
+    c.ctx.ssprk33(0.0, c.dt, 5, flags=L.MF_STEP_LIMITER)
+    ug = c.ctx.download(L.MF_BUF_U)
+    uo = u0.copy()
+    c.prob.ssprk33(uo, 0.0, c.dt, 5, limiter=lim, stencil=(O.ell_ptr(c.N, c.k), c.idx))
+    assert rel_inf(ug, uo) < 1e-12
+
+
+def test_ssprk33_c1_full_run(mf):
+    """C1: advection of a Gaussian pulse, 4,096 nodes, 200 SSPRK33 steps."""
+    _say("C1 200 steps")
+    spec = configs.c1()
+    c = cases.build(spec)
+    c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+    c.ctx.ssprk33(0.0, c.dt, spec.nsteps, flags=L.MF_STEP_GRAPH)
+    ug = c.ctx.download(L.MF_BUF_U)
+    uo = c.u0.copy()
+    c.prob.ssprk33(uo, 0.0, c.dt, spec.nsteps)
+    err = rel_inf(ug, uo)
+    _say("  C1 rel err %.2e" % err)
+    assert err < 1e-11
+
+
+@pytest.mark.parametrize("prec", [L.MF_PREC_F64, L.MF_PREC_SPLIT32])
+def test_graph_replay_matches_eager(mf, prec):
+    """The captured step, replayed over several calls that continue the
+    trajectory, gives the same bits as eager stepping."""
+    _say("graph replay vs eager prec %d" % prec)
+    c = cases.build(configs.c2(), side=48, visc="residual", hv=(2, 1e-4))
+    flags = L.MF_STEP_HISTORY
+    out = {}
+    for graph in (False, True):
+        c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+        t = 0.0
+        for call in range(4):
+            c.ctx.ssprk33(t, c.dt, 3, prec, flags | (L.MF_STEP_GRAPH if graph else 0), resume=(call > 0))
+            t += 3 * c.dt
+            # eager work between replays
+            c.ctx.synchronize()
+        out[graph] = c.ctx.download(L.MF_BUF_U)
+    assert np.array_equal(out[False], out[True])
+
+
+def test_adjoint_vs_finite_differences_c4(mf):
+    """C4: dL/drho0 of L = sum rho(T)^2 through 100 SSPRK33 steps on 65,536
+    nodes (checkpoints every 10) against central differences along random
+    unit directions and single coordinates."""
+    _say("C4 adjoint vs finite differences")
+    from bench import setup_euler
+    spec = configs.c4()
+    ctx, info = setup_euler(L, spec, spec.side, visc=False)
+    N, dt = info["N"], info["dt"]
+    u0 = configs.perturbed_density(info["u0"], spec, N)
+    ctx.upload(L.MF_BUF_U, u0)
+    loss = ctx.ssprk33_vjp(0.0, dt, spec.nsteps, spec.checkpoint_every)
+    grad = ctx.download(L.MF_BUF_AUX1)[0]
+
+    def L_of(u):
+        ctx.upload(L.MF_BUF_U, u)
+        ctx.ssprk33(0.0, dt, spec.nsteps)
+        return float(np.sum(ctx.download(L.MF_BUF_U)[0] ** 2))
+
+    assert abs(L_of(u0) - loss) <= 1e-13 * abs(loss)
+    rng = np.random.default_rng(spec.noise_seed)
+    h = 1e-4
+    errs = []
+    dirs = [rng.standard_normal(N) for _ in range(8)]
+    dirs = [d / np.linalg.norm(d) for d in dirs]
+    for i in rng.choice(N, 8, replace=False):
+        e = np.zeros(N)
+        e[i] = 1.0
+        dirs.append(e)
+    for d in dirs:
+        up, um = u0.copy(), u0.copy()
+        up[0] += h * d
+        um[0] -= h * d
+        fd = (L_of(up) - L_of(um)) / (2 * h)
+        ad = float(grad @ d)
+        errs.append(abs(fd - ad) / max(abs(ad), 1e-300))
+    _say("  FD rel errors max %.2e" % max(errs))
+    assert max(errs) < 1e-6

@@ -237,6 +237,12 @@ def main():
     traffic, traffic_kernel = measured_traffic(dom, tuning, args.side)
     log("rhs %.4f ms (%.3e node-evals/s); stencil %.4f ms, transpose %.4f ms, reductions %.4f ms, bc %.4f ms"
         % (rhs_ms, value, parts[0], parts[1], parts[2], parts[3]))
+    # the same call with the neighbour gathers from global memory (no LDS tiles)
+    ctx.set_tuning(L.MF_TUNE_TILES, 0)
+    nt_ms, nt_parts = ctx.time_rhs(t_now, L.MF_PREC_F64, max(200, args.steps // 10))
+    ctx.set_tuning(L.MF_TUNE_TILES, 1)
+    ctx.time_rhs(t_now, L.MF_PREC_F64, 3)
+    log("without LDS tiles: rhs %.4f ms; stencil %.4f ms, transpose %.4f ms" % (nt_ms, nt_parts[0], nt_parts[1]))
     # split-FP32 and the reference-order FP64
     log("timing %d rhs! (split-FP32, FP64 reference order)" % args.steps)
     s_ms, s_parts = ctx.time_rhs(t_now, L.MF_PREC_SPLIT32, args.steps)
@@ -276,6 +282,8 @@ def main():
         "f64_ref": {"value": N / (r_ms * 1e-3), "ms_per_step": r_ms,
                     "kernels_ms": {"stencil": r_parts[0], "transpose": r_parts[1]},
                     "rhs_frac": rbytes / (r_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "f64_global_gathers": {"ms_per_step": nt_ms,
+                               "kernels_ms": {"stencil": nt_parts[0], "transpose": nt_parts[1]}},
         "setup_ms": {"knn": 1e3 * info["t_knn"], "weights": 1e3 * info["t_weights"]},
         "tuning": tuning,
     }

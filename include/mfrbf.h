@@ -183,8 +183,16 @@ int mf_set_source_viscosity(mf_ctx* ctx, int mode, double c_rv, double c_uw, dou
 /* Copies nvar host fields into / out of a context buffer (MF_BUF_*). */
 int mf_state_upload(mf_ctx* ctx, int buf, const double* const* fields);
 int mf_state_download(mf_ctx* ctx, int buf, double* const* fields);
-/* Device address of a context buffer (node-major, nvar doubles per node). */
+/* Device address of a context buffer: node-major, nvar doubles per node,
+ * nodes in the context's internal order (see mf_node_order). */
 int mf_state_device_ptr(mf_ctx* ctx, int buf, double** dptr);
+/* The internal node order: order_out[p] = the caller's index of internal
+ * node p (n_points int32).  The context stores nodes along a Morton curve
+ * so that stencil neighbours are close in memory; every other entry point
+ * takes and returns data in the caller's order.  No counterpart in the
+ * reference (its arrays are in the caller's order); needed only with
+ * mf_state_device_ptr. */
+int mf_node_order(mf_ctx* ctx, int32_t* order_out);
 
 /* ----------------------------------------------------------------- rhs! */
 /* du = rhs(u, t) on device buffers; u (MF_BUF_U) is mutated at boundary
@@ -238,6 +246,10 @@ int mf_ssprk33_vjp(mf_ctx* ctx, double t0, double dt, int n_steps, int checkpoin
 /* ------------------------------------------------------------- tuning */
 #define MF_TUNE_FLUX_BATCH 0      /* stencil entries loaded per batch: 3, 5, 6, 10 (0 = default) */
 #define MF_TUNE_TRANSPOSE_BATCH 1 /* transposed entries loaded per batch: 2, 4, 8 (0 = default) */
+#define MF_TUNE_NODE_ORDER 2      /* internal node order for the next mf_set_points: 0 Morton
+                                     (default), 1 the caller's order; precision is ignored */
+#define MF_TUNE_TILES 3           /* 1 (default): stage stencil halos in LDS where they fit;
+                                     0: gather from global memory; precision is ignored */
 /* Selects a compiled kernel variant for one precision (MF_PREC_*, or -1 for
  * all).  A batch size that does not divide the stencil size falls back to
  * the default.  Results do not depend on it (the per-node operation order is
@@ -249,8 +261,9 @@ int mf_set_tuning(mf_ctx* ctx, int key, int precision, int value);
 /* Average device time of one rhs! (ms) over n_calls back-to-back calls on
  * MF_BUF_U, measured with HIP events on the context stream (chains of the
  * captured call).  kernel_ms (4 doubles, may be NULL) receives each part's
- * average time inside n_calls further whole calls (events recorded between
- * the kernels, so each part sees the caches as a full call leaves them):
+ * average kernel time inside n_calls further whole calls (the dispatches
+ * carry start/stop events, so each kernel is timed alone while it sees the
+ * caches as a full call leaves them):
  * [0] the stencil kernel (flux divergence, sources, phase A of the
  * viscosity), [1] the transposed gather of the viscosity, [2] the two
  * global reductions of the residual viscosity (with the first boundary pass

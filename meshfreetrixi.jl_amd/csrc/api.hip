@@ -43,6 +43,7 @@ template int dalloc<int64_t>(mf_ctx*, int64_t**, size_t);
 template int dalloc<int16_t>(mf_ctx*, int16_t**, size_t);
 template int dalloc<int8_t>(mf_ctx*, int8_t**, size_t);
 template int dalloc<uint8_t>(mf_ctx*, uint8_t**, size_t);
+template int dalloc<uint16_t>(mf_ctx*, uint16_t**, size_t);
 
 void dfree(void* p)
 {
@@ -189,6 +190,7 @@ static void free_stencil(mf_ctx* c)
     c->d_perm_s = nullptr;
     c->k = 0;
     c->didx_ok = false;
+    free_tiles(c);
     free_ops(c);
 }
 
@@ -303,6 +305,8 @@ int mf_ctx_destroy(mf_ctx* c)
     free_bc(c);
     dfree(c->d_x);
     dfree(c->d_y);
+    dfree(c->d_uid);
+    dfree(c->d_inv);
     dfree(c->d_sc);
     dfree(c->d_rptr);
     dfree(c->d_ridx);
@@ -331,22 +335,31 @@ int mf_set_points(mf_ctx* c, const double* x, const double* y, int64_t n)
     if (!x || !y || n < 2) return fail(c, MF_ERR_ARGUMENT, "mf_set_points: need >= 2 points");
     if (n > (int64_t)INT32_MAX) return fail(c, MF_ERR_ARGUMENT, "mf_set_points: too many points");
     MF_HIP(c, hipStreamSynchronize(c->stream));
+    // a new cloud (or a new order) invalidates everything built on the old one
+    free_state(c);
+    free_stencil(c);
+    free_bc(c);
     if (n != c->N) {
-        free_state(c);
-        free_stencil(c);
-        free_bc(c);
         dfree(c->d_x);
         dfree(c->d_y);
+        dfree(c->d_uid);
+        dfree(c->d_inv);
         c->d_x = c->d_y = nullptr;
+        c->d_uid = c->d_inv = nullptr;
         c->N = n;
-        if ((rc = dalloc(c, &c->d_x, n)) || (rc = dalloc(c, &c->d_y, n))) return rc;
-    } else {
-        free_graph(c);
+        if ((rc = dalloc(c, &c->d_x, n)) || (rc = dalloc(c, &c->d_y, n)) || (rc = dalloc(c, &c->d_uid, n)) ||
+            (rc = dalloc(c, &c->d_inv, n)))
+            return rc;
     }
-    MF_HIP(c, hipMemcpyAsync(c->d_x, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    MF_HIP(c, hipMemcpyAsync(c->d_y, y, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    MF_HIP(c, hipStreamSynchronize(c->stream));
-    return MF_OK;
+    double *xu = nullptr, *yu = nullptr;
+    if ((rc = dalloc(c, &xu, n)) || (rc = dalloc(c, &yu, n))) return rc;
+    MF_HIP(c, hipMemcpyAsync(xu, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    MF_HIP(c, hipMemcpyAsync(yu, y, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    rc = launch_node_order(c, xu, yu);
+    (void)hipStreamSynchronize(c->stream);
+    dfree(xu);
+    dfree(yu);
+    return rc;
 }
 
 static int set_k(mf_ctx* c, int k)
@@ -447,6 +460,7 @@ static void op_installed(mf_ctx* c, int target)
     if (target == MF_OP_FLUX) {
         c->have_flux = true;
         c->have_T = false;
+        free_tiles_b(c);
     } else {
         c->have_hv = true;
     }
@@ -553,8 +567,10 @@ int mf_weights_ragged_set(mf_ctx* c, const int64_t* ptr, const int32_t* idx, int
         (rc = dalloc(c, &c->d_rwx, nnz)) || (rc = dalloc(c, &c->d_rwy, nnz)))
         return rc;
     if (!c->d_status && (rc = dalloc(c, &c->d_status, 1))) return rc;
+    std::vector<int32_t> ridx(nnz);
+    for (int64_t q = 0; q < nnz; ++q) ridx[q] = c->h_inv[idx[q]];
     MF_HIP(c, hipMemcpyAsync(c->d_rptr, ptr, sizeof(int64_t) * (ns + 1), hipMemcpyHostToDevice, c->stream));
-    MF_HIP(c, hipMemcpyAsync(c->d_ridx, idx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, c->stream));
+    MF_HIP(c, hipMemcpyAsync(c->d_ridx, ridx.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, c->stream));
     MF_HIP(c, hipStreamSynchronize(c->stream));
     c->rn = ns;
     c->rnnz = nnz;
@@ -602,7 +618,10 @@ int mf_set_equations(mf_ctx* c, int equations, double p0, double p1)
         return fail(c, MF_ERR_ARGUMENT, "mf_set_equations: unknown equations");
     }
     MF_HIP(c, hipStreamSynchronize(c->stream));
-    if (nv != c->nvar) free_state(c);
+    if (nv != c->nvar) {
+        free_state(c);
+        free_tiles(c);  // halo sizes that fit LDS depend on nvar
+    }
     free_graph(c);
     c->eq = equations;
     c->nvar = nv;
@@ -632,6 +651,7 @@ int mf_set_boundary(mf_ctx* c, int n_tags, const int32_t* tag_ptr, const int32_t
     if (tag_ptr[0] != 0) return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: tag_ptr[0] must be 0");
     const int nb = tag_ptr[n_tags];
     std::vector<int32_t> btag(std::max(nb, 1));
+    std::vector<int32_t> bnode(std::max(nb, 1));
     std::vector<uint8_t> flag(c->N, 0);
     std::vector<int32_t> entry(c->N, -1);
     for (int g = 0; g < n_tags; ++g) {
@@ -652,8 +672,10 @@ int mf_set_boundary(mf_ctx* c, int n_tags, const int32_t* tag_ptr, const int32_t
         b.tag_ptr[g] = tag_ptr[g];
         if (kind == MF_BC_SLIP_WALL) b.any_slip = 1;
         for (int e = tag_ptr[g]; e < tag_ptr[g + 1]; ++e) {
-            const int32_t node = node_idx[e];
-            if (node < 0 || node >= c->N) return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: node index out of range");
+            if (node_idx[e] < 0 || node_idx[e] >= c->N)
+                return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: node index out of range");
+            const int32_t node = c->h_inv[node_idx[e]];
+            bnode[e] = node;
             btag[e] = g;
             if (entry[node] >= 0) b.multi = 1;
             entry[node] = e;
@@ -668,7 +690,7 @@ int mf_set_boundary(mf_ctx* c, int n_tags, const int32_t* tag_ptr, const int32_t
         return rc;
     std::vector<double> nrm(2 * (size_t)std::max(nb, 1), 0.0);
     if (normals) std::copy(normals, normals + 2 * (size_t)nb, nrm.begin());
-    MF_HIP(c, hipMemcpyAsync(b.bidx, node_idx, sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream));
+    MF_HIP(c, hipMemcpyAsync(b.bidx, bnode.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream));
     MF_HIP(c, hipMemcpyAsync(b.btag, btag.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream));
     MF_HIP(c, hipMemcpyAsync(b.bnormal, nrm.data(), sizeof(double) * 2 * nb, hipMemcpyHostToDevice, c->stream));
     MF_HIP(c, hipMemsetAsync(b.btable, 0, sizeof(double) * nb * 4, c->stream));
@@ -754,12 +776,28 @@ int mf_state_device_ptr(mf_ctx* c, int buf, double** dptr)
     return MF_OK;
 }
 
+int mf_node_order(mf_ctx* c, int32_t* order_out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->N == 0) return fail(c, MF_ERR_STATE, "mf_node_order: no points");
+    if (!order_out) return fail(c, MF_ERR_ARGUMENT, "mf_node_order: null output");
+    MF_HIP(c, hipMemcpyAsync(order_out, c->d_uid, sizeof(int32_t) * c->N, hipMemcpyDeviceToHost, c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    return MF_OK;
+}
+
 static int prepare_rhs(mf_ctx* c)
 {
     int rc;
     if ((rc = ensure_state(c))) return rc;
     if ((rc = ensure_transpose(c))) return rc;
     if (c->hv_gamma != 0.0 && !c->have_hv) return fail(c, MF_ERR_STATE, "hyperviscosity operator missing");
+    const bool want_b = c->visc_mode != MF_VISC_NONE && c->nvar == 4;
+    if (c->use_tiles && c->have_flux && (!c->tried_a || (want_b && c->tiles_a && !c->tried_b))) {
+        free_graph(c);
+        if ((rc = build_tiles(c, want_b))) return rc;
+    }
     return MF_OK;
 }
 
@@ -792,9 +830,20 @@ int mf_get_viscosity(mf_ctx* c, double* eps, int8_t* eps_c)
     int rc = check_ctx(c);
     if (rc) return rc;
     if (c->visc_mode == MF_VISC_NONE || !c->d_eps) return fail(c, MF_ERR_STATE, "no viscosity source evaluated");
-    if (eps) MF_HIP(c, hipMemcpyAsync(eps, c->d_eps, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream));
-    if (eps_c) MF_HIP(c, hipMemcpyAsync(eps_c, c->d_epsc, c->N, hipMemcpyDeviceToHost, c->stream));
+    double* tmp = nullptr;
+    if ((rc = dalloc(c, &tmp, c->N))) return rc;
+    if (eps) {
+        if ((rc = launch_gather_nodes_d(c, c->d_eps, tmp))) return rc;
+        MF_HIP(c, hipMemcpyAsync(eps, tmp, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream));
+        MF_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    if (eps_c) {
+        int8_t* t8 = reinterpret_cast<int8_t*>(tmp);
+        if ((rc = launch_gather_nodes_i8(c, c->d_epsc, t8))) return rc;
+        MF_HIP(c, hipMemcpyAsync(eps_c, t8, c->N, hipMemcpyDeviceToHost, c->stream));
+    }
     MF_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(tmp);
     return MF_OK;
 }
 
@@ -951,6 +1000,17 @@ int mf_set_tuning(mf_ctx* c, int key, int precision, int value)
         if (value != 0 && value != 3 && value != 5 && value != 6 && value != 10)
             return fail(c, MF_ERR_ARGUMENT, "mf_set_tuning: flux batch must be 0, 3, 5, 6 or 10");
         for (int p = p0; p <= p1; ++p) c->tune_flux_batch[p] = value;
+    } else if (key == MF_TUNE_TILES) {
+        if (value != 0 && value != 1) return fail(c, MF_ERR_ARGUMENT, "mf_set_tuning: tiles must be 0 or 1");
+        MF_HIP(c, hipStreamSynchronize(c->stream));
+        c->use_tiles = value;
+        free_tiles(c);
+        free_graph(c);
+        return MF_OK;
+    } else if (key == MF_TUNE_NODE_ORDER) {
+        if (value != 0 && value != 1) return fail(c, MF_ERR_ARGUMENT, "mf_set_tuning: node order must be 0 or 1");
+        c->node_order = value;
+        return MF_OK;
     } else if (key == MF_TUNE_TRANSPOSE_BATCH) {
         if (value != 0 && value != 2 && value != 4 && value != 8)
             return fail(c, MF_ERR_ARGUMENT, "mf_set_tuning: transpose batch must be 0, 2, 4 or 8");
@@ -1011,11 +1071,11 @@ int mf_time_rhs(mf_ctx* c, double t, int precision, int n_calls, double* rhs_ms,
     double full = 0.0;
     if ((rc = time_parts(RHS_ALL, &full))) return rc;
     if (rhs_ms) *rhs_ms = full;
-    // each kernel's share of whole rhs! calls: events between the kernels
-    // of n_calls eager calls (so every kernel sees the caches as the full
-    // call leaves them), in batches of up to 32 calls
+    // each kernel's own duration inside n_calls further whole rhs! calls:
+    // the dispatches carry start/stop events (see launch_k), so every kernel
+    // sees the caches as a full call leaves them; batches of 32 calls
     if (kernel_ms) {
-        constexpr int kB = 32, kE = 6;
+        constexpr int kB = 32, kS = 6, kE = 2 * kS;
         std::vector<hipEvent_t> ev(kB * kE, nullptr);
         for (auto& e : ev) MF_HIP(c, hipEventCreate(&e));
         double acc[4] = {0, 0, 0, 0};
@@ -1023,23 +1083,25 @@ int mf_time_rhs(mf_ctx* c, double t, int precision, int n_calls, double* rhs_ms,
         while (done < n_calls && rc == MF_OK) {
             const int nb = std::min(kB, n_calls - done);
             for (int q = 0; q < nb && rc == MF_OK; ++q) {
+                // slots a call does not use keep start == stop
+                for (int e = 0; e < kE; ++e) (void)hipEventRecord(ev[q * kE + e], c->stream);
                 c->probe = &ev[q * kE];
                 rc = run_rhs(c, precision, c->d_buf[MF_BUF_U], c->d_buf[MF_BUF_DU], 0.0, none);
             }
             c->probe = nullptr;
             if (rc) break;
-            hipError_t e = hipEventSynchronize(ev[(nb - 1) * kE + kE - 1]);
-            if (e != hipSuccess) { rc = hip_fail(c, e, "hipEventSynchronize"); break; }
+            hipError_t e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) { rc = hip_fail(c, e, "hipStreamSynchronize"); break; }
             for (int q = 0; q < nb; ++q) {
-                float m[kE - 1];
-                for (int s = 0; s + 1 < kE; ++s) {
-                    e = hipEventElapsedTime(&m[s], ev[q * kE + s], ev[q * kE + s + 1]);
-                    if (e != hipSuccess) m[s] = 0.f;
+                float m[kS];
+                for (int sl = 0; sl < kS; ++sl) {
+                    e = hipEventElapsedTime(&m[sl], ev[q * kE + 2 * sl], ev[q * kE + 2 * sl + 1]);
+                    if (e != hipSuccess) m[sl] = 0.f;
                 }
-                acc[0] += m[2];         // stencil kernel
-                acc[1] += m[3];         // transposed gather
-                acc[2] += m[1];         // reductions
-                acc[3] += m[0] + m[4];  // boundary passes
+                acc[0] += m[0];         // stencil kernel
+                acc[1] += m[1];         // transposed gather
+                acc[2] += m[2] + m[3];  // reductions
+                acc[3] += m[4] + m[5];  // boundary passes
             }
             done += nb;
         }

@@ -67,10 +67,16 @@ struct mf_ctx {
     std::string err;
     int cu_count = 0;
 
-    // cloud
+    // cloud, in the internal node order: internal node p is the caller's
+    // node d_uid[p]; d_inv is the inverse map.  Every array below is
+    // indexed by internal node; the ABI converts at its edges.
     int64_t N = 0;
     double* d_x = nullptr;
     double* d_y = nullptr;
+    int32_t* d_uid = nullptr;
+    int32_t* d_inv = nullptr;
+    std::vector<int32_t> h_inv;  // host copy of d_inv (boundary and stencil indices)
+    int node_order = 0;          // MF_TUNE_NODE_ORDER value for the next mf_set_points
 
     // stencil (ELL, slot-major: entry s of node i at s*N + i)
     int k = 0;
@@ -152,14 +158,34 @@ struct mf_ctx {
     double* d_adj = nullptr;
     int64_t adj_len = 0;
 
+    // LDS tiles: tile t = internal nodes [256 t, 256 t + 256).  Phase A
+    // (stencil kernel) stages the states of the tile's stencil halo in LDS
+    // and addresses them by 16-bit halo positions; phase B (transposed
+    // gather) stages G of its halo and walks rows sorted by degree.
+    int use_tiles = 1;  // MF_TUNE_TILES
+    bool tiles_a = false, tiles_b = false;  // built and fitting in LDS
+    bool tried_a = false, tried_b = false;  // build attempted for this stencil / operator
+    int64_t n_tiles = 0;
+    int ha_max = 0, hb_max = 0;          // largest halo of a tile
+    int64_t* d_ha_off = nullptr;         // n_tiles + 1
+    int32_t* d_ha_ids = nullptr;         // halo node ids, sorted per tile
+    uint16_t* d_la = nullptr;            // slot-major halo positions (like d_idx)
+    int64_t* d_hb_off = nullptr;
+    int32_t* d_hb_ids = nullptr;
+    int64_t* d_sb_off = nullptr;         // 4 slices (64 rows) per tile
+    int32_t* d_sb_w = nullptr;
+    uint8_t* d_rowb = nullptr;           // tile position -> tile row
+    uint16_t* d_lb = nullptr;            // halo positions, slice layout
+    double2* d_twb = nullptr;            // (Dx, Dy) of the transposed entries
+    float2* d_twb32 = nullptr;
+
     // launch tuning (mf_set_tuning)
     // per precision (MF_PREC_*); 0 = the default
     int tune_flux_batch[3] = {0, 0, 0};       // stencil entries per load batch
     int tune_transpose_batch[3] = {0, 0, 0};  // transposed entries per load batch
 
-    // when set, run_rhs_parts records these 6 events at its part boundaries
-    // (start, after boundary pass 1, reductions, stencil kernel, transposed
-    // gather, boundary pass 2)
+    // when set, the rhs! kernels are dispatched with start/stop events:
+    // probe[2 s], probe[2 s + 1] for slot s (see launch_k in rhs.hip)
     hipEvent_t* probe = nullptr;
 };
 
@@ -197,6 +223,12 @@ int launch_rows_to_ell_d(mf_ctx* c, const double* d_rows, double* d_ell, int k);
 int launch_soa_to_aos(mf_ctx* c, const double* d_soa, double* d_aos);
 int launch_aos_to_soa(mf_ctx* c, const double* d_aos, double* d_soa);
 int launch_build_didx(mf_ctx* c);
+int launch_node_order(mf_ctx* c, const double* d_xu, const double* d_yu);
+int build_tiles(mf_ctx* c, bool want_b);
+void free_tiles(mf_ctx* c);
+void free_tiles_b(mf_ctx* c);
+int launch_gather_nodes_d(mf_ctx* c, const double* d_src, double* d_dst);
+int launch_gather_nodes_i8(mf_ctx* c, const int8_t* d_src, int8_t* d_dst);
 int launch_sort_stencil(mf_ctx* c);
 int launch_pack_sorted(mf_ctx* c, int target, const double* d_wx, const double* d_wy);
 

@@ -7,6 +7,7 @@
 //
 // Layout: node-major state (V doubles per node), ELL stencils slot-major
 // (entry s of node i at s*N + i), operator weights (Dx, Dy) interleaved.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -55,6 +56,19 @@ struct RhsArgs {
     const int32_t* __restrict__ trow;
     const double2* __restrict__ tw;
     const float2* __restrict__ tw32;
+    // LDS tiles (phase A: halo of the stencils; phase B: halo of D^T)
+    int64_t n_tiles;
+    const int64_t* __restrict__ ha_off;
+    const int32_t* __restrict__ ha_ids;
+    const uint16_t* __restrict__ la;
+    const int64_t* __restrict__ hb_off;
+    const int32_t* __restrict__ hb_ids;
+    const int64_t* __restrict__ sb_off;
+    const int32_t* __restrict__ sb_w;
+    const uint8_t* __restrict__ rowb;
+    const uint16_t* __restrict__ lb;
+    const double2* __restrict__ twb;
+    const float2* __restrict__ twb32;
     // stage output
     int st_kind;
     double dt;
@@ -184,19 +198,53 @@ __device__ __forceinline__ void block_norms(const RhsArgs& a, double* ninf)
         for (int v = 0; v < 4; ++v) a.norms_out[4 + v] = ninf[v];
 }
 
+// XCD-aware logical block (the remap of remapped_node).
+__device__ __forceinline__ int64_t remapped_block()
+{
+    const unsigned b = blockIdx.x, per = gridDim.x >> 3;
+    return (int64_t)((b & 7u) * per + (b >> 3));
+}
+
+// Node of this thread and, with LDS tiles (TL), the tile's stencil halo
+// staged in lds: halo entry q holds the state of node ha_ids[ha_off[t] + q].
+// Returns -1 for threads without a node (after the block barrier).
+template <int V, bool TL>
+__device__ __forceinline__ int64_t stage_halo_a(const RhsArgs& a, double* lds)
+{
+    if constexpr (TL) {
+        const int64_t t = remapped_block();
+        if (t >= a.n_tiles) return -1;  // whole block
+        const int64_t h0 = a.ha_off[t];
+        const int nh = (int)(a.ha_off[t + 1] - h0);
+        for (int q = threadIdx.x; q < nh; q += kBlock) {
+            double s[V];
+            load_state<V>(a.u, a.ha_ids[h0 + q], s);
+            store_state<V>(lds, q, s);
+        }
+        __syncthreads();
+        const int64_t i = t * kBlock + threadIdx.x;
+        return i < a.N ? i : -1;
+    } else {
+        const int64_t i = remapped_node();
+        return i < a.N ? i : -1;
+    }
+}
+
 // Batches of BT stencil entries: every load of a batch (indices, weights,
 // then the gathered states) is issued before any of it is used, so BT
-// gathers are in flight per thread.  Entries past K read the node itself
-// with weight 0.
-template <int V, int NS, bool HV, bool RV, int BT>
+// gathers are in flight per thread.  Entries past K read a valid node with
+// weight 0.  TL: neighbour states come from the LDS halo, addressed by the
+// 16-bit halo positions la (same slot-major layout as idx).
+template <int V, int NS, bool HV, bool RV, int BT, bool TL>
 __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
 {
+    extern __shared__ double lds[];
     double ninf[4] = {1.0, 1.0, 1.0, 1.0};
     if constexpr (RV) {
         if (a.visc == MF_VISC_RESIDUAL) block_norms(a, ninf);
     }
-    const int64_t i = remapped_node();
-    if (i >= a.N) return;
+    const int64_t i = stage_halo_a<V, TL>(a, lds);
+    if (i < 0) return;
     const int64_t N = a.N;
     const int K = NS ? NS : a.k;
     double ax[V], ay[V], gx[V], gy[V], hh[V];
@@ -204,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
     for (int v = 0; v < V; ++v) ax[v] = ay[v] = gx[v] = gy[v] = hh[v] = 0.0;
 #pragma unroll 1
     for (int s0 = 0; s0 < K; s0 += BT) {
-        int32_t j[BT];
+        int64_t j[BT];
         double2 w[BT];
         double hw[BT];
 #pragma unroll
@@ -212,18 +260,26 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
             const int s = s0 + b;
             if ((NS && NS % BT == 0) || s < K) {
                 const int64_t q = (int64_t)s * N + i;
-                j[b] = a.idx[q];
+                if constexpr (TL)
+                    j[b] = a.la[q];
+                else
+                    j[b] = a.idx[q];
                 w[b] = a.wxy[q];
                 if constexpr (HV) hw[b] = a.hv[q];
             } else {
-                j[b] = (int32_t)i;
+                j[b] = TL ? 0 : i;
                 w[b] = make_double2(0.0, 0.0);
                 hw[b] = 0.0;
             }
         }
         double uj[BT][V];
 #pragma unroll
-        for (int b = 0; b < BT; ++b) load_state<V>(a.u, j[b], uj[b]);
+        for (int b = 0; b < BT; ++b) {
+            if constexpr (TL)
+                load_state<V>(lds, j[b], uj[b]);
+            else
+                load_state<V>(a.u, j[b], uj[b]);
+        }
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
             double f1[V], f2[V];
@@ -379,15 +435,16 @@ __global__ __launch_bounds__(kBlock) void k_rhs_ref(const RhsArgs a)
 // formed in FP64: sum_j w_ij F(u_j) ~= sum_j w_ij (F(u_j) - F(u_i)),
 // exact up to F(u_i) sum_j w_ij, which vanishes with the weights' row sums.
 // Weights FP32, neighbour indices as int16 offsets when the cloud allows.
-template <int V, int NS, bool HV, bool RV, bool D16, int BT>
+template <int V, int NS, bool HV, bool RV, bool D16, int BT, bool TL>
 __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
 {
+    extern __shared__ double lds[];
     double ninf[4] = {1.0, 1.0, 1.0, 1.0};
     if constexpr (RV) {
         if (a.visc == MF_VISC_RESIDUAL) block_norms(a, ninf);
     }
-    const int64_t i = remapped_node();
-    if (i >= a.N) return;
+    const int64_t i = stage_halo_a<V, TL>(a, lds);
+    if (i < 0) return;
     const int64_t N = a.N;
     const int K = NS ? NS : a.k;
     double ui[V], fi1[V], fi2[V];
@@ -406,21 +463,28 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
             const int s = s0 + b;
             if ((NS && NS % BT == 0) || s < K) {
                 const int64_t q = (int64_t)s * N + i;
-                if constexpr (D16)
+                if constexpr (TL)
+                    jj[b] = a.la[q];
+                else if constexpr (D16)
                     jj[b] = i + (int64_t)a.didx[q];
                 else
                     jj[b] = a.idx[q];
                 ww[b] = a.wxy32[q];
                 if constexpr (HV) hw32[b] = a.hv32[q];
             } else {
-                jj[b] = i;
+                jj[b] = TL ? 0 : i;
                 ww[b] = make_float2(0.0f, 0.0f);
                 hw32[b] = 0.0f;
             }
         }
         double ub[BT][V];
 #pragma unroll
-        for (int b = 0; b < BT; ++b) load_state<V>(a.u, jj[b], ub[b]);
+        for (int b = 0; b < BT; ++b) {
+            if constexpr (TL)
+                load_state<V>(lds, jj[b], ub[b]);
+            else
+                load_state<V>(a.u, jj[b], ub[b]);
+        }
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
         const float2 w = ww[b];
@@ -493,15 +557,44 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
 // TM: 0 FP64 (fused multiply-add), 1 FP32 weights, 2 FP64 in the reference's
 // order (tmp = tmp + w*G over ascending rows, then du - tmp; a Flyer source
 // that follows the viscosity in keyword order is added last).
-template <int V, int TM, int BT>
+// TL: the tile's rows in the order of rowb (by degree), SELL-64 per
+// wavefront (sb_off, sb_w), entries addressing G staged in LDS (lb: halo
+// positions; halo entry q holds G of node hb_ids[hb_off[t] + q]).
+template <int V, int TM, int BT, bool TL>
 __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
 {
-    const int64_t i = remapped_node();
-    if (i >= a.N) return;
-    const int64_t slice = i >> 6;
-    const int lane = (int)(i & 63);
-    const int64_t off = a.sell_off[slice];
-    const int W = a.sell_w[slice];
+    extern __shared__ double lds[];
+    int64_t i, off;
+    int lane, W;
+    if constexpr (TL) {
+        const int64_t t = remapped_block();
+        if (t >= a.n_tiles) return;  // whole block
+        const int64_t h0 = a.hb_off[t];
+        const int nh = (int)(a.hb_off[t + 1] - h0);
+        for (int q = threadIdx.x; q < nh; q += kBlock) {
+            const int64_t j = a.hb_ids[h0 + q];
+            double g[2 * V];
+            load_state<V>(a.G, 2 * j, g);
+            load_state<V>(a.G, 2 * j + 1, g + V);
+            store_state<V>(lds, 2 * q, g);
+            store_state<V>(lds, 2 * q + 1, g + V);
+        }
+        __syncthreads();
+        const int pos = threadIdx.x;
+        i = t * kBlock + a.rowb[t * kBlock + pos];
+        if (i >= a.N) return;
+        lane = pos & 63;
+        const int64_t sl = t * (kBlock / kSellC) + (pos >> 6);
+        off = a.sb_off[sl];
+        W = a.sb_w[sl];
+    } else {
+        i = remapped_node();
+        if (i >= a.N) return;
+        const int64_t slice = i >> 6;
+        lane = (int)(i & 63);
+        off = a.sell_off[slice];
+        W = a.sell_w[slice];
+    }
     double tx[V], ty[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) tx[v] = ty[v] = 0.0;
@@ -513,19 +606,27 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
         for (int b = 0; b < BT; ++b) {
             const int e = (e0 + b < W) ? e0 + b : e0;
             const int64_t q = off + (int64_t)e * kSellC + lane;
-            r[b] = a.trow[q];
+            if constexpr (TL)
+                r[b] = a.lb[q];
+            else
+                r[b] = a.trow[q];
             if constexpr (TM == 1) {
-                const float2 w32 = a.tw32[q];
+                const float2 w32 = TL ? a.twb32[q] : a.tw32[q];
                 w[b] = make_double2((double)w32.x, (double)w32.y);
             } else {
-                w[b] = a.tw[q];
+                w[b] = TL ? a.twb[q] : a.tw[q];
             }
         }
         double g0[BT][V], g1[BT][V];
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
-            load_state<V>(a.G, 2 * (int64_t)r[b], g0[b]);
-            load_state<V>(a.G, 2 * (int64_t)r[b] + 1, g1[b]);
+            if constexpr (TL) {
+                load_state<V>(lds, 2 * (int64_t)r[b], g0[b]);
+                load_state<V>(lds, 2 * (int64_t)r[b] + 1, g1[b]);
+            } else {
+                load_state<V>(a.G, 2 * (int64_t)r[b], g0[b]);
+                load_state<V>(a.G, 2 * (int64_t)r[b] + 1, g1[b]);
+            }
         }
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
@@ -932,6 +1033,18 @@ static RhsArgs make_args(mf_ctx* c, double* u, double* du)
     a.trow = c->d_trow;
     a.tw = c->d_tw;
     a.tw32 = c->d_tw32;
+    a.n_tiles = c->n_tiles;
+    a.ha_off = c->d_ha_off;
+    a.ha_ids = c->d_ha_ids;
+    a.la = c->d_la;
+    a.hb_off = c->d_hb_off;
+    a.hb_ids = c->d_hb_ids;
+    a.sb_off = c->d_sb_off;
+    a.sb_w = c->d_sb_w;
+    a.rowb = c->d_rowb;
+    a.lb = c->d_lb;
+    a.twb = c->d_twb;
+    a.twb32 = c->d_twb32;
     return a;
 }
 
@@ -960,6 +1073,21 @@ static BcArgs make_bc(mf_ctx* c, double dt_off, int homog)
 
 // One boundary pass on a state buffer; tags run one after another when a
 // node belongs to several (the reference applies them in tag order).
+// Launch on the context stream.  While the context is being timed
+// (c->probe set), the dispatch of probe slot `slot` carries its own start
+// and stop events (hipExtLaunchKernelGGL), which time the kernel alone.
+// Slots: 0 stencil kernel, 1 transposed gather, 2 sum pass, 3 max pass,
+// 4 first boundary pass, 5 second boundary pass.
+template <typename... KArgs, typename... Args>
+static void launch_k(mf_ctx* c, int slot, void (*kernel)(KArgs...), dim3 G, dim3 B, size_t lds, Args... args)
+{
+    if (c->probe && slot >= 0)
+        hipExtLaunchKernelGGL(kernel, G, B, (uint32_t)lds, c->stream, c->probe[2 * slot], c->probe[2 * slot + 1], 0,
+                              args...);
+    else
+        hipLaunchKernelGGL(kernel, G, B, lds, c->stream, args...);
+}
+
 static int bc_state_pass(mf_ctx* c, double* u, double dt_off, int homog, bool slip_only)
 {
     if (!c->have_bc || c->bc.nb == 0) return MF_OK;
@@ -973,10 +1101,11 @@ static int bc_state_pass(mf_ctx* c, double* u, double dt_off, int homog, bool sl
         b.e0 = e0;
         b.e1 = e1;
         const unsigned g = (unsigned)((e1 - e0 + kBlock - 1) / kBlock);
+        const int slot = homog ? -1 : (slip_only ? 5 : 4);
         if (c->nvar == 4)
-            hipLaunchKernelGGL(k_bc_state<4>, dim3(g), dim3(kBlock), 0, c->stream, b, u);
+            launch_k(c, slot, k_bc_state<4>, dim3(g), dim3(kBlock), 0, b, u);
         else
-            hipLaunchKernelGGL(k_bc_state<1>, dim3(g), dim3(kBlock), 0, c->stream, b, u);
+            launch_k(c, slot, k_bc_state<1>, dim3(g), dim3(kBlock), 0, b, u);
     };
     if (c->bc.multi) {
         for (int g = 0; g < c->bc.ntags; ++g) launch(c->bc.tag_ptr[g], c->bc.tag_ptr[g + 1]);
@@ -987,31 +1116,51 @@ static int bc_state_pass(mf_ctx* c, double* u, double dt_off, int homog, bool sl
     return MF_OK;
 }
 
+template <int V, int NS, int BT, bool TL>
+static void launch_f64(mf_ctx* c, const RhsArgs& a, bool hv, bool rv, const dim3& G, size_t lds)
+{
+    const dim3 B(kBlock);
+    hipStream_t s = c->stream;
+    if (hv && rv) launch_k(c, 0, k_rhs_f64<V, NS, true, true, BT, TL>, G, B, lds, a);
+    else if (hv) launch_k(c, 0, k_rhs_f64<V, NS, true, false, BT, TL>, G, B, lds, a);
+    else if (rv) launch_k(c, 0, k_rhs_f64<V, NS, false, true, BT, TL>, G, B, lds, a);
+    else launch_k(c, 0, k_rhs_f64<V, NS, false, false, BT, TL>, G, B, lds, a);
+}
+
+template <int V, int NS, int BT, bool D16, bool TL>
+static void launch_s32(mf_ctx* c, const RhsArgs& a, bool hv, bool rv, const dim3& G, size_t lds)
+{
+    const dim3 B(kBlock);
+    hipStream_t s = c->stream;
+    if (hv && rv) launch_k(c, 0, k_rhs_s32<V, NS, true, true, D16, BT, TL>, G, B, lds, a);
+    else if (hv) launch_k(c, 0, k_rhs_s32<V, NS, true, false, D16, BT, TL>, G, B, lds, a);
+    else if (rv) launch_k(c, 0, k_rhs_s32<V, NS, false, true, D16, BT, TL>, G, B, lds, a);
+    else launch_k(c, 0, k_rhs_s32<V, NS, false, false, D16, BT, TL>, G, B, lds, a);
+}
+
 template <int V, int NS, int BT>
 static void launch_flux_bt(mf_ctx* c, int prec, const RhsArgs& a, bool hv, bool rv, unsigned grid)
 {
     const dim3 G(grid), B(kBlock);
     hipStream_t s = c->stream;
+    const bool tl = c->tiles_a && prec != MF_PREC_F64_REF;
+    const size_t lds = tl ? (size_t)c->ha_max * V * sizeof(double) : 0;
     if (prec == MF_PREC_F64) {
-        if (hv && rv) hipLaunchKernelGGL((k_rhs_f64<V, NS, true, true, BT>), G, B, 0, s, a);
-        else if (hv) hipLaunchKernelGGL((k_rhs_f64<V, NS, true, false, BT>), G, B, 0, s, a);
-        else if (rv) hipLaunchKernelGGL((k_rhs_f64<V, NS, false, true, BT>), G, B, 0, s, a);
-        else hipLaunchKernelGGL((k_rhs_f64<V, NS, false, false, BT>), G, B, 0, s, a);
+        if (tl)
+            launch_f64<V, NS, BT, true>(c, a, hv, rv, G, lds);
+        else
+            launch_f64<V, NS, BT, false>(c, a, hv, rv, G, 0);
     } else if (prec == MF_PREC_F64_REF) {
-        if (hv && rv) hipLaunchKernelGGL((k_rhs_ref<V, NS, true, true, BT>), G, B, 0, s, a);
-        else if (hv) hipLaunchKernelGGL((k_rhs_ref<V, NS, true, false, BT>), G, B, 0, s, a);
-        else if (rv) hipLaunchKernelGGL((k_rhs_ref<V, NS, false, true, BT>), G, B, 0, s, a);
-        else hipLaunchKernelGGL((k_rhs_ref<V, NS, false, false, BT>), G, B, 0, s, a);
+        if (hv && rv) launch_k(c, 0, k_rhs_ref<V, NS, true, true, BT>, G, B, 0, a);
+        else if (hv) launch_k(c, 0, k_rhs_ref<V, NS, true, false, BT>, G, B, 0, a);
+        else if (rv) launch_k(c, 0, k_rhs_ref<V, NS, false, true, BT>, G, B, 0, a);
+        else launch_k(c, 0, k_rhs_ref<V, NS, false, false, BT>, G, B, 0, a);
+    } else if (tl) {
+        launch_s32<V, NS, BT, false, true>(c, a, hv, rv, G, lds);
     } else if (c->didx_ok) {
-        if (hv && rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, true, true, BT>), G, B, 0, s, a);
-        else if (hv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, false, true, BT>), G, B, 0, s, a);
-        else if (rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, false, true, true, BT>), G, B, 0, s, a);
-        else hipLaunchKernelGGL((k_rhs_s32<V, NS, false, false, true, BT>), G, B, 0, s, a);
+        launch_s32<V, NS, BT, true, false>(c, a, hv, rv, G, 0);
     } else {
-        if (hv && rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, true, false, BT>), G, B, 0, s, a);
-        else if (hv) hipLaunchKernelGGL((k_rhs_s32<V, NS, true, false, false, BT>), G, B, 0, s, a);
-        else if (rv) hipLaunchKernelGGL((k_rhs_s32<V, NS, false, true, false, BT>), G, B, 0, s, a);
-        else hipLaunchKernelGGL((k_rhs_s32<V, NS, false, false, false, BT>), G, B, 0, s, a);
+        launch_s32<V, NS, BT, false, false>(c, a, hv, rv, G, 0);
     }
 }
 
@@ -1040,25 +1189,29 @@ static void launch_flux_v(mf_ctx* c, int prec, const RhsArgs& a, bool hv, bool r
     }
 }
 
-template <int TM>
-static void launch_transpose(mf_ctx* c, const RhsArgs& a, unsigned grid)
+template <int TM, bool TL>
+static void launch_transpose_tl(mf_ctx* c, const RhsArgs& a, unsigned grid, size_t lds)
 {
     const dim3 G(grid), B(kBlock);
     switch (c->tune_transpose_batch[TM]) {
-        case 2: hipLaunchKernelGGL((k_rhs_transpose<4, TM, 2>), G, B, 0, c->stream, a); break;
-        case 8: hipLaunchKernelGGL((k_rhs_transpose<4, TM, 8>), G, B, 0, c->stream, a); break;
-        default: hipLaunchKernelGGL((k_rhs_transpose<4, TM, 4>), G, B, 0, c->stream, a); break;
+        case 2: launch_k(c, 1, k_rhs_transpose<4, TM, 2, TL>, G, B, lds, a); break;
+        case 8: launch_k(c, 1, k_rhs_transpose<4, TM, 8, TL>, G, B, lds, a); break;
+        default: launch_k(c, 1, k_rhs_transpose<4, TM, 4, TL>, G, B, lds, a); break;
     }
+}
+
+template <int TM>
+static void launch_transpose(mf_ctx* c, const RhsArgs& a, unsigned grid)
+{
+    if (c->tiles_b)
+        launch_transpose_tl<TM, true>(c, a, grid, (size_t)c->hb_max * 8 * sizeof(double));
+    else
+        launch_transpose_tl<TM, false>(c, a, grid, 0);
 }
 
 int run_rhs(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const StageOut& st)
 {
     return run_rhs_parts(c, prec, d_u, d_du, dt_off, st, RHS_ALL);
-}
-
-static inline void probe(mf_ctx* c, int slot)
-{
-    if (c->probe) (void)hipEventRecord(c->probe[slot], c->stream);
 }
 
 int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const StageOut& st,
@@ -1074,10 +1227,8 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
     // first boundary pass on u
     // with the residual viscosity the first boundary pass rides on the sum pass
     const bool fused_bc = rv && c->visc_mode == MF_VISC_RESIDUAL && c->have_bc && !c->bc.multi;
-    probe(c, 0);
     int rc = ((parts & RHS_BC) && !fused_bc) ? bc_state_pass(c, d_u, dt_off, 0, false) : MF_OK;
     if (rc) return rc;
-    probe(c, 1);
     RhsArgs a = make_args(c, d_u, d_du);
     a.st_kind = st.kind;
     a.dt = st.dt;
@@ -1091,17 +1242,15 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
             const int64_t chunk = (c->N + nparts - 1) / nparts;
             BcArgs bc = make_bc(c, dt_off, 0);
             if (!((parts & RHS_BC) && fused_bc)) bc.flag = nullptr;
-            hipLaunchKernelGGL(k_rv_sum, dim3(nparts), dim3(kBlock), 0, c->stream, c->N, chunk, d_u,
+            launch_k(c, 2, k_rv_sum, dim3(nparts), dim3(kBlock), 0, c->N, chunk, d_u,
                                c->d_partials, bc);
-            hipLaunchKernelGGL(k_rv_max, dim3(nparts), dim3(kBlock), 0, c->stream, c->N, chunk,
+            launch_k(c, 3, k_rv_max, dim3(nparts), dim3(kBlock), 0, c->N, chunk,
                                (const double*)d_u, (const double*)c->d_partials, nparts,
                                c->d_partials + 4 * nparts, c->d_sc);
         }
-        probe(c, 2);
         RhsArgs pa = a;
         pa.st_kind = ST_NONE;
         if (parts & RHS_FLUX) launch_flux_v<4>(c, prec, pa, hv, true, grid);
-        probe(c, 3);
         if (parts & RHS_TRANSPOSE) {
             if (prec == MF_PREC_F64)
                 launch_transpose<0>(c, a, grid);
@@ -1110,22 +1259,17 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
             else
                 launch_transpose<2>(c, a, grid);
         }
-        probe(c, 4);
     } else {
-        probe(c, 2);
         if (parts & RHS_FLUX) {
             if (c->nvar == 4)
                 launch_flux_v<4>(c, prec, a, hv, false, grid);
             else
                 launch_flux_v<1>(c, prec, a, hv, false, grid);
         }
-        probe(c, 3);
-        probe(c, 4);
     }
     MF_HIP(c, hipGetLastError());
     // second boundary pass on u: only the slip projection changes anything
     rc = (parts & RHS_BC) ? bc_state_pass(c, d_u, dt_off, 0, true) : MF_OK;
-    probe(c, 5);
     return rc;
 }
 

@@ -46,6 +46,94 @@ __global__ void k_bbox(int64_t N, const double* x, const double* y, double* out)
         for (int q = 0; q < 4; ++q) out[q] = r[q][0];
 }
 
+// ========================================================== node order
+// Internal node order: the Morton (Z-order) code of each position,
+// quantised to 16 bits per axis over the bounding square, ties kept in the
+// caller's order (the radix sort is stable).  Stencil neighbours then sit
+// close together in memory, so a block's neighbour gathers touch few cache
+// lines.  The order changes no result: kNN ties and the reference-order
+// sorts compare the caller's indices.
+__device__ __forceinline__ uint32_t spread16(uint32_t v)
+{
+    v &= 0xFFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+__global__ void k_morton(int64_t N, const double* x, const double* y, const double* box, uint32_t* code,
+                         int32_t* iota)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double ext = fmax(box[1] - box[0], box[3] - box[2]);
+    if (!(ext > 0.0)) ext = 1.0;
+    const double fx = (x[i] - box[0]) / ext * 65536.0, fy = (y[i] - box[2]) / ext * 65536.0;
+    const uint32_t qx = (uint32_t)fmin(fmax(fx, 0.0), 65535.0);
+    const uint32_t qy = (uint32_t)fmin(fmax(fy, 0.0), 65535.0);
+    code[i] = spread16(qx) | (spread16(qy) << 1);
+    iota[i] = (int32_t)i;
+}
+
+__global__ void k_iota(int64_t N, int32_t* v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) v[i] = (int32_t)i;
+}
+
+__global__ void k_apply_order(int64_t N, const int32_t* uid, const double* xu, const double* yu, double* x,
+                              double* y, int32_t* inv)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const int32_t u = uid[p];
+    x[p] = xu[u];
+    y[p] = yu[u];
+    inv[u] = (int32_t)p;
+}
+
+int launch_node_order(mf_ctx* c, const double* d_xu, const double* d_yu)
+{
+    const int64_t N = c->N;
+    const unsigned g = (unsigned)((N + 255) / 256);
+    int rc;
+    if (c->node_order == 1) {
+        hipLaunchKernelGGL(k_iota, dim3(g), dim3(256), 0, c->stream, N, c->d_uid);
+    } else {
+        double* box = nullptr;
+        int32_t *code = nullptr, *code_s = nullptr, *iota = nullptr;
+        if ((rc = dalloc(c, &box, 4)) || (rc = dalloc(c, &code, N)) || (rc = dalloc(c, &code_s, N)) ||
+            (rc = dalloc(c, &iota, N)))
+            return rc;
+        uint32_t* ucode = reinterpret_cast<uint32_t*>(code);
+        uint32_t* ucode_s = reinterpret_cast<uint32_t*>(code_s);
+        hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, c->stream, N, d_xu, d_yu, box);
+        hipLaunchKernelGGL(k_morton, dim3(g), dim3(256), 0, c->stream, N, d_xu, d_yu, box, ucode, iota);
+        size_t tb = 0;
+        MF_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, ucode, ucode_s, iota, c->d_uid, (int)N, 0, 32,
+                                                     c->stream));
+        void* tmp = nullptr;
+        MF_HIP(c, hipMalloc(&tmp, tb + 16));
+        MF_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, ucode, ucode_s, iota, c->d_uid, (int)N, 0, 32,
+                                                     c->stream));
+        MF_HIP(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(tmp);
+        dfree(box);
+        dfree(code);
+        dfree(code_s);
+        dfree(iota);
+    }
+    hipLaunchKernelGGL(k_apply_order, dim3(g), dim3(256), 0, c->stream, N, c->d_uid, d_xu, d_yu, c->d_x, c->d_y,
+                       c->d_inv);
+    MF_HIP(c, hipGetLastError());
+    c->h_inv.resize(N);
+    MF_HIP(c, hipMemcpyAsync(c->h_inv.data(), c->d_inv, sizeof(int32_t) * N, hipMemcpyDeviceToHost, c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    return MF_OK;
+}
+
 struct Grid {
     double x0, y0, cs;
     int64_t ncx, ncy;
@@ -119,8 +207,10 @@ struct KnnArgs {
     const int32_t* count;
     const int32_t* order;
     const double2* xys;
-    int32_t* out_idx;   // slot-major
-    double* out_d2;     // slot-major
+    const int32_t* uid;  // internal -> caller index (ties compare caller indices)
+    const int32_t* inv;  // caller -> internal index
+    int32_t* out_idx;    // slot-major, internal indices
+    double* out_d2;      // slot-major
 };
 
 __global__ __launch_bounds__(64) void k_knn_query(const KnnArgs a)
@@ -152,7 +242,7 @@ __global__ __launch_bounds__(64) void k_knn_query(const KnnArgs a)
                     const double2 cp = a.xys[pp];
                     const double dx = qx - cp.x, dy = qy - cp.y;
                     const double d2 = dx * dx + dy * dy;
-                    const int32_t j = a.order[pp];
+                    const int32_t j = a.uid[a.order[pp]];
                     if (cnt == K && !knn_before(d2, j, bd[(K - 1) * 64 + lane], bi[(K - 1) * 64 + lane]))
                         continue;
                     int pos = (cnt < K) ? cnt : K - 1;
@@ -180,7 +270,7 @@ __global__ __launch_bounds__(64) void k_knn_query(const KnnArgs a)
         }
     }
     for (int s = 0; s < K; ++s) {
-        a.out_idx[(int64_t)s * a.N + q] = bi[s * 64 + lane];
+        a.out_idx[(int64_t)s * a.N + q] = a.inv[bi[s * 64 + lane]];
         if (a.out_d2) a.out_d2[(int64_t)s * a.N + q] = bd[s * 64 + lane];
     }
 }
@@ -233,6 +323,8 @@ int launch_knn(mf_ctx* c, int k, double* d_d2)
     a.count = count;
     a.order = order;
     a.xys = xys;
+    a.uid = c->d_uid;
+    a.inv = c->d_inv;
     a.out_idx = c->d_idx;
     a.out_d2 = d_d2;
     const size_t lds = (size_t)64 * k * (sizeof(double) + sizeof(int32_t));
@@ -724,7 +816,8 @@ int launch_pack_ops(mf_ctx* c, int target, const double* d_wx, const double* d_w
 // Column i of D lists the rows r whose stencil contains i; stored as
 // SELL-64 (slices of 64 consecutive columns, padded to the slice's
 // largest count, entry e of column i at off[slice] + e*64 + i%64), each
-// column sorted by row like the reference's CSC (compute_operators.jl:443-452).
+// column sorted by the caller's row index like the reference's CSC
+// (compute_operators.jl:443-452).
 __global__ void k_t_count(int64_t N, int K, const int32_t* idx, int32_t* deg)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -759,7 +852,7 @@ __global__ void k_t_fill(int64_t N, int K, const int32_t* idx, const double2* wx
 }
 
 __global__ void k_t_sort(int64_t N, const int32_t* deg, const int32_t* w, const int64_t* off,
-                         int32_t* trow, double2* tw, float2* tw32)
+                         const int32_t* uid, int32_t* trow, double2* tw, float2* tw32)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -767,9 +860,10 @@ __global__ void k_t_sort(int64_t N, const int32_t* deg, const int32_t* w, const 
     const int n = deg[i];
     for (int a2 = 1; a2 < n; ++a2) {
         const int32_t rv = trow[base + (int64_t)a2 * kSellC];
+        const int32_t rk = uid[rv];
         const double2 wv = tw[base + (int64_t)a2 * kSellC];
         int b = a2 - 1;
-        while (b >= 0 && trow[base + (int64_t)b * kSellC] > rv) {
+        while (b >= 0 && uid[trow[base + (int64_t)b * kSellC]] > rk) {
             trow[base + (int64_t)(b + 1) * kSellC] = trow[base + (int64_t)b * kSellC];
             tw[base + (int64_t)(b + 1) * kSellC] = tw[base + (int64_t)b * kSellC];
             --b;
@@ -829,7 +923,7 @@ int launch_build_transpose(mf_ctx* c)
     hipLaunchKernelGGL(k_t_fill, dim3(gq), dim3(256), 0, c->stream, N, (int)K, c->d_idx, c->d_wxy,
                        c->d_sell_off, cursor, c->d_trow, c->d_tw);
     hipLaunchKernelGGL(k_t_sort, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c->stream, N, deg,
-                       c->d_sell_w, c->d_sell_off, c->d_trow, c->d_tw, c->d_tw32);
+                       c->d_sell_w, c->d_sell_off, c->d_uid, c->d_trow, c->d_tw, c->d_tw32);
     MF_HIP(c, hipGetLastError());
     MF_HIP(c, hipStreamSynchronize(c->stream));
     dfree(deg);
@@ -873,16 +967,18 @@ int launch_build_didx(mf_ctx* c)
 }
 
 // ------------------------------------------ column-sorted stencil copy
-// Sorted slot order of every node's stencil (ascending column index), the
-// order in which the reference's CSC products accumulate a row.
-__global__ void k_sort_stencil(int64_t N, int K, const int32_t* idx, int32_t* idx_s, uint8_t* perm)
+// Sorted slot order of every node's stencil (ascending column index in
+// the caller's numbering), the order in which the reference's CSC products
+// accumulate a row.
+__global__ void k_sort_stencil(int64_t N, int K, const int32_t* idx, const int32_t* uid, int32_t* idx_s,
+                               uint8_t* perm)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     int32_t key[kMaxStencilM];
     uint8_t pos[kMaxStencilM];
     for (int s = 0; s < K; ++s) {
-        key[s] = idx[(int64_t)s * N + i];
+        key[s] = uid[idx[(int64_t)s * N + i]];
         pos[s] = (uint8_t)s;
     }
     for (int a2 = 1; a2 < K; ++a2) {
@@ -898,7 +994,7 @@ __global__ void k_sort_stencil(int64_t N, int K, const int32_t* idx, int32_t* id
         pos[b + 1] = pv;
     }
     for (int s = 0; s < K; ++s) {
-        idx_s[(int64_t)s * N + i] = key[s];
+        idx_s[(int64_t)s * N + i] = idx[(int64_t)pos[s] * N + i];
         perm[(int64_t)s * N + i] = pos[s];
     }
 }
@@ -925,7 +1021,7 @@ int launch_sort_stencil(mf_ctx* c)
 {
     const int64_t N = c->N;
     hipLaunchKernelGGL(k_sort_stencil, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c->stream, N, c->k,
-                       c->d_idx, c->d_idx_s, c->d_perm_s);
+                       c->d_idx, c->d_uid, c->d_idx_s, c->d_perm_s);
     MF_HIP(c, hipGetLastError());
     return MF_OK;
 }
@@ -949,78 +1045,89 @@ int launch_pack_sorted(mf_ctx* c, int target, const double* d_wx, const double* 
 }
 
 // ------------------------------------------------- layout conversions
-template <class T>
-__global__ void k_ell_to_rows(int64_t N, int K, const T* ell, T* rows)
+// Between the caller's row-major arrays (node u, entry s at u*K + s; field
+// v of node u at v*N + u) and the internal layouts (slot-major ELL, node-
+// major state), with the node order applied; MAP also maps index values.
+template <class T, bool MAP>
+__global__ void k_ell_to_rows(int64_t N, int K, const T* ell, T* rows, const int32_t* inv, const int32_t* uid)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= N * K) return;
-    const int64_t i = q / K, s = q % K;
-    rows[q] = ell[s * N + i];
+    const int64_t u = q / K, s = q % K;
+    const T v = ell[s * N + inv[u]];
+    if constexpr (MAP)
+        rows[q] = uid[v];
+    else
+        rows[q] = v;
 }
 
-template <class T>
-__global__ void k_rows_to_ell(int64_t N, int K, const T* rows, T* ell)
+template <class T, bool MAP>
+__global__ void k_rows_to_ell(int64_t N, int K, const T* rows, T* ell, const int32_t* uid, const int32_t* inv)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= N * K) return;
-    const int64_t i = q / K, s = q % K;
-    ell[s * N + i] = rows[q];
+    const int64_t s = q / N, p = q % N;
+    const T v = rows[(int64_t)uid[p] * K + s];
+    if constexpr (MAP)
+        ell[q] = inv[v];
+    else
+        ell[q] = v;
 }
 
 int launch_ell_to_rows(mf_ctx* c, const int32_t* d_ell, int32_t* d_rows, int k)
 {
     const int64_t n = c->N * k;
-    hipLaunchKernelGGL(k_ell_to_rows<int32_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                       c->N, k, d_ell, d_rows);
+    hipLaunchKernelGGL((k_ell_to_rows<int32_t, true>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       c->stream, c->N, k, d_ell, d_rows, c->d_inv, c->d_uid);
     MF_HIP(c, hipGetLastError());
     return MF_OK;
 }
 int launch_ell_to_rows_d(mf_ctx* c, const double* d_ell, double* d_rows, int k)
 {
     const int64_t n = c->N * k;
-    hipLaunchKernelGGL(k_ell_to_rows<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                       c->N, k, d_ell, d_rows);
+    hipLaunchKernelGGL((k_ell_to_rows<double, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       c->stream, c->N, k, d_ell, d_rows, c->d_inv, c->d_uid);
     MF_HIP(c, hipGetLastError());
     return MF_OK;
 }
 int launch_rows_to_ell(mf_ctx* c, const int32_t* d_rows, int32_t* d_ell, int k)
 {
     const int64_t n = c->N * k;
-    hipLaunchKernelGGL(k_rows_to_ell<int32_t>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                       c->N, k, d_rows, d_ell);
+    hipLaunchKernelGGL((k_rows_to_ell<int32_t, true>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       c->stream, c->N, k, d_rows, d_ell, c->d_uid, c->d_inv);
     MF_HIP(c, hipGetLastError());
     return MF_OK;
 }
 int launch_rows_to_ell_d(mf_ctx* c, const double* d_rows, double* d_ell, int k)
 {
     const int64_t n = c->N * k;
-    hipLaunchKernelGGL(k_rows_to_ell<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                       c->N, k, d_rows, d_ell);
+    hipLaunchKernelGGL((k_rows_to_ell<double, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       c->stream, c->N, k, d_rows, d_ell, c->d_uid, c->d_inv);
     MF_HIP(c, hipGetLastError());
     return MF_OK;
 }
 
-__global__ void k_soa_to_aos(int64_t N, int V, const double* soa, double* aos)
+__global__ void k_soa_to_aos(int64_t N, int V, const double* soa, double* aos, const int32_t* uid)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= N * V) return;
-    const int64_t i = q / V, v = q % V;
-    aos[q] = soa[v * N + i];
+    const int64_t p = q / V, v = q % V;
+    aos[q] = soa[v * N + uid[p]];
 }
 
-__global__ void k_aos_to_soa(int64_t N, int V, const double* aos, double* soa)
+__global__ void k_aos_to_soa(int64_t N, int V, const double* aos, double* soa, const int32_t* inv)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= N * V) return;
-    const int64_t i = q / V, v = q % V;
-    soa[v * N + i] = aos[q];
+    const int64_t v = q / N, u = q % N;
+    soa[q] = aos[(int64_t)inv[u] * V + v];
 }
 
 int launch_soa_to_aos(mf_ctx* c, const double* d_soa, double* d_aos)
 {
     const int64_t n = c->N * c->nvar;
     hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->N,
-                       c->nvar, d_soa, d_aos);
+                       c->nvar, d_soa, d_aos, c->d_uid);
     MF_HIP(c, hipGetLastError());
     return MF_OK;
 }
@@ -1028,7 +1135,30 @@ int launch_aos_to_soa(mf_ctx* c, const double* d_aos, double* d_soa)
 {
     const int64_t n = c->N * c->nvar;
     hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->N,
-                       c->nvar, d_aos, d_soa);
+                       c->nvar, d_aos, d_soa, c->d_inv);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+// per-node arrays back to the caller's order: dst[u] = src[inv[u]]
+template <class T>
+__global__ void k_gather_nodes(int64_t N, const int32_t* inv, const T* src, T* dst)
+{
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < N) dst[u] = src[inv[u]];
+}
+
+int launch_gather_nodes_d(mf_ctx* c, const double* d_src, double* d_dst)
+{
+    hipLaunchKernelGGL(k_gather_nodes<double>, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream,
+                       c->N, c->d_inv, d_src, d_dst);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+int launch_gather_nodes_i8(mf_ctx* c, const int8_t* d_src, int8_t* d_dst)
+{
+    hipLaunchKernelGGL(k_gather_nodes<int8_t>, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream,
+                       c->N, c->d_inv, d_src, d_dst);
     MF_HIP(c, hipGetLastError());
     return MF_OK;
 }

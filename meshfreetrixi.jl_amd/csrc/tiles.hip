@@ -1,0 +1,243 @@
+// LDS tiles of the stencil operators (built once per stencil / operator).
+//
+// The neighbour gathers of the rhs! kernels touch one cache line per lane
+// per load: a wavefront's 64 lanes read 64 scattered 32- or 64-byte node
+// records.  With the nodes in Morton order, the 256 nodes of a workgroup
+// form a compact patch whose stencils reach only ~2x as many distinct
+// nodes.  Each workgroup therefore loads that halo once into LDS (runs of
+// consecutive node records) and the stencil loops read LDS through 16-bit
+// halo positions stored in place of the 32-bit node indices.
+//
+// Phase A (flux divergence, sources, viscosity phase A): halo = the union
+// of the tile's stencils; positions slot-major like the node indices.
+// Phase B (transposed gather D^T G): rows of the tile sorted by their
+// transposed degree (long rows first) so that the 64 rows of a wavefront
+// have similar lengths, stored SELL-64 per wavefront; entries of a row in
+// ascending caller index (the reference's CSC order); halo = the union of
+// the rows' entries.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "mf_device.h"
+
+namespace mf {
+
+constexpr int kTile = 256;
+constexpr int kSlicesPerTile = kTile / kSellC;  // 4
+constexpr size_t kTileLdsBytes = 48 * 1024;     // largest halo staged in LDS
+
+void free_tiles_b(mf_ctx* c)
+{
+    dfree(c->d_hb_off);
+    dfree(c->d_hb_ids);
+    dfree(c->d_sb_off);
+    dfree(c->d_sb_w);
+    dfree(c->d_rowb);
+    dfree(c->d_lb);
+    dfree(c->d_twb);
+    dfree(c->d_twb32);
+    c->d_hb_off = c->d_sb_off = nullptr;
+    c->d_hb_ids = c->d_sb_w = nullptr;
+    c->d_lb = nullptr;
+    c->d_rowb = nullptr;
+    c->d_twb = nullptr;
+    c->d_twb32 = nullptr;
+    c->tiles_b = false;
+    c->tried_b = false;
+    c->hb_max = 0;
+}
+
+void free_tiles(mf_ctx* c)
+{
+    free_tiles_b(c);
+    dfree(c->d_ha_off);
+    dfree(c->d_ha_ids);
+    dfree(c->d_la);
+    c->d_ha_off = nullptr;
+    c->d_ha_ids = nullptr;
+    c->d_la = nullptr;
+    c->tiles_a = false;
+    c->tried_a = false;
+    c->n_tiles = 0;
+    c->ha_max = 0;
+}
+
+// (Dx, Dy) of each transposed entry from the forward operator: src[e] is
+// the slot-major position s*N + j of the entry, or -1 for padding.
+__global__ void k_fill_tile_weights(int64_t n, const int64_t* src, const double2* wxy, double2* tw, float2* tw32)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int64_t q = src[e];
+    const double2 w = q >= 0 ? wxy[q] : make_double2(0.0, 0.0);
+    tw[e] = w;
+    tw32[e] = make_float2((float)w.x, (float)w.y);
+}
+
+template <class T>
+static int upload(mf_ctx* c, T** dst, const std::vector<T>& v)
+{
+    int rc = dalloc(c, dst, v.size());
+    if (rc) return rc;
+    if (!v.empty())
+        MF_HIP(c, hipMemcpyAsync(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream));
+    return MF_OK;
+}
+
+int build_tiles(mf_ctx* c, bool want_b)
+{
+    const int64_t N = c->N;
+    const int K = c->k;
+    const int V = c->nvar;
+    const int64_t nt = (N + kTile - 1) / kTile;
+    int rc;
+    std::vector<int32_t> idx((size_t)N * K);
+    MF_HIP(c, hipMemcpyAsync(idx.data(), c->d_idx, sizeof(int32_t) * idx.size(), hipMemcpyDeviceToHost,
+                             c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<int32_t> mark(N, -1);
+    std::vector<int32_t> ids;
+    ids.reserve(4 * kTile);
+
+    if (!c->tried_a) {
+        free_tiles(c);
+        c->tried_a = true;
+        std::vector<int64_t> off(nt + 1, 0);
+        std::vector<int32_t> all;
+        all.reserve((size_t)N * 2);
+        std::vector<uint16_t> la((size_t)N * K);
+        int hmax = 0;
+        bool fits = true;
+        for (int64_t t = 0; t < nt && fits; ++t) {
+            const int64_t i0 = t * kTile, i1 = std::min<int64_t>(N, i0 + kTile);
+            ids.clear();
+            for (int s = 0; s < K; ++s)
+                for (int64_t i = i0; i < i1; ++i) {
+                    const int32_t j = idx[(size_t)s * N + i];
+                    if (mark[j] < 0) {
+                        mark[j] = 0;
+                        ids.push_back(j);
+                    }
+                }
+            std::sort(ids.begin(), ids.end());
+            if (ids.size() > 65535 || ids.size() * V * sizeof(double) > kTileLdsBytes) fits = false;
+            for (size_t q = 0; q < ids.size(); ++q) mark[ids[q]] = (int32_t)q;
+            for (int s = 0; s < K; ++s)
+                for (int64_t i = i0; i < i1; ++i) la[(size_t)s * N + i] = (uint16_t)mark[idx[(size_t)s * N + i]];
+            for (int32_t j : ids) mark[j] = -1;
+            hmax = std::max<int>(hmax, (int)ids.size());
+            all.insert(all.end(), ids.begin(), ids.end());
+            off[t + 1] = (int64_t)all.size();
+        }
+        if (!fits) return MF_OK;  // global-memory kernels
+        if ((rc = upload(c, &c->d_ha_off, off)) || (rc = upload(c, &c->d_ha_ids, all)) ||
+            (rc = upload(c, &c->d_la, la)))
+            return rc;
+        MF_HIP(c, hipStreamSynchronize(c->stream));
+        c->n_tiles = nt;
+        c->ha_max = hmax;
+        c->tiles_a = true;
+    }
+    if (!want_b || !c->tiles_a || c->tried_b) return MF_OK;
+    free_tiles_b(c);
+    c->tried_b = true;
+
+    // transposed adjacency: the entries (j, s) with idx[s][j] = i, per row i,
+    // in ascending caller index of j
+    std::vector<int32_t> uid(N);
+    for (int64_t u = 0; u < N; ++u) uid[c->h_inv[u]] = (int32_t)u;
+    std::vector<int64_t> rp(N + 1, 0);
+    for (size_t q = 0; q < idx.size(); ++q) ++rp[idx[q] + 1];
+    for (int64_t i = 0; i < N; ++i) rp[i + 1] += rp[i];
+    std::vector<int64_t> ent(idx.size());  // slot-major position s*N + j
+    {
+        std::vector<int64_t> cur(rp.begin(), rp.end() - 1);
+        for (int s = 0; s < K; ++s)
+            for (int64_t j = 0; j < N; ++j) {
+                const int32_t i = idx[(size_t)s * N + j];
+                ent[cur[i]++] = (int64_t)s * N + j;
+            }
+    }
+    for (int64_t i = 0; i < N; ++i)
+        std::sort(ent.begin() + rp[i], ent.begin() + rp[i + 1], [&](int64_t a, int64_t b) {
+            const int32_t ua = uid[a % N], ub = uid[b % N];
+            return ua < ub || (ua == ub && a < b);
+        });
+    std::vector<int64_t> hoff(nt + 1, 0), soff(nt * kSlicesPerTile + 1, 0);
+    std::vector<int32_t> hall, sw(nt * kSlicesPerTile, 0);
+    std::vector<uint8_t> rowb((size_t)nt * kTile, 0);
+    std::vector<uint16_t> lb;
+    std::vector<int64_t> src;
+    int hmax = 0;
+    for (int64_t t = 0; t < nt; ++t) {
+        const int64_t i0 = t * kTile, i1 = std::min<int64_t>(N, i0 + kTile);
+        ids.clear();
+        for (int64_t i = i0; i < i1; ++i)
+            for (int64_t e = rp[i]; e < rp[i + 1]; ++e) {
+                const int32_t j = (int32_t)(ent[e] % N);
+                if (mark[j] < 0) {
+                    mark[j] = 0;
+                    ids.push_back(j);
+                }
+            }
+        std::sort(ids.begin(), ids.end());
+        if (ids.size() > 65535 || ids.size() * 2 * V * sizeof(double) > kTileLdsBytes) {
+            for (int32_t j : ids) mark[j] = -1;
+            return MF_OK;  // global-memory transposed gather
+        }
+        for (size_t q = 0; q < ids.size(); ++q) mark[ids[q]] = (int32_t)q;
+        // rows by degree, longest first (rows past N have degree 0)
+        int order[kTile];
+        int deg[kTile];
+        for (int r = 0; r < kTile; ++r) {
+            order[r] = r;
+            deg[r] = (i0 + r < i1) ? (int)(rp[i0 + r + 1] - rp[i0 + r]) : 0;
+        }
+        std::stable_sort(order, order + kTile, [&](int a, int b) { return deg[a] > deg[b]; });
+        for (int p = 0; p < kTile; ++p) rowb[(size_t)t * kTile + p] = (uint8_t)order[p];
+        for (int w = 0; w < kSlicesPerTile; ++w) {
+            int W = 0;
+            for (int l = 0; l < kSellC; ++l) W = std::max(W, deg[order[w * kSellC + l]]);
+            const int64_t sl = t * kSlicesPerTile + w;
+            sw[sl] = W;
+            const size_t base = lb.size();
+            lb.resize(base + (size_t)W * kSellC, 0);
+            src.resize(base + (size_t)W * kSellC, -1);
+            for (int l = 0; l < kSellC; ++l) {
+                const int r = order[w * kSellC + l];
+                if (i0 + r >= i1) continue;
+                const int64_t i = i0 + r;
+                for (int64_t e = rp[i]; e < rp[i + 1]; ++e) {
+                    const size_t at = base + (size_t)(e - rp[i]) * kSellC + l;
+                    lb[at] = (uint16_t)mark[ent[e] % N];
+                    src[at] = ent[e];
+                }
+            }
+            soff[sl + 1] = (int64_t)lb.size();
+        }
+        for (int32_t j : ids) mark[j] = -1;
+        hmax = std::max<int>(hmax, (int)ids.size());
+        hall.insert(hall.end(), ids.begin(), ids.end());
+        hoff[t + 1] = (int64_t)hall.size();
+    }
+    int64_t* d_src = nullptr;
+    if ((rc = upload(c, &c->d_hb_off, hoff)) || (rc = upload(c, &c->d_hb_ids, hall)) ||
+        (rc = upload(c, &c->d_sb_off, soff)) || (rc = upload(c, &c->d_sb_w, sw)) ||
+        (rc = upload(c, &c->d_rowb, rowb)) || (rc = upload(c, &c->d_lb, lb)) || (rc = upload(c, &d_src, src)) ||
+        (rc = dalloc(c, &c->d_twb, lb.size())) || (rc = dalloc(c, &c->d_twb32, lb.size())))
+        return rc;
+    const int64_t ne = (int64_t)lb.size();
+    hipLaunchKernelGGL(k_fill_tile_weights, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, c->stream, ne, d_src,
+                       c->d_wxy, c->d_twb, c->d_twb32);
+    MF_HIP(c, hipGetLastError());
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(d_src);
+    c->hb_max = hmax;
+    c->tiles_b = true;
+    return MF_OK;
+}
+
+}  // namespace mf

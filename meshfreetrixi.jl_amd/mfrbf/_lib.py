@@ -23,7 +23,7 @@ MF_PREC_F64, MF_PREC_SPLIT32, MF_PREC_F64_REF = 0, 1, 2
 MF_BUF_U, MF_BUF_DU, MF_BUF_AUX0, MF_BUF_AUX1 = 0, 1, 2, 3
 MF_VAR_DENSITY, MF_VAR_PRESSURE = 0, 1
 MF_STEP_GRAPH, MF_STEP_LIMITER, MF_STEP_HISTORY = 1, 2, 4
-MF_TUNE_FLUX_BATCH, MF_TUNE_TRANSPOSE_BATCH = 0, 1
+MF_TUNE_FLUX_BATCH, MF_TUNE_TRANSPOSE_BATCH, MF_TUNE_NODE_ORDER, MF_TUNE_TILES = 0, 1, 2, 3
 
 
 class MfError(RuntimeError):
@@ -71,6 +71,7 @@ _SIGS = {
     "mf_ssprk33_vjp": [_ctx, C.c_double, C.c_double, C.c_int, C.c_int, _dp],
     "mf_time_rhs": [_ctx, C.c_double, C.c_int, C.c_int, _dp, _dp],
     "mf_set_tuning": [_ctx, C.c_int, C.c_int, C.c_int],
+    "mf_node_order": [_ctx, _i32p],
 }
 
 _lib = None
@@ -276,6 +277,12 @@ class Context:
 
     def set_tuning(self, key, value, precision=-1):
         self._chk(self._L.mf_set_tuning(self.h, key, precision, value))
+
+    def node_order(self):
+        """order[p] = caller index of internal node p."""
+        out = np.zeros(self.N, np.int32)
+        self._chk(self._L.mf_node_order(self.h, i32ptr(out)))
+        return out
 
     def time_rhs(self, t, precision, n_calls):
         """(ms per rhs!, [ms of stencil kernel, transposed gather, reductions, boundary])."""

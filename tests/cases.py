@@ -12,7 +12,7 @@ class Case:
 
 
 def build(spec, side=None, use_gpu=True, weights="oracle", hv=None, visc=None, bc=("dirichlet", "nothing"),
-          knn="grid"):
+          knn="grid", node_order=0):
     """weights: 'oracle' installs the oracle's weights on the GPU (so the
     rhs! comparison isolates the stencil kernels), 'gpu' keeps the GPU's own."""
     c = Case()
@@ -76,6 +76,7 @@ def build(spec, side=None, use_gpu=True, weights="oracle", hv=None, visc=None, b
         ctx.set_equations(L.MF_EQ_EULER_2D, spec.gamma)
     else:
         ctx.set_equations(L.MF_EQ_ADVECTION_2D, spec.adv[0], spec.adv[1])
+    ctx.set_tuning(L.MF_TUNE_NODE_ORDER, node_order)
     ctx.set_points(x, y)
     ctx.set_stencil(idx)
     if weights == "oracle":

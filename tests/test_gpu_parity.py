@@ -314,3 +314,92 @@ def test_rhs_generic_stencil_size(mf, k):
     for prec, tol in ((L.MF_PREC_F64, TOL64), (L.MF_PREC_F64_REF, TOL64), (L.MF_PREC_SPLIT32, TOL32)):
         u, du, uo, duo, _, _ = _rhs_pair(c, prec=prec, t=0.001)
         assert rel_inf(du, duo) < tol, prec
+
+
+# ------------------------------------------------------------ node order
+def test_node_order_changes_no_result(mf):
+    """The context stores nodes in Morton order; every result in the
+    caller's order must equal the one computed in the caller's own order
+    (kNN, weights, rhs! in all precisions, viscosity, step, adjoint)."""
+    _say("node order: Morton vs caller order")
+    from mfrbf.clouds import cloud_square
+    x, y, _, _, _ = cloud_square(64, 20261207)
+    res = {}
+    for order in (0, 1):
+        ctx = L.Context(0)
+        ctx.set_tuning(L.MF_TUNE_NODE_ORDER, order)
+        ctx.set_points(x, y)
+        idx, dist, dmin, davg = ctx.knn(30, want_dist=True)
+        perm = ctx.node_order()
+        assert sorted(perm.tolist()) == list(range(len(x)))
+        if order == 1:
+            assert np.array_equal(perm, np.arange(len(x)))
+        wx, wy = ctx.compute_operator(5, 3, 1, L.MF_OP_FLUX, want=True, k=30)
+        res[order] = dict(idx=idx, dist=dist, dmin=dmin, wx=wx, wy=wy, perm=perm)
+    a, b = res[0], res[1]
+    assert not np.array_equal(a["perm"], b["perm"])
+    for key in ("idx", "dist", "wx", "wy"):
+        assert np.array_equal(a[key], b[key]), key
+    assert a["dmin"] == b["dmin"]
+    outs = {}
+    for order in (0, 1):
+        c = cases.build(configs.c2(), side=48, hv=(2, 1e-4), visc="upwind", bc=("dirichlet", "nothing"),
+                        node_order=order)
+        o = {}
+        for prec in (L.MF_PREC_F64, L.MF_PREC_SPLIT32, L.MF_PREC_F64_REF):
+            u = c.u0.copy()
+            du = np.zeros_like(u)
+            c.ctx.rhs_host(0.01, u, du, prec)
+            o[prec] = (u, du)
+        o["eps"] = c.ctx.viscosity()
+        c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+        c.ctx.ssprk33(0.0, c.dt, 3, flags=L.MF_STEP_GRAPH)
+        o["step"] = c.ctx.download(L.MF_BUF_U)
+        # the adjoint covers the flux divergence and boundary conditions
+        c = cases.build(configs.c2(), side=48, node_order=order)
+        lam = np.random.default_rng(2).standard_normal(c.u0.shape)
+        c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+        c.ctx.upload(L.MF_BUF_AUX0, lam)
+        c.ctx.rhs_vjp(0.0)
+        o["vjp"] = c.ctx.download(L.MF_BUF_AUX1)
+        outs[order] = o
+    for key in outs[0]:
+        va, vb = outs[0][key], outs[1][key]
+        for p, q in zip(va if isinstance(va, tuple) else (va,), vb if isinstance(vb, tuple) else (vb,)):
+            assert np.array_equal(p, q), key
+
+
+# ------------------------------------------------------------- LDS tiles
+@pytest.mark.parametrize("cfg", ["c1", "euler_rv_hv", "euler_rv_generic_k"])
+def test_lds_tiles_change_no_result(mf, cfg):
+    """Staging the stencil halos in LDS keeps every node's operation order:
+    rhs! and SSPRK33 steps give the same bits with and without tiles."""
+    _say("LDS tiles on/off %s" % cfg)
+    outs = {}
+    for tiles in (1, 0):
+        if cfg == "c1":
+            c = cases.build(configs.c1(), side=48)
+        elif cfg == "euler_rv_hv":
+            c = cases.build(configs.c2(), side=56, visc="residual", hv=(2, 1e-4))
+        else:
+            spec = configs.c2()
+            spec.k, spec.p, spec.deg = 25, 3, 2
+            c = cases.build(spec, side=40, visc="residual")
+        c.ctx.set_tuning(L.MF_TUNE_TILES, tiles)
+        o = {}
+        if c.visc == "residual":
+            c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+            c.ctx.history_update(0.0, 0)
+            c.ctx.upload(L.MF_BUF_U, np.ascontiguousarray(c.u0 * 1.0001))
+            c.ctx.history_update(0.001, 1)
+        for prec in (L.MF_PREC_F64, L.MF_PREC_SPLIT32, L.MF_PREC_F64_REF):
+            u = c.u0.copy()
+            du = np.zeros_like(u)
+            c.ctx.rhs_host(0.001, u, du, prec)
+            o[prec] = du
+        c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+        c.ctx.ssprk33(0.0, c.dt, 3, flags=L.MF_STEP_GRAPH | L.MF_STEP_HISTORY)
+        o["step"] = c.ctx.download(L.MF_BUF_U)
+        outs[tiles] = o
+    for key in outs[0]:
+        assert np.array_equal(outs[0][key], outs[1][key]), key

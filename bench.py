@@ -76,17 +76,29 @@ def rhs_bytes(N, k, V=4, visc=True):
     return flux
 
 
-def kernel_bytes(N, k, V=4):
+def kernel_bytes(N, k, ti, V=4):
     """Bytes the stencil kernel (phase A) and the transposed gather (phase B)
-    must move: phase A reads u, idx, (Dx, Dy), approx_du and writes du, G,
-    eps, eps code; phase B reads the transposed (row, Dx, Dy), G, du and
-    writes du."""
-    a = N * (V * 8 + k * 4 + k * 16 + V * 8 + V * 8 + 2 * V * 8 + 9)
-    b = N * (k * 4 + k * 16 + 2 * V * 8 + V * 8 + V * 8 + 1)
+    must move in the format they use, each array once (gathers of the halo
+    not counted).  Phase A reads u, the stencil positions, (Dx, Dy) and
+    approx_du and writes du, G, eps and its code; phase B reads the
+    transposed positions and (Dx, Dy) (N k entries, padding not counted), G
+    and du and writes du.  With LDS tiles the positions are 16-bit and each
+    tile's halo list (int32) and offsets are read once; without, the
+    positions are int32 node indices."""
+    if ti["tiles_a"]:
+        a = N * (V * 8 + 2 * k + 16 * k + V * 8 + V * 8 + 2 * V * 8 + 9) + 4 * ti["halo_a_total"] \
+            + 8 * (ti["n_tiles"] + 1)
+    else:
+        a = N * (V * 8 + 4 * k + 16 * k + V * 8 + V * 8 + 2 * V * 8 + 9)
+    if ti["tiles_b"]:
+        b = N * k * (2 + 16) + N * (2 * V * 8 + V * 8 + V * 8 + 1) + 4 * ti["halo_b_total"] \
+            + ti["n_tiles"] * (256 + 4 * (8 + 4) + 8)
+    else:
+        b = N * (4 * k + 16 * k + 2 * V * 8 + V * 8 + V * 8 + 1)
     return a, b
 
 
-def measured_traffic(kernel, tuning, side):
+def measured_traffic(kernel, tuning, side, tiled):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (FETCH_SIZE
     scaled by the copy-kernel calibration recorded beside it), or None."""
@@ -95,13 +107,15 @@ def measured_traffic(kernel, tuning, side):
         return None, None
     with open(path) as f:
         tab = json.load(f)
+    tl = "true" if tiled else "false"
     if kernel == "transpose":
-        want = "k_rhs_transpose<4, 0, %d>" % (tuning["f64"]["transpose"] or 4)
+        want = "k_rhs_transpose<4, 0, %d, %s>" % (tuning["f64"]["transpose"] or 4, tl)
         family = "k_rhs_transpose<4, 0,"
     else:
-        want = "k_rhs_f64<4, 30, false, true, %d>" % (tuning["f64"]["flux"] or 5)
+        want = "k_rhs_f64<4, 30, false, true, %d, %s>" % (tuning["f64"]["flux"] or 5, tl)
         family = "k_rhs_f64<4, 30, false, true,"
-    names = [n for n in tab["kernels"] if want in n] or [n for n in tab["kernels"] if family in n]
+    names = [n for n in tab["kernels"] if want in n] or \
+        [n for n in tab["kernels"] if family in n and n.endswith("%s>(mf::RhsArgs)" % tl)]
     if not names:
         return None, None
     return tab["kernels"][names[0]]["bytes"], names[0]
@@ -228,13 +242,15 @@ def main():
     ctx.synchronize()
     wall = time.perf_counter() - w0
     value = N / (rhs_ms * 1e-3)
-    ba, bb = kernel_bytes(N, k)
+    tinfo = ctx.tile_info()
+    ba, bb = kernel_bytes(N, k, tinfo)
     kernels = {"stencil": (parts[0], ba), "transpose": (parts[1], bb)}
     dom = max(kernels, key=lambda q: kernels[q][0])
     kms, kbytes = kernels[dom]
     achieved = kbytes / (kms * 1e-3) / 1e9
     rbytes = rhs_bytes(N, k)
-    traffic, traffic_kernel = measured_traffic(dom, tuning, args.side)
+    traffic, traffic_kernel = measured_traffic(dom, tuning, args.side,
+                                               tinfo["tiles_b"] if dom == "transpose" else tinfo["tiles_a"])
     log("rhs %.4f ms (%.3e node-evals/s); stencil %.4f ms, transpose %.4f ms, reductions %.4f ms, bc %.4f ms"
         % (rhs_ms, value, parts[0], parts[1], parts[2], parts[3]))
     # the same call with the neighbour gathers from global memory (no LDS tiles)
@@ -282,6 +298,7 @@ def main():
         "f64_ref": {"value": N / (r_ms * 1e-3), "ms_per_step": r_ms,
                     "kernels_ms": {"stencil": r_parts[0], "transpose": r_parts[1]},
                     "rhs_frac": rbytes / (r_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "lds_tiles": tinfo,
         "f64_global_gathers": {"ms_per_step": nt_ms,
                                "kernels_ms": {"stencil": nt_parts[0], "transpose": nt_parts[1]}},
         "setup_ms": {"knn": 1e3 * info["t_knn"], "weights": 1e3 * info["t_weights"]},

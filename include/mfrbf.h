@@ -257,6 +257,13 @@ int mf_ssprk33_vjp(mf_ctx* ctx, double t0, double dt, int n_steps, int checkpoin
  * path has no launch shapes); set by the autotuner of the Python package. */
 int mf_set_tuning(mf_ctx* ctx, int key, int precision, int value);
 
+/* State of the LDS tiles after the last rhs! preparation: info[0] phase A
+ * tiles in use, [1] phase B tiles in use, [2] tiles, [3] total phase-A halo
+ * entries, [4] total phase-B halo entries, [5] largest phase-A halo, [6]
+ * largest phase-B halo, [7] phase-B entries including padding (8 int64).
+ * No counterpart in the reference; for performance accounting. */
+int mf_tile_info(mf_ctx* ctx, int64_t* info);
+
 /* ------------------------------------------------------------- timing */
 /* Average device time of one rhs! (ms) over n_calls back-to-back calls on
  * MF_BUF_U, measured with HIP events on the context stream (chains of the

@@ -1022,6 +1022,22 @@ int mf_set_tuning(mf_ctx* c, int key, int precision, int value)
     return MF_OK;
 }
 
+int mf_tile_info(mf_ctx* c, int64_t* info)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!info) return fail(c, MF_ERR_ARGUMENT, "mf_tile_info: null output");
+    info[0] = c->tiles_a;
+    info[1] = c->tiles_b;
+    info[2] = c->n_tiles;
+    info[3] = c->tiles_a ? c->ha_total : 0;
+    info[4] = c->tiles_b ? c->hb_total : 0;
+    info[5] = c->tiles_a ? c->ha_max : 0;
+    info[6] = c->tiles_b ? c->hb_max : 0;
+    info[7] = c->tiles_b ? c->nb_entries : 0;
+    return MF_OK;
+}
+
 int mf_time_rhs(mf_ctx* c, double t, int precision, int n_calls, double* rhs_ms, double* kernel_ms)
 {
     int rc = check_ctx(c);

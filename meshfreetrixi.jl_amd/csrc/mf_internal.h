@@ -167,9 +167,12 @@ struct mf_ctx {
     bool tried_a = false, tried_b = false;  // build attempted for this stencil / operator
     int64_t n_tiles = 0;
     int ha_max = 0, hb_max = 0;          // largest halo of a tile
+    int64_t ha_total = 0, hb_total = 0;  // all tiles' halo entries
+    int64_t nb_entries = 0;              // phase-B entries with padding
     int64_t* d_ha_off = nullptr;         // n_tiles + 1
     int32_t* d_ha_ids = nullptr;         // halo node ids, sorted per tile
     uint16_t* d_la = nullptr;            // slot-major halo positions (like d_idx)
+    uint16_t* d_la_s = nullptr;          // the same for the column-sorted stencil d_idx_s
     int64_t* d_hb_off = nullptr;
     int32_t* d_hb_ids = nullptr;
     int64_t* d_sb_off = nullptr;         // 4 slices (64 rows) per tile

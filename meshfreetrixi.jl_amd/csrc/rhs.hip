@@ -61,6 +61,7 @@ struct RhsArgs {
     const int64_t* __restrict__ ha_off;
     const int32_t* __restrict__ ha_ids;
     const uint16_t* __restrict__ la;
+    const uint16_t* __restrict__ la_s;  // halo positions of the column-sorted stencil
     const int64_t* __restrict__ hb_off;
     const int32_t* __restrict__ hb_ids;
     const int64_t* __restrict__ sb_off;
@@ -350,8 +351,9 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
 // One pass over the column-sorted stencil in batches of BT entries (loads
 // first, then the adds in entry order).  PASS 0: x flux (and D_x u),
 // 1: y flux (and D_y u), 2: Flyer term.
-template <int V, int NS, bool RV, int BT, int PASS>
-__device__ __forceinline__ void ref_pass(const RhsArgs& a, int64_t i, int K, double* du, double* g)
+template <int V, int NS, bool RV, int BT, int PASS, bool TL>
+__device__ __forceinline__ void ref_pass(const RhsArgs& a, int64_t i, int K, double* du, double* g,
+                                         const double* lds)
 {
     const int64_t N = a.N;
     const double* __restrict__ wsrc = PASS == 0 ? a.wx_s : (PASS == 1 ? a.wy_s : a.hv_s);
@@ -365,12 +367,20 @@ __device__ __forceinline__ void ref_pass(const RhsArgs& a, int64_t i, int K, dou
             const int s = s0 + b;
             const bool in = (NS && NS % BT == 0) || s < K;
             const int64_t q = (int64_t)(in ? s : s0) * N + i;
-            j[b] = a.idx_s[q];
+            if constexpr (TL)
+                j[b] = a.la_s[q];
+            else
+                j[b] = a.idx_s[q];
             w[b] = wsrc[q];
         }
         double uj[BT][V];
 #pragma unroll
-        for (int b = 0; b < BT; ++b) load_state<V>(a.u, j[b], uj[b]);
+        for (int b = 0; b < BT; ++b) {
+            if constexpr (TL)
+                load_state<V>(lds, j[b], uj[b]);
+            else
+                load_state<V>(a.u, j[b], uj[b]);
+        }
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
             if (!((NS && NS % BT == 0) || s0 + b < K)) continue;
@@ -390,23 +400,24 @@ __device__ __forceinline__ void ref_pass(const RhsArgs& a, int64_t i, int K, dou
     }
 }
 
-template <int V, int NS, bool HV, bool RV, int BT>
+template <int V, int NS, bool HV, bool RV, int BT, bool TL>
 __global__ __launch_bounds__(kBlock) void k_rhs_ref(const RhsArgs a)
 {
+    extern __shared__ double lds[];
     double ninf[4] = {1.0, 1.0, 1.0, 1.0};
     if constexpr (RV) {
         if (a.visc == MF_VISC_RESIDUAL) block_norms(a, ninf);
     }
-    const int64_t i = remapped_node();
-    if (i >= a.N) return;
+    const int64_t i = stage_halo_a<V, TL>(a, lds);
+    if (i < 0) return;
     const int K = NS ? NS : a.k;
     double du[V], gx[V], gy[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) du[v] = gx[v] = gy[v] = 0.0;
-    ref_pass<V, NS, RV, BT, 0>(a, i, K, du, gx);
-    ref_pass<V, NS, RV, BT, 1>(a, i, K, du, gy);
+    ref_pass<V, NS, RV, BT, 0, TL>(a, i, K, du, gx, lds);
+    ref_pass<V, NS, RV, BT, 1, TL>(a, i, K, du, gy, lds);
     if constexpr (HV) {
-        if (!RV || a.hv_first) ref_pass<V, NS, false, BT, 2>(a, i, K, du, nullptr);
+        if (!RV || a.hv_first) ref_pass<V, NS, false, BT, 2, TL>(a, i, K, du, nullptr, lds);
     }
     if constexpr (RV) {
         double ui[V];
@@ -732,11 +743,13 @@ __global__ __launch_bounds__(kBlock) void k_bc_state(const BcArgs b, double* u)
 // mean(u) = sum(u) / (V N)  (ode_mean, auxiliary/mpi.jl:40-52) and the
 // lexicographic maximum of |u_i - mean| (ode_maximum, mpi.jl:71-82) with
 // zeros replaced by eps() (hyperviscosity.jl:305-311).  Each block reduces
-// a contiguous node range in a fixed order; the last block to finish
-// combines the partials.
+// a contiguous node range in a fixed order into one partial; the next
+// kernel combines the partials (every block the same way, same bits).
 // Sum pass: block b reduces nodes [b*chunk, (b+1)*chunk) in a fixed order.
 // With bc.flag set it also applies the first boundary pass to the nodes it
 // visits (each node is touched by exactly one thread here).
+constexpr int kRed = 4;
+
 __global__ __launch_bounds__(kBlock) void k_rv_sum(int64_t N, int64_t chunk, double* u, double* psum,
                                                    const BcArgs bc)
 {
@@ -744,14 +757,24 @@ __global__ __launch_bounds__(kBlock) void k_rv_sum(int64_t N, int64_t chunk, dou
     const int64_t lo = (int64_t)blockIdx.x * chunk;
     const int64_t hi = min(N, lo + chunk);
     double acc[4] = {0, 0, 0, 0};
-    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        double s[4];
-        load_state<4>(u, i, s);
-        if (bc.flag && bc.flag[i]) {
-            if (bc_entry<4>(bc, bc.entry[i], (int32_t)i, s)) store_state<4>(u, i, s);
+    // kRed nodes per thread and trip, all loads issued first
+    for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (int64_t)kBlock * kRed) {
+        double s[kRed][4];
+#pragma unroll
+        for (int r = 0; r < kRed; ++r) {
+            const int64_t i = i0 + (int64_t)r * kBlock;
+            if (i < hi) load_state<4>(u, i, s[r]);
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) acc[v] += s[v];
+        for (int r = 0; r < kRed; ++r) {
+            const int64_t i = i0 + (int64_t)r * kBlock;
+            if (i >= hi) break;
+            if (bc.flag && bc.flag[i]) {
+                if (bc_entry<4>(bc, bc.entry[i], (int32_t)i, s[r])) store_state<4>(u, i, s[r]);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[v] += s[r][v];
+        }
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = acc[v];
@@ -777,14 +800,23 @@ __global__ __launch_bounds__(kBlock) void k_rv_max(int64_t N, int64_t chunk, con
     const int64_t lo = (int64_t)blockIdx.x * chunk;
     const int64_t hi = min(N, lo + chunk);
     double best[4] = {-1.0, -1.0, -1.0, -1.0};  // below every |.|
-    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        double s[4], c[4];
-        load_state<4>(u, i, s);
+    for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (int64_t)kBlock * kRed) {
+        double s[kRed][4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) c[v] = fabs(s[v] - mean[v]);
-        if (lex_less(best, c))
+        for (int r = 0; r < kRed; ++r) {
+            const int64_t i = i0 + (int64_t)r * kBlock;
+            if (i < hi) load_state<4>(u, i, s[r]);
+        }
 #pragma unroll
-            for (int v = 0; v < 4; ++v) best[v] = c[v];
+        for (int r = 0; r < kRed; ++r) {
+            if (i0 + (int64_t)r * kBlock >= hi) break;
+            double c[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) c[v] = fabs(s[r][v] - mean[v]);
+            if (lex_less(best, c))
+#pragma unroll
+                for (int v = 0; v < 4; ++v) best[v] = c[v];
+        }
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = best[v];
@@ -1037,6 +1069,7 @@ static RhsArgs make_args(mf_ctx* c, double* u, double* du)
     a.ha_off = c->d_ha_off;
     a.ha_ids = c->d_ha_ids;
     a.la = c->d_la;
+    a.la_s = c->d_la_s;
     a.hb_off = c->d_hb_off;
     a.hb_ids = c->d_hb_ids;
     a.sb_off = c->d_sb_off;
@@ -1138,12 +1171,22 @@ static void launch_s32(mf_ctx* c, const RhsArgs& a, bool hv, bool rv, const dim3
     else launch_k(c, 0, k_rhs_s32<V, NS, false, false, D16, BT, TL>, G, B, lds, a);
 }
 
+template <int V, int NS, int BT, bool TL>
+static void launch_ref(mf_ctx* c, const RhsArgs& a, bool hv, bool rv, const dim3& G, size_t lds)
+{
+    const dim3 B(kBlock);
+    if (hv && rv) launch_k(c, 0, k_rhs_ref<V, NS, true, true, BT, TL>, G, B, lds, a);
+    else if (hv) launch_k(c, 0, k_rhs_ref<V, NS, true, false, BT, TL>, G, B, lds, a);
+    else if (rv) launch_k(c, 0, k_rhs_ref<V, NS, false, true, BT, TL>, G, B, lds, a);
+    else launch_k(c, 0, k_rhs_ref<V, NS, false, false, BT, TL>, G, B, lds, a);
+}
+
 template <int V, int NS, int BT>
 static void launch_flux_bt(mf_ctx* c, int prec, const RhsArgs& a, bool hv, bool rv, unsigned grid)
 {
     const dim3 G(grid), B(kBlock);
     hipStream_t s = c->stream;
-    const bool tl = c->tiles_a && prec != MF_PREC_F64_REF;
+    const bool tl = c->tiles_a;
     const size_t lds = tl ? (size_t)c->ha_max * V * sizeof(double) : 0;
     if (prec == MF_PREC_F64) {
         if (tl)
@@ -1151,10 +1194,10 @@ static void launch_flux_bt(mf_ctx* c, int prec, const RhsArgs& a, bool hv, bool 
         else
             launch_f64<V, NS, BT, false>(c, a, hv, rv, G, 0);
     } else if (prec == MF_PREC_F64_REF) {
-        if (hv && rv) launch_k(c, 0, k_rhs_ref<V, NS, true, true, BT>, G, B, 0, a);
-        else if (hv) launch_k(c, 0, k_rhs_ref<V, NS, true, false, BT>, G, B, 0, a);
-        else if (rv) launch_k(c, 0, k_rhs_ref<V, NS, false, true, BT>, G, B, 0, a);
-        else launch_k(c, 0, k_rhs_ref<V, NS, false, false, BT>, G, B, 0, a);
+        if (tl)
+            launch_ref<V, NS, BT, true>(c, a, hv, rv, G, lds);
+        else
+            launch_ref<V, NS, BT, false>(c, a, hv, rv, G, 0);
     } else if (tl) {
         launch_s32<V, NS, BT, false, true>(c, a, hv, rv, G, lds);
     } else if (c->didx_ok) {

@@ -56,9 +56,11 @@ void free_tiles(mf_ctx* c)
     dfree(c->d_ha_off);
     dfree(c->d_ha_ids);
     dfree(c->d_la);
+    dfree(c->d_la_s);
     c->d_ha_off = nullptr;
     c->d_ha_ids = nullptr;
     c->d_la = nullptr;
+    c->d_la_s = nullptr;
     c->tiles_a = false;
     c->tried_a = false;
     c->n_tiles = 0;
@@ -94,8 +96,10 @@ int build_tiles(mf_ctx* c, bool want_b)
     const int V = c->nvar;
     const int64_t nt = (N + kTile - 1) / kTile;
     int rc;
-    std::vector<int32_t> idx((size_t)N * K);
+    std::vector<int32_t> idx((size_t)N * K), idx_s((size_t)N * K);
     MF_HIP(c, hipMemcpyAsync(idx.data(), c->d_idx, sizeof(int32_t) * idx.size(), hipMemcpyDeviceToHost,
+                             c->stream));
+    MF_HIP(c, hipMemcpyAsync(idx_s.data(), c->d_idx_s, sizeof(int32_t) * idx_s.size(), hipMemcpyDeviceToHost,
                              c->stream));
     MF_HIP(c, hipStreamSynchronize(c->stream));
     std::vector<int32_t> mark(N, -1);
@@ -108,7 +112,7 @@ int build_tiles(mf_ctx* c, bool want_b)
         std::vector<int64_t> off(nt + 1, 0);
         std::vector<int32_t> all;
         all.reserve((size_t)N * 2);
-        std::vector<uint16_t> la((size_t)N * K);
+        std::vector<uint16_t> la((size_t)N * K), la_s((size_t)N * K);
         int hmax = 0;
         bool fits = true;
         for (int64_t t = 0; t < nt && fits; ++t) {
@@ -126,7 +130,10 @@ int build_tiles(mf_ctx* c, bool want_b)
             if (ids.size() > 65535 || ids.size() * V * sizeof(double) > kTileLdsBytes) fits = false;
             for (size_t q = 0; q < ids.size(); ++q) mark[ids[q]] = (int32_t)q;
             for (int s = 0; s < K; ++s)
-                for (int64_t i = i0; i < i1; ++i) la[(size_t)s * N + i] = (uint16_t)mark[idx[(size_t)s * N + i]];
+                for (int64_t i = i0; i < i1; ++i) {
+                    la[(size_t)s * N + i] = (uint16_t)mark[idx[(size_t)s * N + i]];
+                    la_s[(size_t)s * N + i] = (uint16_t)mark[idx_s[(size_t)s * N + i]];
+                }
             for (int32_t j : ids) mark[j] = -1;
             hmax = std::max<int>(hmax, (int)ids.size());
             all.insert(all.end(), ids.begin(), ids.end());
@@ -134,11 +141,12 @@ int build_tiles(mf_ctx* c, bool want_b)
         }
         if (!fits) return MF_OK;  // global-memory kernels
         if ((rc = upload(c, &c->d_ha_off, off)) || (rc = upload(c, &c->d_ha_ids, all)) ||
-            (rc = upload(c, &c->d_la, la)))
+            (rc = upload(c, &c->d_la, la)) || (rc = upload(c, &c->d_la_s, la_s)))
             return rc;
         MF_HIP(c, hipStreamSynchronize(c->stream));
         c->n_tiles = nt;
         c->ha_max = hmax;
+        c->ha_total = (int64_t)all.size();
         c->tiles_a = true;
     }
     if (!want_b || !c->tiles_a || c->tried_b) return MF_OK;
@@ -236,6 +244,8 @@ int build_tiles(mf_ctx* c, bool want_b)
     MF_HIP(c, hipStreamSynchronize(c->stream));
     dfree(d_src);
     c->hb_max = hmax;
+    c->hb_total = (int64_t)hall.size();
+    c->nb_entries = ne;
     c->tiles_b = true;
     return MF_OK;
 }

@@ -72,6 +72,7 @@ _SIGS = {
     "mf_time_rhs": [_ctx, C.c_double, C.c_int, C.c_int, _dp, _dp],
     "mf_set_tuning": [_ctx, C.c_int, C.c_int, C.c_int],
     "mf_node_order": [_ctx, _i32p],
+    "mf_tile_info": [_ctx, _i64p],
 }
 
 _lib = None
@@ -277,6 +278,14 @@ class Context:
 
     def set_tuning(self, key, value, precision=-1):
         self._chk(self._L.mf_set_tuning(self.h, key, precision, value))
+
+    def tile_info(self):
+        """dict of the LDS tile state (mf_tile_info)."""
+        v = np.zeros(8, np.int64)
+        self._chk(self._L.mf_tile_info(self.h, i64ptr(v)))
+        keys = ("tiles_a", "tiles_b", "n_tiles", "halo_a_total", "halo_b_total", "halo_a_max", "halo_b_max",
+                "b_entries_padded")
+        return {k: int(x) for k, x in zip(keys, v)}
 
     def node_order(self):
         """order[p] = caller index of internal node p."""

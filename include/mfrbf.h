@@ -183,6 +183,10 @@ int mf_set_source_viscosity(mf_ctx* ctx, int mode, double c_rv, double c_uw, dou
 /* Copies nvar host fields into / out of a context buffer (MF_BUF_*). */
 int mf_state_upload(mf_ctx* ctx, int buf, const double* const* fields);
 int mf_state_download(mf_ctx* ctx, int buf, double* const* fields);
+/* Device-side copy of one context buffer into another (e.g. keep a state
+ * while stepping on); a plain streaming copy of nvar * n_points doubles.
+ * No counterpart in the reference (copyto! on host arrays there). */
+int mf_state_copy(mf_ctx* ctx, int dst_buf, int src_buf);
 /* Device address of a context buffer: node-major, nvar doubles per node,
  * nodes in the context's internal order (see mf_node_order). */
 int mf_state_device_ptr(mf_ctx* ctx, int buf, double** dptr);

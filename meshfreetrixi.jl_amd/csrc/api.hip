@@ -214,9 +214,11 @@ int ensure_state(mf_ctx* c)
         }
         if (!c->d_hist && (rc = dalloc(c, &c->d_hist, nv * (size_t)(c->approx_order + 1)))) return rc;
         if (!c->d_partials) {
-            // 256 blocks for the two reduction passes: sums, then maxima
-            c->n_partials = (int)std::min<int64_t>(256, (c->N + 255) / 256);
-            if ((rc = dalloc(c, &c->d_partials, 8 * (size_t)c->n_partials))) return rc;
+            // reduction passes: the sum pass one node per thread (up to 2048
+            // blocks), the max pass up to 256 blocks of 4 nodes per thread
+            c->n_psum = (int)std::min<int64_t>(2048, (c->N + 255) / 256);
+            c->n_pmax = (int)std::min<int64_t>(256, (c->N + 1023) / 1024);
+            if ((rc = dalloc(c, &c->d_partials, 4 * (size_t)(c->n_psum + c->n_pmax)))) return rc;
         }
     }
     if (!c->d_lim && (rc = dalloc(c, &c->d_lim, nv))) return rc;
@@ -776,6 +778,19 @@ int mf_state_device_ptr(mf_ctx* c, int buf, double** dptr)
     return MF_OK;
 }
 
+int mf_state_copy(mf_ctx* c, int dst_buf, int src_buf)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (dst_buf < 0 || dst_buf > 3 || src_buf < 0 || src_buf > 3) return fail(c, MF_ERR_ARGUMENT, "bad buffer");
+    if ((rc = ensure_state(c))) return rc;
+    if (dst_buf == src_buf) return MF_OK;
+    if ((rc = launch_copy_buffer(c, c->d_buf[src_buf], c->d_buf[dst_buf], c->N * c->nvar))) return rc;
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    if (dst_buf == MF_BUF_U) c->fsal_valid = false;
+    return MF_OK;
+}
+
 int mf_node_order(mf_ctx* c, int32_t* order_out)
 {
     int rc = check_ctx(c);
@@ -1095,14 +1110,15 @@ int mf_time_rhs(mf_ctx* c, double t, int precision, int n_calls, double* rhs_ms,
         std::vector<hipEvent_t> ev(kB * kE, nullptr);
         for (auto& e : ev) MF_HIP(c, hipEventCreate(&e));
         double acc[4] = {0, 0, 0, 0};
+        unsigned used[kB];
         int done = 0;
         while (done < n_calls && rc == MF_OK) {
             const int nb = std::min(kB, n_calls - done);
             for (int q = 0; q < nb && rc == MF_OK; ++q) {
-                // slots a call does not use keep start == stop
-                for (int e = 0; e < kE; ++e) (void)hipEventRecord(ev[q * kE + e], c->stream);
                 c->probe = &ev[q * kE];
+                c->probe_used = 0;
                 rc = run_rhs(c, precision, c->d_buf[MF_BUF_U], c->d_buf[MF_BUF_DU], 0.0, none);
+                used[q] = c->probe_used;
             }
             c->probe = nullptr;
             if (rc) break;
@@ -1111,6 +1127,8 @@ int mf_time_rhs(mf_ctx* c, double t, int precision, int n_calls, double* rhs_ms,
             for (int q = 0; q < nb; ++q) {
                 float m[kS];
                 for (int sl = 0; sl < kS; ++sl) {
+                    m[sl] = 0.f;
+                    if (!(used[q] & (1u << sl))) continue;  // no kernel in this slot
                     e = hipEventElapsedTime(&m[sl], ev[q * kE + 2 * sl], ev[q * kE + 2 * sl + 1]);
                     if (e != hipSuccess) m[sl] = 0.f;
                 }

@@ -127,8 +127,8 @@ struct mf_ctx {
     int8_t* d_epsc = nullptr;
     double* d_approx = nullptr;
     double* d_hist = nullptr;
-    double* d_partials = nullptr;
-    int n_partials = 0;
+    double* d_partials = nullptr;  // 4 n_psum sums, then 4 n_pmax lexicographic maxima
+    int n_psum = 0, n_pmax = 0;
     mf::StepScalars* d_sc = nullptr;
     int host_success_iter = 0;
     bool fsal_valid = false;
@@ -190,6 +190,7 @@ struct mf_ctx {
     // when set, the rhs! kernels are dispatched with start/stop events:
     // probe[2 s], probe[2 s + 1] for slot s (see launch_k in rhs.hip)
     hipEvent_t* probe = nullptr;
+    unsigned probe_used = 0;  // slots dispatched with events during the probed call
 };
 
 namespace mf {
@@ -231,6 +232,7 @@ int build_tiles(mf_ctx* c, bool want_b);
 void free_tiles(mf_ctx* c);
 void free_tiles_b(mf_ctx* c);
 int launch_gather_nodes_d(mf_ctx* c, const double* d_src, double* d_dst);
+int launch_copy_buffer(mf_ctx* c, const double* d_src, double* d_dst, int64_t n);
 int launch_gather_nodes_i8(mf_ctx* c, const int8_t* d_src, int8_t* d_dst);
 int launch_sort_stencil(mf_ctx* c);
 int launch_pack_sorted(mf_ctx* c, int target, const double* d_wx, const double* d_wy);

@@ -10,6 +10,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 
 #include "mf_device.h"
@@ -190,9 +191,16 @@ __device__ __forceinline__ double node_viscosity(const RhsArgs& a, int64_t i, co
 // G_i = eps_i (Dx u, Dy u)_i for the transposed pass (hyperviscosity.jl:401-407).
 // Residual-viscosity norms: every block reduces the max pass's partials
 // (same bits everywhere); block 0 publishes them.
-__device__ __forceinline__ void block_norms(const RhsArgs& a, double* ninf)
+template <bool TL>
+__device__ __forceinline__ void block_norms(const RhsArgs& a, double* ninf, double* lds)
 {
-    __shared__ double red[kBlock][4];
+    double(*red)[4];
+    if constexpr (TL) {
+        red = reinterpret_cast<double(*)[4]>(lds);  // before the halo is staged
+    } else {
+        __shared__ double red_s[kBlock][4];
+        red = red_s;
+    }
     ninf_from_partials(a.pmax, a.nparts, red, ninf);
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.norms_out)
 #pragma unroll
@@ -206,8 +214,87 @@ __device__ __forceinline__ int64_t remapped_block()
     return (int64_t)((b & 7u) * per + (b >> 3));
 }
 
+// Halo record of a node in LDS: for the Euler equations (rho, m1, m2, E,
+// v1, v2, p, 0) with the velocity and pressure evaluated exactly as
+// flux_xy does (so the fluxes built from a record have the same bits),
+// which takes the divisions out of the per-entry work; otherwise the state.
+template <int V>
+constexpr int rec_len()
+{
+    return V == 4 ? 8 : V;
+}
+
+template <int V>
+__device__ __forceinline__ void make_record(const EqParams& e, const double* u, double* r)
+{
+    if constexpr (V == 4) {
+        const double rho = u[0], m1 = u[1], m2 = u[2], E = u[3];
+        const double v1 = m1 / rho, v2 = m2 / rho;
+        r[0] = rho;
+        r[1] = m1;
+        r[2] = m2;
+        r[3] = E;
+        r[4] = v1;
+        r[5] = v2;
+        r[6] = (e.gamma - 1.0) * fma(-0.5, fma(m1, v1, m2 * v2), E);
+        r[7] = 0.0;
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v) r[v] = u[v];
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void flux_record(const EqParams& e, const double* r, double* f1, double* f2)
+{
+    if constexpr (V == 4) {
+        const double m1 = r[1], m2 = r[2], E = r[3], v1 = r[4], v2 = r[5], p = r[6];
+        const double ep = E + p;
+        f1[0] = m1;
+        f1[1] = fma(m1, v1, p);
+        f1[2] = m1 * v2;
+        f1[3] = ep * v1;
+        f2[0] = m2;
+        f2[1] = m2 * v1;
+        f2[2] = fma(m2, v2, p);
+        f2[3] = ep * v2;
+    } else {
+        flux_xy<V>(e, r, f1, f2);
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void load_record(const double* lds, int64_t l, double* r)
+{
+    if constexpr (V == 4) {
+        load_state<4>(lds, 2 * l, r);
+        load_state<4>(lds, 2 * l + 1, r + 4);
+    } else {
+        load_state<V>(lds, l, r);
+    }
+}
+
+// Neighbour j of a stencil loop: its halo record (TL) or its state.
+template <int V, bool TL>
+__device__ __forceinline__ void load_nb(const RhsArgs& a, const double* lds, int64_t j, double* r)
+{
+    if constexpr (TL)
+        load_record<V>(lds, j, r);
+    else
+        load_state<V>(a.u, j, r);
+}
+
+template <int V, bool TL>
+__device__ __forceinline__ void flux_nb(const EqParams& e, const double* r, double* f1, double* f2)
+{
+    if constexpr (TL)
+        flux_record<V>(e, r, f1, f2);
+    else
+        flux_xy<V>(e, r, f1, f2);
+}
+
 // Node of this thread and, with LDS tiles (TL), the tile's stencil halo
-// staged in lds: halo entry q holds the state of node ha_ids[ha_off[t] + q].
+// staged in lds: halo entry q holds the record of node ha_ids[ha_off[t] + q].
 // Returns -1 for threads without a node (after the block barrier).
 template <int V, bool TL>
 __device__ __forceinline__ int64_t stage_halo_a(const RhsArgs& a, double* lds)
@@ -218,9 +305,15 @@ __device__ __forceinline__ int64_t stage_halo_a(const RhsArgs& a, double* lds)
         const int64_t h0 = a.ha_off[t];
         const int nh = (int)(a.ha_off[t + 1] - h0);
         for (int q = threadIdx.x; q < nh; q += kBlock) {
-            double s[V];
+            double s[V], r[rec_len<V>()];
             load_state<V>(a.u, a.ha_ids[h0 + q], s);
-            store_state<V>(lds, q, s);
+            make_record<V>(a.eq, s, r);
+            if constexpr (V == 4) {
+                store_state<4>(lds, 2 * q, r);
+                store_state<4>(lds, 2 * q + 1, r + 4);
+            } else {
+                store_state<V>(lds, q, r);
+            }
         }
         __syncthreads();
         const int64_t i = t * kBlock + threadIdx.x;
@@ -236,13 +329,18 @@ __device__ __forceinline__ int64_t stage_halo_a(const RhsArgs& a, double* lds)
 // gathers are in flight per thread.  Entries past K read a valid node with
 // weight 0.  TL: neighbour states come from the LDS halo, addressed by the
 // 16-bit halo positions la (same slot-major layout as idx).
+// Batches of BT stencil entries: every load of a batch (positions,
+// weights, then the gathered states) is issued before any of it is used,
+// so BT gathers are in flight per thread.  Entries past K read a valid node
+// with weight 0.  TL: neighbour records come from the LDS halo, addressed
+// by 16-bit halo positions (same slot-major layout as idx).
 template <int V, int NS, bool HV, bool RV, int BT, bool TL>
 __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
 {
     extern __shared__ double lds[];
     double ninf[4] = {1.0, 1.0, 1.0, 1.0};
     if constexpr (RV) {
-        if (a.visc == MF_VISC_RESIDUAL) block_norms(a, ninf);
+        if (a.visc == MF_VISC_RESIDUAL) block_norms<TL>(a, ninf, lds);
     }
     const int64_t i = stage_halo_a<V, TL>(a, lds);
     if (i < 0) return;
@@ -253,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
     for (int v = 0; v < V; ++v) ax[v] = ay[v] = gx[v] = gy[v] = hh[v] = 0.0;
 #pragma unroll 1
     for (int s0 = 0; s0 < K; s0 += BT) {
-        int64_t j[BT];
+        int32_t j[BT];
         double2 w[BT];
         double hw[BT];
 #pragma unroll
@@ -268,23 +366,18 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
                 w[b] = a.wxy[q];
                 if constexpr (HV) hw[b] = a.hv[q];
             } else {
-                j[b] = TL ? 0 : i;
+                j[b] = TL ? 0 : (int32_t)i;
                 w[b] = make_double2(0.0, 0.0);
                 hw[b] = 0.0;
             }
         }
-        double uj[BT][V];
+        double uj[BT][TL ? rec_len<V>() : V];
 #pragma unroll
-        for (int b = 0; b < BT; ++b) {
-            if constexpr (TL)
-                load_state<V>(lds, j[b], uj[b]);
-            else
-                load_state<V>(a.u, j[b], uj[b]);
-        }
+        for (int b = 0; b < BT; ++b) load_nb<V, TL>(a, lds, j[b], uj[b]);
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
             double f1[V], f2[V];
-            flux_xy<V>(a.eq, uj[b], f1, f2);
+            flux_nb<V, TL>(a.eq, uj[b], f1, f2);
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 ax[v] = fma(w[b].x, f1[v], ax[v]);
@@ -373,14 +466,9 @@ __device__ __forceinline__ void ref_pass(const RhsArgs& a, int64_t i, int K, dou
                 j[b] = a.idx_s[q];
             w[b] = wsrc[q];
         }
-        double uj[BT][V];
+        double uj[BT][TL ? rec_len<V>() : V];
 #pragma unroll
-        for (int b = 0; b < BT; ++b) {
-            if constexpr (TL)
-                load_state<V>(lds, j[b], uj[b]);
-            else
-                load_state<V>(a.u, j[b], uj[b]);
-        }
+        for (int b = 0; b < BT; ++b) load_nb<V, TL>(a, lds, j[b], uj[b]);
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
             if (!((NS && NS % BT == 0) || s0 + b < K)) continue;
@@ -389,7 +477,7 @@ __device__ __forceinline__ void ref_pass(const RhsArgs& a, int64_t i, int K, dou
                 for (int v = 0; v < V; ++v) du[v] = du[v] + w[b] * (uj[b][v] * ng);
             } else {
                 double f1[V], f2[V];
-                flux_xy<V>(a.eq, uj[b], f1, f2);
+                flux_nb<V, TL>(a.eq, uj[b], f1, f2);
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
                     du[v] = du[v] - w[b] * (PASS == 0 ? f1[v] : f2[v]);
@@ -406,7 +494,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_ref(const RhsArgs a)
     extern __shared__ double lds[];
     double ninf[4] = {1.0, 1.0, 1.0, 1.0};
     if constexpr (RV) {
-        if (a.visc == MF_VISC_RESIDUAL) block_norms(a, ninf);
+        if (a.visc == MF_VISC_RESIDUAL) block_norms<TL>(a, ninf, lds);
     }
     const int64_t i = stage_halo_a<V, TL>(a, lds);
     if (i < 0) return;
@@ -452,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
     extern __shared__ double lds[];
     double ninf[4] = {1.0, 1.0, 1.0, 1.0};
     if constexpr (RV) {
-        if (a.visc == MF_VISC_RESIDUAL) block_norms(a, ninf);
+        if (a.visc == MF_VISC_RESIDUAL) block_norms<TL>(a, ninf, lds);
     }
     const int64_t i = stage_halo_a<V, TL>(a, lds);
     if (i < 0) return;
@@ -488,20 +576,15 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
                 hw32[b] = 0.0f;
             }
         }
-        double ub[BT][V];
+        double ub[BT][TL ? rec_len<V>() : V];
 #pragma unroll
-        for (int b = 0; b < BT; ++b) {
-            if constexpr (TL)
-                load_state<V>(lds, jj[b], ub[b]);
-            else
-                load_state<V>(a.u, jj[b], ub[b]);
-        }
+        for (int b = 0; b < BT; ++b) load_nb<V, TL>(a, lds, jj[b], ub[b]);
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
         const float2 w = ww[b];
         const double* uj = ub[b];
         double f1[V], f2[V];
-        flux_xy<V>(a.eq, uj, f1, f2);
+        flux_nb<V, TL>(a.eq, uj, f1, f2);
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             ax[v] = fmaf(w.x, (float)(f1[v] - fi1[v]), ax[v]);
@@ -760,16 +843,21 @@ __global__ __launch_bounds__(kBlock) void k_rv_sum(int64_t N, int64_t chunk, dou
     // kRed nodes per thread and trip, all loads issued first
     for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (int64_t)kBlock * kRed) {
         double s[kRed][4];
+        uint8_t f[kRed];
 #pragma unroll
         for (int r = 0; r < kRed; ++r) {
             const int64_t i = i0 + (int64_t)r * kBlock;
-            if (i < hi) load_state<4>(u, i, s[r]);
+            f[r] = 0;
+            if (i < hi) {
+                load_state<4>(u, i, s[r]);
+                if (bc.flag) f[r] = bc.flag[i];
+            }
         }
 #pragma unroll
         for (int r = 0; r < kRed; ++r) {
             const int64_t i = i0 + (int64_t)r * kBlock;
             if (i >= hi) break;
-            if (bc.flag && bc.flag[i]) {
+            if (f[r]) {
                 if (bc_entry<4>(bc, bc.entry[i], (int32_t)i, s[r])) store_state<4>(u, i, s[r]);
             }
 #pragma unroll
@@ -1054,8 +1142,8 @@ static RhsArgs make_args(mf_ctx* c, double* u, double* du)
     a.wx_s = c->d_wx_s;
     a.wy_s = c->d_wy_s;
     a.hv_s = c->d_hv_s;
-    a.pmax = c->d_partials ? c->d_partials + 4 * c->n_partials : nullptr;
-    a.nparts = c->n_partials;
+    a.pmax = c->d_partials ? c->d_partials + 4 * c->n_psum : nullptr;
+    a.nparts = c->n_pmax;
     a.norms_out = c->d_sc ? c->d_sc->norms : nullptr;
     a.G = c->d_G;
     a.eps = c->d_eps;
@@ -1114,11 +1202,13 @@ static BcArgs make_bc(mf_ctx* c, double dt_off, int homog)
 template <typename... KArgs, typename... Args>
 static void launch_k(mf_ctx* c, int slot, void (*kernel)(KArgs...), dim3 G, dim3 B, size_t lds, Args... args)
 {
-    if (c->probe && slot >= 0)
+    if (c->probe && slot >= 0) {
+        c->probe_used |= 1u << slot;
         hipExtLaunchKernelGGL(kernel, G, B, (uint32_t)lds, c->stream, c->probe[2 * slot], c->probe[2 * slot + 1], 0,
                               args...);
-    else
+    } else {
         hipLaunchKernelGGL(kernel, G, B, lds, c->stream, args...);
+    }
 }
 
 static int bc_state_pass(mf_ctx* c, double* u, double dt_off, int homog, bool slip_only)
@@ -1187,7 +1277,10 @@ static void launch_flux_bt(mf_ctx* c, int prec, const RhsArgs& a, bool hv, bool 
     const dim3 G(grid), B(kBlock);
     hipStream_t s = c->stream;
     const bool tl = c->tiles_a;
-    const size_t lds = tl ? (size_t)c->ha_max * V * sizeof(double) : 0;
+    // halo records; at least the 256 x 4 doubles the norm reduction borrows
+    const size_t lds = tl ? std::max<size_t>((size_t)c->ha_max * rec_len<V>() * sizeof(double),
+                                             (size_t)kBlock * 4 * sizeof(double))
+                          : 0;
     if (prec == MF_PREC_F64) {
         if (tl)
             launch_f64<V, NS, BT, true>(c, a, hv, rv, G, lds);
@@ -1281,15 +1374,13 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
     const unsigned grid = xcd_grid(nblk);
     if (rv) {
         if (c->visc_mode == MF_VISC_RESIDUAL && (parts & RHS_REDUCE)) {
-            const int nparts = c->n_partials;
-            const int64_t chunk = (c->N + nparts - 1) / nparts;
+            const int ns = c->n_psum, nm = c->n_pmax;
             BcArgs bc = make_bc(c, dt_off, 0);
             if (!((parts & RHS_BC) && fused_bc)) bc.flag = nullptr;
-            launch_k(c, 2, k_rv_sum, dim3(nparts), dim3(kBlock), 0, c->N, chunk, d_u,
-                               c->d_partials, bc);
-            launch_k(c, 3, k_rv_max, dim3(nparts), dim3(kBlock), 0, c->N, chunk,
-                               (const double*)d_u, (const double*)c->d_partials, nparts,
-                               c->d_partials + 4 * nparts, c->d_sc);
+            launch_k(c, 2, k_rv_sum, dim3(ns), dim3(kBlock), 0, c->N, (c->N + ns - 1) / ns, d_u, c->d_partials,
+                     bc);
+            launch_k(c, 3, k_rv_max, dim3(nm), dim3(kBlock), 0, c->N, (c->N + nm - 1) / nm, (const double*)d_u,
+                     (const double*)c->d_partials, ns, c->d_partials + 4 * ns, c->d_sc);
         }
         RhsArgs pa = a;
         pa.st_kind = ST_NONE;

@@ -2,6 +2,7 @@
 // transposed operator used by the viscosity sources.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <hipcub/hipcub.hpp>
 
@@ -1136,6 +1137,25 @@ int launch_aos_to_soa(mf_ctx* c, const double* d_aos, double* d_soa)
     const int64_t n = c->N * c->nvar;
     hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->N,
                        c->nvar, d_aos, d_soa, c->d_inv);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+// plain streaming copy of n doubles (n a multiple of 4 or not)
+__global__ void k_copy_buffer(int64_t n, const double* __restrict__ src, double* __restrict__ dst)
+{
+    const int64_t n4 = n / 4;
+    const double4* s4 = reinterpret_cast<const double4*>(src);
+    double4* d4 = reinterpret_cast<double4*>(dst);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += stride) d4[q] = s4[q];
+    for (int64_t q = 4 * n4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) dst[q] = src[q];
+}
+
+int launch_copy_buffer(mf_ctx* c, const double* d_src, double* d_dst, int64_t n)
+{
+    const int64_t blocks = std::min<int64_t>(4096, (n / 4 + 255) / 256 + 1);
+    hipLaunchKernelGGL(k_copy_buffer, dim3((unsigned)blocks), dim3(256), 0, c->stream, n, d_src, d_dst);
     MF_HIP(c, hipGetLastError());
     return MF_OK;
 }

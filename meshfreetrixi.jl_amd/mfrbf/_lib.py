@@ -73,6 +73,7 @@ _SIGS = {
     "mf_set_tuning": [_ctx, C.c_int, C.c_int, C.c_int],
     "mf_node_order": [_ctx, _i32p],
     "mf_tile_info": [_ctx, _i64p],
+    "mf_state_copy": [_ctx, C.c_int, C.c_int],
 }
 
 _lib = None
@@ -278,6 +279,10 @@ class Context:
 
     def set_tuning(self, key, value, precision=-1):
         self._chk(self._L.mf_set_tuning(self.h, key, precision, value))
+
+    def copy(self, dst_buf, src_buf):
+        """Device-side copy of buffer src_buf into dst_buf."""
+        self._chk(self._L.mf_state_copy(self.h, dst_buf, src_buf))
 
     def tile_info(self):
         """dict of the LDS tile state (mf_tile_info)."""

@@ -403,3 +403,10 @@ def test_lds_tiles_change_no_result(mf, cfg):
         outs[tiles] = o
     for key in outs[0]:
         assert np.array_equal(outs[0][key], outs[1][key]), key
+
+
+def test_state_copy(mf):
+    c = cases.build(configs.c2(), side=24)
+    c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+    c.ctx.copy(L.MF_BUF_AUX0, L.MF_BUF_U)
+    assert np.array_equal(c.ctx.download(L.MF_BUF_AUX0), c.u0)

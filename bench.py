@@ -14,6 +14,7 @@ Printed as one JSON line, with:
   split32       the split-FP32 rhs! rate, speed-up and error vs FP64
   adjoint       reverse-sweep / forward time of 100 SSPRK33 steps (65,536 nodes)
   weights       batched weight generation for 1,048,576 ragged stencils
+  ensemble      64 clouds (5.7M nodes), 50 split-FP32 SSPRK33 steps each
 """
 import argparse
 import json
@@ -174,6 +175,51 @@ def adjoint_ratio(L, configs):
                 ratio_reverse=(t_adj - t_fwd) / t_fwd, loss=loss)
 
 
+def ensemble(L, configs, steps):
+    """C5: 64 Euler clouds (16k..262k nodes) with Flyer hyperviscosity,
+    split-FP32 rhs!, SSPRK33 steps replayed from each context's captured
+    graph.  All contexts are enqueued from this thread (each has its own
+    stream) before any is synchronized; the same steps one context at a
+    time are timed for comparison."""
+    specs = configs.c5()
+    t0 = time.perf_counter()
+    clouds = []
+    for spec in specs:
+        ctx, info = setup_euler(L, spec, spec.side, visc=False)
+        ctx.compute_operator(spec.p, spec.deg, 2 * spec.hv["k"], L.MF_OP_HYPERVISCOSITY, want=False)
+        ctx.set_source_hyperviscosity(spec.hv["c"] * info["dx_min"] ** (2 * spec.hv["k"]))
+        clouds.append((spec, ctx, info))
+    t_setup = time.perf_counter() - t0
+    flags = L.MF_STEP_GRAPH | L.MF_STEP_NOSYNC
+
+    def run(concurrent):
+        for _, ctx, info in clouds:
+            ctx.upload(L.MF_BUF_U, info["u0"])
+        w0 = time.perf_counter()
+        for _, ctx, info in clouds:
+            ctx.ssprk33(0.0, info["dt"], steps, L.MF_PREC_SPLIT32, flags if concurrent else L.MF_STEP_GRAPH)
+        for _, ctx, _ in clouds:
+            ctx.synchronize()
+        return time.perf_counter() - w0
+
+    run(True)  # capture the graphs, build the LDS tiles
+    wall_c = run(True)
+    wall_s = run(False)
+    total = sum(info["N"] for _, _, info in clouds)
+    evals = sum(info["N"] * (3 * steps + 1) for _, _, info in clouds)  # + the first (FSAL) rhs!
+    # the density wave is an exact solution: report the largest density error at T
+    from mfrbf.equations import CompressibleEulerEquations2D
+    err = 0.0
+    for spec, ctx, info in clouds:
+        T = steps * info["dt"]
+        ex = spec.ic(np.c_[info["x"], info["y"]], T, CompressibleEulerEquations2D(spec.gamma))
+        u = ctx.download(L.MF_BUF_U)
+        err = max(err, float(np.abs(u[0] - ex[0]).max()))
+    return dict(clouds=len(clouds), nodes=total, steps=steps, setup_s=t_setup, ms_concurrent=1e3 * wall_c,
+                ms_one_at_a_time=1e3 * wall_s, node_evals_per_s=evals / wall_c,
+                node_evals_per_s_one_at_a_time=evals / wall_s, max_density_error_vs_exact=err)
+
+
 def weights_batch(L, configs, side):
     from mfrbf.clouds import cloud_square
 
@@ -314,6 +360,8 @@ def main():
         out["adjoint"] = adjoint_ratio(L, configs)
         log("batched weights: 1,048,576 ragged stencils")
         out["weights"] = weights_batch(L, configs, 1024)
+        log("ensemble: 64 clouds, 50 SSPRK33 steps each (split-FP32, graphs, concurrent streams)")
+        out["ensemble"] = ensemble(L, configs, 50)
     if not args.no_cpu:
         log("cpu baseline (oracle, %.0f s budget)" % args.cpu_seconds)
         cb, du_cpu = cpu_baseline(L, ctx, info, spec, t_now, args.cpu_seconds)

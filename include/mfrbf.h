@@ -91,6 +91,9 @@ typedef struct mf_ctx mf_ctx;
 #define MF_STEP_GRAPH 1   /* capture one step as a hipGraph and replay it */
 #define MF_STEP_LIMITER 2 /* apply the configured stage limiter */
 #define MF_STEP_HISTORY 4 /* HistoryCallback at init and after every step */
+#define MF_STEP_NOSYNC 8  /* return once the steps are enqueued on the context's stream
+                             (mf_synchronize before touching its buffers); lets one host
+                             thread keep several contexts (an ensemble) busy at once */
 
 /* ------------------------------------------------------------- context */
 /* Creates a context on HIP device `device` with its own stream. */

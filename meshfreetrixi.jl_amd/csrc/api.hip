@@ -959,6 +959,7 @@ int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int 
     if (n_steps < 0 || !(dt > 0.0)) return fail(c, MF_ERR_ARGUMENT, "mf_ssprk33: bad step");
     if ((rc = prepare_rhs(c))) return rc;
     const bool limiter = (flags & MF_STEP_LIMITER) && c->lim_n > 0;
+    const int gflags = flags & ~MF_STEP_NOSYNC;  // what the captured step depends on
     const bool history = (flags & MF_STEP_HISTORY) && c->visc_mode == MF_VISC_RESIDUAL;
     if (resume) {
         if (!c->fsal_valid || c->graph_dt != dt)
@@ -975,7 +976,7 @@ int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int 
     }
     c->graph_dt = dt;
     if (flags & MF_STEP_GRAPH) {
-        if (!c->step_exec || c->graph_prec != precision || c->graph_flags != flags || c->graph_dt != dt) {
+        if (!c->step_exec || c->graph_prec != precision || c->graph_flags != gflags || c->graph_dt != dt) {
             free_graph(c);
             MF_HIP(c, hipStreamSynchronize(c->stream));
             MF_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -990,7 +991,7 @@ int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int 
             c->step_graph = g;
             MF_HIP(c, hipGraphInstantiate(&c->step_exec, g, nullptr, nullptr, 0));
             c->graph_prec = precision;
-            c->graph_flags = flags;
+            c->graph_flags = gflags;
             c->graph_dt = dt;
         }
         for (int n = 0; n < n_steps; ++n) MF_HIP(c, hipGraphLaunch(c->step_exec, c->stream));
@@ -1000,7 +1001,7 @@ int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int 
     }
     c->host_success_iter += n_steps;
     c->fsal_valid = true;
-    MF_HIP(c, hipStreamSynchronize(c->stream));
+    if (!(flags & MF_STEP_NOSYNC)) MF_HIP(c, hipStreamSynchronize(c->stream));
     return MF_OK;
 }
 

@@ -22,7 +22,7 @@ MF_VISC_NONE, MF_VISC_RESIDUAL, MF_VISC_UPWIND = 0, 1, 2
 MF_PREC_F64, MF_PREC_SPLIT32, MF_PREC_F64_REF = 0, 1, 2
 MF_BUF_U, MF_BUF_DU, MF_BUF_AUX0, MF_BUF_AUX1 = 0, 1, 2, 3
 MF_VAR_DENSITY, MF_VAR_PRESSURE = 0, 1
-MF_STEP_GRAPH, MF_STEP_LIMITER, MF_STEP_HISTORY = 1, 2, 4
+MF_STEP_GRAPH, MF_STEP_LIMITER, MF_STEP_HISTORY, MF_STEP_NOSYNC = 1, 2, 4, 8
 MF_TUNE_FLUX_BATCH, MF_TUNE_TRANSPOSE_BATCH, MF_TUNE_NODE_ORDER, MF_TUNE_TILES = 0, 1, 2, 3
 
 

@@ -60,6 +60,17 @@ def c5_sizes(n_clouds=64, seed=SEED0 + 5):
     return sides, phases
 
 
+def c5(n_clouds=64):
+    """C5 ensemble: Euler + Flyer hyperviscosity (k = 2, c = 1e-4) on clouds
+    of 16k..262k nodes (log-uniform), density waves with random phases,
+    split-FP32 rhs!, 50 SSPRK33 steps each."""
+    sides, phases = c5_sizes(n_clouds)
+    return [Spec(name="ensemble_%d" % r, side=int(sides[r]), seed=SEED0 + 500 + r, eq="euler", adv=(0.0, 0.0),
+                 gamma=1.4, p=5, deg=3, k=30, ic=DensityWave(1.0, 0.2, (1.0, 1.0), 1.0, float(phases[r])),
+                 cfl=0.1, nsteps=50, hv=dict(k=2, c=1e-4), rv=None)
+            for r in range(n_clouds)]
+
+
 def c3_stencils(side=1024, seed=SEED0 + 3):
     """Ragged stencil sizes for the batched-weights configuration."""
     return ragged_sizes(side * side, seed)

@@ -198,3 +198,27 @@ def test_adjoint_vs_finite_differences_c4(mf):
         errs.append(abs(fd - ad) / max(abs(ad), 1e-300))
     _say("  FD rel errors max %.2e" % max(errs))
     assert max(errs) < 1e-6
+
+
+def test_ensemble_nosync_matches_one_at_a_time(mf):
+    """Contexts stepped concurrently (MF_STEP_NOSYNC, one stream each) give
+    the same bits as the same steps one context at a time."""
+    _say("ensemble: concurrent vs one at a time")
+    specs = configs.c5(3)
+    cs = []
+    for spec in specs:
+        spec.side = min(spec.side, 96)
+        c = cases.build(spec, hv=(2, 1e-4), bc=("dirichlet", "nothing"))
+        cs.append(c)
+    res = {}
+    for conc in (True, False):
+        for c in cs:
+            c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+        for c in cs:
+            f = L.MF_STEP_GRAPH | (L.MF_STEP_NOSYNC if conc else 0)
+            c.ctx.ssprk33(0.0, c.dt, 7, L.MF_PREC_SPLIT32, f)
+        for c in cs:
+            c.ctx.synchronize()
+        res[conc] = [c.ctx.download(L.MF_BUF_U) for c in cs]
+    for a, b in zip(res[True], res[False]):
+        assert np.array_equal(a, b)

@@ -312,6 +312,7 @@ int mf_ctx_destroy(mf_ctx* c)
     dfree(c->d_sc);
     dfree(c->d_rptr);
     dfree(c->d_ridx);
+    dfree(c->d_rorder);
     dfree(c->d_rwx);
     dfree(c->d_rwy);
     dfree(c->d_status);
@@ -560,19 +561,40 @@ int mf_weights_ragged_set(mf_ctx* c, const int64_t* ptr, const int32_t* idx, int
         if (idx[q] < 0 || idx[q] >= c->N) return fail(c, MF_ERR_ARGUMENT, "mf_weights_ragged_set: index out of range");
     dfree(c->d_rptr);
     dfree(c->d_ridx);
+    dfree(c->d_rorder);
     dfree(c->d_rwx);
     dfree(c->d_rwy);
     c->d_rptr = nullptr;
     c->d_ridx = nullptr;
+    c->d_rorder = nullptr;
     c->d_rwx = c->d_rwy = nullptr;
     if ((rc = dalloc(c, &c->d_rptr, ns + 1)) || (rc = dalloc(c, &c->d_ridx, nnz)) ||
+        (rc = dalloc(c, &c->d_rorder, ns)) ||
         (rc = dalloc(c, &c->d_rwx, nnz)) || (rc = dalloc(c, &c->d_rwy, nnz)))
         return rc;
     if (!c->d_status && (rc = dalloc(c, &c->d_status, 1))) return rc;
     std::vector<int32_t> ridx(nnz);
     for (int64_t q = 0; q < nnz; ++q) ridx[q] = c->h_inv[idx[q]];
+    // group the stencils by size: one launch per size, LDS sized to it
+    std::vector<int32_t> order(ns);
+    c->rclass_start.clear();
+    c->rclass_n.clear();
+    {
+        std::vector<int64_t> cnt(kMaxStencilM + 1, 0);
+        for (int64_t e = 0; e < ns; ++e) ++cnt[ptr[e + 1] - ptr[e]];
+        std::vector<int64_t> at(kMaxStencilM + 2, 0);
+        for (int n = 1; n <= kMaxStencilM; ++n) at[n + 1] = at[n] + cnt[n];
+        for (int n = 1; n <= kMaxStencilM; ++n)
+            if (cnt[n]) {
+                c->rclass_start.push_back(at[n]);
+                c->rclass_n.push_back(n);
+            }
+        c->rclass_start.push_back(ns);
+        for (int64_t e = 0; e < ns; ++e) order[at[ptr[e + 1] - ptr[e]]++] = (int32_t)e;
+    }
     MF_HIP(c, hipMemcpyAsync(c->d_rptr, ptr, sizeof(int64_t) * (ns + 1), hipMemcpyHostToDevice, c->stream));
     MF_HIP(c, hipMemcpyAsync(c->d_ridx, ridx.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, c->stream));
+    MF_HIP(c, hipMemcpyAsync(c->d_rorder, order.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, c->stream));
     MF_HIP(c, hipStreamSynchronize(c->stream));
     c->rn = ns;
     c->rnnz = nnz;
@@ -588,8 +610,13 @@ int mf_weights_ragged_run(mf_ctx* c, int phs_power, int poly_degree, int deriv_o
     if (phs_power < 1 || poly_degree < 0 || deriv_order < 1 || deriv_order > 8)
         return fail(c, MF_ERR_ARGUMENT, "mf_weights_ragged_run: bad basis");
     MF_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
-    return launch_weights(c, c->d_rptr, c->d_ridx, 0, c->rn, c->rmaxn, phs_power, poly_degree, deriv_order,
-                          c->d_rwx, c->d_rwy, c->d_status);
+    for (size_t q = 0; q < c->rclass_n.size(); ++q) {
+        const int64_t b0 = c->rclass_start[q], b1 = c->rclass_start[q + 1];
+        if ((rc = launch_weights(c, c->d_rptr, c->d_ridx, 0, b1 - b0, c->rclass_n[q], phs_power, poly_degree,
+                                 deriv_order, c->d_rwx, c->d_rwy, c->d_status, c->d_rorder, b0)))
+            return rc;
+    }
+    return MF_OK;
 }
 
 int mf_weights_ragged_get(mf_ctx* c, double* wx, double* wy)

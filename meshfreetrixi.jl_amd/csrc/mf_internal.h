@@ -150,6 +150,9 @@ struct mf_ctx {
     int rmaxn = 0;
     int64_t* d_rptr = nullptr;
     int32_t* d_ridx = nullptr;
+    int32_t* d_rorder = nullptr;          // stencils grouped by size
+    std::vector<int64_t> rclass_start;    // per size class: first entry of d_rorder
+    std::vector<int> rclass_n;            // per size class: stencil size
     double* d_rwx = nullptr;
     double* d_rwy = nullptr;
     int* d_status = nullptr;
@@ -214,9 +217,11 @@ inline unsigned xcd_grid(int64_t nblocks) { return (unsigned)(((nblocks + 7) / 8
 
 // kernels / launchers (defined in the .hip files)
 int launch_knn(mf_ctx* c, int k, double* d_dist2);
+// n_stencils stencils: all of them (d_order null) or d_order[base + b],
+// b < n_stencils, each with at most maxn nodes (sizes the LDS)
 int launch_weights(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k,
                    int64_t n_stencils, int maxn, int p, int deg, int kd, double* d_wx,
-                   double* d_wy, int* d_status);
+                   double* d_wy, int* d_status, const int32_t* d_order = nullptr, int64_t base = 0);
 int launch_pack_ops(mf_ctx* c, int target, const double* d_wx, const double* d_wy);
 int launch_build_transpose(mf_ctx* c);
 int launch_dx_stats(mf_ctx* c, const double* d_dist2, double* dx_min, double* dx_avg);

@@ -396,6 +396,8 @@ struct WeightsArgs {
     const double* x;
     const double* y;
     const int64_t* ptr;
+    const int32_t* order;  // stencils of this launch (ragged batches), or null
+    int64_t base;          // first entry of order for this launch
     const int32_t* ridx;
     const int32_t* ell;
     int ell_k;
@@ -468,16 +470,43 @@ __device__ __forceinline__ void argmax_wave(double& v, int& idx)
     }
 }
 
+// Position (i, j), 0 <= i <= j, of the f-th element of an upper triangle
+// stored column by column (column j holds j + 1 elements), and the step to
+// the element 64 further on.  The rank-1 and rank-2 updates of the
+// factorization are spread over the wavefront in this order; each element
+// gets exactly the reference's arithmetic, so the order changes no bits.
+__device__ __forceinline__ void tri_pos(int f, int* i, int* j)
+{
+    int c = 0;
+    while (f > c) {
+        f -= c + 1;
+        ++c;
+    }
+    *i = f;
+    *j = c;
+}
+
+__device__ __forceinline__ void tri_next64(int* i, int* j)
+{
+    *i += 64;
+    while (*i > *j) {
+        *i -= *j + 1;
+        ++*j;
+    }
+}
+
 __global__ __launch_bounds__(64) void k_weights(const WeightsArgs a)
 {
 #pragma clang fp contract(off)
     extern __shared__ double sm[];
     const int ld = a.ld;
+    // the upper triangle of the saddle matrix, packed column by column
+    // (DSYTF2/DSYTRS with UPLO = 'U' never touch the lower one)
     double* A = sm;
-    double* B = A + (size_t)ld * ld;
+    double* B = A + (size_t)ld * (ld + 1) / 2;
     double* X = B + 2 * ld;
     int* ipiv = reinterpret_cast<int*>(X + 2 * ld);
-    const int64_t e = blockIdx.x;
+    const int64_t e = a.order ? (int64_t)a.order[a.base + blockIdx.x] : (int64_t)blockIdx.x;
     const int lane = threadIdx.x;
     int64_t off;
     int n;
@@ -490,7 +519,7 @@ __global__ __launch_bounds__(64) void k_weights(const WeightsArgs a)
     }
     auto sidx = [&](int j) -> int32_t { return a.ptr ? a.ridx[off + j] : a.ell[(int64_t)j * a.N + e]; };
     const int m = n + a.np;
-#define AA(i, j) A[((i)-1) + (size_t)((j)-1) * ld]
+#define AA(i, j) A[(((j)-1) * (j)) / 2 + (i)-1]
 #define BB(i, c) B[((i)-1) + (size_t)(c) * ld]
     // shift_stencil (:225-246): centre on node 0, scale each dimension
     const int32_t c0 = sidx(0);
@@ -516,14 +545,14 @@ __global__ __launch_bounds__(64) void k_weights(const WeightsArgs a)
         const int i = lane;
         if (i < n) {
             const double xi = X[2 * i], yi = X[2 * i + 1];
-            for (int j = 0; j < n; ++j) {
+            for (int j = i; j < n; ++j) {
                 const double dx = xi - X[2 * j], dy = yi - X[2 * j + 1];
-                A[i + (size_t)j * ld] = ipow(sqrt(dx * dx + dy * dy), a.p);
+                AA(i + 1, j + 1) = ipow(sqrt(dx * dx + dy * dy), a.p);
             }
             for (int t = 0; t < a.np; ++t) {
                 int ax, ay;
                 mono_exp(t, &ax, &ay);
-                A[i + (size_t)(n + t) * ld] = ipow(xi, ax) * ipow(yi, ay);
+                AA(i + 1, n + t + 1) = ipow(xi, ax) * ipow(yi, ay);
             }
             double qx = -xi, qy = -yi;
             if (i == 0) qx = qy = DBL_EPSILON;
@@ -533,8 +562,7 @@ __global__ __launch_bounds__(64) void k_weights(const WeightsArgs a)
             const int t = i - n;
             int ax, ay;
             mono_exp(t, &ax, &ay);
-            for (int j = 0; j < n; ++j) A[i + (size_t)j * ld] = ipow(X[2 * j], ax) * ipow(X[2 * j + 1], ay);
-            for (int j = n; j < m; ++j) A[i + (size_t)j * ld] = 0.0;
+            for (int j = i; j < m; ++j) AA(i + 1, j + 1) = 0.0;
             B[i] = (ax == a.kd && ay == 0) ? a.kfact : 0.0;
             B[i + ld] = (ay == a.kd && ax == 0) ? a.kfact : 0.0;
         }
@@ -544,6 +572,8 @@ __global__ __launch_bounds__(64) void k_weights(const WeightsArgs a)
     const double alpha = (1.0 + sqrt(17.0)) / 8.0;
     int info = 0;
     int k = m;
+    int ti0, tj0;  // this lane's first element of the flattened upper triangle
+    tri_pos(lane, &ti0, &tj0);
     double* W = X;  // reused: 2 x ld scratch for the 2x2-pivot multipliers
     while (k >= 1) {
         int kstep = 1;
@@ -607,19 +637,20 @@ __global__ __launch_bounds__(64) void k_weights(const WeightsArgs a)
             }
             __syncthreads();
             if (kstep == 1) {
+                // A(1:k-1, 1:k-1) += x x' * (-1/d), x = A(1:k-1, k) (DSYR, upper)
                 const double r1 = 1.0 / AA(k, k);
-                const int i = lane + 1;
-                if (i <= k - 1) {
-                    const double xi = AA(i, k);
-                    for (int j = i; j <= k - 1; ++j) {
-                        const double xj = AA(j, k);
-                        if (xj != 0.0) {
-                            const double temp = -r1 * xj;
-                            AA(i, j) = AA(i, j) + xi * temp;
-                        }
+                const int T = (k - 1) * k / 2;
+                int ti = ti0, tj = tj0;
+                for (int f = lane; f < T; f += 64) {
+                    const double xj = AA(tj + 1, k);
+                    if (xj != 0.0) {
+                        const double temp = -r1 * xj;
+                        AA(ti + 1, tj + 1) = AA(ti + 1, tj + 1) + AA(ti + 1, k) * temp;
                     }
+                    tri_next64(&ti, &tj);
                 }
                 __syncthreads();
+                const int i = lane + 1;
                 if (i <= k - 1) AA(i, k) = r1 * AA(i, k);
             } else if (k > 2) {
                 double d12 = AA(k - 1, k);
@@ -633,11 +664,13 @@ __global__ __launch_bounds__(64) void k_weights(const WeightsArgs a)
                     W[ld + j - 1] = d12 * (d22 * AA(j, k) - AA(j, k - 1));  // wk
                 }
                 __syncthreads();
-                const int i = lane + 1;
-                if (i <= k - 2) {
-                    const double aik = AA(i, k), aik1 = AA(i, k - 1);
-                    for (int jc = i; jc <= k - 2; ++jc)
-                        AA(i, jc) = AA(i, jc) - aik * W[ld + jc - 1] - aik1 * W[jc - 1];
+                // A(1:k-2, 1:k-2) -= [A(:,k) A(:,k-1)] [wk wkm1]' (upper)
+                const int T = (k - 2) * (k - 1) / 2;
+                int ti = ti0, tj = tj0;
+                for (int f = lane; f < T; f += 64) {
+                    const int ii = ti + 1, jc = tj + 1;
+                    AA(ii, jc) = AA(ii, jc) - AA(ii, k) * W[ld + jc - 1] - AA(ii, k - 1) * W[jc - 1];
+                    tri_next64(&ti, &tj);
                 }
                 __syncthreads();
                 if (j <= k - 2) {
@@ -719,11 +752,13 @@ __global__ __launch_bounds__(64) void k_weights(const WeightsArgs a)
         const bool one = ipiv[k - 1] > 0;
         if (lane < 2) {
             double temp = 0.0;
+#pragma unroll 8
             for (int i = 1; i <= k - 1; ++i) temp = temp + BB(i, lane) * AA(i, k);
             const double nk = BB(k, lane) + -1.0 * temp;
             double nk1 = 0.0;
             if (!one) {
                 double temp2 = 0.0;
+#pragma unroll 8
                 for (int i = 1; i <= k - 1; ++i) temp2 = temp2 + BB(i, lane) * AA(i, k + 1);
                 nk1 = BB(k + 1, lane) + -1.0 * temp2;
             }
@@ -751,12 +786,15 @@ __global__ __launch_bounds__(64) void k_weights(const WeightsArgs a)
 }
 
 int launch_weights(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k, int64_t n_stencils,
-                   int maxn, int p, int deg, int kd, double* d_wx, double* d_wy, int* d_status)
+                   int maxn, int p, int deg, int kd, double* d_wx, double* d_wy, int* d_status,
+                   const int32_t* d_order, int64_t base)
 {
     WeightsArgs a;
     a.x = c->d_x;
     a.y = c->d_y;
     a.ptr = d_ptr;
+    a.order = d_order;
+    a.base = base;
     a.ridx = d_ptr ? d_idx : nullptr;
     a.ell = d_ptr ? nullptr : d_idx;
     a.ell_k = ell_k;
@@ -775,7 +813,7 @@ int launch_weights(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int el
     a.wx = d_wx;
     a.wy = d_wy;
     a.status = d_status;
-    const size_t lds = sizeof(double) * ((size_t)a.ld * a.ld + 4 * a.ld) + sizeof(int) * a.ld;
+    const size_t lds = sizeof(double) * ((size_t)a.ld * (a.ld + 1) / 2 + 4 * a.ld) + sizeof(int) * a.ld;
     if (n_stencils > 0x7fffffff) return fail(c, MF_ERR_ARGUMENT, "weights: too many stencils");
     hipLaunchKernelGGL(k_weights, dim3((unsigned)n_stencils), dim3(64), lds, c->stream, a);
     MF_HIP(c, hipGetLastError());

@@ -149,6 +149,8 @@ static void free_state(mf_ctx* c)
     dfree(c->d_S);
     dfree(c->d_T);
     dfree(c->d_G);
+    dfree(c->d_du32);
+    c->d_du32 = nullptr;
     dfree(c->d_eps);
     dfree(c->d_epsc);
     dfree(c->d_approx);
@@ -206,6 +208,7 @@ int ensure_state(mf_ctx* c)
     if (!c->d_T && (rc = dalloc(c, &c->d_T, nv))) return rc;
     if (c->visc_mode != MF_VISC_NONE) {
         if (!c->d_G && (rc = dalloc(c, &c->d_G, 2 * nv))) return rc;
+        if (!c->d_du32 && (rc = dalloc(c, &c->d_du32, nv))) return rc;
         if (!c->d_eps && (rc = dalloc(c, &c->d_eps, c->N))) return rc;
         if (!c->d_epsc && (rc = dalloc(c, &c->d_epsc, c->N))) return rc;
         if (!c->d_approx) {

@@ -134,61 +134,80 @@ __device__ __forceinline__ bool lex_less(const double* a, const double* b)
 }
 
 // Fixed-order block reductions over 4-vectors held in LDS (kBlock entries).
-__device__ __forceinline__ void block_sum4(double (*red)[4])
+// Block-wide sum of every thread's 4 values: butterfly within each
+// wavefront (a + b and b + a round alike, so all lanes hold the same bits),
+// then the 4 wavefront sums in a fixed order.  Every thread gets the result.
+__device__ __forceinline__ void block_sum4(double* v)
 {
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
+    __shared__ double wres[kBlock / 64][4];
+    for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) red[threadIdx.x][v] += red[threadIdx.x + w][v];
-        __syncthreads();
+        for (int q = 0; q < 4; ++q) v[q] += __shfl_xor(v[q], o);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wres[threadIdx.x >> 6][q] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double t = wres[0][q];
+        for (int w = 1; w < kBlock / 64; ++w) t = t + wres[w][q];
+        v[q] = t;
     }
+    __syncthreads();
 }
 
-__device__ __forceinline__ void block_lexmax4(double (*red)[4])
+// Block-wide lexicographic maximum of every thread's 4-vector, the same way.
+__device__ __forceinline__ void block_lexmax4(double* v)
 {
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w && lex_less(red[threadIdx.x], red[threadIdx.x + w]))
+    __shared__ double wres[kBlock / 64][4];
+    for (int o = 32; o > 0; o >>= 1) {
+        double c[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = red[threadIdx.x + w][v];
-        __syncthreads();
+        for (int q = 0; q < 4; ++q) c[q] = __shfl_xor(v[q], o);
+        if (lex_less(v, c))
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = c[q];
     }
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wres[threadIdx.x >> 6][q] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = wres[0][q];
+    for (int w = 1; w < kBlock / 64; ++w)
+        if (lex_less(v, wres[w]))
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = wres[w][q];
+    __syncthreads();
 }
 
 // mean(u) from the per-block partial sums (every caller gets the same bits).
-__device__ __forceinline__ void mean_from_partials(const double* psum, int nparts, int64_t N,
-                                                   double (*red)[4], double* mean)
+__device__ __forceinline__ void mean_from_partials(const double* psum, int nparts, int64_t N, double* mean)
 {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = 0.0;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nparts; b += kBlock)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) red[threadIdx.x][v] += psum[4 * b + v];
-    block_sum4(red);
+        for (int v = 0; v < 4; ++v) acc[v] += psum[4 * b + v];
+    block_sum4(acc);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) mean[v] = red[0][v] / (double)(4 * N);
-    __syncthreads();
+    for (int v = 0; v < 4; ++v) mean[v] = acc[v] / (double)(4 * N);
 }
 
 // ninf from the per-block partial lexicographic maxima, zeros -> eps().
-__device__ __forceinline__ void ninf_from_partials(const double* pmax, int nparts, double (*red)[4],
-                                                   double* ninf)
+__device__ __forceinline__ void ninf_from_partials(const double* pmax, int nparts, double* ninf)
 {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = -1.0;
+    double best[4] = {-1.0, -1.0, -1.0, -1.0};
     for (int b = threadIdx.x; b < nparts; b += kBlock) {
         double c[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) c[v] = pmax[4 * b + v];
-        if (lex_less(red[threadIdx.x], c))
+        if (lex_less(best, c))
 #pragma unroll
-            for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = c[v];
+            for (int v = 0; v < 4; ++v) best[v] = c[v];
     }
-    block_lexmax4(red);
+    block_lexmax4(best);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) ninf[v] = (red[0][v] == 0.0) ? DBL_EPSILON : red[0][v];
-    __syncthreads();
+    for (int v = 0; v < 4; ++v) ninf[v] = (best[v] == 0.0) ? DBL_EPSILON : best[v];
 }
 
 }  // namespace mf

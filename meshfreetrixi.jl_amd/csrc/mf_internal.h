@@ -123,6 +123,7 @@ struct mf_ctx {
     double* d_S = nullptr;
     double* d_T = nullptr;
     double* d_G = nullptr;
+    float* d_du32 = nullptr;  // split-FP32 partial du between the two viscosity phases
     double* d_eps = nullptr;
     int8_t* d_epsc = nullptr;
     double* d_approx = nullptr;

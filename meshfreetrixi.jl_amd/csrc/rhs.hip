@@ -44,6 +44,8 @@ struct RhsArgs {
     int nparts;
     double* norms_out;  // ninf published here (inspection)
     double* __restrict__ G;
+    float* __restrict__ G32;   // split-FP32: G in FP32 (same bytes as the first half of G)
+    float* __restrict__ du32;  // split-FP32: phase A's du in FP32
     double* __restrict__ eps;
     int8_t* __restrict__ epsc;
     // column-sorted operators (reference operation order)
@@ -88,6 +90,33 @@ __device__ __forceinline__ void slip_project(const double* nrm2, double* s)
     const double dot = m1 * e1 + m2 * e2;
     s[1] = fma(-dot, e1, m1);
     s[2] = fma(-dot, e2, m2);
+}
+
+// FP32 node records (split-FP32 intermediates): V floats per record.
+template <int V>
+__device__ __forceinline__ void store_state32(float* __restrict__ p, int64_t j, const double* s)
+{
+    if constexpr (V == 4) {
+        reinterpret_cast<float4*>(p)[j] = make_float4((float)s[0], (float)s[1], (float)s[2], (float)s[3]);
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v) p[j * V + v] = (float)s[v];
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void load_state32(const float* __restrict__ p, int64_t j, double* s)
+{
+    if constexpr (V == 4) {
+        const float4 q = reinterpret_cast<const float4*>(p)[j];
+        s[0] = q.x;
+        s[1] = q.y;
+        s[2] = q.z;
+        s[3] = q.w;
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v) s[v] = p[j * V + v];
+    }
 }
 
 // Second boundary pass on du (rbfsolver.jl:422-425): Dirichlet nodes get
@@ -191,17 +220,27 @@ __device__ __forceinline__ double node_viscosity(const RhsArgs& a, int64_t i, co
 // G_i = eps_i (Dx u, Dy u)_i for the transposed pass (hyperviscosity.jl:401-407).
 // Residual-viscosity norms: every block reduces the max pass's partials
 // (same bits everywhere); block 0 publishes them.
-template <bool TL>
-__device__ __forceinline__ void block_norms(const RhsArgs& a, double* ninf, double* lds)
+// Residual-viscosity norms in the stencil kernels: each thread loads its
+// share of the max pass's per-block partials at kernel start (the loads
+// overlap the halo staging and the stencil loop) and the block reduces them
+// just before the viscosity is needed; every block gets the same bits.
+__device__ __forceinline__ void prefetch_max_partials(const RhsArgs& a, double* pm)
 {
-    double(*red)[4];
-    if constexpr (TL) {
-        red = reinterpret_cast<double(*)[4]>(lds);  // before the halo is staged
-    } else {
-        __shared__ double red_s[kBlock][4];
-        red = red_s;
+    for (int b = threadIdx.x; b < a.nparts; b += kBlock) {
+        double c[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) c[v] = a.pmax[4 * b + v];
+        if (lex_less(pm, c))
+#pragma unroll
+            for (int v = 0; v < 4; ++v) pm[v] = c[v];
     }
-    ninf_from_partials(a.pmax, a.nparts, red, ninf);
+}
+
+__device__ __forceinline__ void finish_norms(const RhsArgs& a, double* pm, double* ninf)
+{
+    block_lexmax4(pm);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ninf[v] = (pm[v] == 0.0) ? DBL_EPSILON : pm[v];
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.norms_out)
 #pragma unroll
         for (int v = 0; v < 4; ++v) a.norms_out[4 + v] = ninf[v];
@@ -295,13 +334,16 @@ __device__ __forceinline__ void flux_nb(const EqParams& e, const double* r, doub
 
 // Node of this thread and, with LDS tiles (TL), the tile's stencil halo
 // staged in lds: halo entry q holds the record of node ha_ids[ha_off[t] + q].
-// Returns -1 for threads without a node (after the block barrier).
+// Returns -1 for threads without a node (after the block barrier) and
+// kNoBlock when the whole block has none.
+constexpr int64_t kNoBlock = -2;
+
 template <int V, bool TL>
 __device__ __forceinline__ int64_t stage_halo_a(const RhsArgs& a, double* lds)
 {
     if constexpr (TL) {
         const int64_t t = remapped_block();
-        if (t >= a.n_tiles) return -1;  // whole block
+        if (t >= a.n_tiles) return kNoBlock;  // whole block
         const int64_t h0 = a.ha_off[t];
         const int nh = (int)(a.ha_off[t + 1] - h0);
         for (int q = threadIdx.x; q < nh; q += kBlock) {
@@ -319,6 +361,7 @@ __device__ __forceinline__ int64_t stage_halo_a(const RhsArgs& a, double* lds)
         const int64_t i = t * kBlock + threadIdx.x;
         return i < a.N ? i : -1;
     } else {
+        if (remapped_block() * kBlock >= a.N) return kNoBlock;  // whole block
         const int64_t i = remapped_node();
         return i < a.N ? i : -1;
     }
@@ -338,12 +381,16 @@ template <int V, int NS, bool HV, bool RV, int BT, bool TL>
 __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
 {
     extern __shared__ double lds[];
-    double ninf[4] = {1.0, 1.0, 1.0, 1.0};
-    if constexpr (RV) {
-        if (a.visc == MF_VISC_RESIDUAL) block_norms<TL>(a, ninf, lds);
-    }
-    const int64_t i = stage_halo_a<V, TL>(a, lds);
-    if (i < 0) return;
+    double pm[4] = {-1.0, -1.0, -1.0, -1.0};
+    const bool rvres = RV && a.visc == MF_VISC_RESIDUAL;
+    if (rvres) prefetch_max_partials(a, pm);
+    const int64_t i_node = stage_halo_a<V, TL>(a, lds);
+    if (i_node == kNoBlock) return;  // whole block
+    // a thread without a node shadows the last node until the block-wide
+    // norm reduction, then leaves
+    const bool live = i_node >= 0;
+    if (!rvres && !live) return;
+    const int64_t i = live ? i_node : a.N - 1;
     const int64_t N = a.N;
     const int K = NS ? NS : a.k;
     double ax[V], ay[V], gx[V], gy[V], hh[V];
@@ -409,6 +456,9 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
         double ui[V];
         load_state<V>(a.u, i, ui);
         int8_t code = 1;
+        double ninf[4] = {1.0, 1.0, 1.0, 1.0};
+        if (rvres) finish_norms(a, pm, ninf);
+        if (!live) return;
         const double e = node_viscosity(a, i, ui, du, ninf, &code);
         a.eps[i] = e;
         a.epsc[i] = code;
@@ -492,12 +542,16 @@ template <int V, int NS, bool HV, bool RV, int BT, bool TL>
 __global__ __launch_bounds__(kBlock) void k_rhs_ref(const RhsArgs a)
 {
     extern __shared__ double lds[];
-    double ninf[4] = {1.0, 1.0, 1.0, 1.0};
-    if constexpr (RV) {
-        if (a.visc == MF_VISC_RESIDUAL) block_norms<TL>(a, ninf, lds);
-    }
-    const int64_t i = stage_halo_a<V, TL>(a, lds);
-    if (i < 0) return;
+    double pm[4] = {-1.0, -1.0, -1.0, -1.0};
+    const bool rvres = RV && a.visc == MF_VISC_RESIDUAL;
+    if (rvres) prefetch_max_partials(a, pm);
+    const int64_t i_node = stage_halo_a<V, TL>(a, lds);
+    if (i_node == kNoBlock) return;  // whole block
+    // a thread without a node shadows the last node until the block-wide
+    // norm reduction, then leaves
+    const bool live = i_node >= 0;
+    if (!rvres && !live) return;
+    const int64_t i = live ? i_node : a.N - 1;
     const int K = NS ? NS : a.k;
     double du[V], gx[V], gy[V];
 #pragma unroll
@@ -511,6 +565,9 @@ __global__ __launch_bounds__(kBlock) void k_rhs_ref(const RhsArgs a)
         double ui[V];
         load_state<V>(a.u, i, ui);
         int8_t code = 1;
+        double ninf[4] = {1.0, 1.0, 1.0, 1.0};
+        if (rvres) finish_norms(a, pm, ninf);
+        if (!live) return;
         const double e = node_viscosity(a, i, ui, du, ninf, &code);
         a.eps[i] = e;
         a.epsc[i] = code;
@@ -538,12 +595,16 @@ template <int V, int NS, bool HV, bool RV, bool D16, int BT, bool TL>
 __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
 {
     extern __shared__ double lds[];
-    double ninf[4] = {1.0, 1.0, 1.0, 1.0};
-    if constexpr (RV) {
-        if (a.visc == MF_VISC_RESIDUAL) block_norms<TL>(a, ninf, lds);
-    }
-    const int64_t i = stage_halo_a<V, TL>(a, lds);
-    if (i < 0) return;
+    double pm[4] = {-1.0, -1.0, -1.0, -1.0};
+    const bool rvres = RV && a.visc == MF_VISC_RESIDUAL;
+    if (rvres) prefetch_max_partials(a, pm);
+    const int64_t i_node = stage_halo_a<V, TL>(a, lds);
+    if (i_node == kNoBlock) return;  // whole block
+    // a thread without a node shadows the last node until the block-wide
+    // norm reduction, then leaves
+    const bool live = i_node >= 0;
+    if (!rvres && !live) return;
+    const int64_t i = live ? i_node : a.N - 1;
     const int64_t N = a.N;
     const int K = NS ? NS : a.k;
     double ui[V], fi1[V], fi2[V];
@@ -620,6 +681,9 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
     }
     if constexpr (RV) {
         int8_t code = 1;
+        double ninf[4] = {1.0, 1.0, 1.0, 1.0};
+        if (rvres) finish_norms(a, pm, ninf);
+        if (!live) return;
         const double e = node_viscosity(a, i, ui, du, ninf, &code);
         a.eps[i] = e;
         a.epsc[i] = code;
@@ -629,15 +693,16 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
             g[v] = e * (double)gx[v];
             g[V + v] = e * (double)gy[v];
         }
-        store_state<V>(a.G, 2 * i, g);
-        store_state<V>(a.G, 2 * i + 1, g + V);
+        // split-FP32: G and the partial du go to phase B in FP32
+        store_state32<V>(a.G32, 2 * i, g);
+        store_state32<V>(a.G32, 2 * i + 1, g + V);
         if constexpr (HV) {
             if (!a.hv_first) {
 #pragma unroll
                 for (int v = 0; v < V; ++v) du[v] = fma(-a.hv_gamma, (double)hh[v], du[v]);
             }
         }
-        store_state<V>(a.du, i, du);
+        store_state32<V>(a.du32, i, du);
         return;
     }
     mask_du<V>(a, i, du);
@@ -667,11 +732,17 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
         const int nh = (int)(a.hb_off[t + 1] - h0);
         for (int q = threadIdx.x; q < nh; q += kBlock) {
             const int64_t j = a.hb_ids[h0 + q];
-            double g[2 * V];
-            load_state<V>(a.G, 2 * j, g);
-            load_state<V>(a.G, 2 * j + 1, g + V);
-            store_state<V>(lds, 2 * q, g);
-            store_state<V>(lds, 2 * q + 1, g + V);
+            if constexpr (TM == 1) {  // FP32 G: 8 floats per halo node
+                float* ldsf = reinterpret_cast<float*>(lds);
+                reinterpret_cast<float4*>(ldsf)[2 * q] = reinterpret_cast<const float4*>(a.G32)[2 * j];
+                reinterpret_cast<float4*>(ldsf)[2 * q + 1] = reinterpret_cast<const float4*>(a.G32)[2 * j + 1];
+            } else {
+                double g[2 * V];
+                load_state<V>(a.G, 2 * j, g);
+                load_state<V>(a.G, 2 * j + 1, g + V);
+                store_state<V>(lds, 2 * q, g);
+                store_state<V>(lds, 2 * q + 1, g + V);
+            }
         }
         __syncthreads();
         const int pos = threadIdx.x;
@@ -714,7 +785,11 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
         double g0[BT][V], g1[BT][V];
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
-            if constexpr (TL) {
+            if constexpr (TM == 1) {
+                const float* gsrc = TL ? reinterpret_cast<const float*>(lds) : a.G32;
+                load_state32<V>(gsrc, 2 * (int64_t)r[b], g0[b]);
+                load_state32<V>(gsrc, 2 * (int64_t)r[b] + 1, g1[b]);
+            } else if constexpr (TL) {
                 load_state<V>(lds, 2 * (int64_t)r[b], g0[b]);
                 load_state<V>(lds, 2 * (int64_t)r[b] + 1, g1[b]);
             } else {
@@ -738,7 +813,10 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
         }
     }
     double du[V];
-    load_state<V>(a.du, i, du);
+    if constexpr (TM == 1)
+        load_state32<V>(a.du32, i, du);
+    else
+        load_state<V>(a.du, i, du);
 #pragma unroll
     for (int v = 0; v < V; ++v) du[v] = (du[v] - tx[v]) - ty[v];
     if constexpr (TM == 2) {
@@ -836,7 +914,6 @@ constexpr int kRed = 4;
 __global__ __launch_bounds__(kBlock) void k_rv_sum(int64_t N, int64_t chunk, double* u, double* psum,
                                                    const BcArgs bc)
 {
-    __shared__ double red[kBlock][4];
     const int64_t lo = (int64_t)blockIdx.x * chunk;
     const int64_t hi = min(N, lo + chunk);
     double acc[4] = {0, 0, 0, 0};
@@ -864,12 +941,10 @@ __global__ __launch_bounds__(kBlock) void k_rv_sum(int64_t N, int64_t chunk, dou
             for (int v = 0; v < 4; ++v) acc[v] += s[r][v];
         }
     }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = acc[v];
-    block_sum4(red);
+    block_sum4(acc);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) psum[4 * blockIdx.x + v] = red[0][v];
+        for (int v = 0; v < 4; ++v) psum[4 * blockIdx.x + v] = acc[v];
 }
 
 // Max pass: mean from the sum pass's partials (recomputed identically in
@@ -879,22 +954,25 @@ __global__ __launch_bounds__(kBlock) void k_rv_max(int64_t N, int64_t chunk, con
                                                    const double* psum, int nparts, double* pmax,
                                                    StepScalars* sc)
 {
-    __shared__ double red[kBlock][4];
+    const int64_t lo = (int64_t)blockIdx.x * chunk;
+    const int64_t hi = min(N, lo + chunk);
+    double s[kRed][4];
+    auto load_trip = [&](int64_t base) {
+#pragma unroll
+        for (int r = 0; r < kRed; ++r) {
+            const int64_t i = base + (int64_t)r * kBlock;
+            if (i < hi) load_state<4>(u, i, s[r]);
+        }
+    };
+    load_trip(lo + threadIdx.x);  // in flight while the mean is reduced
     double mean[4];
-    mean_from_partials(psum, nparts, N, red, mean);
+    mean_from_partials(psum, nparts, N, mean);
     if (blockIdx.x == 0 && threadIdx.x == 0)
 #pragma unroll
         for (int v = 0; v < 4; ++v) sc->norms[v] = mean[v];
-    const int64_t lo = (int64_t)blockIdx.x * chunk;
-    const int64_t hi = min(N, lo + chunk);
     double best[4] = {-1.0, -1.0, -1.0, -1.0};  // below every |.|
     for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (int64_t)kBlock * kRed) {
-        double s[kRed][4];
-#pragma unroll
-        for (int r = 0; r < kRed; ++r) {
-            const int64_t i = i0 + (int64_t)r * kBlock;
-            if (i < hi) load_state<4>(u, i, s[r]);
-        }
+        if (i0 != lo + threadIdx.x) load_trip(i0);
 #pragma unroll
         for (int r = 0; r < kRed; ++r) {
             if (i0 + (int64_t)r * kBlock >= hi) break;
@@ -906,12 +984,10 @@ __global__ __launch_bounds__(kBlock) void k_rv_max(int64_t N, int64_t chunk, con
                 for (int v = 0; v < 4; ++v) best[v] = c[v];
         }
     }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) red[threadIdx.x][v] = best[v];
-    block_lexmax4(red);
+    block_lexmax4(best);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) pmax[4 * blockIdx.x + v] = red[0][v];
+        for (int v = 0; v < 4; ++v) pmax[4 * blockIdx.x + v] = best[v];
 }
 
 // ------------------------------------------------------ small kernels
@@ -1146,6 +1222,8 @@ static RhsArgs make_args(mf_ctx* c, double* u, double* du)
     a.nparts = c->n_pmax;
     a.norms_out = c->d_sc ? c->d_sc->norms : nullptr;
     a.G = c->d_G;
+    a.G32 = reinterpret_cast<float*>(c->d_G);
+    a.du32 = c->d_du32;
     a.eps = c->d_eps;
     a.epsc = c->d_epsc;
     a.sell_off = c->d_sell_off;
@@ -1277,10 +1355,7 @@ static void launch_flux_bt(mf_ctx* c, int prec, const RhsArgs& a, bool hv, bool 
     const dim3 G(grid), B(kBlock);
     hipStream_t s = c->stream;
     const bool tl = c->tiles_a;
-    // halo records; at least the 256 x 4 doubles the norm reduction borrows
-    const size_t lds = tl ? std::max<size_t>((size_t)c->ha_max * rec_len<V>() * sizeof(double),
-                                             (size_t)kBlock * 4 * sizeof(double))
-                          : 0;
+    const size_t lds = tl ? (size_t)c->ha_max * rec_len<V>() * sizeof(double) : 0;  // halo records
     if (prec == MF_PREC_F64) {
         if (tl)
             launch_f64<V, NS, BT, true>(c, a, hv, rv, G, lds);
@@ -1340,7 +1415,7 @@ template <int TM>
 static void launch_transpose(mf_ctx* c, const RhsArgs& a, unsigned grid)
 {
     if (c->tiles_b)
-        launch_transpose_tl<TM, true>(c, a, grid, (size_t)c->hb_max * 8 * sizeof(double));
+        launch_transpose_tl<TM, true>(c, a, grid, (size_t)c->hb_max * 8 * (TM == 1 ? sizeof(float) : sizeof(double)));
     else
         launch_transpose_tl<TM, false>(c, a, grid, 0);
 }

@@ -5,12 +5,14 @@ Workload (the headline configuration): compressible Euler (gamma = 1.4) with
 residual viscosity on 262,144 jittered nodes (512 x 512, +-0.3h, seeded),
 PHS r^5 + degree-3 polynomials, 30-point stencils, density-wave state,
 Dirichlet inflow / do-nothing outflow.  One step = one rhs! over the whole
-cloud (262,144 node-evaluations) on device-resident state, FP64.
+cloud (262,144 node-evaluations) on device-resident state, FP64 in the
+reference's operation order (the mode within 1e-12 of the reference).
 
 Printed as one JSON line, with:
   roofline      the dominant kernel's algorithmic bytes / its event-timed
                 duration vs 8 TB/s (whole-rhs! fraction reported beside it)
   cpu_baseline  the FP64 CPU oracle timed on this host (1 thread)
+  f64_fma       the fused-multiply-add FP64 rhs! (faster, within 5e-12)
   split32       the split-FP32 rhs! rate, speed-up and error vs FP64
   adjoint       reverse-sweep / forward time of 100 SSPRK33 steps (65,536 nodes)
   weights       batched weight generation for 1,048,576 ragged stencils
@@ -77,7 +79,7 @@ def rhs_bytes(N, k, V=4, visc=True):
     return flux
 
 
-def kernel_bytes(N, k, ti, V=4):
+def kernel_bytes(N, k, ti, V=4, passes=1):
     """Bytes the stencil kernel (phase A) and the transposed gather (phase B)
     must move in the format they use, each array once (gathers of the halo
     not counted).  Phase A reads u, the stencil positions, (Dx, Dy) and
@@ -85,12 +87,13 @@ def kernel_bytes(N, k, ti, V=4):
     transposed positions and (Dx, Dy) (N k entries, padding not counted), G
     and du and writes du.  With LDS tiles the positions are 16-bit and each
     tile's halo list (int32) and offsets are read once; without, the
-    positions are int32 node indices."""
+    positions are int32 node indices.  passes: times the stencil positions
+    are read (2 in the reference-order kernel: x pass, then y pass)."""
     if ti["tiles_a"]:
-        a = N * (V * 8 + 2 * k + 16 * k + V * 8 + V * 8 + 2 * V * 8 + 9) + 4 * ti["halo_a_total"] \
+        a = N * (V * 8 + 2 * k * passes + 16 * k + V * 8 + V * 8 + 2 * V * 8 + 9) + 4 * ti["halo_a_total"] \
             + 8 * (ti["n_tiles"] + 1)
     else:
-        a = N * (V * 8 + 4 * k + 16 * k + V * 8 + V * 8 + 2 * V * 8 + 9)
+        a = N * (V * 8 + 4 * k * passes + 16 * k + V * 8 + V * 8 + 2 * V * 8 + 9)
     if ti["tiles_b"]:
         b = N * k * (2 + 16) + N * (2 * V * 8 + V * 8 + V * 8 + 1) + 4 * ti["halo_b_total"] \
             + ti["n_tiles"] * (256 + 4 * (8 + 4) + 8)
@@ -99,7 +102,7 @@ def kernel_bytes(N, k, ti, V=4):
     return a, b
 
 
-def measured_traffic(kernel, tuning, side, tiled):
+def measured_traffic(kernel, tuning, side, tiled, mode="f64_ref"):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (FETCH_SIZE
     scaled by the copy-kernel calibration recorded beside it), or None."""
@@ -109,12 +112,14 @@ def measured_traffic(kernel, tuning, side, tiled):
     with open(path) as f:
         tab = json.load(f)
     tl = "true" if tiled else "false"
+    tm = 2 if mode == "f64_ref" else 0
     if kernel == "transpose":
-        want = "k_rhs_transpose<4, 0, %d, %s>" % (tuning["f64"]["transpose"] or 4, tl)
-        family = "k_rhs_transpose<4, 0,"
+        want = "k_rhs_transpose<4, %d, %d, %s>" % (tm, tuning[mode]["transpose"] or 4, tl)
+        family = "k_rhs_transpose<4, %d," % tm
     else:
-        want = "k_rhs_f64<4, 30, false, true, %d, %s>" % (tuning["f64"]["flux"] or 5, tl)
-        family = "k_rhs_f64<4, 30, false, true,"
+        kern = "k_rhs_ref" if mode == "f64_ref" else "k_rhs_f64"
+        want = "%s<4, 30, false, true, %d, %s>" % (kern, tuning[mode]["flux"] or 5, tl)
+        family = "%s<4, 30, false, true," % kern
     names = [n for n in tab["kernels"] if want in n] or \
         [n for n in tab["kernels"] if family in n and n.endswith("%s>(mf::RhsArgs)" % tl)]
     if not names:
@@ -280,21 +285,24 @@ def main():
     t_now = 3 * info["dt"]
     log("knn %.1f ms, weights %.1f ms; warmup %d rhs!" % (1e3 * info["t_knn"], 1e3 * info["t_weights"],
                                                           args.warmup))
-    ctx.time_rhs(t_now, L.MF_PREC_F64, max(1, args.warmup))
+    ctx.time_rhs(t_now, L.MF_PREC_F64_REF, max(1, args.warmup))
     ctx.synchronize()
     log("autotune: load-batch sizes of the stencil and transposed kernels")
     tuning = tune.autotune(ctx, t_now, N, k, L.MF_EQ_EULER_2D, L.MF_VISC_RESIDUAL,
                            precisions=(L.MF_PREC_F64, L.MF_PREC_SPLIT32, L.MF_PREC_F64_REF),
                            use_cache=False)
     log("tuning: " + json.dumps({p: (v["flux"], v["transpose"]) for p, v in tuning.items()}))
-    log("timing %d rhs! (FP64)" % args.steps)
+    # the headline FP64 mode is the reference's operation order (the one within
+    # 1e-12 of the reference at this size); the fused-multiply-add FP64 and
+    # the split-FP32 modes are timed beside it
+    log("timing %d rhs! (FP64, reference operation order)" % args.steps)
     w0 = time.perf_counter()
-    rhs_ms, parts = ctx.time_rhs(t_now, L.MF_PREC_F64, args.steps)
+    rhs_ms, parts = ctx.time_rhs(t_now, L.MF_PREC_F64_REF, args.steps)
     ctx.synchronize()
     wall = time.perf_counter() - w0
     value = N / (rhs_ms * 1e-3)
     tinfo = ctx.tile_info()
-    ba, bb = kernel_bytes(N, k, tinfo)
+    ba, bb = kernel_bytes(N, k, tinfo, passes=2)
     kernels = {"stencil": (parts[0], ba), "transpose": (parts[1], bb)}
     dom = max(kernels, key=lambda q: kernels[q][0])
     kms, kbytes = kernels[dom]
@@ -304,21 +312,22 @@ def main():
                                                tinfo["tiles_b"] if dom == "transpose" else tinfo["tiles_a"])
     log("rhs %.4f ms (%.3e node-evals/s); stencil %.4f ms, transpose %.4f ms, reductions %.4f ms, bc %.4f ms"
         % (rhs_ms, value, parts[0], parts[1], parts[2], parts[3]))
+    log("timing %d rhs! (FP64 fused multiply-add, split-FP32)" % args.steps)
+    f_ms, f_parts = ctx.time_rhs(t_now, L.MF_PREC_F64, args.steps)
+    s_ms, s_parts = ctx.time_rhs(t_now, L.MF_PREC_SPLIT32, args.steps)
+    fa, fb = kernel_bytes(N, k, tinfo)
+    log("fma rhs %.4f ms; split-FP32 rhs %.4f ms" % (f_ms, s_ms))
     # the same call with the neighbour gathers from global memory (no LDS tiles)
     ctx.set_tuning(L.MF_TUNE_TILES, 0)
-    nt_ms, nt_parts = ctx.time_rhs(t_now, L.MF_PREC_F64, max(200, args.steps // 10))
+    nt_ms, nt_parts = ctx.time_rhs(t_now, L.MF_PREC_F64_REF, max(200, args.steps // 10))
     ctx.set_tuning(L.MF_TUNE_TILES, 1)
-    ctx.time_rhs(t_now, L.MF_PREC_F64, 3)
+    ctx.time_rhs(t_now, L.MF_PREC_F64_REF, 3)
     log("without LDS tiles: rhs %.4f ms; stencil %.4f ms, transpose %.4f ms" % (nt_ms, nt_parts[0], nt_parts[1]))
-    # split-FP32 and the reference-order FP64
-    log("timing %d rhs! (split-FP32, FP64 reference order)" % args.steps)
-    s_ms, s_parts = ctx.time_rhs(t_now, L.MF_PREC_SPLIT32, args.steps)
-    r_ms, r_parts = ctx.time_rhs(t_now, L.MF_PREC_F64_REF, args.steps)
     dus = {}
-    for name, prec in (("f64", L.MF_PREC_F64), ("split32", L.MF_PREC_SPLIT32), ("f64_ref", L.MF_PREC_F64_REF)):
+    for name, prec in (("f64_ref", L.MF_PREC_F64_REF), ("f64_fma", L.MF_PREC_F64), ("split32", L.MF_PREC_SPLIT32)):
         ctx.rhs(t_now, prec)
         dus[name] = ctx.download(L.MF_BUF_DU)
-    du64 = dus["f64"]
+    du64 = dus["f64_ref"]
     s_err = float(np.abs(dus["split32"] - du64).max() / np.abs(du64).max())
     out = {
         "metric": METRIC,
@@ -334,8 +343,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "Euler + residual viscosity rhs!, %d jittered nodes (%dx%d), 30-point "
-                               "PHS r^5 + degree-3 stencils, FP64" % (N, args.side, args.side),
-                   "nodes": N, "stencil": k, "precision": "f64", "timed_wall_s": wall},
+                               "PHS r^5 + degree-3 stencils, FP64 in the reference's operation order"
+                               % (N, args.side, args.side),
+                   "nodes": N, "stencil": k, "precision": "f64_ref", "timed_wall_s": wall},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                      "traffic_kernel": traffic_kernel,
@@ -343,12 +353,14 @@ def main():
                      "rhs_frac": rbytes / (rhs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "rhs_bytes": rbytes},
         "kernels_ms": {"stencil": parts[0], "transpose": parts[1], "reductions": parts[2],
                        "boundary": parts[3]},
+        "f64_fma": {"value": N / (f_ms * 1e-3), "ms_per_step": f_ms,
+                    "kernels_ms": {"stencil": f_parts[0], "transpose": f_parts[1], "reductions": f_parts[2]},
+                    "stencil_frac": fa / (f_parts[0] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "transpose_frac": fb / (f_parts[1] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "rhs_frac": rbytes / (f_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "split32": {"value": N / (s_ms * 1e-3), "ms_per_step": s_ms, "speedup_vs_f64": rhs_ms / s_ms,
-                    "rel_err_vs_f64": s_err,
+                    "speedup_vs_f64_fma": f_ms / s_ms, "rel_err_vs_f64": s_err,
                     "kernels_ms": {"stencil": s_parts[0], "transpose": s_parts[1]}},
-        "f64_ref": {"value": N / (r_ms * 1e-3), "ms_per_step": r_ms,
-                    "kernels_ms": {"stencil": r_parts[0], "transpose": r_parts[1]},
-                    "rhs_frac": rbytes / (r_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "lds_tiles": tinfo,
         "f64_global_gathers": {"ms_per_step": nt_ms,
                                "kernels_ms": {"stencil": nt_parts[0], "transpose": nt_parts[1]}},

@@ -911,12 +911,24 @@ int64_t or_rhs_scratch_len(const or_rhs_cfg* c)
     return (int64_t)c->nvar * c->N + 4 * c->N;
 }
 
-/* Flyer hyperviscosity source, hyperviscosity.jl:52-64 */
-static void or_src_hv(const or_rhs_cfg* c, const double* u, double* du)
+/* Hyperviscosity sources.
+ * Flyer, hyperviscosity.jl:52-64: du += (H u) * (-gamma), H = Dx^2k + Dy^2k.
+ * Tominec, hyperviscosity.jl:80-134: the reference forms H = lap' * lap
+ * (lap = Dxx + Dyy) explicitly as a sparse product and applies it the same
+ * way; here it is applied as lap' (lap u): local = lap u (CSC product), then
+ * du += (lap' local) * (-gamma).  The two forms agree to rounding; the
+ * oracle tests pin this one against scipy's explicit lap.T @ lap. */
+static void or_src_hv(const or_rhs_cfg* c, const double* u, double* du, double* local)
 {
-    for (int v = 0; v < c->nvar; ++v)
-        or_csc_mul(c->N, c->colptr, c->rowval, c->hv, u + v * c->N, du + v * c->N,
-                   -c->hv_gamma, 0);
+    for (int v = 0; v < c->nvar; ++v) {
+        if (c->hv_kind == 1) {
+            or_csc_mul(c->N, c->colptr, c->rowval, c->hv, u + v * c->N, local, 1.0, 1);
+            or_csc_tmul(c->N, c->colptr, c->rowval, c->hv, local, du + v * c->N, -c->hv_gamma);
+        } else {
+            or_csc_mul(c->N, c->colptr, c->rowval, c->hv, u + v * c->N, du + v * c->N,
+                       -c->hv_gamma, 0);
+        }
+    }
 }
 
 /* residual / upwind viscosity source, hyperviscosity.jl:351-409 */
@@ -979,9 +991,9 @@ int or_rhs(const or_rhs_cfg* c, double t, double* u, double* du, double* eps_out
         if (!eps) eps = own_e;
         if (!eps_c) eps_c = own_c;
     }
-    if (c->hv && c->hv_first) or_src_hv(c, u, du);
+    if (c->hv && c->hv_first) or_src_hv(c, u, du, rv_scr + 3 * N);
     if (c->rv_mode != OR_RV_NONE) or_src_rv(c, u, du, eps, eps_c, rv_scr);
-    if (c->hv && !c->hv_first) or_src_hv(c, u, du);
+    if (c->hv && !c->hv_first) or_src_hv(c, u, du, rv_scr + 3 * N);
     or_bc_pass(c, t, u, du); /* second pass, :422-425 */
     free(own_e);
     free(own_c);

@@ -88,6 +88,9 @@ typedef struct {
     /* residual-viscosity state */
     const double* approx_du;
     int success_iter;
+    /* hyperviscosity kind: 0 Flyer (du -= gamma H u with H = hv),
+     * 1 Tominec (du -= gamma lap' (lap u) with lap = hv) */
+    int hv_kind;
 } or_rhs_cfg;
 
 void or_flux(int eq, const double* adv_a, double gamma, int orientation, const double* u,

@@ -18,6 +18,7 @@ BC_DIRICHLET, BC_SLIP, BC_NOTHING = 0, 1, 2
 FN_CONST, FN_PULSE, FN_WAVE, FN_GRADIENT, FN_TABLE = 0, 1, 2, 3, 4
 RV_NONE, RV_RESIDUAL, RV_UPWIND = 0, 1, 2
 VAR_DENSITY, VAR_PRESSURE = 0, 1
+HV_FLYER, HV_TOMINEC = 0, 1
 
 
 def build():
@@ -56,7 +57,7 @@ class RhsCfg(C.Structure):
         ("ntags", C.c_int), ("tag_ptr", _i32p), ("tag_kind", _i32p),
         ("tag_fn", _i32p), ("tag_fn_params", _dp), ("bidx", _i32p),
         ("bnormal", _dp), ("btable", _dp),
-        ("approx_du", _dp), ("success_iter", C.c_int),
+        ("approx_du", _dp), ("success_iter", C.c_int), ("hv_kind", C.c_int),
     ]
 
 
@@ -241,7 +242,7 @@ class Problem:
 
     def __init__(self, x, y, ptr, idx, wx, wy, eq, nvar, gamma=1.4, adv=(1.0, 1.0),
                  boundary=(), hv=None, hv_gamma=0.0, rv_mode=RV_NONE, c_rv=1.0, c_uw=1.0,
-                 dx_avg=0.0, hv_first=True):
+                 dx_avg=0.0, hv_first=True, hv_kind=HV_FLYER):
         self.N = N = len(x)
         self.x = np.ascontiguousarray(x, dtype=np.float64)
         self.y = np.ascontiguousarray(y, dtype=np.float64)
@@ -290,6 +291,7 @@ class Problem:
         c.bidx, c.bnormal, c.btable = i32(self.bidx), d(self.bnormal), d(self.btable)
         c.approx_du = d(self.approx_du)
         c.success_iter = 0
+        c.hv_kind = hv_kind
         self.cfg = c
         self.scratch = np.zeros(lib().or_rhs_scratch_len(C.byref(c)) + 3 * nvar * N)
 

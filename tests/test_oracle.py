@@ -415,3 +415,38 @@ def test_vjp_matches_finite_differences():
     jv = (fp - fm) / (2 * eps)
     lhs, rhs = float((lam * jv).sum()), float((ub * v).sum())
     assert abs(lhs - rhs) < 1e-6 * abs(rhs)
+
+
+def test_tominec_hyperviscosity_matches_explicit_product():
+    """SourceHyperviscosityTominec (hyperviscosity.jl:80-134): the reference
+    builds H = lap' * lap explicitly (lap = Dxx + Dyy) and adds -gamma H u.
+    The oracle applies lap' (lap u); pinned here against scipy's explicit
+    product, and lap against exact second derivatives of quadratics."""
+    spec = configs.c2()
+    x, y, _, _, _ = O.cloud_square(24, spec.seed)
+    N, k = len(x), 30
+    idx, _ = O.knn_grid(x, y, k)
+    ptr = O.ell_ptr(N, k)
+    hx, hy, rc = O.weights(x, y, ptr, idx, 5, 3, 2)
+    assert rc == 0
+    lap = (hx + hy).reshape(N, k)
+    wx, wy, rc = O.weights(x, y, ptr, idx, 5, 3, 1)
+    assert rc == 0
+    Lm = sp.csr_matrix((lap.ravel(), idx.ravel(), np.arange(N + 1) * k), shape=(N, N))
+    # lap is exact on quadratics: lap(x^2 + 3xy - y^2) = 2 - 2 = 0, lap(x^2 + y^2) = 4
+    assert np.abs(Lm @ (x * x + y * y) - 4.0).max() < 1e-6
+    assert np.abs(Lm @ (x * x + 3 * x * y - y * y)).max() < 1e-6
+    g = 1e-4 * (1.0 / 23) ** 4.5
+    u0 = np.stack([1.0 + 0.2 * np.sin(2 * np.pi * (x + y)), 1.0 + 0 * x, 1.0 + 0 * x, 2.5 + x * y])
+    plain = O.Problem(x, y, ptr, idx, wx, wy, O.EQ_EULER, 4)
+    tom = O.Problem(x, y, ptr, idx, wx, wy, O.EQ_EULER, 4, hv=lap, hv_gamma=g, hv_kind=O.HV_TOMINEC)
+    du0, _, _ = plain.rhs(u0.copy(), 0.0)
+    du1, _, _ = tom.rhs(u0.copy(), 0.0)
+    H = (Lm.T @ Lm).tocsr()
+    ref = -g * (H @ u0.T).T
+    assert rel_inf(du1 - du0, ref) < 1e-12
+    # a constant state feels no hyperviscosity
+    c = np.ones_like(u0)
+    dc1, _, _ = tom.rhs(c.copy(), 0.0)
+    dc0, _, _ = plain.rhs(c.copy(), 0.0)
+    assert np.abs(dc1 - dc0).max() < 1e-12 * np.abs(g * (abs(H) @ np.ones(N))).max() + 1e-18

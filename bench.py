@@ -114,17 +114,49 @@ def measured_traffic(kernel, tuning, side, tiled, mode="f64_ref"):
     tl = "true" if tiled else "false"
     tm = 2 if mode == "f64_ref" else 0
     if kernel == "transpose":
-        want = "k_rhs_transpose<4, %d, %d, %s>" % (tm, tuning[mode]["transpose"] or 4, tl)
-        family = "k_rhs_transpose<4, %d," % tm
+        want = "k_rhs_transpose<4, %d, %d, %s, 1>" % (tm, tuning[mode]["transpose"] or 4, tl)
+        family, tail = "k_rhs_transpose<4, %d," % tm, "%s, 1>(mf::RhsArgs)" % tl
     else:
         kern = "k_rhs_ref" if mode == "f64_ref" else "k_rhs_f64"
-        want = "%s<4, 30, false, true, %d, %s>" % (kern, tuning[mode]["flux"] or 5, tl)
-        family = "%s<4, 30, false, true," % kern
+        want = "%s<4, 30, 0, true, %d, %s>" % (kern, tuning[mode]["flux"] or 5, tl)
+        family, tail = "%s<4, 30, 0, true," % kern, "%s>(mf::RhsArgs)" % tl
     names = [n for n in tab["kernels"] if want in n] or \
-        [n for n in tab["kernels"] if family in n and n.endswith("%s>(mf::RhsArgs)" % tl)]
+        [n for n in tab["kernels"] if family in n and n.endswith(tail)]
     if not names:
         return None, None
     return tab["kernels"][names[0]]["bytes"], names[0]
+
+
+def tominec(L, ctx, info, spec, t_now, steps):
+    """The same rhs! with the Tominec hyperviscosity added after the residual
+    viscosity (lap computed on the device, gamma = dx_min^4.5): per-precision
+    rhs! time, kernel times and the two kernels' HBM fractions.  The Tominec
+    source adds to phase A its lap weights (N k doubles) and the store of
+    L = lap u (N V doubles), to phase B the lap of the transposed entries
+    (N k doubles) and the read of L (N V doubles)."""
+    N, k, V = info["N"], spec.k, 4
+    ctx.compute_operator(spec.p, spec.deg, 2, L.MF_OP_LAPLACIAN, want=False)
+    ctx.set_source_hyperviscosity_tominec(info["dx_min"] ** 4.5)
+    ctx.set_source_order([L.MF_SRC_VISCOSITY, L.MF_SRC_TOMINEC])
+    res = {}
+    try:
+        for name, prec in (("f64_ref", L.MF_PREC_F64_REF), ("f64_fma", L.MF_PREC_F64),
+                           ("split32", L.MF_PREC_SPLIT32)):
+            ctx.time_rhs(t_now, prec, 20)
+            ms, parts = ctx.time_rhs(t_now, prec, steps)
+            ti = ctx.tile_info()
+            ba, bb = kernel_bytes(N, k, ti, passes=2 if name == "f64_ref" else 1)
+            ba += N * k * 8 + N * V * 8
+            bb += N * k * 8 + N * V * 8
+            res[name] = {"value": N / (ms * 1e-3), "ms_per_step": ms,
+                         "kernels_ms": {"stencil": parts[0], "transpose": parts[1], "reductions": parts[2]}}
+            if name != "split32":
+                res[name]["stencil_frac"] = ba / (parts[0] * 1e-3) / 1e9 / HBM_PEAK_GBS
+                res[name]["transpose_frac"] = bb / (parts[1] * 1e-3) / 1e9 / HBM_PEAK_GBS
+        res["lds_tiles"] = ctx.tile_info()
+    finally:
+        ctx.set_source_hyperviscosity_tominec(0.0)
+    return res
 
 
 def cpu_baseline(L, ctx, info, spec, t_now, budget_s):
@@ -374,6 +406,9 @@ def main():
         out["weights"] = weights_batch(L, configs, 1024)
         log("ensemble: 64 clouds, 50 SSPRK33 steps each (split-FP32, graphs, concurrent streams)")
         out["ensemble"] = ensemble(L, configs, 50)
+        log("Tominec hyperviscosity after the residual viscosity, %d nodes" % N)
+        out["tominec"] = tominec(L, ctx, info, spec, t_now, max(200, args.steps // 4))
+        ctx.time_rhs(t_now, L.MF_PREC_F64_REF, 3)
     if not args.no_cpu:
         log("cpu baseline (oracle, %.0f s budget)" % args.cpu_seconds)
         cb, du_cpu = cpu_baseline(L, ctx, info, spec, t_now, args.cpu_seconds)

@@ -61,6 +61,12 @@ typedef struct mf_ctx mf_ctx;
 /* operators */
 #define MF_OP_FLUX 0            /* Dx, Dy (compute_operators.jl:409-453) */
 #define MF_OP_HYPERVISCOSITY 1  /* Dx^2k + Dy^2k (hyperviscosity.jl:43-44) */
+#define MF_OP_LAPLACIAN 2       /* lap = Dxx + Dyy of the Tominec source (hyperviscosity.jl:111-113) */
+
+/* sources, for mf_set_source_order (SourceTerms keyword order) */
+#define MF_SRC_FLYER 1     /* SourceHyperviscosityFlyer */
+#define MF_SRC_TOMINEC 2   /* SourceHyperviscosityTominec */
+#define MF_SRC_VISCOSITY 3 /* SourceResidual/UpwindViscosityTominec */
 
 /* viscosity sources (hyperviscosity.jl) */
 #define MF_VISC_NONE 0
@@ -129,7 +135,9 @@ int mf_set_stencil(mf_ctx* ctx, const int32_t* idx, int64_t n_points, int k);
  * domain) (pointcloudsolver/compute_operators.jl:409-453); target
  * MF_OP_HYPERVISCOSITY (deriv_order 2k) replaces compute_flux_operator(
  * solver, domain, 2k) summed into one operator (compute_operators.jl:549-594,
- * sources/hyperviscosity.jl:43-44).  Weights stay on the device; wx_out /
+ * sources/hyperviscosity.jl:43-44); target MF_OP_LAPLACIAN (deriv_order 2)
+ * replaces the lap = sum(compute_flux_operator(solver, domain, 2)) of the
+ * Tominec source (hyperviscosity.jl:111-113).  Weights stay on the device; wx_out /
  * wy_out (n_points*k, row per node, may be NULL) receive the x / y operator.
  * Returns MF_ERR_SINGULAR if some stencil's system is singular. */
 int mf_compute_operator(mf_ctx* ctx, int phs_power, int poly_degree, int deriv_order,
@@ -137,7 +145,7 @@ int mf_compute_operator(mf_ctx* ctx, int phs_power, int poly_degree, int deriv_o
 
 /* Installs host weights for the current stencil (n_points*k each, row per
  * node).  MF_OP_FLUX: wx = Dx, wy = Dy.  MF_OP_HYPERVISCOSITY: wx = H,
- * wy must be NULL.  Replaces the cached rbf_differentiation_matrices /
+ * MF_OP_LAPLACIAN: wx = lap; wy must be NULL for both.  Replaces the cached rbf_differentiation_matrices /
  * hv_differentiation_matrix (pointcloudsolver/rbfsolver.jl:139,162). */
 int mf_set_operator(mf_ctx* ctx, int target, const double* wx, const double* wy);
 
@@ -181,6 +189,25 @@ int mf_set_source_hyperviscosity(mf_ctx* ctx, double gamma);
  * order (generic_sources.jl:7-18, rbfsolver.jl:388-395). */
 int mf_set_source_viscosity(mf_ctx* ctx, int mode, double c_rv, double c_uw, double h,
                             int approx_order, int hv_first);
+
+/* Tominec hyperviscosity: du -= gamma * lap' (lap u) with the
+ * MF_OP_LAPLACIAN operator; gamma = c * dx_min^4.5.  Replaces
+ * SourceHyperviscosityTominec (hyperviscosity.jl:66-134), which forms
+ * H = lap' * lap explicitly and applies du += H (u * -gamma); here H is
+ * applied as its two factors (the same operator; rounding differs).
+ * gamma = 0 removes the source.  Euler equations only. */
+int mf_set_source_hyperviscosity_tominec(mf_ctx* ctx, double gamma);
+
+/* The order in which the sources add to du, as the keys of the reference's
+ * SourceTerms NamedTuple (generic_sources.jl:7-18, rbfsolver.jl:388-395):
+ * kinds[0..n) is a list of distinct MF_SRC_* values; sources not listed
+ * follow in the order FLYER, TOMINEC, VISCOSITY.  It decides what the
+ * residual viscosity sees in du and, for MF_PREC_F64_REF, the rounding
+ * order.  A Tominec source before a residual viscosity source is
+ * MF_ERR_UNSUPPORTED at the next rhs! (its du would need lap'(lap u)
+ * before the viscosity's gathers).  mf_set_source_viscosity's hv_first
+ * sets the order of FLYER and VISCOSITY only. */
+int mf_set_source_order(mf_ctx* ctx, const int32_t* kinds, int n);
 
 /* --------------------------------------------------------------- state */
 /* Copies nvar host fields into / out of a context buffer (MF_BUF_*). */

@@ -212,7 +212,7 @@ int mf_rhs_vjp(mf_ctx* c, double t)
     c->err.clear();
     MF_HIP(c, hipSetDevice(c->device));
     int rc;
-    if (c->visc_mode != MF_VISC_NONE || c->hv_gamma != 0.0)
+    if (c->visc_mode != MF_VISC_NONE || c->hv_gamma != 0.0 || c->tom_gamma != 0.0)
         return fail(c, MF_ERR_UNSUPPORTED, "mf_rhs_vjp: sources have no adjoint in this build");
     if ((rc = ensure_state(c))) return rc;
     if ((rc = ensure_transpose_any(c))) return rc;
@@ -243,7 +243,7 @@ int mf_ssprk33_vjp(mf_ctx* c, double t0, double dt, int n_steps, int checkpoint_
     int rc;
     if (n_steps < 1 || checkpoint_every < 1 || !(dt > 0.0))
         return fail(c, MF_ERR_ARGUMENT, "mf_ssprk33_vjp: bad arguments");
-    if (c->visc_mode != MF_VISC_NONE || c->hv_gamma != 0.0)
+    if (c->visc_mode != MF_VISC_NONE || c->hv_gamma != 0.0 || c->tom_gamma != 0.0)
         return fail(c, MF_ERR_UNSUPPORTED, "mf_ssprk33_vjp: sources have no adjoint in this build");
     if ((rc = ensure_state(c))) return rc;
     if ((rc = ensure_transpose_any(c))) return rc;
@@ -259,8 +259,8 @@ int mf_ssprk33_vjp(mf_ctx* c, double t0, double dt, int n_steps, int checkpoint_
         if ((rc = dalloc(c, &c->d_adj, (size_t)(nbuf * NV)))) return rc;
         c->adj_len = nbuf * NV;
     }
+    // checkpoint s at buf(s); the tape and work states after them
     auto buf = [&](int64_t q) { return c->d_adj + q * NV; };
-    double* ck0 = buf(0);
     double* tape = buf(nck);
     double* Ub = buf(nck + 3 * C);
     double* Ub2 = buf(nck + 3 * C + 1);

@@ -128,7 +128,16 @@ static void free_ops(mf_ctx* c)
     dfree(c->d_wy_s);
     dfree(c->d_hv_s);
     c->d_wx_s = c->d_wy_s = c->d_hv_s = nullptr;
-    c->have_flux = c->have_hv = c->have_T = false;
+    dfree(c->d_lap);
+    dfree(c->d_lap32);
+    dfree(c->d_lap_s);
+    dfree(c->d_tsrc);
+    dfree(c->d_tl);
+    dfree(c->d_tl32);
+    c->d_lap = c->d_lap_s = c->d_tl = nullptr;
+    c->d_lap32 = c->d_tl32 = nullptr;
+    c->d_tsrc = nullptr;
+    c->have_flux = c->have_hv = c->have_lap = c->have_T = c->lap_T = false;
 }
 
 static void free_graph(mf_ctx* c)
@@ -151,6 +160,8 @@ static void free_state(mf_ctx* c)
     dfree(c->d_G);
     dfree(c->d_du32);
     c->d_du32 = nullptr;
+    dfree(c->d_L);
+    c->d_L = nullptr;
     dfree(c->d_eps);
     dfree(c->d_epsc);
     dfree(c->d_approx);
@@ -230,7 +241,7 @@ int ensure_state(mf_ctx* c)
 
 int ensure_transpose(mf_ctx* c)
 {
-    if (c->visc_mode == MF_VISC_NONE || c->have_T) return MF_OK;
+    if ((c->visc_mode == MF_VISC_NONE && c->tom_gamma == 0.0) || c->have_T) return MF_OK;
     if (!c->have_flux) return fail(c, MF_ERR_STATE, "no flux operator");
     return launch_build_transpose(c);
 }
@@ -452,10 +463,14 @@ static int alloc_op(mf_ctx* c, int target)
         if (!c->d_wx_s && (rc = dalloc(c, &c->d_wx_s, n))) return rc;
         if (!c->d_wy_s && (rc = dalloc(c, &c->d_wy_s, n))) return rc;
         if (!c->d_wxy32 && (rc = dalloc(c, &c->d_wxy32, n))) return rc;
-    } else {
+    } else if (target == MF_OP_HYPERVISCOSITY) {
         if (!c->d_hv && (rc = dalloc(c, &c->d_hv, n))) return rc;
         if (!c->d_hv_s && (rc = dalloc(c, &c->d_hv_s, n))) return rc;
         if (!c->d_hv32 && (rc = dalloc(c, &c->d_hv32, n))) return rc;
+    } else {
+        if (!c->d_lap && (rc = dalloc(c, &c->d_lap, n))) return rc;
+        if (!c->d_lap_s && (rc = dalloc(c, &c->d_lap_s, n))) return rc;
+        if (!c->d_lap32 && (rc = dalloc(c, &c->d_lap32, n))) return rc;
     }
     return MF_OK;
 }
@@ -467,8 +482,12 @@ static void op_installed(mf_ctx* c, int target)
         c->have_flux = true;
         c->have_T = false;
         free_tiles_b(c);
-    } else {
+    } else if (target == MF_OP_HYPERVISCOSITY) {
         c->have_hv = true;
+    } else {
+        c->have_lap = true;
+        c->lap_T = false;
+        c->lap_tb = false;
     }
 }
 
@@ -478,7 +497,7 @@ int mf_compute_operator(mf_ctx* c, int phs_power, int poly_degree, int deriv_ord
     int rc = check_ctx(c);
     if (rc) return rc;
     if (c->k == 0) return fail(c, MF_ERR_STATE, "mf_compute_operator: no stencil");
-    if (target != MF_OP_FLUX && target != MF_OP_HYPERVISCOSITY)
+    if (target != MF_OP_FLUX && target != MF_OP_HYPERVISCOSITY && target != MF_OP_LAPLACIAN)
         return fail(c, MF_ERR_ARGUMENT, "mf_compute_operator: bad target");
     if (phs_power < 1 || poly_degree < 0 || deriv_order < 1 || deriv_order > 8)
         return fail(c, MF_ERR_ARGUMENT, "mf_compute_operator: bad basis");
@@ -521,8 +540,8 @@ int mf_set_operator(mf_ctx* c, int target, const double* wx, const double* wy)
     int rc = check_ctx(c);
     if (rc) return rc;
     if (c->k == 0) return fail(c, MF_ERR_STATE, "mf_set_operator: no stencil");
-    if (!wx || (target == MF_OP_FLUX && !wy) || (target == MF_OP_HYPERVISCOSITY && wy) ||
-        (target != MF_OP_FLUX && target != MF_OP_HYPERVISCOSITY))
+    if (!wx || (target == MF_OP_FLUX && !wy) || (target != MF_OP_FLUX && wy) ||
+        (target != MF_OP_FLUX && target != MF_OP_HYPERVISCOSITY && target != MF_OP_LAPLACIAN))
         return fail(c, MF_ERR_ARGUMENT, "mf_set_operator: bad arguments");
     const size_t n = (size_t)c->N * c->k;
     double *rx = nullptr, *ex = nullptr, *ey = nullptr;
@@ -773,7 +792,43 @@ int mf_set_source_viscosity(mf_ctx* c, int mode, double c_rv, double c_uw, doubl
     c->c_uw = c_uw;
     c->h_visc = h;
     c->approx_order = approx_order;
+    // hv_first orders the Flyer and viscosity sources only
+    int* rk = c->src_rank;
+    if ((rk[MF_SRC_FLYER] < rk[MF_SRC_VISCOSITY]) != (hv_first != 0)) std::swap(rk[MF_SRC_FLYER], rk[MF_SRC_VISCOSITY]);
     c->hv_first = hv_first ? 1 : 0;
+    return MF_OK;
+}
+
+int mf_set_source_hyperviscosity_tominec(mf_ctx* c, double gamma)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (gamma != 0.0 && !c->have_lap)
+        return fail(c, MF_ERR_STATE, "Tominec hyperviscosity without its operator (MF_OP_LAPLACIAN)");
+    if (gamma != 0.0 && c->eq != MF_EQ_EULER_2D)
+        return fail(c, MF_ERR_UNSUPPORTED, "Tominec hyperviscosity needs the Euler equations");
+    free_graph(c);
+    c->tom_gamma = gamma;
+    return MF_OK;
+}
+
+int mf_set_source_order(mf_ctx* c, const int32_t* kinds, int n)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (n < 0 || n > 3 || (n > 0 && !kinds)) return fail(c, MF_ERR_ARGUMENT, "mf_set_source_order: bad list");
+    bool seen[4] = {false, false, false, false};
+    for (int q = 0; q < n; ++q) {
+        if (kinds[q] < MF_SRC_FLYER || kinds[q] > MF_SRC_VISCOSITY || seen[kinds[q]])
+            return fail(c, MF_ERR_ARGUMENT, "mf_set_source_order: kinds must be distinct MF_SRC_* values");
+        seen[kinds[q]] = true;
+    }
+    free_graph(c);
+    int r = 0;
+    for (int q = 0; q < n; ++q) c->src_rank[kinds[q]] = r++;
+    for (int k = MF_SRC_FLYER; k <= MF_SRC_VISCOSITY; ++k)
+        if (!seen[k]) c->src_rank[k] = r++;
+    c->hv_first = c->src_rank[MF_SRC_FLYER] < c->src_rank[MF_SRC_VISCOSITY] ? 1 : 0;
     return MF_OK;
 }
 
@@ -838,10 +893,41 @@ static int prepare_rhs(mf_ctx* c)
     if ((rc = ensure_state(c))) return rc;
     if ((rc = ensure_transpose(c))) return rc;
     if (c->hv_gamma != 0.0 && !c->have_hv) return fail(c, MF_ERR_STATE, "hyperviscosity operator missing");
-    const bool want_b = c->visc_mode != MF_VISC_NONE && c->nvar == 4;
+    const bool rv = c->visc_mode != MF_VISC_NONE;
+    const bool tom = c->tom_gamma != 0.0;
+    if (tom && !c->have_lap) return fail(c, MF_ERR_STATE, "Tominec operator missing");
+    if (tom) {
+        if (!c->d_L && (rc = dalloc(c, &c->d_L, (size_t)c->N * c->nvar))) return rc;
+        if (!c->d_du32 && (rc = dalloc(c, &c->d_du32, (size_t)c->N * c->nvar))) return rc;
+        if (!c->lap_T) {
+            free_graph(c);
+            if ((!c->d_tl && (rc = dalloc(c, &c->d_tl, c->t_entries))) ||
+                (!c->d_tl32 && (rc = dalloc(c, &c->d_tl32, c->t_entries))))
+                return rc;
+            if ((rc = launch_fill_lap_t(c, c->t_entries, c->d_tsrc, c->d_tl, c->d_tl32))) return rc;
+            MF_HIP(c, hipStreamSynchronize(c->stream));
+            c->lap_T = true;
+        }
+    }
+    const bool want_b = (rv || tom) && c->nvar == 4;
+    // phase-B halo record: G (2 nvar doubles) and / or lap u (nvar)
+    const int rec_b = (rv ? 2 * c->nvar : 0) + (tom ? c->nvar : 0);
+    if (want_b && c->tried_b && c->tiles_b_rec != rec_b) {
+        free_graph(c);
+        free_tiles_b(c);
+    }
     if (c->use_tiles && c->have_flux && (!c->tried_a || (want_b && c->tiles_a && !c->tried_b))) {
         free_graph(c);
-        if ((rc = build_tiles(c, want_b))) return rc;
+        if ((rc = build_tiles(c, want_b, rec_b))) return rc;
+    }
+    if (tom && c->tiles_b && !c->lap_tb) {
+        free_graph(c);
+        if ((!c->d_twl && (rc = dalloc(c, &c->d_twl, c->nb_entries))) ||
+            (!c->d_twl32 && (rc = dalloc(c, &c->d_twl32, c->nb_entries))))
+            return rc;
+        if ((rc = launch_fill_lap_t(c, c->nb_entries, c->d_tb_src, c->d_twl, c->d_twl32))) return rc;
+        MF_HIP(c, hipStreamSynchronize(c->stream));
+        c->lap_tb = true;
     }
     return MF_OK;
 }

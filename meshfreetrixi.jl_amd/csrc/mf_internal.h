@@ -90,21 +90,29 @@ struct mf_ctx {
     double* d_wx_s = nullptr;
     double* d_wy_s = nullptr;
     double* d_hv_s = nullptr;
+    double* d_lap_s = nullptr;
 
     // operators
     double2* d_wxy = nullptr;  // (Dx, Dy)
     double* d_hv = nullptr;    // Dx^2k + Dy^2k
     float2* d_wxy32 = nullptr;
     float* d_hv32 = nullptr;
-    bool have_flux = false, have_hv = false;
+    double* d_lap = nullptr;  // Dxx + Dyy (Tominec source)
+    float* d_lap32 = nullptr;
+    bool have_flux = false, have_hv = false, have_lap = false;
     // transposed flux operator, SELL-64
     int64_t n_slices = 0;
+    int64_t t_entries = 0;  // entries with padding
     int64_t* d_sell_off = nullptr;  // n_slices + 1
     int32_t* d_sell_w = nullptr;    // n_slices
     int32_t* d_trow = nullptr;
     double2* d_tw = nullptr;
     float2* d_tw32 = nullptr;
+    int64_t* d_tsrc = nullptr;  // slot-major stencil position of each entry (-1: padding)
+    double* d_tl = nullptr;     // lap of the transposed entries (Tominec)
+    float* d_tl32 = nullptr;
     bool have_T = false;
+    bool lap_T = false;  // d_tl filled from the current lap and transpose
 
     // physics
     int eq = MF_EQ_ADVECTION_2D;
@@ -117,6 +125,10 @@ struct mf_ctx {
     double c_rv = 1.0, c_uw = 1.0, h_visc = 0.0;
     int approx_order = 0;
     int hv_first = 1;
+    double tom_gamma = 0.0;
+    // rank of each source kind (MF_SRC_FLYER, _TOMINEC, _VISCOSITY) in the
+    // SourceTerms keyword order; smaller adds to du first
+    int src_rank[4] = {0, 0, 1, 2};
 
     // state
     double* d_buf[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -124,6 +136,7 @@ struct mf_ctx {
     double* d_T = nullptr;
     double* d_G = nullptr;
     float* d_du32 = nullptr;  // split-FP32 partial du between the two viscosity phases
+    double* d_L = nullptr;    // lap u between the phases (Tominec); FP32 in split-FP32
     double* d_eps = nullptr;
     int8_t* d_epsc = nullptr;
     double* d_approx = nullptr;
@@ -185,6 +198,11 @@ struct mf_ctx {
     uint16_t* d_lb = nullptr;            // halo positions, slice layout
     double2* d_twb = nullptr;            // (Dx, Dy) of the transposed entries
     float2* d_twb32 = nullptr;
+    int64_t* d_tb_src = nullptr;         // slot-major stencil position of each entry (-1: padding)
+    double* d_twl = nullptr;             // lap of the transposed entries (Tominec)
+    float* d_twl32 = nullptr;
+    int tiles_b_rec = 0;                 // doubles per phase-B halo record the tiles were sized for
+    bool lap_tb = false;                 // d_twl filled from the current lap
 
     // launch tuning (mf_set_tuning)
     // per precision (MF_PREC_*); 0 = the default
@@ -234,7 +252,6 @@ int launch_soa_to_aos(mf_ctx* c, const double* d_soa, double* d_aos);
 int launch_aos_to_soa(mf_ctx* c, const double* d_aos, double* d_soa);
 int launch_build_didx(mf_ctx* c);
 int launch_node_order(mf_ctx* c, const double* d_xu, const double* d_yu);
-int build_tiles(mf_ctx* c, bool want_b);
 void free_tiles(mf_ctx* c);
 void free_tiles_b(mf_ctx* c);
 int launch_gather_nodes_d(mf_ctx* c, const double* d_src, double* d_dst);
@@ -242,6 +259,8 @@ int launch_copy_buffer(mf_ctx* c, const double* d_src, double* d_dst, int64_t n)
 int launch_gather_nodes_i8(mf_ctx* c, const int8_t* d_src, int8_t* d_dst);
 int launch_sort_stencil(mf_ctx* c);
 int launch_pack_sorted(mf_ctx* c, int target, const double* d_wx, const double* d_wy);
+int launch_fill_lap_t(mf_ctx* c, int64_t n, const int64_t* d_src, double* d_w, float* d_w32);
+int build_tiles(mf_ctx* c, bool want_b, int rec_b);
 
 // rhs
 enum StageKind { ST_NONE = 0, ST_FSAL = 1, ST_TWO = 2, ST_THREE = 3 };

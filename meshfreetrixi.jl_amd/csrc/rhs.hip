@@ -59,6 +59,22 @@ struct RhsArgs {
     const int32_t* __restrict__ trow;
     const double2* __restrict__ tw;
     const float2* __restrict__ tw32;
+    // Tominec hyperviscosity: phase A stores L = lap u, phase B gathers lap' L
+    const double* __restrict__ lap;
+    const float* __restrict__ lap32;
+    const double* __restrict__ lap_s;  // column-sorted
+    double tom_gamma;
+    double* __restrict__ L;
+    float* __restrict__ L32;           // split-FP32: L in FP32 (same buffer)
+    const double* __restrict__ tl;     // lap of the transposed entries
+    const float* __restrict__ tl32;
+    const double* __restrict__ twl;    // the same in the tile layout
+    const float* __restrict__ twl32;
+    // reference order: Flyer term in phase A (it precedes every phase-B
+    // source); phase-B sources in keyword order, 2 bits each from the low
+    // end (kSeqViscosity, kSeqTominec, kSeqFlyer)
+    int flyer_a;
+    int seq_b;
     // LDS tiles (phase A: halo of the stencils; phase B: halo of D^T)
     int64_t n_tiles;
     const int64_t* __restrict__ ha_off;
@@ -213,11 +229,17 @@ __device__ __forceinline__ double node_viscosity(const RhsArgs& a, int64_t i, co
     return eps_rv < eps_uw ? eps_rv : eps_uw;
 }
 
+constexpr int kSeqViscosity = 1, kSeqTominec = 2, kSeqFlyer = 3;
+
 // ------------------------------------------------------------ FP64 kernel
 // du_i = -sum_j Dx_ij F1(u_j) - sum_j Dy_ij F2(u_j)     (calc_fluxes!, :247-265)
 //        - gamma sum_j H_ij u_j                         (Flyer HV, hyperviscosity.jl:52-64)
+// HVK bit 0: Flyer source; bit 1: Tominec source.
 // With a viscosity source this is phase A: it also stores
 // G_i = eps_i (Dx u, Dy u)_i for the transposed pass (hyperviscosity.jl:401-407).
+// With the Tominec source it is phase A too and stores L_i = sum_j lap_ij u_j
+// for the transposed pass du_i -= gamma sum_r lap_ri L_r (hyperviscosity.jl:
+// 111-134, the factors of H = lap' lap).
 // Residual-viscosity norms: every block reduces the max pass's partials
 // (same bits everywhere); block 0 publishes them.
 // Residual-viscosity norms in the stencil kernels: each thread loads its
@@ -377,9 +399,10 @@ __device__ __forceinline__ int64_t stage_halo_a(const RhsArgs& a, double* lds)
 // so BT gathers are in flight per thread.  Entries past K read a valid node
 // with weight 0.  TL: neighbour records come from the LDS halo, addressed
 // by 16-bit halo positions (same slot-major layout as idx).
-template <int V, int NS, bool HV, bool RV, int BT, bool TL>
+template <int V, int NS, int HVK, bool RV, int BT, bool TL>
 __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
 {
+    constexpr bool FLY = HVK & 1, TOM = HVK & 2;
     extern __shared__ double lds[];
     double pm[4] = {-1.0, -1.0, -1.0, -1.0};
     const bool rvres = RV && a.visc == MF_VISC_RESIDUAL;
@@ -393,14 +416,14 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
     const int64_t i = live ? i_node : a.N - 1;
     const int64_t N = a.N;
     const int K = NS ? NS : a.k;
-    double ax[V], ay[V], gx[V], gy[V], hh[V];
+    double ax[V], ay[V], gx[V], gy[V], hh[V], ll[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) ax[v] = ay[v] = gx[v] = gy[v] = hh[v] = 0.0;
+    for (int v = 0; v < V; ++v) ax[v] = ay[v] = gx[v] = gy[v] = hh[v] = ll[v] = 0.0;
 #pragma unroll 1
     for (int s0 = 0; s0 < K; s0 += BT) {
         int32_t j[BT];
         double2 w[BT];
-        double hw[BT];
+        double hw[BT], lw[BT];
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
             const int s = s0 + b;
@@ -411,11 +434,12 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
                 else
                     j[b] = a.idx[q];
                 w[b] = a.wxy[q];
-                if constexpr (HV) hw[b] = a.hv[q];
+                if constexpr (FLY) hw[b] = a.hv[q];
+                if constexpr (TOM) lw[b] = a.lap[q];
             } else {
                 j[b] = TL ? 0 : (int32_t)i;
                 w[b] = make_double2(0.0, 0.0);
-                hw[b] = 0.0;
+                hw[b] = lw[b] = 0.0;
             }
         }
         double uj[BT][TL ? rec_len<V>() : V];
@@ -437,45 +461,52 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
                     gy[v] = fma(w[b].y, uj[b][v], gy[v]);
                 }
             }
-            if constexpr (HV) {
+            if constexpr (FLY) {
 #pragma unroll
                 for (int v = 0; v < V; ++v) hh[v] = fma(hw[b], uj[b][v], hh[v]);
+            }
+            if constexpr (TOM) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) ll[v] = fma(lw[b], uj[b][v], ll[v]);
             }
         }
     }
     double du[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) du[v] = -ax[v] - ay[v];
-    if constexpr (HV) {
+    if constexpr (FLY) {
         if (!RV || a.hv_first) {
 #pragma unroll
             for (int v = 0; v < V; ++v) du[v] = fma(-a.hv_gamma, hh[v], du[v]);
         }
     }
-    if constexpr (RV) {
-        double ui[V];
-        load_state<V>(a.u, i, ui);
-        int8_t code = 1;
-        double ninf[4] = {1.0, 1.0, 1.0, 1.0};
-        if (rvres) finish_norms(a, pm, ninf);
-        if (!live) return;
-        const double e = node_viscosity(a, i, ui, du, ninf, &code);
-        a.eps[i] = e;
-        a.epsc[i] = code;
-        double g[2 * V];
+    if constexpr (RV || TOM) {
+        if constexpr (RV) {
+            double ui[V];
+            load_state<V>(a.u, i, ui);
+            int8_t code = 1;
+            double ninf[4] = {1.0, 1.0, 1.0, 1.0};
+            if (rvres) finish_norms(a, pm, ninf);
+            if (!live) return;
+            const double e = node_viscosity(a, i, ui, du, ninf, &code);
+            a.eps[i] = e;
+            a.epsc[i] = code;
+            double g[2 * V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-            g[v] = e * gx[v];
-            g[V + v] = e * gy[v];
-        }
-        store_state<V>(a.G, 2 * i, g);
-        store_state<V>(a.G, 2 * i + 1, g + V);
-        if constexpr (HV) {
-            if (!a.hv_first) {
+            for (int v = 0; v < V; ++v) {
+                g[v] = e * gx[v];
+                g[V + v] = e * gy[v];
+            }
+            store_state<V>(a.G, 2 * i, g);
+            store_state<V>(a.G, 2 * i + 1, g + V);
+            if constexpr (FLY) {
+                if (!a.hv_first) {
 #pragma unroll
-                for (int v = 0; v < V; ++v) du[v] = fma(-a.hv_gamma, hh[v], du[v]);
+                    for (int v = 0; v < V; ++v) du[v] = fma(-a.hv_gamma, hh[v], du[v]);
+                }
             }
         }
+        if constexpr (TOM) store_state<V>(a.L, i, ll);
         store_state<V>(a.du, i, du);
         return;
     }
@@ -493,13 +524,14 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
 // reproducible against a CSC implementation with the same weights.
 // One pass over the column-sorted stencil in batches of BT entries (loads
 // first, then the adds in entry order).  PASS 0: x flux (and D_x u),
-// 1: y flux (and D_y u), 2: Flyer term.
+// 1: y flux (and D_y u), 2: Flyer term, 3: L = lap u of the Tominec source
+// (lap * u in CSC order, as the first factor of its H = lap' lap).
 template <int V, int NS, bool RV, int BT, int PASS, bool TL>
 __device__ __forceinline__ void ref_pass(const RhsArgs& a, int64_t i, int K, double* du, double* g,
                                          const double* lds)
 {
     const int64_t N = a.N;
-    const double* __restrict__ wsrc = PASS == 0 ? a.wx_s : (PASS == 1 ? a.wy_s : a.hv_s);
+    const double* __restrict__ wsrc = PASS == 0 ? a.wx_s : (PASS == 1 ? a.wy_s : (PASS == 2 ? a.hv_s : a.lap_s));
     const double ng = -a.hv_gamma;
 #pragma unroll 1
     for (int s0 = 0; s0 < K; s0 += BT) {
@@ -525,6 +557,9 @@ __device__ __forceinline__ void ref_pass(const RhsArgs& a, int64_t i, int K, dou
             if constexpr (PASS == 2) {
 #pragma unroll
                 for (int v = 0; v < V; ++v) du[v] = du[v] + w[b] * (uj[b][v] * ng);
+            } else if constexpr (PASS == 3) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) du[v] = du[v] + w[b] * uj[b][v];
             } else {
                 double f1[V], f2[V];
                 flux_nb<V, TL>(a.eq, uj[b], f1, f2);
@@ -538,9 +573,10 @@ __device__ __forceinline__ void ref_pass(const RhsArgs& a, int64_t i, int K, dou
     }
 }
 
-template <int V, int NS, bool HV, bool RV, int BT, bool TL>
+template <int V, int NS, int HVK, bool RV, int BT, bool TL>
 __global__ __launch_bounds__(kBlock) void k_rhs_ref(const RhsArgs a)
 {
+    constexpr bool FLY = HVK & 1, TOM = HVK & 2;
     extern __shared__ double lds[];
     double pm[4] = {-1.0, -1.0, -1.0, -1.0};
     const bool rvres = RV && a.visc == MF_VISC_RESIDUAL;
@@ -558,27 +594,36 @@ __global__ __launch_bounds__(kBlock) void k_rhs_ref(const RhsArgs a)
     for (int v = 0; v < V; ++v) du[v] = gx[v] = gy[v] = 0.0;
     ref_pass<V, NS, RV, BT, 0, TL>(a, i, K, du, gx, lds);
     ref_pass<V, NS, RV, BT, 1, TL>(a, i, K, du, gy, lds);
-    if constexpr (HV) {
-        if (!RV || a.hv_first) ref_pass<V, NS, false, BT, 2, TL>(a, i, K, du, nullptr, lds);
+    if constexpr (FLY) {
+        if (a.flyer_a) ref_pass<V, NS, false, BT, 2, TL>(a, i, K, du, nullptr, lds);
     }
-    if constexpr (RV) {
-        double ui[V];
-        load_state<V>(a.u, i, ui);
-        int8_t code = 1;
-        double ninf[4] = {1.0, 1.0, 1.0, 1.0};
-        if (rvres) finish_norms(a, pm, ninf);
-        if (!live) return;
-        const double e = node_viscosity(a, i, ui, du, ninf, &code);
-        a.eps[i] = e;
-        a.epsc[i] = code;
-        double g[2 * V];
+    if constexpr (RV || TOM) {
+        if constexpr (RV) {
+            double ui[V];
+            load_state<V>(a.u, i, ui);
+            int8_t code = 1;
+            double ninf[4] = {1.0, 1.0, 1.0, 1.0};
+            if (rvres) finish_norms(a, pm, ninf);
+            if (!live) return;
+            const double e = node_viscosity(a, i, ui, du, ninf, &code);
+            a.eps[i] = e;
+            a.epsc[i] = code;
+            double g[2 * V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-            g[v] = e * gx[v];
-            g[V + v] = e * gy[v];
+            for (int v = 0; v < V; ++v) {
+                g[v] = e * gx[v];
+                g[V + v] = e * gy[v];
+            }
+            store_state<V>(a.G, 2 * i, g);
+            store_state<V>(a.G, 2 * i + 1, g + V);
         }
-        store_state<V>(a.G, 2 * i, g);
-        store_state<V>(a.G, 2 * i + 1, g + V);
+        if constexpr (TOM) {
+            double ll[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) ll[v] = 0.0;
+            ref_pass<V, NS, false, BT, 3, TL>(a, i, K, ll, nullptr, lds);
+            store_state<V>(a.L, i, ll);
+        }
         store_state<V>(a.du, i, du);
         return;
     }
@@ -591,9 +636,10 @@ __global__ __launch_bounds__(kBlock) void k_rhs_ref(const RhsArgs a)
 // formed in FP64: sum_j w_ij F(u_j) ~= sum_j w_ij (F(u_j) - F(u_i)),
 // exact up to F(u_i) sum_j w_ij, which vanishes with the weights' row sums.
 // Weights FP32, neighbour indices as int16 offsets when the cloud allows.
-template <int V, int NS, bool HV, bool RV, bool D16, int BT, bool TL>
+template <int V, int NS, int HVK, bool RV, bool D16, int BT, bool TL>
 __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
 {
+    constexpr bool FLY = HVK & 1, TOM = HVK & 2;
     extern __shared__ double lds[];
     double pm[4] = {-1.0, -1.0, -1.0, -1.0};
     const bool rvres = RV && a.visc == MF_VISC_RESIDUAL;
@@ -610,14 +656,14 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
     double ui[V], fi1[V], fi2[V];
     load_state<V>(a.u, i, ui);
     flux_xy<V>(a.eq, ui, fi1, fi2);
-    float ax[V], ay[V], gx[V], gy[V], hh[V];
+    float ax[V], ay[V], gx[V], gy[V], hh[V], ll[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) ax[v] = ay[v] = gx[v] = gy[v] = hh[v] = 0.0f;
+    for (int v = 0; v < V; ++v) ax[v] = ay[v] = gx[v] = gy[v] = hh[v] = ll[v] = 0.0f;
 #pragma unroll 1
     for (int s0 = 0; s0 < K; s0 += BT) {
         int64_t jj[BT];
         float2 ww[BT];
-        float hw32[BT];
+        float hw32[BT], lw32[BT];
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
             const int s = s0 + b;
@@ -630,11 +676,12 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
                 else
                     jj[b] = a.idx[q];
                 ww[b] = a.wxy32[q];
-                if constexpr (HV) hw32[b] = a.hv32[q];
+                if constexpr (FLY) hw32[b] = a.hv32[q];
+                if constexpr (TOM) lw32[b] = a.lap32[q];
             } else {
                 jj[b] = TL ? 0 : i;
                 ww[b] = make_float2(0.0f, 0.0f);
-                hw32[b] = 0.0f;
+                hw32[b] = lw32[b] = 0.0f;
             }
         }
         double ub[BT][TL ? rec_len<V>() : V];
@@ -642,65 +689,78 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
         for (int b = 0; b < BT; ++b) load_nb<V, TL>(a, lds, jj[b], ub[b]);
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
-        const float2 w = ww[b];
-        const double* uj = ub[b];
-        double f1[V], f2[V];
-        flux_nb<V, TL>(a.eq, uj, f1, f2);
+            const float2 w = ww[b];
+            const double* uj = ub[b];
+            double f1[V], f2[V];
+            flux_nb<V, TL>(a.eq, uj, f1, f2);
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-            ax[v] = fmaf(w.x, (float)(f1[v] - fi1[v]), ax[v]);
-            ay[v] = fmaf(w.y, (float)(f2[v] - fi2[v]), ay[v]);
-        }
-        if constexpr (RV || HV) {
-            float dd[V];
+            for (int v = 0; v < V; ++v) {
+                ax[v] = fmaf(w.x, (float)(f1[v] - fi1[v]), ax[v]);
+                ay[v] = fmaf(w.y, (float)(f2[v] - fi2[v]), ay[v]);
+            }
+            if constexpr (RV || FLY || TOM) {
+                float dd[V];
 #pragma unroll
-            for (int v = 0; v < V; ++v) dd[v] = (float)(uj[v] - ui[v]);
-            if constexpr (RV) {
+                for (int v = 0; v < V; ++v) dd[v] = (float)(uj[v] - ui[v]);
+                if constexpr (RV) {
 #pragma unroll
-                for (int v = 0; v < V; ++v) {
-                    gx[v] = fmaf(w.x, dd[v], gx[v]);
-                    gy[v] = fmaf(w.y, dd[v], gy[v]);
+                    for (int v = 0; v < V; ++v) {
+                        gx[v] = fmaf(w.x, dd[v], gx[v]);
+                        gy[v] = fmaf(w.y, dd[v], gy[v]);
+                    }
+                }
+                if constexpr (FLY) {
+                    const float hw = hw32[b];
+#pragma unroll
+                    for (int v = 0; v < V; ++v) hh[v] = fmaf(hw, dd[v], hh[v]);
+                }
+                if constexpr (TOM) {
+                    const float lw = lw32[b];
+#pragma unroll
+                    for (int v = 0; v < V; ++v) ll[v] = fmaf(lw, dd[v], ll[v]);
                 }
             }
-            if constexpr (HV) {
-                const float hw = hw32[b];
-#pragma unroll
-                for (int v = 0; v < V; ++v) hh[v] = fmaf(hw, dd[v], hh[v]);
-            }
-        }
         }
     }
     double du[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) du[v] = -(double)ax[v] - (double)ay[v];
-    if constexpr (HV) {
+    if constexpr (FLY) {
         if (!RV || a.hv_first) {
 #pragma unroll
             for (int v = 0; v < V; ++v) du[v] = fma(-a.hv_gamma, (double)hh[v], du[v]);
         }
     }
-    if constexpr (RV) {
-        int8_t code = 1;
-        double ninf[4] = {1.0, 1.0, 1.0, 1.0};
-        if (rvres) finish_norms(a, pm, ninf);
-        if (!live) return;
-        const double e = node_viscosity(a, i, ui, du, ninf, &code);
-        a.eps[i] = e;
-        a.epsc[i] = code;
-        double g[2 * V];
+    if constexpr (RV || TOM) {
+        if constexpr (RV) {
+            int8_t code = 1;
+            double ninf[4] = {1.0, 1.0, 1.0, 1.0};
+            if (rvres) finish_norms(a, pm, ninf);
+            if (!live) return;
+            const double e = node_viscosity(a, i, ui, du, ninf, &code);
+            a.eps[i] = e;
+            a.epsc[i] = code;
+            double g[2 * V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-            g[v] = e * (double)gx[v];
-            g[V + v] = e * (double)gy[v];
-        }
-        // split-FP32: G and the partial du go to phase B in FP32
-        store_state32<V>(a.G32, 2 * i, g);
-        store_state32<V>(a.G32, 2 * i + 1, g + V);
-        if constexpr (HV) {
-            if (!a.hv_first) {
-#pragma unroll
-                for (int v = 0; v < V; ++v) du[v] = fma(-a.hv_gamma, (double)hh[v], du[v]);
+            for (int v = 0; v < V; ++v) {
+                g[v] = e * (double)gx[v];
+                g[V + v] = e * (double)gy[v];
             }
+            // split-FP32: G, L and the partial du go to phase B in FP32
+            store_state32<V>(a.G32, 2 * i, g);
+            store_state32<V>(a.G32, 2 * i + 1, g + V);
+            if constexpr (FLY) {
+                if (!a.hv_first) {
+#pragma unroll
+                    for (int v = 0; v < V; ++v) du[v] = fma(-a.hv_gamma, (double)hh[v], du[v]);
+                }
+            }
+        }
+        if constexpr (TOM) {
+            double l[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) l[v] = (double)ll[v];
+            store_state32<V>(a.L32, i, l);
         }
         store_state32<V>(a.du32, i, du);
         return;
@@ -711,17 +771,23 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
 
 // ------------------------------------------ phase B: transposed gather
 // du_i -= sum_r Dx_ri Gx_r ; du_i -= sum_r Dy_ri Gy_r   (the D_d' products of
-// hyperviscosity.jl:405), over the sorted SELL-64 transpose, then the
-// second boundary pass and the output.
-// TM: 0 FP64 (fused multiply-add), 1 FP32 weights, 2 FP64 in the reference's
-// order (tmp = tmp + w*G over ascending rows, then du - tmp; a Flyer source
-// that follows the viscosity in keyword order is added last).
+// hyperviscosity.jl:405; GT bit 0) and du_i -= gamma sum_r lap_ri L_r (the
+// Tominec source, hyperviscosity.jl:111-134; GT bit 1), over the sorted
+// SELL-64 transpose, then the second boundary pass and the output.
+// TM: 0 FP64 (fused multiply-add), 1 FP32 weights, G and L, 2 FP64 in the
+// reference's order (tmp = tmp + w*G over ascending rows, then du - tmp;
+// the phase-B sources added in keyword order, a Flyer source among them by
+// its own gather of u).
 // TL: the tile's rows in the order of rowb (by degree), SELL-64 per
-// wavefront (sb_off, sb_w), entries addressing G staged in LDS (lb: halo
-// positions; halo entry q holds G of node hb_ids[hb_off[t] + q]).
-template <int V, int TM, int BT, bool TL>
+// wavefront (sb_off, sb_w), entries addressing halo records in LDS (lb:
+// halo positions; record q belongs to node hb_ids[hb_off[t] + q] and holds
+// its G (2V values) and / or L (V values)).
+template <int V, int TM, int BT, bool TL, int GT>
 __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
 {
+    constexpr bool HG = GT & 1, HL = GT & 2;
+    constexpr int RU = (HG ? 2 : 0) + (HL ? 1 : 0);  // V-value units per halo record
+    constexpr int LU = HG ? 2 : 0;                   // unit of L in a record
     extern __shared__ double lds[];
     int64_t i, off;
     int lane, W;
@@ -732,16 +798,31 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
         const int nh = (int)(a.hb_off[t + 1] - h0);
         for (int q = threadIdx.x; q < nh; q += kBlock) {
             const int64_t j = a.hb_ids[h0 + q];
-            if constexpr (TM == 1) {  // FP32 G: 8 floats per halo node
+            if constexpr (TM == 1) {  // FP32 records
                 float* ldsf = reinterpret_cast<float*>(lds);
-                reinterpret_cast<float4*>(ldsf)[2 * q] = reinterpret_cast<const float4*>(a.G32)[2 * j];
-                reinterpret_cast<float4*>(ldsf)[2 * q + 1] = reinterpret_cast<const float4*>(a.G32)[2 * j + 1];
+                double g[V];
+                if constexpr (HG) {
+                    load_state32<V>(a.G32, 2 * j, g);
+                    store_state32<V>(ldsf, RU * q, g);
+                    load_state32<V>(a.G32, 2 * j + 1, g);
+                    store_state32<V>(ldsf, RU * q + 1, g);
+                }
+                if constexpr (HL) {
+                    load_state32<V>(a.L32, j, g);
+                    store_state32<V>(ldsf, RU * q + LU, g);
+                }
             } else {
-                double g[2 * V];
-                load_state<V>(a.G, 2 * j, g);
-                load_state<V>(a.G, 2 * j + 1, g + V);
-                store_state<V>(lds, 2 * q, g);
-                store_state<V>(lds, 2 * q + 1, g + V);
+                double g[V];
+                if constexpr (HG) {
+                    load_state<V>(a.G, 2 * j, g);
+                    store_state<V>(lds, RU * q, g);
+                    load_state<V>(a.G, 2 * j + 1, g);
+                    store_state<V>(lds, RU * q + 1, g);
+                }
+                if constexpr (HL) {
+                    load_state<V>(a.L, j, g);
+                    store_state<V>(lds, RU * q + LU, g);
+                }
             }
         }
         __syncthreads();
@@ -760,13 +841,14 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
         off = a.sell_off[slice];
         W = a.sell_w[slice];
     }
-    double tx[V], ty[V];
+    double tx[V], ty[V], th[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) tx[v] = ty[v] = 0.0;
+    for (int v = 0; v < V; ++v) tx[v] = ty[v] = th[v] = 0.0;
 #pragma unroll 1
     for (int e0 = 0; e0 < W; e0 += BT) {
         int32_t r[BT];
         double2 w[BT];
+        double wl[BT];
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
             const int e = (e0 + b < W) ? e0 + b : e0;
@@ -775,26 +857,48 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
                 r[b] = a.lb[q];
             else
                 r[b] = a.trow[q];
-            if constexpr (TM == 1) {
-                const float2 w32 = TL ? a.twb32[q] : a.tw32[q];
-                w[b] = make_double2((double)w32.x, (double)w32.y);
-            } else {
-                w[b] = TL ? a.twb[q] : a.tw[q];
+            if constexpr (HG) {
+                if constexpr (TM == 1) {
+                    const float2 w32 = TL ? a.twb32[q] : a.tw32[q];
+                    w[b] = make_double2((double)w32.x, (double)w32.y);
+                } else {
+                    w[b] = TL ? a.twb[q] : a.tw[q];
+                }
+            }
+            if constexpr (HL) {
+                if constexpr (TM == 1)
+                    wl[b] = (double)(TL ? a.twl32[q] : a.tl32[q]);
+                else
+                    wl[b] = TL ? a.twl[q] : a.tl[q];
             }
         }
-        double g0[BT][V], g1[BT][V];
+        double g0[BT][V], g1[BT][V], lv[BT][V];
 #pragma unroll
         for (int b = 0; b < BT; ++b) {
+            const int64_t rb = r[b];
             if constexpr (TM == 1) {
-                const float* gsrc = TL ? reinterpret_cast<const float*>(lds) : a.G32;
-                load_state32<V>(gsrc, 2 * (int64_t)r[b], g0[b]);
-                load_state32<V>(gsrc, 2 * (int64_t)r[b] + 1, g1[b]);
-            } else if constexpr (TL) {
-                load_state<V>(lds, 2 * (int64_t)r[b], g0[b]);
-                load_state<V>(lds, 2 * (int64_t)r[b] + 1, g1[b]);
+                const float* ldsf = reinterpret_cast<const float*>(lds);
+                if constexpr (HG) {
+                    if constexpr (TL) {
+                        load_state32<V>(ldsf, RU * rb, g0[b]);
+                        load_state32<V>(ldsf, RU * rb + 1, g1[b]);
+                    } else {
+                        load_state32<V>(a.G32, 2 * rb, g0[b]);
+                        load_state32<V>(a.G32, 2 * rb + 1, g1[b]);
+                    }
+                }
+                if constexpr (HL) load_state32<V>(TL ? ldsf : a.L32, TL ? RU * rb + LU : rb, lv[b]);
             } else {
-                load_state<V>(a.G, 2 * (int64_t)r[b], g0[b]);
-                load_state<V>(a.G, 2 * (int64_t)r[b] + 1, g1[b]);
+                if constexpr (HG) {
+                    if constexpr (TL) {
+                        load_state<V>(lds, RU * rb, g0[b]);
+                        load_state<V>(lds, RU * rb + 1, g1[b]);
+                    } else {
+                        load_state<V>(a.G, 2 * rb, g0[b]);
+                        load_state<V>(a.G, 2 * rb + 1, g1[b]);
+                    }
+                }
+                if constexpr (HL) load_state<V>(TL ? lds : a.L, TL ? RU * rb + LU : rb, lv[b]);
             }
         }
 #pragma unroll
@@ -803,11 +907,17 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 if constexpr (TM == 2) {
-                    tx[v] = tx[v] + w[b].x * g0[b][v];
-                    ty[v] = ty[v] + w[b].y * g1[b][v];
+                    if constexpr (HG) {
+                        tx[v] = tx[v] + w[b].x * g0[b][v];
+                        ty[v] = ty[v] + w[b].y * g1[b][v];
+                    }
+                    if constexpr (HL) th[v] = th[v] + wl[b] * lv[b][v];
                 } else {
-                    tx[v] = fma(w[b].x, g0[b][v], tx[v]);
-                    ty[v] = fma(w[b].y, g1[b][v], ty[v]);
+                    if constexpr (HG) {
+                        tx[v] = fma(w[b].x, g0[b][v], tx[v]);
+                        ty[v] = fma(w[b].y, g1[b][v], ty[v]);
+                    }
+                    if constexpr (HL) th[v] = fma(wl[b], lv[b][v], th[v]);
                 }
             }
         }
@@ -817,19 +927,37 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
         load_state32<V>(a.du32, i, du);
     else
         load_state<V>(a.du, i, du);
-#pragma unroll
-    for (int v = 0; v < V; ++v) du[v] = (du[v] - tx[v]) - ty[v];
     if constexpr (TM == 2) {
-        if (a.hv_s && a.hv_gamma != 0.0 && !a.hv_first) {
-            const double ng = -a.hv_gamma;
-            for (int s = 0; s < a.k; ++s) {
-                const int64_t q = (int64_t)s * a.N + i;
-                double uj[V];
-                load_state<V>(a.u, a.idx_s[q], uj);
-                const double h = a.hv_s[q];
+        // the phase-B sources in keyword order
+        for (int sq = a.seq_b; sq != 0; sq >>= 2) {
+            const int kind = sq & 3;
+            if (kind == kSeqViscosity) {
 #pragma unroll
-                for (int v = 0; v < V; ++v) du[v] = du[v] + h * (uj[v] * ng);
+                for (int v = 0; v < V; ++v) du[v] = (du[v] - tx[v]) - ty[v];
+            } else if (kind == kSeqTominec) {
+                const double ng = -a.tom_gamma;
+#pragma unroll
+                for (int v = 0; v < V; ++v) du[v] = du[v] + th[v] * ng;
+            } else {
+                const double ng = -a.hv_gamma;
+                for (int s = 0; s < a.k; ++s) {
+                    const int64_t q = (int64_t)s * a.N + i;
+                    double uj[V];
+                    load_state<V>(a.u, a.idx_s[q], uj);
+                    const double h = a.hv_s[q];
+#pragma unroll
+                    for (int v = 0; v < V; ++v) du[v] = du[v] + h * (uj[v] * ng);
+                }
             }
+        }
+    } else {
+        if constexpr (HG) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) du[v] = (du[v] - tx[v]) - ty[v];
+        }
+        if constexpr (HL) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) du[v] = fma(-a.tom_gamma, th[v], du[v]);
         }
     }
     mask_du<V>(a, i, du);
@@ -1244,6 +1372,35 @@ static RhsArgs make_args(mf_ctx* c, double* u, double* du)
     a.lb = c->d_lb;
     a.twb = c->d_twb;
     a.twb32 = c->d_twb32;
+    a.lap = c->d_lap;
+    a.lap32 = c->d_lap32;
+    a.lap_s = c->d_lap_s;
+    a.tom_gamma = c->tom_gamma;
+    a.L = c->d_L;
+    a.L32 = reinterpret_cast<float*>(c->d_L);
+    a.tl = c->d_tl;
+    a.tl32 = c->d_tl32;
+    a.twl = c->d_twl;
+    a.twl32 = c->d_twl32;
+    // the active sources in keyword order
+    const bool fly = c->have_hv && c->hv_gamma != 0.0;
+    const bool tom = c->have_lap && c->tom_gamma != 0.0;
+    const bool rv = c->visc_mode != MF_VISC_NONE;
+    int order[3], n = 0;
+    for (int k : {MF_SRC_FLYER, MF_SRC_TOMINEC, MF_SRC_VISCOSITY})
+        if ((k == MF_SRC_FLYER && fly) || (k == MF_SRC_TOMINEC && tom) || (k == MF_SRC_VISCOSITY && rv))
+            order[n++] = k;
+    std::sort(order, order + n, [&](int x, int y) { return c->src_rank[x] < c->src_rank[y]; });
+    a.flyer_a = fly && order[0] == MF_SRC_FLYER;
+    a.seq_b = 0;
+    int sh = 0;
+    for (int q = 0; q < n; ++q) {
+        if (order[q] == MF_SRC_FLYER && a.flyer_a) continue;
+        const int code = order[q] == MF_SRC_VISCOSITY ? kSeqViscosity
+                                                      : (order[q] == MF_SRC_TOMINEC ? kSeqTominec : kSeqFlyer);
+        a.seq_b |= code << sh;
+        sh += 2;
+    }
     return a;
 }
 
@@ -1317,107 +1474,126 @@ static int bc_state_pass(mf_ctx* c, double* u, double dt_off, int homog, bool sl
     return MF_OK;
 }
 
+// Stencil batch of the Tominec variants (compiled for one batch size only).
+constexpr int def_bt(int ns) { return ns ? 5 : 4; }
+
+// hvk bit 0: Flyer source, bit 1: Tominec source.
+#define MF_LAUNCH_HVK(KERNEL, ...)                                                      \
+    do {                                                                                \
+        if (hvk & 2) {                                                                  \
+            if constexpr (V == 4 && BT == def_bt(NS)) {                                 \
+                if (hvk == 3 && rv) launch_k(c, 0, KERNEL<V, NS, 3, true, __VA_ARGS__>, G, B, lds, a);  \
+                else if (hvk == 3) launch_k(c, 0, KERNEL<V, NS, 3, false, __VA_ARGS__>, G, B, lds, a);  \
+                else if (rv) launch_k(c, 0, KERNEL<V, NS, 2, true, __VA_ARGS__>, G, B, lds, a);         \
+                else launch_k(c, 0, KERNEL<V, NS, 2, false, __VA_ARGS__>, G, B, lds, a);                \
+            }                                                                           \
+        } else if (hvk && rv) launch_k(c, 0, KERNEL<V, NS, 1, true, __VA_ARGS__>, G, B, lds, a);         \
+        else if (hvk) launch_k(c, 0, KERNEL<V, NS, 1, false, __VA_ARGS__>, G, B, lds, a);                \
+        else if (rv) launch_k(c, 0, KERNEL<V, NS, 0, true, __VA_ARGS__>, G, B, lds, a);                  \
+        else launch_k(c, 0, KERNEL<V, NS, 0, false, __VA_ARGS__>, G, B, lds, a);                         \
+    } while (0)
+
 template <int V, int NS, int BT, bool TL>
-static void launch_f64(mf_ctx* c, const RhsArgs& a, bool hv, bool rv, const dim3& G, size_t lds)
+static void launch_f64(mf_ctx* c, const RhsArgs& a, int hvk, bool rv, const dim3& G, size_t lds)
 {
     const dim3 B(kBlock);
-    hipStream_t s = c->stream;
-    if (hv && rv) launch_k(c, 0, k_rhs_f64<V, NS, true, true, BT, TL>, G, B, lds, a);
-    else if (hv) launch_k(c, 0, k_rhs_f64<V, NS, true, false, BT, TL>, G, B, lds, a);
-    else if (rv) launch_k(c, 0, k_rhs_f64<V, NS, false, true, BT, TL>, G, B, lds, a);
-    else launch_k(c, 0, k_rhs_f64<V, NS, false, false, BT, TL>, G, B, lds, a);
+    MF_LAUNCH_HVK(k_rhs_f64, BT, TL);
 }
 
 template <int V, int NS, int BT, bool D16, bool TL>
-static void launch_s32(mf_ctx* c, const RhsArgs& a, bool hv, bool rv, const dim3& G, size_t lds)
+static void launch_s32(mf_ctx* c, const RhsArgs& a, int hvk, bool rv, const dim3& G, size_t lds)
 {
     const dim3 B(kBlock);
-    hipStream_t s = c->stream;
-    if (hv && rv) launch_k(c, 0, k_rhs_s32<V, NS, true, true, D16, BT, TL>, G, B, lds, a);
-    else if (hv) launch_k(c, 0, k_rhs_s32<V, NS, true, false, D16, BT, TL>, G, B, lds, a);
-    else if (rv) launch_k(c, 0, k_rhs_s32<V, NS, false, true, D16, BT, TL>, G, B, lds, a);
-    else launch_k(c, 0, k_rhs_s32<V, NS, false, false, D16, BT, TL>, G, B, lds, a);
+    MF_LAUNCH_HVK(k_rhs_s32, D16, BT, TL);
 }
 
 template <int V, int NS, int BT, bool TL>
-static void launch_ref(mf_ctx* c, const RhsArgs& a, bool hv, bool rv, const dim3& G, size_t lds)
+static void launch_ref(mf_ctx* c, const RhsArgs& a, int hvk, bool rv, const dim3& G, size_t lds)
 {
     const dim3 B(kBlock);
-    if (hv && rv) launch_k(c, 0, k_rhs_ref<V, NS, true, true, BT, TL>, G, B, lds, a);
-    else if (hv) launch_k(c, 0, k_rhs_ref<V, NS, true, false, BT, TL>, G, B, lds, a);
-    else if (rv) launch_k(c, 0, k_rhs_ref<V, NS, false, true, BT, TL>, G, B, lds, a);
-    else launch_k(c, 0, k_rhs_ref<V, NS, false, false, BT, TL>, G, B, lds, a);
+    MF_LAUNCH_HVK(k_rhs_ref, BT, TL);
 }
+#undef MF_LAUNCH_HVK
 
 template <int V, int NS, int BT>
-static void launch_flux_bt(mf_ctx* c, int prec, const RhsArgs& a, bool hv, bool rv, unsigned grid)
+static void launch_flux_bt(mf_ctx* c, int prec, const RhsArgs& a, int hvk, bool rv, unsigned grid)
 {
-    const dim3 G(grid), B(kBlock);
-    hipStream_t s = c->stream;
+    const dim3 G(grid);
     const bool tl = c->tiles_a;
     const size_t lds = tl ? (size_t)c->ha_max * rec_len<V>() * sizeof(double) : 0;  // halo records
     if (prec == MF_PREC_F64) {
         if (tl)
-            launch_f64<V, NS, BT, true>(c, a, hv, rv, G, lds);
+            launch_f64<V, NS, BT, true>(c, a, hvk, rv, G, lds);
         else
-            launch_f64<V, NS, BT, false>(c, a, hv, rv, G, 0);
+            launch_f64<V, NS, BT, false>(c, a, hvk, rv, G, 0);
     } else if (prec == MF_PREC_F64_REF) {
         if (tl)
-            launch_ref<V, NS, BT, true>(c, a, hv, rv, G, lds);
+            launch_ref<V, NS, BT, true>(c, a, hvk, rv, G, lds);
         else
-            launch_ref<V, NS, BT, false>(c, a, hv, rv, G, 0);
+            launch_ref<V, NS, BT, false>(c, a, hvk, rv, G, 0);
     } else if (tl) {
-        launch_s32<V, NS, BT, false, true>(c, a, hv, rv, G, lds);
+        launch_s32<V, NS, BT, false, true>(c, a, hvk, rv, G, lds);
     } else if (c->didx_ok) {
-        launch_s32<V, NS, BT, true, false>(c, a, hv, rv, G, 0);
+        launch_s32<V, NS, BT, true, false>(c, a, hvk, rv, G, 0);
     } else {
-        launch_s32<V, NS, BT, false, false>(c, a, hv, rv, G, 0);
+        launch_s32<V, NS, BT, false, false>(c, a, hvk, rv, G, 0);
     }
 }
 
 // Stencil batch sizes compiled per stencil size: the tuning value picks
-// one (it must divide the stencil size; 0 = the default, the first).
+// one (it must divide the stencil size; 0 = the default, the first).  The
+// Tominec variants exist for the default only.
 template <int V>
-static void launch_flux_v(mf_ctx* c, int prec, const RhsArgs& a, bool hv, bool rv, unsigned grid)
+static void launch_flux_v(mf_ctx* c, int prec, const RhsArgs& a, int hvk, bool rv, unsigned grid)
 {
-    const int bt = c->tune_flux_batch[prec];
+    const int bt = (hvk & 2) ? 0 : c->tune_flux_batch[prec];
     switch (c->k) {
         case 15:
-            if (bt == 3) launch_flux_bt<V, 15, 3>(c, prec, a, hv, rv, grid);
-            else launch_flux_bt<V, 15, 5>(c, prec, a, hv, rv, grid);
+            if (bt == 3) launch_flux_bt<V, 15, 3>(c, prec, a, hvk, rv, grid);
+            else launch_flux_bt<V, 15, 5>(c, prec, a, hvk, rv, grid);
             break;
         case 20:
-            if (bt == 10) launch_flux_bt<V, 20, 10>(c, prec, a, hv, rv, grid);
-            else launch_flux_bt<V, 20, 5>(c, prec, a, hv, rv, grid);
+            if (bt == 10) launch_flux_bt<V, 20, 10>(c, prec, a, hvk, rv, grid);
+            else launch_flux_bt<V, 20, 5>(c, prec, a, hvk, rv, grid);
             break;
         case 30:
-            if (bt == 3) launch_flux_bt<V, 30, 3>(c, prec, a, hv, rv, grid);
-            else if (bt == 6) launch_flux_bt<V, 30, 6>(c, prec, a, hv, rv, grid);
-            else if (bt == 10) launch_flux_bt<V, 30, 10>(c, prec, a, hv, rv, grid);
-            else launch_flux_bt<V, 30, 5>(c, prec, a, hv, rv, grid);
+            if (bt == 3) launch_flux_bt<V, 30, 3>(c, prec, a, hvk, rv, grid);
+            else if (bt == 6) launch_flux_bt<V, 30, 6>(c, prec, a, hvk, rv, grid);
+            else if (bt == 10) launch_flux_bt<V, 30, 10>(c, prec, a, hvk, rv, grid);
+            else launch_flux_bt<V, 30, 5>(c, prec, a, hvk, rv, grid);
             break;
-        default: launch_flux_bt<V, 0, 4>(c, prec, a, hv, rv, grid); break;
+        default: launch_flux_bt<V, 0, 4>(c, prec, a, hvk, rv, grid); break;
     }
 }
 
+// gt bit 0: G of the viscosity source, bit 1: L of the Tominec source (its
+// variants exist for the default batch only).
 template <int TM, bool TL>
-static void launch_transpose_tl(mf_ctx* c, const RhsArgs& a, unsigned grid, size_t lds)
+static void launch_transpose_tl(mf_ctx* c, const RhsArgs& a, unsigned grid, size_t lds, int gt)
 {
     const dim3 G(grid), B(kBlock);
-    switch (c->tune_transpose_batch[TM]) {
-        case 2: launch_k(c, 1, k_rhs_transpose<4, TM, 2, TL>, G, B, lds, a); break;
-        case 8: launch_k(c, 1, k_rhs_transpose<4, TM, 8, TL>, G, B, lds, a); break;
-        default: launch_k(c, 1, k_rhs_transpose<4, TM, 4, TL>, G, B, lds, a); break;
+    if (gt == 2) {
+        launch_k(c, 1, k_rhs_transpose<4, TM, 4, TL, 2>, G, B, lds, a);
+    } else if (gt == 3) {
+        launch_k(c, 1, k_rhs_transpose<4, TM, 4, TL, 3>, G, B, lds, a);
+    } else {
+        switch (c->tune_transpose_batch[TM]) {
+            case 2: launch_k(c, 1, k_rhs_transpose<4, TM, 2, TL, 1>, G, B, lds, a); break;
+            case 8: launch_k(c, 1, k_rhs_transpose<4, TM, 8, TL, 1>, G, B, lds, a); break;
+            default: launch_k(c, 1, k_rhs_transpose<4, TM, 4, TL, 1>, G, B, lds, a); break;
+        }
     }
 }
 
 template <int TM>
-static void launch_transpose(mf_ctx* c, const RhsArgs& a, unsigned grid)
+static void launch_transpose(mf_ctx* c, const RhsArgs& a, unsigned grid, int gt)
 {
-    if (c->tiles_b)
-        launch_transpose_tl<TM, true>(c, a, grid, (size_t)c->hb_max * 8 * (TM == 1 ? sizeof(float) : sizeof(double)));
+    const int ru = ((gt & 1) ? 2 : 0) + ((gt & 2) ? 1 : 0);  // 4-value units per halo record
+    if (c->tiles_b && c->tiles_b_rec == 4 * ru)
+        launch_transpose_tl<TM, true>(c, a, grid,
+                                      (size_t)c->hb_max * 4 * ru * (TM == 1 ? sizeof(float) : sizeof(double)), gt);
     else
-        launch_transpose_tl<TM, false>(c, a, grid, 0);
+        launch_transpose_tl<TM, false>(c, a, grid, 0, gt);
 }
 
 int run_rhs(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const StageOut& st)
@@ -1432,9 +1608,17 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
     if (prec != MF_PREC_F64 && prec != MF_PREC_SPLIT32 && prec != MF_PREC_F64_REF)
         return fail(c, MF_ERR_ARGUMENT, "rhs!: unknown precision");
     const bool hv = c->have_hv && c->hv_gamma != 0.0;
+    const bool tom = c->have_lap && c->tom_gamma != 0.0;
     const bool rv = c->visc_mode != MF_VISC_NONE;
     if (rv && c->nvar != 4) return fail(c, MF_ERR_UNSUPPORTED, "viscosity sources need the Euler equations");
-    if (rv && !c->have_T) return fail(c, MF_ERR_STATE, "viscosity source without transposed operator");
+    if (tom && c->nvar != 4) return fail(c, MF_ERR_UNSUPPORTED, "the Tominec source needs the Euler equations");
+    if ((rv || tom) && !c->have_T) return fail(c, MF_ERR_STATE, "viscosity source without transposed operator");
+    if (tom && !c->lap_T) return fail(c, MF_ERR_STATE, "Tominec source: transposed lap not prepared");
+    if (tom && rv && c->visc_mode == MF_VISC_RESIDUAL &&
+        c->src_rank[MF_SRC_TOMINEC] < c->src_rank[MF_SRC_VISCOSITY])
+        return fail(c, MF_ERR_UNSUPPORTED,
+                    "a Tominec source before the residual viscosity source (its du would need lap'(lap u) first)");
+    const int hvk = (hv ? 1 : 0) | (tom ? 2 : 0);
     // first boundary pass on u
     // with the residual viscosity the first boundary pass rides on the sum pass
     const bool fused_bc = rv && c->visc_mode == MF_VISC_RESIDUAL && c->have_bc && !c->bc.multi;
@@ -1447,8 +1631,8 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
     a.out = st.out;
     const int64_t nblk = (c->N + kBlock - 1) / kBlock;
     const unsigned grid = xcd_grid(nblk);
-    if (rv) {
-        if (c->visc_mode == MF_VISC_RESIDUAL && (parts & RHS_REDUCE)) {
+    if (rv || tom) {
+        if (rv && c->visc_mode == MF_VISC_RESIDUAL && (parts & RHS_REDUCE)) {
             const int ns = c->n_psum, nm = c->n_pmax;
             BcArgs bc = make_bc(c, dt_off, 0);
             if (!((parts & RHS_BC) && fused_bc)) bc.flag = nullptr;
@@ -1459,21 +1643,22 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
         }
         RhsArgs pa = a;
         pa.st_kind = ST_NONE;
-        if (parts & RHS_FLUX) launch_flux_v<4>(c, prec, pa, hv, true, grid);
+        if (parts & RHS_FLUX) launch_flux_v<4>(c, prec, pa, hvk, rv, grid);
         if (parts & RHS_TRANSPOSE) {
+            const int gt = (rv ? 1 : 0) | (tom ? 2 : 0);
             if (prec == MF_PREC_F64)
-                launch_transpose<0>(c, a, grid);
+                launch_transpose<0>(c, a, grid, gt);
             else if (prec == MF_PREC_SPLIT32)
-                launch_transpose<1>(c, a, grid);
+                launch_transpose<1>(c, a, grid, gt);
             else
-                launch_transpose<2>(c, a, grid);
+                launch_transpose<2>(c, a, grid, gt);
         }
     } else {
         if (parts & RHS_FLUX) {
             if (c->nvar == 4)
-                launch_flux_v<4>(c, prec, a, hv, false, grid);
+                launch_flux_v<4>(c, prec, a, hvk, false, grid);
             else
-                launch_flux_v<1>(c, prec, a, hv, false, grid);
+                launch_flux_v<1>(c, prec, a, hvk, false, grid);
         }
     }
     MF_HIP(c, hipGetLastError());

@@ -845,8 +845,10 @@ int launch_pack_ops(mf_ctx* c, int target, const double* d_wx, const double* d_w
     const unsigned g = (unsigned)((n + 255) / 256);
     if (target == MF_OP_FLUX)
         hipLaunchKernelGGL(k_pack_flux, dim3(g), dim3(256), 0, c->stream, n, d_wx, d_wy, c->d_wxy, c->d_wxy32);
-    else
+    else if (target == MF_OP_HYPERVISCOSITY)
         hipLaunchKernelGGL(k_pack_hv, dim3(g), dim3(256), 0, c->stream, n, d_wx, d_wy, c->d_hv, c->d_hv32);
+    else
+        hipLaunchKernelGGL(k_pack_hv, dim3(g), dim3(256), 0, c->stream, n, d_wx, d_wy, c->d_lap, c->d_lap32);
     MF_HIP(c, hipGetLastError());
     return MF_OK;
 }
@@ -878,7 +880,7 @@ __global__ void k_t_width(int64_t N, int64_t ns, const int32_t* deg, int32_t* w,
 }
 
 __global__ void k_t_fill(int64_t N, int K, const int32_t* idx, const double2* wxy, const int64_t* off,
-                         int32_t* cursor, int32_t* trow, double2* tw)
+                         int32_t* cursor, int32_t* trow, double2* tw, int64_t* tsrc)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= N * K) return;
@@ -888,10 +890,11 @@ __global__ void k_t_fill(int64_t N, int K, const int32_t* idx, const double2* wx
     const int64_t t = off[j / kSellC] + (int64_t)pos * kSellC + (j % kSellC);
     trow[t] = (int32_t)r;
     tw[t] = wxy[q];
+    tsrc[t] = q;
 }
 
 __global__ void k_t_sort(int64_t N, const int32_t* deg, const int32_t* w, const int64_t* off,
-                         const int32_t* uid, int32_t* trow, double2* tw, float2* tw32)
+                         const int32_t* uid, int32_t* trow, double2* tw, float2* tw32, int64_t* tsrc)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -901,19 +904,23 @@ __global__ void k_t_sort(int64_t N, const int32_t* deg, const int32_t* w, const 
         const int32_t rv = trow[base + (int64_t)a2 * kSellC];
         const int32_t rk = uid[rv];
         const double2 wv = tw[base + (int64_t)a2 * kSellC];
+        const int64_t sv = tsrc[base + (int64_t)a2 * kSellC];
         int b = a2 - 1;
         while (b >= 0 && uid[trow[base + (int64_t)b * kSellC]] > rk) {
             trow[base + (int64_t)(b + 1) * kSellC] = trow[base + (int64_t)b * kSellC];
             tw[base + (int64_t)(b + 1) * kSellC] = tw[base + (int64_t)b * kSellC];
+            tsrc[base + (int64_t)(b + 1) * kSellC] = tsrc[base + (int64_t)b * kSellC];
             --b;
         }
         trow[base + (int64_t)(b + 1) * kSellC] = rv;
         tw[base + (int64_t)(b + 1) * kSellC] = wv;
+        tsrc[base + (int64_t)(b + 1) * kSellC] = sv;
     }
     const int W = w[i / kSellC];
     for (int a2 = n; a2 < W; ++a2) {
         trow[base + (int64_t)a2 * kSellC] = (int32_t)i;
         tw[base + (int64_t)a2 * kSellC] = make_double2(0.0, 0.0);
+        tsrc[base + (int64_t)a2 * kSellC] = -1;
     }
     for (int a2 = 0; a2 < W; ++a2) {
         const double2 v = tw[base + (int64_t)a2 * kSellC];
@@ -933,11 +940,18 @@ int launch_build_transpose(mf_ctx* c)
     dfree(c->d_trow);
     dfree(c->d_tw);
     dfree(c->d_tw32);
+    dfree(c->d_tsrc);
+    dfree(c->d_tl);
+    dfree(c->d_tl32);
     c->d_sell_off = nullptr;
     c->d_sell_w = nullptr;
     c->d_trow = nullptr;
     c->d_tw = nullptr;
     c->d_tw32 = nullptr;
+    c->d_tsrc = nullptr;
+    c->d_tl = nullptr;
+    c->d_tl32 = nullptr;
+    c->lap_T = false;
     if ((rc = dalloc(c, &deg, N)) || (rc = dalloc(c, &cursor, N)) || (rc = dalloc(c, &wsize, ns + 1)) ||
         (rc = dalloc(c, &c->d_sell_off, ns + 1)) || (rc = dalloc(c, &c->d_sell_w, ns)))
         return rc;
@@ -957,18 +971,22 @@ int launch_build_transpose(mf_ctx* c)
     MF_HIP(c, hipStreamSynchronize(c->stream));
     (void)hipFree(tmp);
     if ((rc = dalloc(c, &c->d_trow, total)) || (rc = dalloc(c, &c->d_tw, total)) ||
-        (rc = dalloc(c, &c->d_tw32, total)))
+        (rc = dalloc(c, &c->d_tw32, total)) || (rc = dalloc(c, &c->d_tsrc, total)))
         return rc;
+    // entries of lanes past N in the last slice are never filled: mark
+    // every entry as padding first
+    MF_HIP(c, hipMemsetAsync(c->d_tsrc, 0xff, sizeof(int64_t) * total, c->stream));
     hipLaunchKernelGGL(k_t_fill, dim3(gq), dim3(256), 0, c->stream, N, (int)K, c->d_idx, c->d_wxy,
-                       c->d_sell_off, cursor, c->d_trow, c->d_tw);
+                       c->d_sell_off, cursor, c->d_trow, c->d_tw, c->d_tsrc);
     hipLaunchKernelGGL(k_t_sort, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c->stream, N, deg,
-                       c->d_sell_w, c->d_sell_off, c->d_uid, c->d_trow, c->d_tw, c->d_tw32);
+                       c->d_sell_w, c->d_sell_off, c->d_uid, c->d_trow, c->d_tw, c->d_tw32, c->d_tsrc);
     MF_HIP(c, hipGetLastError());
     MF_HIP(c, hipStreamSynchronize(c->stream));
     dfree(deg);
     dfree(cursor);
     dfree(wsize);
     c->n_slices = ns;
+    c->t_entries = total;
     c->have_T = true;
     return MF_OK;
 }
@@ -1077,8 +1095,30 @@ int launch_pack_sorted(mf_ctx* c, int target, const double* d_wx, const double* 
                            c->d_wy_s);
     } else {
         hipLaunchKernelGGL(k_permute_sum, dim3(g), dim3(256), 0, c->stream, c->N, c->k, c->d_perm_s, d_wx, d_wy,
-                           c->d_hv_s);
+                           target == MF_OP_HYPERVISCOSITY ? c->d_hv_s : c->d_lap_s);
     }
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+// lap of the transposed entries (Tominec source): w[e] = lap at the
+// slot-major stencil position src[e], 0 for padding (src[e] = -1).
+__global__ void k_fill_lap_t(int64_t n, const int64_t* src, const double* lap, int64_t n_lap, double* w,
+                             float* w32)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int64_t q = src[e];
+    const double v = (q >= 0 && q < n_lap) ? lap[q] : 0.0;
+    w[e] = v;
+    w32[e] = (float)v;
+}
+
+int launch_fill_lap_t(mf_ctx* c, int64_t n, const int64_t* d_src, double* d_w, float* d_w32)
+{
+    if (n <= 0) return MF_OK;
+    hipLaunchKernelGGL(k_fill_lap_t, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n, d_src,
+                       c->d_lap, c->N * (int64_t)c->k, d_w, d_w32);
     MF_HIP(c, hipGetLastError());
     return MF_OK;
 }

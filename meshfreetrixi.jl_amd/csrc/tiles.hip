@@ -10,7 +10,7 @@
 //
 // Phase A (flux divergence, sources, viscosity phase A): halo = the union
 // of the tile's stencils; positions slot-major like the node indices.
-// Phase B (transposed gather D^T G): rows of the tile sorted by their
+// Phase B (transposed gathers D^T G, lap^T L): rows of the tile sorted by their
 // transposed degree (long rows first) so that the 64 rows of a wavefront
 // have similar lengths, stored SELL-64 per wavefront; entries of a row in
 // ascending caller index (the reference's CSC order); halo = the union of
@@ -27,7 +27,8 @@ namespace mf {
 
 constexpr int kTile = 256;
 constexpr int kSlicesPerTile = kTile / kSellC;  // 4
-constexpr size_t kTileLdsBytes = 48 * 1024;     // largest halo staged in LDS
+constexpr size_t kTileLdsBytes = 48 * 1024;     // largest phase-A halo staged in LDS
+constexpr size_t kTileLdsBytesB = 64 * 1024;    // phase B (records of G and lap u)
 
 void free_tiles_b(mf_ctx* c)
 {
@@ -39,6 +40,14 @@ void free_tiles_b(mf_ctx* c)
     dfree(c->d_lb);
     dfree(c->d_twb);
     dfree(c->d_twb32);
+    dfree(c->d_tb_src);
+    dfree(c->d_twl);
+    dfree(c->d_twl32);
+    c->d_tb_src = nullptr;
+    c->d_twl = nullptr;
+    c->d_twl32 = nullptr;
+    c->lap_tb = false;
+    c->tiles_b_rec = 0;
     c->d_hb_off = c->d_sb_off = nullptr;
     c->d_hb_ids = c->d_sb_w = nullptr;
     c->d_lb = nullptr;
@@ -89,7 +98,9 @@ static int upload(mf_ctx* c, T** dst, const std::vector<T>& v)
     return MF_OK;
 }
 
-int build_tiles(mf_ctx* c, bool want_b)
+// rec_b: doubles per phase-B halo record (G: 2 nvar, L: nvar), which
+// decides whether a tile's phase-B halo fits in LDS.
+int build_tiles(mf_ctx* c, bool want_b, int rec_b)
 {
     const int64_t N = c->N;
     const int K = c->k;
@@ -154,6 +165,7 @@ int build_tiles(mf_ctx* c, bool want_b)
     if (!want_b || !c->tiles_a || c->tried_b) return MF_OK;
     free_tiles_b(c);
     c->tried_b = true;
+    c->tiles_b_rec = rec_b;
 
     // transposed adjacency: the entries (j, s) with idx[s][j] = i, per row i,
     // in ascending caller index of j
@@ -194,7 +206,7 @@ int build_tiles(mf_ctx* c, bool want_b)
                 }
             }
         std::sort(ids.begin(), ids.end());
-        if (ids.size() > 65535 || ids.size() * 2 * V * sizeof(double) > kTileLdsBytes) {
+        if (ids.size() > 65535 || ids.size() * (size_t)rec_b * sizeof(double) > kTileLdsBytesB) {
             for (int32_t j : ids) mark[j] = -1;
             return MF_OK;  // global-memory transposed gather
         }
@@ -233,18 +245,17 @@ int build_tiles(mf_ctx* c, bool want_b)
         hall.insert(hall.end(), ids.begin(), ids.end());
         hoff[t + 1] = (int64_t)hall.size();
     }
-    int64_t* d_src = nullptr;
     if ((rc = upload(c, &c->d_hb_off, hoff)) || (rc = upload(c, &c->d_hb_ids, hall)) ||
         (rc = upload(c, &c->d_sb_off, soff)) || (rc = upload(c, &c->d_sb_w, sw)) ||
-        (rc = upload(c, &c->d_rowb, rowb)) || (rc = upload(c, &c->d_lb, lb)) || (rc = upload(c, &d_src, src)) ||
-        (rc = dalloc(c, &c->d_twb, lb.size())) || (rc = dalloc(c, &c->d_twb32, lb.size())))
+        (rc = upload(c, &c->d_rowb, rowb)) || (rc = upload(c, &c->d_lb, lb)) ||
+        (rc = upload(c, &c->d_tb_src, src)) || (rc = dalloc(c, &c->d_twb, lb.size())) ||
+        (rc = dalloc(c, &c->d_twb32, lb.size())))
         return rc;
     const int64_t ne = (int64_t)lb.size();
-    hipLaunchKernelGGL(k_fill_tile_weights, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, c->stream, ne, d_src,
-                       c->d_wxy, c->d_twb, c->d_twb32);
+    hipLaunchKernelGGL(k_fill_tile_weights, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, c->stream, ne,
+                       c->d_tb_src, c->d_wxy, c->d_twb, c->d_twb32);
     MF_HIP(c, hipGetLastError());
     MF_HIP(c, hipStreamSynchronize(c->stream));
-    dfree(d_src);
     c->hb_max = hmax;
     c->hb_total = (int64_t)hall.size();
     c->nb_entries = ne;

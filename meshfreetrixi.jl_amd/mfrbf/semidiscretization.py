@@ -15,8 +15,8 @@ import numpy as np
 
 from . import _lib as L
 from .equations import CompressibleEulerEquations2D, LinearScalarAdvectionEquation2D
-from .sources import (SourceHyperviscosityFlyer, SourceResidualViscosityTominec,
-                      SourceTerms, SourceUpwindViscosityTominec)
+from .sources import (SourceHyperviscosityFlyer, SourceHyperviscosityTominec,
+                      SourceResidualViscosityTominec, SourceTerms, SourceUpwindViscosityTominec)
 
 PRECISIONS = {"f64": L.MF_PREC_F64, "split32": L.MF_PREC_SPLIT32, "f64_ref": L.MF_PREC_F64_REF}
 
@@ -91,8 +91,18 @@ class SemidiscretizationHyperbolic:
         hv_seen = False
         self.visc_source = None
         self.hv_source = None
+        self.tominec_source = None
+        order = []
         for s in self.source_terms.values():
-            if isinstance(s, SourceHyperviscosityFlyer):
+            if isinstance(s, SourceHyperviscosityTominec):
+                if self.tominec_source is not None:
+                    raise NotImplementedError("more than one Tominec hyperviscosity source")
+                # lap = sum(compute_flux_operator(solver, domain, 2)), hyperviscosity.jl:111-113
+                self.ctx.compute_operator(basis.phs_power, basis.N, 2, L.MF_OP_LAPLACIAN, want=False)
+                self.ctx.set_source_hyperviscosity_tominec(s.gamma)
+                self.tominec_source = s
+                order.append(L.MF_SRC_TOMINEC)
+            elif isinstance(s, SourceHyperviscosityFlyer):
                 if self.hv_source is not None:
                     raise NotImplementedError("more than one Flyer hyperviscosity source")
                 self.ctx.compute_operator(basis.phs_power, basis.N, 2 * s.k, L.MF_OP_HYPERVISCOSITY,
@@ -100,13 +110,16 @@ class SemidiscretizationHyperbolic:
                 self.ctx.set_source_hyperviscosity(s.gamma)
                 self.hv_source = s
                 hv_seen = True
+                order.append(L.MF_SRC_FLYER)
             elif isinstance(s, (SourceResidualViscosityTominec, SourceUpwindViscosityTominec)):
                 if self.visc_source is not None:
                     raise NotImplementedError("more than one viscosity source")
                 self.ctx.set_source_viscosity(s.mode, s.c_rv, s.c_uw, s.h, s.polydeg, hv_seen)
                 self.visc_source = s
+                order.append(L.MF_SRC_VISCOSITY)
             else:
                 raise NotImplementedError("source %r" % type(s).__name__)
+        self.ctx.set_source_order(order)
 
     def _update_host_bc(self, t):
         if not self._host_bc:

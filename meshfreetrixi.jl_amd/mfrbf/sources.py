@@ -64,12 +64,14 @@ class SourceUpwindViscosityTominec:
 
 
 class SourceHyperviscosityTominec:
-    """lap' * lap hyperviscosity (hyperviscosity.jl:80-134).  Its operator
-    has the two-ring pattern of a sparse product, outside the fixed-width
-    stencil layout of this build."""
+    """du -= gamma * lap' (lap u), lap = Dxx + Dyy from the solver's basis,
+    gamma = c * dx_min^4.5 (hyperviscosity.jl:66-134).  The reference forms
+    H = lap' * lap as a sparse product; the engine applies its two factors
+    (the stencil pass stores lap u, the transposed pass gathers lap')."""
 
-    def __init__(self, *a, **kw):
-        raise NotImplementedError("SourceHyperviscosityTominec is not provided by the HIP engine")
+    def __init__(self, solver, equations, domain, c=1.0):
+        self.c = float(c)
+        self.gamma = self.c * domain.pd.dx_min ** 4.5
 
 
 class SourceIGR:

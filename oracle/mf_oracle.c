@@ -911,23 +911,24 @@ int64_t or_rhs_scratch_len(const or_rhs_cfg* c)
     return (int64_t)c->nvar * c->N + 4 * c->N;
 }
 
-/* Hyperviscosity sources.
- * Flyer, hyperviscosity.jl:52-64: du += (H u) * (-gamma), H = Dx^2k + Dy^2k.
- * Tominec, hyperviscosity.jl:80-134: the reference forms H = lap' * lap
- * (lap = Dxx + Dyy) explicitly as a sparse product and applies it the same
- * way; here it is applied as lap' (lap u): local = lap u (CSC product), then
+/* Flyer hyperviscosity, hyperviscosity.jl:52-64: du += (H u) * (-gamma),
+ * H = Dx^2k + Dy^2k, applied like mul_by_accum! (one CSC product). */
+static void or_src_flyer(const or_rhs_cfg* c, const double* u, double* du)
+{
+    for (int v = 0; v < c->nvar; ++v)
+        or_csc_mul(c->N, c->colptr, c->rowval, c->hv, u + v * c->N, du + v * c->N, -c->hv_gamma, 0);
+}
+
+/* Tominec hyperviscosity, hyperviscosity.jl:80-134: the reference forms
+ * H = lap' * lap (lap = Dxx + Dyy) explicitly as a sparse product and
+ * applies it the same way as Flyer; here H u = lap' (lap u): local = lap u,
  * du += (lap' local) * (-gamma).  The two forms agree to rounding; the
  * oracle tests pin this one against scipy's explicit lap.T @ lap. */
-static void or_src_hv(const or_rhs_cfg* c, const double* u, double* du, double* local)
+static void or_src_tominec(const or_rhs_cfg* c, const double* u, double* du, double* local)
 {
     for (int v = 0; v < c->nvar; ++v) {
-        if (c->hv_kind == 1) {
-            or_csc_mul(c->N, c->colptr, c->rowval, c->hv, u + v * c->N, local, 1.0, 1);
-            or_csc_tmul(c->N, c->colptr, c->rowval, c->hv, local, du + v * c->N, -c->hv_gamma);
-        } else {
-            or_csc_mul(c->N, c->colptr, c->rowval, c->hv, u + v * c->N, du + v * c->N,
-                       -c->hv_gamma, 0);
-        }
+        or_csc_mul(c->N, c->colptr, c->rowval, c->lap, u + v * c->N, local, 1.0, 1);
+        or_csc_tmul(c->N, c->colptr, c->rowval, c->lap, local, du + v * c->N, -c->tom_gamma);
     }
 }
 
@@ -991,9 +992,19 @@ int or_rhs(const or_rhs_cfg* c, double t, double* u, double* du, double* eps_out
         if (!eps) eps = own_e;
         if (!eps_c) eps_c = own_c;
     }
-    if (c->hv && c->hv_first) or_src_hv(c, u, du, rv_scr + 3 * N);
-    if (c->rv_mode != OR_RV_NONE) or_src_rv(c, u, du, eps, eps_c, rv_scr);
-    if (c->hv && !c->hv_first) or_src_hv(c, u, du, rv_scr + 3 * N);
+    int order[3] = {0, 0, 0};
+    if (c->src_order[0]) {
+        for (int q = 0; q < 3; ++q) order[q] = c->src_order[q];
+    } else {
+        order[0] = c->hv_first ? OR_SRC_FLYER : OR_SRC_VISCOSITY;
+        order[1] = c->hv_first ? OR_SRC_VISCOSITY : OR_SRC_FLYER;
+        order[2] = OR_SRC_TOMINEC;
+    }
+    for (int q = 0; q < 3 && order[q]; ++q) {
+        if (order[q] == OR_SRC_FLYER && c->hv) or_src_flyer(c, u, du);
+        if (order[q] == OR_SRC_TOMINEC && c->lap) or_src_tominec(c, u, du, rv_scr + 3 * N);
+        if (order[q] == OR_SRC_VISCOSITY && c->rv_mode != OR_RV_NONE) or_src_rv(c, u, du, eps, eps_c, rv_scr);
+    }
     or_bc_pass(c, t, u, du); /* second pass, :422-425 */
     free(own_e);
     free(own_c);

@@ -58,6 +58,10 @@ enum { OR_BC_DIRICHLET = 0, OR_BC_SLIP = 1, OR_BC_NOTHING = 2 };
 enum { OR_FN_CONST = 0, OR_FN_PULSE = 1, OR_FN_WAVE = 2, OR_FN_GRADIENT = 3, OR_FN_TABLE = 4 };
 enum { OR_RV_NONE = 0, OR_RV_RESIDUAL = 1, OR_RV_UPWIND = 2 };
 
+#define OR_SRC_FLYER 1
+#define OR_SRC_TOMINEC 2
+#define OR_SRC_VISCOSITY 3
+
 typedef struct {
     int64_t N;
     int nvar;
@@ -88,9 +92,12 @@ typedef struct {
     /* residual-viscosity state */
     const double* approx_du;
     int success_iter;
-    /* hyperviscosity kind: 0 Flyer (du -= gamma H u with H = hv),
-     * 1 Tominec (du -= gamma lap' (lap u) with lap = hv) */
-    int hv_kind;
+    /* Tominec hyperviscosity: du -= tom_gamma lap' (lap u); NULL = none */
+    const double* lap;
+    double tom_gamma;
+    /* sources in keyword order (OR_SRC_*), 0-terminated; src_order[0] = 0:
+     * Flyer and the viscosity source ordered by hv_first, then Tominec */
+    int src_order[3];
 } or_rhs_cfg;
 
 void or_flux(int eq, const double* adv_a, double gamma, int orientation, const double* u,

@@ -18,7 +18,7 @@ BC_DIRICHLET, BC_SLIP, BC_NOTHING = 0, 1, 2
 FN_CONST, FN_PULSE, FN_WAVE, FN_GRADIENT, FN_TABLE = 0, 1, 2, 3, 4
 RV_NONE, RV_RESIDUAL, RV_UPWIND = 0, 1, 2
 VAR_DENSITY, VAR_PRESSURE = 0, 1
-HV_FLYER, HV_TOMINEC = 0, 1
+SRC_FLYER, SRC_TOMINEC, SRC_VISCOSITY = 1, 2, 3
 
 
 def build():
@@ -57,7 +57,8 @@ class RhsCfg(C.Structure):
         ("ntags", C.c_int), ("tag_ptr", _i32p), ("tag_kind", _i32p),
         ("tag_fn", _i32p), ("tag_fn_params", _dp), ("bidx", _i32p),
         ("bnormal", _dp), ("btable", _dp),
-        ("approx_du", _dp), ("success_iter", C.c_int), ("hv_kind", C.c_int),
+        ("approx_du", _dp), ("success_iter", C.c_int),
+        ("lap", _dp), ("tom_gamma", C.c_double), ("src_order", C.c_int * 3),
     ]
 
 
@@ -242,7 +243,9 @@ class Problem:
 
     def __init__(self, x, y, ptr, idx, wx, wy, eq, nvar, gamma=1.4, adv=(1.0, 1.0),
                  boundary=(), hv=None, hv_gamma=0.0, rv_mode=RV_NONE, c_rv=1.0, c_uw=1.0,
-                 dx_avg=0.0, hv_first=True, hv_kind=HV_FLYER):
+                 dx_avg=0.0, hv_first=True, lap=None, tom_gamma=0.0, order=None):
+        """lap, tom_gamma: Tominec source; order: the sources in keyword order
+        (SRC_* values; default Flyer and viscosity by hv_first, then Tominec)."""
         self.N = N = len(x)
         self.x = np.ascontiguousarray(x, dtype=np.float64)
         self.y = np.ascontiguousarray(y, dtype=np.float64)
@@ -250,6 +253,7 @@ class Problem:
         self.dx = self.csc.values(wx)
         self.dy = self.csc.values(wy)
         self.hv = self.csc.values(hv) if hv is not None else None
+        self.lap = self.csc.values(lap) if lap is not None else None
         self.nvar = nvar
         self.approx_du = np.zeros(nvar * N)
         tag_ptr, kinds, fns, params, bidx, bnorm, btab = [0], [], [], [], [], [], []
@@ -291,7 +295,10 @@ class Problem:
         c.bidx, c.bnormal, c.btable = i32(self.bidx), d(self.bnormal), d(self.btable)
         c.approx_du = d(self.approx_du)
         c.success_iter = 0
-        c.hv_kind = hv_kind
+        c.lap = d(self.lap)
+        c.tom_gamma = tom_gamma
+        for q, k in enumerate(order or []):
+            c.src_order[q] = k
         self.cfg = c
         self.scratch = np.zeros(lib().or_rhs_scratch_len(C.byref(c)) + 3 * nvar * N)
 

@@ -12,9 +12,11 @@ class Case:
 
 
 def build(spec, side=None, use_gpu=True, weights="oracle", hv=None, visc=None, bc=("dirichlet", "nothing"),
-          knn="grid", node_order=0):
+          knn="grid", node_order=0, tom=None, order=None):
     """weights: 'oracle' installs the oracle's weights on the GPU (so the
-    rhs! comparison isolates the stencil kernels), 'gpu' keeps the GPU's own."""
+    rhs! comparison isolates the stencil kernels), 'gpu' keeps the GPU's own.
+    hv = (k, c): Flyer source; tom = c: Tominec source; order: the sources in
+    keyword order, names 'flyer', 'tominec', 'visc' (default: that order)."""
     c = Case()
     c.spec = spec
     side = side or spec.side
@@ -60,6 +62,17 @@ def build(spec, side=None, use_gpu=True, weights="oracle", hv=None, visc=None, b
         assert rc == 0
         c.hv_w = (hx + hy).reshape(N, -1)
         c.hv_gamma = hc * c.dx_min ** (2 * hk)
+    c.lap_w = None
+    c.tom_gamma = 0.0
+    if tom is not None:
+        lx, ly, rc = O.weights(x, y, ptr, idx, spec.p, spec.deg, 2)
+        assert rc == 0
+        c.lap_w = (lx + ly).reshape(N, -1)
+        c.tom_gamma = tom * c.dx_min ** 4.5
+    names = {"flyer": (O.SRC_FLYER, L.MF_SRC_FLYER), "tominec": (O.SRC_TOMINEC, L.MF_SRC_TOMINEC),
+             "visc": (O.SRC_VISCOSITY, L.MF_SRC_VISCOSITY)}
+    order = list(order or ["flyer", "tominec", "visc"])
+    hv_first = order.index("flyer") < order.index("visc") if "visc" in order and "flyer" in order else True
     c.visc = visc
     rv_mode = O.RV_NONE
     if visc is not None:
@@ -67,7 +80,8 @@ def build(spec, side=None, use_gpu=True, weights="oracle", hv=None, visc=None, b
     c.prob = O.Problem(x, y, ptr, idx, wx, wy, c.eq, c.nvar, gamma=spec.gamma, adv=spec.adv,
                        boundary=[(ko, nodes, nrm, fn, params, None) for ((kg, ko), nodes, nrm) in c.bc],
                        hv=c.hv_w, hv_gamma=c.hv_gamma, rv_mode=rv_mode, c_rv=1.0, c_uw=1.0,
-                       dx_avg=c.dx_avg, hv_first=True)
+                       dx_avg=c.dx_avg, hv_first=hv_first, lap=c.lap_w, tom_gamma=c.tom_gamma,
+                       order=[names[n][0] for n in order])
     if not use_gpu:
         return c
     ctx = L.Context(0)
@@ -94,7 +108,14 @@ def build(spec, side=None, use_gpu=True, weights="oracle", hv=None, visc=None, b
         else:
             ctx.compute_operator(spec.p, spec.deg, 2 * hv[0], L.MF_OP_HYPERVISCOSITY, want=False)
         ctx.set_source_hyperviscosity(c.hv_gamma)
+    if tom is not None:
+        if weights == "oracle":
+            ctx.set_operator(L.MF_OP_LAPLACIAN, c.lap_w)
+        else:
+            ctx.compute_operator(spec.p, spec.deg, 2, L.MF_OP_LAPLACIAN, want=False)
+        ctx.set_source_hyperviscosity_tominec(c.tom_gamma)
     if visc is not None:
         mode = {"residual": L.MF_VISC_RESIDUAL, "upwind": L.MF_VISC_UPWIND}[visc]
-        ctx.set_source_viscosity(mode, 1.0, 1.0, c.dx_avg, getattr(spec, "approx_order", 3), True)
+        ctx.set_source_viscosity(mode, 1.0, 1.0, c.dx_avg, getattr(spec, "approx_order", 3), hv_first)
+    ctx.set_source_order([names[n][1] for n in order])
     return c

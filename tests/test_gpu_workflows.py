@@ -40,6 +40,12 @@ def _cyl_semi(visc=None, host_dirichlet=False):
     src = None
     if visc == "upwind":
         src = mfrbf.SourceTerms(uv=mfrbf.SourceUpwindViscosityTominec(solver, eqs, domain, c=1.0, c_uw=1.0))
+    elif visc == "tominec_upwind":  # test/upwind_viscosity_test.jl:50-52
+        src = mfrbf.SourceTerms(hv=mfrbf.SourceHyperviscosityTominec(solver, eqs, domain, c=1.0),
+                                rv=mfrbf.SourceUpwindViscosityTominec(solver, eqs, domain))
+    elif visc == "flyer_tominec":  # test/hyperviscosity_test.jl:42-45
+        src = mfrbf.SourceTerms(hv=mfrbf.SourceHyperviscosityFlyer(solver, eqs, domain, k=2, c=1.0),
+                                hv2=mfrbf.SourceHyperviscosityTominec(solver, eqs, domain, c=1.0))
     semi = mfrbf.SemidiscretizationHyperbolic(domain, eqs, grad, solver, boundary_conditions=bcs,
                                               source_terms=src)
     return semi, domain
@@ -58,15 +64,26 @@ def _cyl_oracle(domain, visc=None):
                 (O.BC_NOTHING, tags["outlet"].idx, tags["outlet"].normals, None, None, None)]
     for name in ("bottom", "top", "cylinder"):
         boundary.append((O.BC_SLIP, tags[name].idx, tags[name].normals, None, None, None))
-    rv = O.RV_UPWIND if visc == "upwind" else O.RV_NONE
+    rv = O.RV_UPWIND if visc in ("upwind", "tominec_upwind") else O.RV_NONE
+    kw = {}
+    if visc in ("tominec_upwind", "flyer_tominec"):
+        lx, ly, rc = O.weights(x, y, O.ell_ptr(N, k), idx, 5, 3, 2)
+        kw.update(lap=(lx + ly).reshape(N, k), tom_gamma=pd.dx_min ** 4.5)
+    if visc == "tominec_upwind":
+        kw.update(order=[O.SRC_TOMINEC, O.SRC_VISCOSITY])
+    if visc == "flyer_tominec":
+        hx, hy, rc = O.weights(x, y, O.ell_ptr(N, k), idx, 5, 3, 4)
+        kw.update(hv=(hx + hy).reshape(N, k), hv_gamma=pd.dx_min ** 4, order=[O.SRC_FLYER, O.SRC_TOMINEC])
     return O.Problem(x, y, O.ell_ptr(N, k), idx, wx, wy, O.EQ_EULER, 4, boundary=boundary, rv_mode=rv,
-                     c_rv=1.0, c_uw=1.0, dx_avg=pd.dx_avg)
+                     c_rv=1.0, c_uw=1.0, dx_avg=pd.dx_avg, **kw)
 
 
-@pytest.mark.parametrize("visc,host_dirichlet", [(None, False), ("upwind", False), (None, True)])
+@pytest.mark.parametrize("visc,host_dirichlet", [(None, False), ("upwind", False), (None, True),
+                                                 ("tominec_upwind", False), ("flyer_tominec", False)])
 def test_host_api_cyl_slip_walls(mf, visc, host_dirichlet):
     """Trixi.rhs! on the reference's Medusa cloud (2,154 nodes, five boundary
-    groups): Dirichlet inlet, do-nothing outlet, three slip walls."""
+    groups): Dirichlet inlet, do-nothing outlet, three slip walls; with the
+    source combinations of the reference's viscosity tests."""
     _say("host API cyl_0_05 visc=%s host_dirichlet=%s" % (visc, host_dirichlet))
     semi, domain = _cyl_semi(visc, host_dirichlet)
     prob = _cyl_oracle(domain, visc)

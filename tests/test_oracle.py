@@ -439,7 +439,7 @@ def test_tominec_hyperviscosity_matches_explicit_product():
     g = 1e-4 * (1.0 / 23) ** 4.5
     u0 = np.stack([1.0 + 0.2 * np.sin(2 * np.pi * (x + y)), 1.0 + 0 * x, 1.0 + 0 * x, 2.5 + x * y])
     plain = O.Problem(x, y, ptr, idx, wx, wy, O.EQ_EULER, 4)
-    tom = O.Problem(x, y, ptr, idx, wx, wy, O.EQ_EULER, 4, hv=lap, hv_gamma=g, hv_kind=O.HV_TOMINEC)
+    tom = O.Problem(x, y, ptr, idx, wx, wy, O.EQ_EULER, 4, lap=lap, tom_gamma=g)
     du0, _, _ = plain.rhs(u0.copy(), 0.0)
     du1, _, _ = tom.rhs(u0.copy(), 0.0)
     H = (Lm.T @ Lm).tocsr()
@@ -450,3 +450,41 @@ def test_tominec_hyperviscosity_matches_explicit_product():
     dc1, _, _ = tom.rhs(c.copy(), 0.0)
     dc0, _, _ = plain.rhs(c.copy(), 0.0)
     assert np.abs(dc1 - dc0).max() < 1e-12 * np.abs(g * (abs(H) @ np.ones(N))).max() + 1e-18
+
+
+def test_source_order_flyer_tominec_upwind():
+    """SourceTerms applies its sources in keyword order (generic_sources.jl:
+    7-18).  With the upwind viscosity (eps from u alone) every order gives
+    the same du up to rounding, and each source adds its own term: the
+    rhs! with all three equals the flux-only rhs! plus the three
+    single-source differences."""
+    spec = configs.c2()
+    x, y, _, _, _ = O.cloud_square(20, spec.seed)
+    N, k = len(x), 30
+    idx, d2 = O.knn_grid(x, y, k)
+    ptr = O.ell_ptr(N, k)
+    dx_min, dx_avg = O.dx_stats(d2[:, 1])
+    wx, wy, rc = O.weights(x, y, ptr, idx, 5, 3, 1)
+    hx, hy, _ = O.weights(x, y, ptr, idx, 5, 3, 4)
+    lx, ly, _ = O.weights(x, y, ptr, idx, 5, 3, 2)
+    H, lap = (hx + hy).reshape(N, k), (lx + ly).reshape(N, k)
+    gf, gt = dx_min ** 4, dx_min ** 4.5
+    u0 = np.stack([1.0 + 0.2 * np.sin(2 * np.pi * (x + y)), 0.3 + 0 * x, -0.2 + 0.1 * y, 2.5 + x * y])
+
+    def du(order, fly=True, tom=True, rv=True):
+        p = O.Problem(x, y, ptr, idx, wx, wy, O.EQ_EULER, 4, hv=H if fly else None, hv_gamma=gf,
+                      rv_mode=O.RV_UPWIND if rv else O.RV_NONE, dx_avg=dx_avg,
+                      lap=lap if tom else None, tom_gamma=gt, order=order)
+        return p.rhs(u0.copy(), 0.0)[0]
+
+    F, T, R = O.SRC_FLYER, O.SRC_TOMINEC, O.SRC_VISCOSITY
+    ref = du([F, T, R])
+    scale = np.abs(ref).max()
+    for order in ([F, R, T], [T, F, R], [T, R, F], [R, F, T], [R, T, F]):
+        assert np.abs(du(order) - ref).max() < 1e-12 * scale
+    base = du([F], fly=False, tom=False, rv=False)
+    parts = (du([F], tom=False, rv=False) - base) + (du([T], fly=False, rv=False) - base) + \
+        (du([R], fly=False, tom=False) - base)
+    assert np.abs(base + parts - ref).max() < 1e-12 * scale
+    # and the Tominec part is not negligible here
+    assert np.abs(du([T], fly=False, rv=False) - base).max() > 1e-6 * scale

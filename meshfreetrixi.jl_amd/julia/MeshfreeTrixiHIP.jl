@@ -33,7 +33,8 @@ const MF_OK = Cint(0)
 const MF_EQ_ADVECTION_2D, MF_EQ_EULER_2D = Cint(0), Cint(1)
 const MF_BC_DIRICHLET, MF_BC_SLIP_WALL, MF_BC_DO_NOTHING = Cint(0), Cint(1), Cint(2)
 const MF_FN_TABLE = Cint(4)
-const MF_OP_FLUX, MF_OP_HYPERVISCOSITY = Cint(0), Cint(1)
+const MF_OP_FLUX, MF_OP_HYPERVISCOSITY, MF_OP_LAPLACIAN = Cint(0), Cint(1), Cint(2)
+const MF_SRC_FLYER, MF_SRC_TOMINEC, MF_SRC_VISCOSITY = Int32(1), Int32(2), Int32(3)
 const MF_VISC_NONE, MF_VISC_RESIDUAL, MF_VISC_UPWIND = Cint(0), Cint(1), Cint(2)
 const MF_PREC_F64, MF_PREC_SPLIT32 = Cint(0), Cint(1)
 
@@ -122,8 +123,19 @@ function install!(e::RBFFDEngineHIP, domain, equations, boundary_conditions, sou
                    (Ptr{Cvoid}, Cint, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Cdouble}, Ptr{Int32}, Ptr{Cdouble}),
                    e.ctx, length(kinds), tag_ptr, kinds, fns, params, nodes, normals))
     hv_first = false
+    order = Int32[]  # the SourceTerms keyword order (generic_sources.jl:7-18)
     for s in values(source_terms)
-        if s isa SourceHyperviscosityFlyer
+        if s isa SourceHyperviscosityTominec
+            # lap = sum(compute_flux_operator(solver, domain, 2)) (hyperviscosity.jl:111-113)
+            basis = solver.basis
+            check(e, ccall((:mf_compute_operator, libmfrbf), Cint,
+                           (Ptr{Cvoid}, Cint, Cint, Cint, Cint, Ptr{Cdouble}, Ptr{Cdouble}),
+                           e.ctx, basis.approximation_type.Nrbf, basis.N, 2, MF_OP_LAPLACIAN,
+                           C_NULL, C_NULL))
+            check(e, ccall((:mf_set_source_hyperviscosity_tominec, libmfrbf), Cint, (Ptr{Cvoid}, Cdouble),
+                           e.ctx, s.cache.gamma))
+            push!(order, MF_SRC_TOMINEC)
+        elseif s isa SourceHyperviscosityFlyer
             basis = solver.basis
             check(e, ccall((:mf_compute_operator, libmfrbf), Cint,
                            (Ptr{Cvoid}, Cint, Cint, Cint, Cint, Ptr{Cdouble}, Ptr{Cdouble}),
@@ -132,14 +144,18 @@ function install!(e::RBFFDEngineHIP, domain, equations, boundary_conditions, sou
             check(e, ccall((:mf_set_source_hyperviscosity, libmfrbf), Cint, (Ptr{Cvoid}, Cdouble),
                            e.ctx, s.cache.gamma))
             hv_first = true
+            push!(order, MF_SRC_FLYER)
         elseif s isa SourceResidualViscosityTominec || s isa SourceUpwindViscosityTominec
             mode = s isa SourceResidualViscosityTominec ? MF_VISC_RESIDUAL : MF_VISC_UPWIND
             P = length(s.cache.time_history) - 1
             check(e, ccall((:mf_set_source_viscosity, libmfrbf), Cint,
                            (Ptr{Cvoid}, Cint, Cdouble, Cdouble, Cdouble, Cint, Cint),
                            e.ctx, mode, s.cache.c_rv, s.cache.c_uw, domain.pd.dx_avg, P, hv_first))
+            push!(order, MF_SRC_VISCOSITY)
         end
     end
+    check(e, ccall((:mf_set_source_order, libmfrbf), Cint, (Ptr{Cvoid}, Ptr{Int32}, Cint),
+                   e.ctx, order, length(order)))
 end
 
 # --- rhs! (replaces pointcloudsolver/rbfsolver.jl:397-428)

@@ -324,12 +324,69 @@ __device__ __forceinline__ void flux_record(const EqParams& e, const double* r, 
     }
 }
 
+// LDS records are read by the 64 lanes of a wavefront at scattered record
+// indices, one 16-byte chunk per lane per instruction.  With a plain stride
+// of NC 16-byte chunks per record (NC = 2 or 4) those reads reach only a few
+// LDS banks; a record's chunks are therefore stored in an order rotated by
+// bits 2-3 of its index, which spreads one instruction's lanes over all the
+// banks.  Layout only: every value and operation order is unchanged.
+constexpr bool kLdsSwizzle = true;
+
+template <int NC>
+__device__ __forceinline__ int64_t lds_chunk(int64_t q, int c)
+{
+    if constexpr (kLdsSwizzle && NC == 4)
+        return 4 * q + (c ^ (int)((q >> 2) & 3));
+    else if constexpr (kLdsSwizzle && NC == 2)
+        return 2 * q + (c ^ (int)((q >> 2) & 1));
+    else
+        return (int64_t)NC * q + c;
+}
+
+// Unit u (4 values) of LDS record q, records of RU units: FP64 (two chunks
+// per unit) and FP32 (one chunk per unit).
+template <int RU>
+__device__ __forceinline__ void lds_store_unit(double* lds, int64_t q, int u, const double* s)
+{
+    double2* p = reinterpret_cast<double2*>(lds);
+    p[lds_chunk<2 * RU>(q, 2 * u)] = make_double2(s[0], s[1]);
+    p[lds_chunk<2 * RU>(q, 2 * u + 1)] = make_double2(s[2], s[3]);
+}
+
+template <int RU>
+__device__ __forceinline__ void lds_load_unit(const double* lds, int64_t q, int u, double* s)
+{
+    const double2* p = reinterpret_cast<const double2*>(lds);
+    const double2 a = p[lds_chunk<2 * RU>(q, 2 * u)], b = p[lds_chunk<2 * RU>(q, 2 * u + 1)];
+    s[0] = a.x;
+    s[1] = a.y;
+    s[2] = b.x;
+    s[3] = b.y;
+}
+
+template <int RU>
+__device__ __forceinline__ void lds_store_unit32(float* lds, int64_t q, int u, const double* s)
+{
+    reinterpret_cast<float4*>(lds)[lds_chunk<RU>(q, u)] = make_float4((float)s[0], (float)s[1], (float)s[2],
+                                                                      (float)s[3]);
+}
+
+template <int RU>
+__device__ __forceinline__ void lds_load_unit32(const float* lds, int64_t q, int u, double* s)
+{
+    const float4 a = reinterpret_cast<const float4*>(lds)[lds_chunk<RU>(q, u)];
+    s[0] = a.x;
+    s[1] = a.y;
+    s[2] = a.z;
+    s[3] = a.w;
+}
+
 template <int V>
 __device__ __forceinline__ void load_record(const double* lds, int64_t l, double* r)
 {
     if constexpr (V == 4) {
-        load_state<4>(lds, 2 * l, r);
-        load_state<4>(lds, 2 * l + 1, r + 4);
+        lds_load_unit<2>(lds, l, 0, r);
+        lds_load_unit<2>(lds, l, 1, r + 4);
     } else {
         load_state<V>(lds, l, r);
     }
@@ -373,8 +430,8 @@ __device__ __forceinline__ int64_t stage_halo_a(const RhsArgs& a, double* lds)
             load_state<V>(a.u, a.ha_ids[h0 + q], s);
             make_record<V>(a.eq, s, r);
             if constexpr (V == 4) {
-                store_state<4>(lds, 2 * q, r);
-                store_state<4>(lds, 2 * q + 1, r + 4);
+                lds_store_unit<2>(lds, q, 0, r);
+                lds_store_unit<2>(lds, q, 1, r + 4);
             } else {
                 store_state<V>(lds, q, r);
             }
@@ -803,25 +860,25 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
                 double g[V];
                 if constexpr (HG) {
                     load_state32<V>(a.G32, 2 * j, g);
-                    store_state32<V>(ldsf, RU * q, g);
+                    lds_store_unit32<RU>(ldsf, q, 0, g);
                     load_state32<V>(a.G32, 2 * j + 1, g);
-                    store_state32<V>(ldsf, RU * q + 1, g);
+                    lds_store_unit32<RU>(ldsf, q, 1, g);
                 }
                 if constexpr (HL) {
                     load_state32<V>(a.L32, j, g);
-                    store_state32<V>(ldsf, RU * q + LU, g);
+                    lds_store_unit32<RU>(ldsf, q, LU, g);
                 }
             } else {
                 double g[V];
                 if constexpr (HG) {
                     load_state<V>(a.G, 2 * j, g);
-                    store_state<V>(lds, RU * q, g);
+                    lds_store_unit<RU>(lds, q, 0, g);
                     load_state<V>(a.G, 2 * j + 1, g);
-                    store_state<V>(lds, RU * q + 1, g);
+                    lds_store_unit<RU>(lds, q, 1, g);
                 }
                 if constexpr (HL) {
                     load_state<V>(a.L, j, g);
-                    store_state<V>(lds, RU * q + LU, g);
+                    lds_store_unit<RU>(lds, q, LU, g);
                 }
             }
         }
@@ -880,25 +937,35 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
                 const float* ldsf = reinterpret_cast<const float*>(lds);
                 if constexpr (HG) {
                     if constexpr (TL) {
-                        load_state32<V>(ldsf, RU * rb, g0[b]);
-                        load_state32<V>(ldsf, RU * rb + 1, g1[b]);
+                        lds_load_unit32<RU>(ldsf, rb, 0, g0[b]);
+                        lds_load_unit32<RU>(ldsf, rb, 1, g1[b]);
                     } else {
                         load_state32<V>(a.G32, 2 * rb, g0[b]);
                         load_state32<V>(a.G32, 2 * rb + 1, g1[b]);
                     }
                 }
-                if constexpr (HL) load_state32<V>(TL ? ldsf : a.L32, TL ? RU * rb + LU : rb, lv[b]);
+                if constexpr (HL) {
+                    if constexpr (TL)
+                        lds_load_unit32<RU>(ldsf, rb, LU, lv[b]);
+                    else
+                        load_state32<V>(a.L32, rb, lv[b]);
+                }
             } else {
                 if constexpr (HG) {
                     if constexpr (TL) {
-                        load_state<V>(lds, RU * rb, g0[b]);
-                        load_state<V>(lds, RU * rb + 1, g1[b]);
+                        lds_load_unit<RU>(lds, rb, 0, g0[b]);
+                        lds_load_unit<RU>(lds, rb, 1, g1[b]);
                     } else {
                         load_state<V>(a.G, 2 * rb, g0[b]);
                         load_state<V>(a.G, 2 * rb + 1, g1[b]);
                     }
                 }
-                if constexpr (HL) load_state<V>(TL ? lds : a.L, TL ? RU * rb + LU : rb, lv[b]);
+                if constexpr (HL) {
+                    if constexpr (TL)
+                        lds_load_unit<RU>(lds, rb, LU, lv[b]);
+                    else
+                        load_state<V>(a.L, rb, lv[b]);
+                }
             }
         }
 #pragma unroll

@@ -35,7 +35,31 @@ struct VjpArgs {
     const double* add[3];  // out = P^T (sum_q add_scale[q] add[q] + J^T ...)
     double add_scale[3];
     double* out;
+    // LDS tiles of the transposed operator (the layout of the rhs! transposed pass)
+    int64_t n_tiles;
+    const int64_t* hb_off;
+    const int32_t* hb_ids;
+    const int64_t* sb_off;
+    const int32_t* sb_w;
+    const uint8_t* rowb;
+    const uint16_t* lb;
+    const double2* twb;
 };
+
+// M^T lam at node r (the second boundary pass is its own transpose):
+// Dirichlet rows vanish, slip nodes drop the momentum.
+template <int V>
+__device__ __forceinline__ void mask_lam(const VjpArgs& a, int64_t r, double* l)
+{
+    if (!a.flag) return;
+    const uint8_t f = a.flag[r];
+    if (f & 1u)
+#pragma unroll
+        for (int v = 0; v < V; ++v) l[v] = 0.0;
+    if constexpr (V == 4) {
+        if (f & 2u) l[1] = l[2] = 0.0;
+    }
+}
 
 // J_d(u)^T g for the Euler fluxes (d = 0: x, 1: y), conservative variables.
 __device__ __forceinline__ void euler_jt(double gamma, const double* u, const double* gx,
@@ -61,39 +85,11 @@ __device__ __forceinline__ void euler_jt(double gamma, const double* u, const do
     o[3] = a13 * gx[1] + a33 * gx[3] + b23 * gy[2] + b33 * gy[3];
 }
 
+// Epilogue of one node's product: scale, flux Jacobian transpose at the
+// (boundary-imposed) state, the added terms, then P^T.
 template <int V>
-__global__ __launch_bounds__(kBlock) void k_vjp(const VjpArgs a)
+__device__ __forceinline__ void vjp_finish(const VjpArgs& a, int64_t i, double* gx, double* gy)
 {
-    const int64_t i = remapped_node();
-    if (i >= a.N) return;
-    const int64_t slice = i >> 6;
-    const int lane = (int)(i & 63);
-    const int64_t off = a.sell_off[slice];
-    const int W = a.sell_w[slice];
-    double gx[V], gy[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) gx[v] = gy[v] = 0.0;
-    for (int e = 0; e < W; ++e) {
-        const int64_t q = off + (int64_t)e * kSellC + lane;
-        const int32_t r = a.trow[q];
-        const double2 w = a.tw[q];
-        double l[V];
-        load_state<V>(a.lam, r, l);
-        if (a.flag) {
-            const uint8_t f = a.flag[r];
-            if (f & 1u)
-#pragma unroll
-                for (int v = 0; v < V; ++v) l[v] = 0.0;
-            if constexpr (V == 4) {
-                if (f & 2u) l[1] = l[2] = 0.0;
-            }
-        }
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-            gx[v] = fma(w.x, l[v], gx[v]);
-            gy[v] = fma(w.y, l[v], gy[v]);
-        }
-    }
 #pragma unroll
     for (int v = 0; v < V; ++v) {
         gx[v] *= a.lam_scale;
@@ -137,6 +133,99 @@ __global__ __launch_bounds__(kBlock) void k_vjp(const VjpArgs a)
     store_state<V>(a.out, i, o);
 }
 
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_vjp(const VjpArgs a)
+{
+    const int64_t i = remapped_node();
+    if (i >= a.N) return;
+    const int64_t slice = i >> 6;
+    const int lane = (int)(i & 63);
+    const int64_t off = a.sell_off[slice];
+    const int W = a.sell_w[slice];
+    double gx[V], gy[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) gx[v] = gy[v] = 0.0;
+    for (int e = 0; e < W; ++e) {
+        const int64_t q = off + (int64_t)e * kSellC + lane;
+        const int32_t r = a.trow[q];
+        const double2 w = a.tw[q];
+        double l[V];
+        load_state<V>(a.lam, r, l);
+        mask_lam<V>(a, r, l);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            gx[v] = fma(w.x, l[v], gx[v]);
+            gy[v] = fma(w.y, l[v], gy[v]);
+        }
+    }
+    vjp_finish<V>(a, i, gx, gy);
+}
+
+// The same product over the LDS tiles of the transposed operator: the
+// tile's halo of M^T lam is staged once, the rows walk their entries in
+// batches of BT (every load of a batch issued first), in the same order as
+// k_vjp, so both give the same bits.
+template <int V, int BT>
+__global__ __launch_bounds__(kBlock) void k_vjp_tl(const VjpArgs a)
+{
+    extern __shared__ double lds[];
+    const int64_t t = remapped_block();
+    if (t >= a.n_tiles) return;  // whole block
+    const int64_t h0 = a.hb_off[t];
+    const int nh = (int)(a.hb_off[t + 1] - h0);
+    for (int q = threadIdx.x; q < nh; q += kBlock) {
+        const int64_t r = a.hb_ids[h0 + q];
+        double l[V];
+        load_state<V>(a.lam, r, l);
+        mask_lam<V>(a, r, l);
+        if constexpr (V == 4)
+            lds_store_unit<1>(lds, q, 0, l);
+        else
+            store_state<V>(lds, q, l);
+    }
+    __syncthreads();
+    const int pos = threadIdx.x;
+    const int64_t i = t * kBlock + a.rowb[t * kBlock + pos];
+    if (i >= a.N) return;
+    const int lane = pos & 63;
+    const int64_t sl = t * (kBlock / kSellC) + (pos >> 6);
+    const int64_t off = a.sb_off[sl];
+    const int W = a.sb_w[sl];
+    double gx[V], gy[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) gx[v] = gy[v] = 0.0;
+#pragma unroll 1
+    for (int e0 = 0; e0 < W; e0 += BT) {
+        int32_t r[BT];
+        double2 w[BT];
+#pragma unroll
+        for (int b = 0; b < BT; ++b) {
+            const int e = (e0 + b < W) ? e0 + b : e0;
+            const int64_t q = off + (int64_t)e * kSellC + lane;
+            r[b] = a.lb[q];
+            w[b] = a.twb[q];
+        }
+        double l[BT][V];
+#pragma unroll
+        for (int b = 0; b < BT; ++b) {
+            if constexpr (V == 4)
+                lds_load_unit<1>(lds, r[b], 0, l[b]);
+            else
+                load_state<V>(lds, r[b], l[b]);
+        }
+#pragma unroll
+        for (int b = 0; b < BT; ++b) {
+            if (e0 + b >= W) continue;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                gx[v] = fma(w[b].x, l[b][v], gx[v]);
+                gy[v] = fma(w[b].y, l[b][v], gy[v]);
+            }
+        }
+    }
+    vjp_finish<V>(a, i, gx, gy);
+}
+
 static VjpArgs vjp_args(mf_ctx* c)
 {
     VjpArgs a{};
@@ -152,16 +241,32 @@ static VjpArgs vjp_args(mf_ctx* c)
     a.entry = c->have_bc ? c->bc.entry : nullptr;
     a.bnormal = c->have_bc ? c->bc.bnormal : nullptr;
     a.lam_scale = 1.0;
+    a.n_tiles = c->n_tiles;
+    a.hb_off = c->d_hb_off;
+    a.hb_ids = c->d_hb_ids;
+    a.sb_off = c->d_sb_off;
+    a.sb_w = c->d_sb_w;
+    a.rowb = c->d_rowb;
+    a.lb = c->d_lb;
+    a.twb = c->d_twb;
     return a;
 }
 
 static int launch_vjp(mf_ctx* c, const VjpArgs& a)
 {
     const unsigned grid = xcd_grid((c->N + kBlock - 1) / kBlock);
-    if (c->nvar == 4)
+    if (c->tiles_b) {
+        // halo records of nvar doubles (the tiles were sized for at least that)
+        const size_t lds = (size_t)c->hb_max * c->nvar * sizeof(double);
+        if (c->nvar == 4)
+            hipLaunchKernelGGL((k_vjp_tl<4, 4>), dim3(grid), dim3(kBlock), lds, c->stream, a);
+        else
+            hipLaunchKernelGGL((k_vjp_tl<1, 4>), dim3(grid), dim3(kBlock), lds, c->stream, a);
+    } else if (c->nvar == 4) {
         hipLaunchKernelGGL(k_vjp<4>, dim3(grid), dim3(kBlock), 0, c->stream, a);
-    else
+    } else {
         hipLaunchKernelGGL(k_vjp<1>, dim3(grid), dim3(kBlock), 0, c->stream, a);
+    }
     MF_HIP(c, hipGetLastError());
     return MF_OK;
 }
@@ -204,6 +309,18 @@ int ensure_transpose_any(mf_ctx* c)
     return launch_build_transpose(c);
 }
 
+// State, transposed operator and, where they fit, the LDS tiles of both
+// phases (the transposed tiles carry nvar doubles of lam per halo node).
+static int prepare_vjp(mf_ctx* c)
+{
+    int rc;
+    if ((rc = ensure_state(c))) return rc;
+    if ((rc = ensure_transpose_any(c))) return rc;
+    if (c->use_tiles && c->have_flux && (!c->tried_a || (c->tiles_a && !c->tried_b)))
+        if ((rc = build_tiles(c, true, c->nvar))) return rc;
+    return MF_OK;
+}
+
 extern "C" {
 
 int mf_rhs_vjp(mf_ctx* c, double t)
@@ -214,8 +331,7 @@ int mf_rhs_vjp(mf_ctx* c, double t)
     int rc;
     if (c->visc_mode != MF_VISC_NONE || c->hv_gamma != 0.0 || c->tom_gamma != 0.0)
         return fail(c, MF_ERR_UNSUPPORTED, "mf_rhs_vjp: sources have no adjoint in this build");
-    if ((rc = ensure_state(c))) return rc;
-    if ((rc = ensure_transpose_any(c))) return rc;
+    if ((rc = prepare_vjp(c))) return rc;
     if ((rc = run_set_time(c, t, -1))) return rc;
     // J is evaluated at P u: boundary pass on a copy of u
     if ((rc = copy_state(c, c->d_S, c->d_buf[MF_BUF_U]))) return rc;
@@ -245,8 +361,7 @@ int mf_ssprk33_vjp(mf_ctx* c, double t0, double dt, int n_steps, int checkpoint_
         return fail(c, MF_ERR_ARGUMENT, "mf_ssprk33_vjp: bad arguments");
     if (c->visc_mode != MF_VISC_NONE || c->hv_gamma != 0.0 || c->tom_gamma != 0.0)
         return fail(c, MF_ERR_UNSUPPORTED, "mf_ssprk33_vjp: sources have no adjoint in this build");
-    if ((rc = ensure_state(c))) return rc;
-    if ((rc = ensure_transpose_any(c))) return rc;
+    if ((rc = prepare_vjp(c))) return rc;
     const int C = std::min(checkpoint_every, n_steps);
     const int nck = (n_steps + C - 1) / C;
     const int64_t NV = c->N * c->nvar;

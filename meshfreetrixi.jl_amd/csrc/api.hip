@@ -144,8 +144,10 @@ static void free_graph(mf_ctx* c)
 {
     if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
     if (c->step_graph) (void)hipGraphDestroy(c->step_graph);
-    c->step_exec = nullptr;
-    c->step_graph = nullptr;
+    if (c->multi_exec) (void)hipGraphExecDestroy(c->multi_exec);
+    if (c->multi_graph) (void)hipGraphDestroy(c->multi_graph);
+    c->step_exec = c->multi_exec = nullptr;
+    c->step_graph = c->multi_graph = nullptr;
     c->graph_prec = -1;
 }
 
@@ -1078,7 +1080,7 @@ int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int 
     const int gflags = flags & ~MF_STEP_NOSYNC;  // what the captured step depends on
     const bool history = (flags & MF_STEP_HISTORY) && c->visc_mode == MF_VISC_RESIDUAL;
     if (resume) {
-        if (!c->fsal_valid || c->graph_dt != dt)
+        if (!c->fsal_valid || c->fsal_dt != dt)
             return fail(c, MF_ERR_STATE, "mf_ssprk33: resume needs the previous call's state and the same dt");
     } else {
         if ((rc = run_set_time(c, t0, 0))) return rc;
@@ -1090,31 +1092,43 @@ int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int 
         if ((rc = run_rhs(c, precision, c->d_buf[MF_BUF_U], c->d_buf[MF_BUF_DU], 0.0, s1))) return rc;
         c->host_success_iter = 0;
     }
-    c->graph_dt = dt;
     if (flags & MF_STEP_GRAPH) {
+        // every step reads its time from device memory, so a graph of
+        // several consecutive steps replays like the same steps one by one
+        auto capture = [&](int steps, hipGraph_t* graph, hipGraphExec_t* exec) -> int {
+            MF_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int r = MF_OK;
+            for (int s = 0; s < steps && r == MF_OK; ++s) r = ssprk_step(c, dt, precision, limiter, history);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (r) {
+                if (g) (void)hipGraphDestroy(g);
+                return r;
+            }
+            if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+            *graph = g;
+            MF_HIP(c, hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+            return MF_OK;
+        };
         if (!c->step_exec || c->graph_prec != precision || c->graph_flags != gflags || c->graph_dt != dt) {
             free_graph(c);
             MF_HIP(c, hipStreamSynchronize(c->stream));
-            MF_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            rc = ssprk_step(c, dt, precision, limiter, history);
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamEndCapture(c->stream, &g);
-            if (rc) {
-                if (g) (void)hipGraphDestroy(g);
-                return rc;
-            }
-            if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
-            c->step_graph = g;
-            MF_HIP(c, hipGraphInstantiate(&c->step_exec, g, nullptr, nullptr, 0));
+            if ((rc = capture(1, &c->step_graph, &c->step_exec))) return rc;
             c->graph_prec = precision;
             c->graph_flags = gflags;
             c->graph_dt = dt;
         }
-        for (int n = 0; n < n_steps; ++n) MF_HIP(c, hipGraphLaunch(c->step_exec, c->stream));
+        if (n_steps >= kGraphSteps && !c->multi_exec &&
+            (rc = capture(kGraphSteps, &c->multi_graph, &c->multi_exec)))
+            return rc;
+        int n = 0;
+        for (; n + kGraphSteps <= n_steps; n += kGraphSteps) MF_HIP(c, hipGraphLaunch(c->multi_exec, c->stream));
+        for (; n < n_steps; ++n) MF_HIP(c, hipGraphLaunch(c->step_exec, c->stream));
     } else {
         for (int n = 0; n < n_steps; ++n)
             if ((rc = ssprk_step(c, dt, precision, limiter, history))) return rc;
     }
+    c->fsal_dt = dt;
     c->host_success_iter += n_steps;
     c->fsal_valid = true;
     if (!(flags & MF_STEP_NOSYNC)) MF_HIP(c, hipStreamSynchronize(c->stream));

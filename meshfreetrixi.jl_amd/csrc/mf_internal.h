@@ -154,9 +154,13 @@ struct mf_ctx {
     double* d_lim = nullptr;  // 2 nvar N scratch
 
     // graph of one SSPRK33 step
+    // graphs of one step and of kGraphSteps consecutive steps (fewer launches)
     hipGraphExec_t step_exec = nullptr;
     hipGraph_t step_graph = nullptr;
-    double graph_dt = 0.0;
+    hipGraphExec_t multi_exec = nullptr;
+    hipGraph_t multi_graph = nullptr;
+    double graph_dt = 0.0;  // dt the graphs were captured with
+    double fsal_dt = 0.0;   // dt of the last mf_ssprk33 call (its FSAL stage)
     int graph_prec = -1, graph_flags = -1;
 
     // ragged weights batch
@@ -216,6 +220,8 @@ struct mf_ctx {
 };
 
 namespace mf {
+
+constexpr int kGraphSteps = 16;  // SSPRK33 steps per multi-step graph
 
 // error helpers
 int fail(mf_ctx* c, int code, const std::string& msg);

@@ -268,12 +268,6 @@ __device__ __forceinline__ void finish_norms(const RhsArgs& a, double* pm, doubl
         for (int v = 0; v < 4; ++v) a.norms_out[4 + v] = ninf[v];
 }
 
-// XCD-aware logical block (the remap of remapped_node).
-__device__ __forceinline__ int64_t remapped_block()
-{
-    const unsigned b = blockIdx.x, per = gridDim.x >> 3;
-    return (int64_t)((b & 7u) * per + (b >> 3));
-}
 
 // Halo record of a node in LDS: for the Euler equations (rho, m1, m2, E,
 // v1, v2, p, 0) with the velocity and pressure evaluated exactly as
@@ -322,63 +316,6 @@ __device__ __forceinline__ void flux_record(const EqParams& e, const double* r, 
     } else {
         flux_xy<V>(e, r, f1, f2);
     }
-}
-
-// LDS records are read by the 64 lanes of a wavefront at scattered record
-// indices, one 16-byte chunk per lane per instruction.  With a plain stride
-// of NC 16-byte chunks per record (NC = 2 or 4) those reads reach only a few
-// LDS banks; a record's chunks are therefore stored in an order rotated by
-// bits 2-3 of its index, which spreads one instruction's lanes over all the
-// banks.  Layout only: every value and operation order is unchanged.
-constexpr bool kLdsSwizzle = true;
-
-template <int NC>
-__device__ __forceinline__ int64_t lds_chunk(int64_t q, int c)
-{
-    if constexpr (kLdsSwizzle && NC == 4)
-        return 4 * q + (c ^ (int)((q >> 2) & 3));
-    else if constexpr (kLdsSwizzle && NC == 2)
-        return 2 * q + (c ^ (int)((q >> 2) & 1));
-    else
-        return (int64_t)NC * q + c;
-}
-
-// Unit u (4 values) of LDS record q, records of RU units: FP64 (two chunks
-// per unit) and FP32 (one chunk per unit).
-template <int RU>
-__device__ __forceinline__ void lds_store_unit(double* lds, int64_t q, int u, const double* s)
-{
-    double2* p = reinterpret_cast<double2*>(lds);
-    p[lds_chunk<2 * RU>(q, 2 * u)] = make_double2(s[0], s[1]);
-    p[lds_chunk<2 * RU>(q, 2 * u + 1)] = make_double2(s[2], s[3]);
-}
-
-template <int RU>
-__device__ __forceinline__ void lds_load_unit(const double* lds, int64_t q, int u, double* s)
-{
-    const double2* p = reinterpret_cast<const double2*>(lds);
-    const double2 a = p[lds_chunk<2 * RU>(q, 2 * u)], b = p[lds_chunk<2 * RU>(q, 2 * u + 1)];
-    s[0] = a.x;
-    s[1] = a.y;
-    s[2] = b.x;
-    s[3] = b.y;
-}
-
-template <int RU>
-__device__ __forceinline__ void lds_store_unit32(float* lds, int64_t q, int u, const double* s)
-{
-    reinterpret_cast<float4*>(lds)[lds_chunk<RU>(q, u)] = make_float4((float)s[0], (float)s[1], (float)s[2],
-                                                                      (float)s[3]);
-}
-
-template <int RU>
-__device__ __forceinline__ void lds_load_unit32(const float* lds, int64_t q, int u, double* s)
-{
-    const float4 a = reinterpret_cast<const float4*>(lds)[lds_chunk<RU>(q, u)];
-    s[0] = a.x;
-    s[1] = a.y;
-    s[2] = a.z;
-    s[3] = a.w;
 }
 
 template <int V>

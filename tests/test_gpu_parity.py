@@ -199,6 +199,29 @@ def test_rhs_vjp_vs_oracle(mf):
     assert rel_inf(ub, ubo) < 1e-12
 
 
+def test_vjp_tiles_change_no_result(mf):
+    """The tiled VJP (lam halo staged in LDS) gives the bits of the global
+    gather, for rhs! and through SSPRK33 steps with slip walls."""
+    _say("vjp tiles on/off")
+    c = cases.build(configs.c2(), side=64, bc=("dirichlet", "slip"))
+    lam = np.random.default_rng(2).standard_normal(c.u0.shape)
+    res = []
+    for tiles in (1, 0):
+        c.ctx.set_tuning(L.MF_TUNE_TILES, tiles)
+        c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+        c.ctx.upload(L.MF_BUF_AUX0, lam)
+        c.ctx.rhs_vjp(0.01)
+        a = c.ctx.download(L.MF_BUF_AUX1)
+        if tiles:
+            assert c.ctx.tile_info()["tiles_b"] == 1
+        c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+        loss = c.ctx.ssprk33_vjp(0.0, c.dt, 6, 3)
+        res.append((a, loss, c.ctx.download(L.MF_BUF_AUX1)))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert res[0][1] == res[1][1] and np.array_equal(res[0][2], res[1][2])
+    assert rel_inf(res[0][0], c.prob.vjp(c.u0.copy(), lam, 0.01)) < 1e-12
+
+
 # ------------------------------------------------ reference operation order
 @pytest.mark.parametrize("cfg,bc", [("c1", ("nothing", "nothing")), ("c2", ("nothing", "nothing")),
                                     ("c2", ("dirichlet", "nothing"))])

@@ -177,6 +177,27 @@ def test_graph_replay_matches_eager(mf, prec):
     assert np.array_equal(out[False], out[True])
 
 
+def test_multi_step_graph_and_new_dt(mf):
+    """20 steps as one 16-step graph plus four 1-step graphs give the eager
+    bits; a later call with another dt recaptures instead of replaying the
+    old step."""
+    _say("multi-step graph, then a new dt")
+    c = cases.build(configs.c2(), side=40, hv=(2, 1e-4))
+    out = {}
+    for graph in (False, True):
+        c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+        c.ctx.ssprk33(0.0, c.dt, 20, L.MF_PREC_F64, L.MF_STEP_GRAPH if graph else 0)
+        a = c.ctx.download(L.MF_BUF_U)
+        c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+        c.ctx.ssprk33(0.0, 0.5 * c.dt, 17, L.MF_PREC_F64, L.MF_STEP_GRAPH if graph else 0)
+        out[graph] = (a, c.ctx.download(L.MF_BUF_U))
+    assert np.array_equal(out[False][0], out[True][0])
+    assert np.array_equal(out[False][1], out[True][1])
+    uo = c.u0.copy()
+    c.prob.ssprk33(uo, 0.0, 0.5 * c.dt, 17)
+    assert rel_inf(out[True][1], uo) < 1e-12
+
+
 def test_adjoint_vs_finite_differences_c4(mf):
     """C4: dL/drho0 of L = sum rho(T)^2 through 100 SSPRK33 steps on 65,536
     nodes (checkpoints every 10) against central differences along random

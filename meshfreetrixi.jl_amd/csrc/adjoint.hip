@@ -409,18 +409,27 @@ int mf_ssprk33_vjp(mf_ctx* c, double t0, double dt, int n_steps, int checkpoint_
         s3.out = Y;
         return run_rhs(c, MF_PREC_F64, T, c->d_buf[MF_BUF_DU], dt / 2.0, s3);
     };
-    // forward pass with checkpoints (raw states at steps 0, C, 2C, ...)
+    // tape of segment step n (first step n0): X at 3 (n - n0), S and T after it
+    auto tape_x = [&](int n, int n0) { return tape + (int64_t)(3 * (n - n0)) * NV; };
+    // forward pass with checkpoints (raw states at steps 0, C, 2C, ...); the
+    // last segment keeps its tape, so the reverse sweep does not recompute it
+    const int last0 = (nck - 1) * C;
     if ((rc = copy_state(c, W0, c->d_buf[MF_BUF_U]))) return rc;
-    for (int n = 0; n < n_steps; ++n) {
+    for (int n = 0; n < last0; ++n) {
         if (n % C == 0 && (rc = copy_state(c, buf(n / C), W0))) return rc;
         if ((rc = fwd(n, W0, tape, tape + NV, W1))) return rc;
         std::swap(W0, W1);
     }
-    // loss on the final state after its boundary pass (the last FSAL call)
+    if ((rc = copy_state(c, tape_x(last0, last0), W0))) return rc;
+    for (int n = last0; n < n_steps; ++n) {
+        double* Xn = tape_x(n, last0);
+        if ((rc = fwd(n, Xn, Xn + NV, Xn + 2 * NV, n + 1 < n_steps ? tape_x(n + 1, last0) : W1))) return rc;
+    }
+    // loss on the final state (W1) after its boundary pass (the last FSAL call)
     if ((rc = run_set_time(c, tn[n_steps], -1))) return rc;
-    if ((rc = bc_pass_state(c, W0, 0.0))) return rc;
-    double* d_loss = W1;
-    hipLaunchKernelGGL(k_loss, dim3(1), dim3(1024), 0, c->stream, c->N, c->nvar, W0, Ub, d_loss);
+    if ((rc = bc_pass_state(c, W1, 0.0))) return rc;
+    double* d_loss = W0;
+    hipLaunchKernelGGL(k_loss, dim3(1), dim3(1024), 0, c->stream, c->N, c->nvar, W1, Ub, d_loss);
     MF_HIP(c, hipGetLastError());
     double hl = 0.0;
     MF_HIP(c, hipMemcpyAsync(&hl, d_loss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -428,11 +437,12 @@ int mf_ssprk33_vjp(mf_ctx* c, double t0, double dt, int n_steps, int checkpoint_
     // reverse sweep, one segment at a time
     for (int sg = nck - 1; sg >= 0; --sg) {
         const int n0 = sg * C, n1 = std::min(n_steps, n0 + C);
-        if ((rc = copy_state(c, W0, buf(sg)))) return rc;
-        for (int n = n0; n < n1; ++n) {
-            double* Xn = tape + (int64_t)(3 * (n - n0)) * NV;
-            if ((rc = copy_state(c, Xn, W0))) return rc;
-            if ((rc = fwd(n, Xn, Xn + NV, Xn + 2 * NV, W0))) return rc;
+        if (sg != nck - 1) {  // recompute the segment's tape from its checkpoint
+            if ((rc = copy_state(c, tape_x(n0, n0), buf(sg)))) return rc;
+            for (int n = n0; n < n1; ++n) {
+                double* Xn = tape_x(n, n0);
+                if ((rc = fwd(n, Xn, Xn + NV, Xn + 2 * NV, n + 1 < n1 ? tape_x(n + 1, n0) : W1))) return rc;
+            }
         }
         for (int n = n1 - 1; n >= n0; --n) {
             double* Xn = tape + (int64_t)(3 * (n - n0)) * NV;

@@ -259,9 +259,9 @@ static int launch_vjp(mf_ctx* c, const VjpArgs& a)
         // halo records of nvar doubles (the tiles were sized for at least that)
         const size_t lds = (size_t)c->hb_max * c->nvar * sizeof(double);
         if (c->nvar == 4)
-            hipLaunchKernelGGL((k_vjp_tl<4, 4>), dim3(grid), dim3(kBlock), lds, c->stream, a);
+            hipLaunchKernelGGL((k_vjp_tl<4, 8>), dim3(grid), dim3(kBlock), lds, c->stream, a);
         else
-            hipLaunchKernelGGL((k_vjp_tl<1, 4>), dim3(grid), dim3(kBlock), lds, c->stream, a);
+            hipLaunchKernelGGL((k_vjp_tl<1, 8>), dim3(grid), dim3(kBlock), lds, c->stream, a);
     } else if (c->nvar == 4) {
         hipLaunchKernelGGL(k_vjp<4>, dim3(grid), dim3(kBlock), 0, c->stream, a);
     } else {

@@ -3,8 +3,8 @@
 Python host mirror of the reference's Trixi-style API over libmfrbf.so:
 PointCloudBasis / PointCloudSolver / PointCloudDomain /
 SemidiscretizationHyperbolic / semidiscretize / rhs_ (Trixi.rhs!) /
-solve with HistoryCallback and the Zhang-Shu stage limiter.  All numerical
-work runs in HIP kernels on the GPU.
+solve with HistoryCallback and the Zhang-Shu stage limiter, and trixi2vtk
+output.  All numerical work runs in HIP kernels on the GPU.
 """
 from . import _lib
 from ._lib import Context, MfError
@@ -23,5 +23,6 @@ from .semidiscretization import (CallbackSet, HistoryCallback, ODEProblem, ODESo
 from .solver import AbstractRBFEngine, PointCloudSolver, RBFFDEngineHIP, RBFSolver
 from .sources import (SourceHyperviscosityFlyer, SourceHyperviscosityTominec, SourceIGR,
                       SourceResidualViscosityTominec, SourceTerms, SourceUpwindViscosityTominec)
+from .vtk import trixi2vtk
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -105,6 +105,33 @@ def test_host_api_cyl_slip_walls(mf, visc, host_dirichlet):
     assert np.all(du[:, domain.boundary_tags["inlet"].idx] == 0.0)
 
 
+def test_trixi2vtk_cyl(mf, tmp_path):
+    """trixi2vtk(u, semi, t) after a rhs! with the upwind viscosity: the file
+    holds the state, the primitive variables, eps and its code, the 1-based
+    index and the time; a second call with iter = 1 appends to the .pvd."""
+    _say("trixi2vtk on cyl_0_05")
+    from mfrbf import vtk
+    semi, domain = _cyl_semi("upwind")
+    ode = mfrbf.semidiscretize(semi, (0.0, 0.1))
+    u = ode.u0.copy()
+    du = np.zeros_like(u)
+    mfrbf.rhs_(du, u, semi, 0.0)
+    path = mfrbf.trixi2vtk(u, semi, 0.0, iter=0, output_directory=str(tmp_path), prefix="sol")
+    assert os.path.basename(path) == "sol_CompressibleEulerEquations2D_1_0.vtu"
+    p, pd, fd = vtk.read_vtu(path)
+    assert np.array_equal(p[:, :2], domain.pd.points)
+    assert np.array_equal(pd["density"], u[0]) and np.array_equal(pd["density_energy"], u[3])
+    assert np.array_equal(pd["momentum"], u[1:3].T)
+    assert np.allclose(pd["velocity"], (u[1:3] / u[0]).T, rtol=1e-15)
+    eps, code = semi.ctx.viscosity()
+    assert np.array_equal(pd["eps"], eps) and np.array_equal(pd["eps_scalar"], code)
+    assert np.array_equal(pd["index"], np.arange(1, u.shape[1] + 1))
+    assert float(fd["time"][0]) == 0.0
+    mfrbf.trixi2vtk(u, semi, 0.5, iter=1, output_directory=str(tmp_path), prefix="sol")
+    pvd = open(os.path.join(str(tmp_path), "sol_CompressibleEulerEquations2D_1.pvd")).read()
+    assert pvd.count("<DataSet") == 2 and 'timestep="0.5"' in pvd
+
+
 def test_limiter_vs_oracle(mf):
     """Trixi.limiter_zhang_shu! (positivity_zhang_shu_point2d.jl:22-83) on
     density and then pressure, with nodes pushed below the thresholds."""

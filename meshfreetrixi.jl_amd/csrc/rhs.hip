@@ -678,6 +678,9 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
                 hw32[b] = lw32[b] = 0.0f;
             }
         }
+        // keep the batch's index and weight loads issued together, ahead of
+        // the neighbour reads (the scheduler otherwise sinks each one to its use)
+        __builtin_amdgcn_sched_barrier(0);
         double ub[BT][TL ? rec_len<V>() : V];
 #pragma unroll
         for (int b = 0; b < BT; ++b) load_nb<V, TL>(a, lds, jj[b], ub[b]);

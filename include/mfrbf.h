@@ -67,6 +67,7 @@ typedef struct mf_ctx mf_ctx;
 #define MF_SRC_FLYER 1     /* SourceHyperviscosityFlyer */
 #define MF_SRC_TOMINEC 2   /* SourceHyperviscosityTominec */
 #define MF_SRC_VISCOSITY 3 /* SourceResidual/UpwindViscosityTominec */
+#define MF_SRC_IGR 4       /* SourceIGR */
 
 /* viscosity sources (hyperviscosity.jl) */
 #define MF_VISC_NONE 0
@@ -208,6 +209,20 @@ int mf_set_source_hyperviscosity_tominec(mf_ctx* ctx, double gamma);
  * before the viscosity's gathers).  mf_set_source_viscosity's hv_first
  * sets the order of FLYER and VISCOSITY only. */
 int mf_set_source_order(mf_ctx* ctx, const int32_t* kinds, int n);
+
+/* IGR source (Euler equations): sigma from maxiter conjugate-gradient
+ * iterations, started at 0, on (1/rho - alpha (Dx diag(1/rho) Dx + Dy
+ * diag(1/rho) Dy)) sigma = alpha (trace^2 + trace_sq), then du_m1 -= Dx
+ * sigma, du_m2 -= Dy sigma.  Replaces SourceIGR (sources/IGR.jl:17-239)
+ * with its CG solve (IterativeSolvers cg! semantics, maxiter = 20 in the
+ * reference); trace = D_x rho + D_y v1 as the reference computes it.  It
+ * must come after the other sources in the keyword order
+ * (MF_ERR_UNSUPPORTED otherwise).  The CG runs in FP64 in every precision.
+ * alpha = 0 removes the source. */
+int mf_set_source_igr(mf_ctx* ctx, double alpha, int maxiter);
+/* sigma of the last rhs! with the IGR source (n_points doubles, the
+ * caller's order); replaces SourceIGR's cache.sigma (IGR.jl:97, :344-349). */
+int mf_get_igr_sigma(mf_ctx* ctx, double* sigma_out);
 
 /* --------------------------------------------------------------- state */
 /* Copies nvar host fields into / out of a context buffer (MF_BUF_*). */

@@ -329,7 +329,7 @@ int mf_rhs_vjp(mf_ctx* c, double t)
     c->err.clear();
     MF_HIP(c, hipSetDevice(c->device));
     int rc;
-    if (c->visc_mode != MF_VISC_NONE || c->hv_gamma != 0.0 || c->tom_gamma != 0.0)
+    if (c->visc_mode != MF_VISC_NONE || c->hv_gamma != 0.0 || c->tom_gamma != 0.0 || c->igr_alpha != 0.0)
         return fail(c, MF_ERR_UNSUPPORTED, "mf_rhs_vjp: sources have no adjoint in this build");
     if ((rc = prepare_vjp(c))) return rc;
     if ((rc = run_set_time(c, t, -1))) return rc;
@@ -359,7 +359,7 @@ int mf_ssprk33_vjp(mf_ctx* c, double t0, double dt, int n_steps, int checkpoint_
     int rc;
     if (n_steps < 1 || checkpoint_every < 1 || !(dt > 0.0))
         return fail(c, MF_ERR_ARGUMENT, "mf_ssprk33_vjp: bad arguments");
-    if (c->visc_mode != MF_VISC_NONE || c->hv_gamma != 0.0 || c->tom_gamma != 0.0)
+    if (c->visc_mode != MF_VISC_NONE || c->hv_gamma != 0.0 || c->tom_gamma != 0.0 || c->igr_alpha != 0.0)
         return fail(c, MF_ERR_UNSUPPORTED, "mf_ssprk33_vjp: sources have no adjoint in this build");
     if ((rc = prepare_vjp(c))) return rc;
     const int C = std::min(checkpoint_every, n_steps);

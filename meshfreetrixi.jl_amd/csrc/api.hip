@@ -44,6 +44,7 @@ template int dalloc<int16_t>(mf_ctx*, int16_t**, size_t);
 template int dalloc<int8_t>(mf_ctx*, int8_t**, size_t);
 template int dalloc<uint8_t>(mf_ctx*, uint8_t**, size_t);
 template int dalloc<uint16_t>(mf_ctx*, uint16_t**, size_t);
+template int dalloc<IgrScalars>(mf_ctx*, IgrScalars**, size_t);
 
 void dfree(void* p)
 {
@@ -164,6 +165,12 @@ static void free_state(mf_ctx* c)
     c->d_du32 = nullptr;
     dfree(c->d_L);
     c->d_L = nullptr;
+    dfree(c->d_igr);
+    dfree(c->d_igr_part);
+    dfree(c->d_igr_sc);
+    c->d_igr = nullptr;
+    c->d_igr_part = nullptr;
+    c->d_igr_sc = nullptr;
     dfree(c->d_eps);
     dfree(c->d_epsc);
     dfree(c->d_approx);
@@ -814,21 +821,49 @@ int mf_set_source_hyperviscosity_tominec(mf_ctx* c, double gamma)
     return MF_OK;
 }
 
+int mf_set_source_igr(mf_ctx* c, double alpha, int maxiter)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (maxiter < 0 || maxiter > 10000) return fail(c, MF_ERR_ARGUMENT, "mf_set_source_igr: maxiter out of range");
+    if (alpha != 0.0 && c->eq != MF_EQ_EULER_2D)
+        return fail(c, MF_ERR_UNSUPPORTED, "the IGR source needs the Euler equations");
+    free_graph(c);
+    c->igr_alpha = alpha;
+    c->igr_maxiter = maxiter;
+    return MF_OK;
+}
+
+int mf_get_igr_sigma(mf_ctx* c, double* sigma)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!sigma) return fail(c, MF_ERR_ARGUMENT, "mf_get_igr_sigma: null output");
+    if (!c->d_igr) return fail(c, MF_ERR_STATE, "mf_get_igr_sigma: no rhs! with the IGR source yet");
+    double* tmp = nullptr;
+    if ((rc = dalloc(c, &tmp, c->N))) return rc;
+    if ((rc = launch_gather_nodes_d(c, c->d_igr + 2 * c->N, tmp))) return rc;
+    MF_HIP(c, hipMemcpyAsync(sigma, tmp, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(tmp);
+    return MF_OK;
+}
+
 int mf_set_source_order(mf_ctx* c, const int32_t* kinds, int n)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
-    if (n < 0 || n > 3 || (n > 0 && !kinds)) return fail(c, MF_ERR_ARGUMENT, "mf_set_source_order: bad list");
-    bool seen[4] = {false, false, false, false};
+    if (n < 0 || n > 4 || (n > 0 && !kinds)) return fail(c, MF_ERR_ARGUMENT, "mf_set_source_order: bad list");
+    bool seen[5] = {false, false, false, false, false};
     for (int q = 0; q < n; ++q) {
-        if (kinds[q] < MF_SRC_FLYER || kinds[q] > MF_SRC_VISCOSITY || seen[kinds[q]])
+        if (kinds[q] < MF_SRC_FLYER || kinds[q] > MF_SRC_IGR || seen[kinds[q]])
             return fail(c, MF_ERR_ARGUMENT, "mf_set_source_order: kinds must be distinct MF_SRC_* values");
         seen[kinds[q]] = true;
     }
     free_graph(c);
     int r = 0;
     for (int q = 0; q < n; ++q) c->src_rank[kinds[q]] = r++;
-    for (int k = MF_SRC_FLYER; k <= MF_SRC_VISCOSITY; ++k)
+    for (int k = MF_SRC_FLYER; k <= MF_SRC_IGR; ++k)
         if (!seen[k]) c->src_rank[k] = r++;
     c->hv_first = c->src_rank[MF_SRC_FLYER] < c->src_rank[MF_SRC_VISCOSITY] ? 1 : 0;
     return MF_OK;
@@ -910,6 +945,13 @@ static int prepare_rhs(mf_ctx* c)
             MF_HIP(c, hipStreamSynchronize(c->stream));
             c->lap_T = true;
         }
+    }
+    if (c->igr_alpha != 0.0) {
+        const size_t nblk = (size_t)((c->N + 255) / 256);
+        if ((!c->d_igr && (rc = dalloc(c, &c->d_igr, 8 * (size_t)c->N))) ||
+            (!c->d_igr_part && (rc = dalloc(c, &c->d_igr_part, nblk))) ||
+            (!c->d_igr_sc && (rc = dalloc(c, &c->d_igr_sc, 1))))
+            return rc;
     }
     const bool want_b = (rv || tom) && c->nvar == 4;
     // phase-B halo record: G (2 nvar doubles) and / or lap u (nvar)

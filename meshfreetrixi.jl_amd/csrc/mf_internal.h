@@ -32,6 +32,12 @@ struct StepScalars {
     int pad[2];
 };
 
+// Conjugate-gradient scalars of the IGR source, in device memory.
+struct IgrScalars {
+    double res, prev, tol, acg;
+    int iter, done;
+};
+
 // Term table of the k-th derivative of r^p along one coordinate s:
 // sum_t c_t s^a_t r^b_t.
 struct PhsTable {
@@ -126,9 +132,11 @@ struct mf_ctx {
     int approx_order = 0;
     int hv_first = 1;
     double tom_gamma = 0.0;
+    double igr_alpha = 0.0;  // IGR source (0: none)
+    int igr_maxiter = 20;
     // rank of each source kind (MF_SRC_FLYER, _TOMINEC, _VISCOSITY) in the
     // SourceTerms keyword order; smaller adds to du first
-    int src_rank[4] = {0, 0, 1, 2};
+    int src_rank[5] = {0, 0, 1, 2, 3};
 
     // state
     double* d_buf[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -137,6 +145,9 @@ struct mf_ctx {
     double* d_G = nullptr;
     float* d_du32 = nullptr;  // split-FP32 partial du between the two viscosity phases
     double* d_L = nullptr;    // lap u between the phases (Tominec); FP32 in split-FP32
+    double* d_igr = nullptr;  // IGR: 8 fields of N (1/rho, b, sigma, r, p, c, t1, t2)
+    double* d_igr_part = nullptr;          // per-block partial sums
+    mf::IgrScalars* d_igr_sc = nullptr;  // CG scalars on the device
     double* d_eps = nullptr;
     int8_t* d_epsc = nullptr;
     double* d_approx = nullptr;
@@ -278,7 +289,7 @@ struct StageOut {
 };
 int run_rhs(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const StageOut& st);
 // parts of one rhs! (for timing single kernels)
-enum RhsPart { RHS_BC = 1, RHS_REDUCE = 2, RHS_FLUX = 4, RHS_TRANSPOSE = 8, RHS_ALL = 15 };
+enum RhsPart { RHS_BC = 1, RHS_REDUCE = 2, RHS_FLUX = 4, RHS_TRANSPOSE = 8, RHS_IGR = 16, RHS_ALL = 31 };
 int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const StageOut& st,
                   int parts);
 int run_set_time(mf_ctx* c, double t, int success_iter);

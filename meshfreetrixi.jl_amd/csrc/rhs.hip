@@ -1320,6 +1320,234 @@ __global__ void k_fill(double* p, int64_t n, double v)
     if (q < n) p[q] = v;
 }
 
+// ------------------------------------------------------------ IGR source
+// SourceIGR (sources/IGR.jl): sigma from 20 conjugate-gradient iterations on
+//   (rho^-1 - alpha (Dx rho^-1 Dx + Dy rho^-1 Dy)) sigma = b,
+//   b = alpha (trace^2 + trace_sq) from the primitive-variable derivatives,
+// then du_m1 -= Dx sigma, du_m2 -= Dy sigma.  Scalar node fields; every
+// reduction is per block (fixed node ranges) into partials, which one block
+// sums in a fixed order, so every run gives the same bits.
+struct IgrArgs {
+    int64_t N;
+    int k;
+    int64_t nblk;
+    const int32_t* __restrict__ idx;
+    const double2* __restrict__ wxy;
+    double alpha;
+    int maxiter;
+    const double* __restrict__ u;
+    double *rinv, *b, *x, *r, *p, *c, *t1, *t2;
+    double* part;
+    IgrScalars* sc;
+};
+
+// Block-wide sum of one value per thread (every thread gets it).
+__device__ __forceinline__ double block_sum1(double v)
+{
+    double w[4] = {v, 0.0, 0.0, 0.0};
+    block_sum4(w);
+    return w[0];
+}
+
+// Sum over the stencil of node i of (Dx, Dy) times the scalar field f.
+template <int BT>
+__device__ __forceinline__ double2 igr_grad(const IgrArgs& a, int64_t i, const double* __restrict__ f)
+{
+    double sx = 0.0, sy = 0.0;
+#pragma unroll 1
+    for (int s0 = 0; s0 < a.k; s0 += BT) {
+        int32_t j[BT];
+        double2 w[BT];
+#pragma unroll
+        for (int q = 0; q < BT; ++q) {
+            const int s = s0 + q < a.k ? s0 + q : s0;
+            j[q] = a.idx[(int64_t)s * a.N + i];
+            w[q] = a.wxy[(int64_t)s * a.N + i];
+        }
+        double fj[BT];
+#pragma unroll
+        for (int q = 0; q < BT; ++q) fj[q] = f[j[q]];
+#pragma unroll
+        for (int q = 0; q < BT; ++q) {
+            if (s0 + q >= a.k) continue;
+            sx = fma(w[q].x, fj[q], sx);
+            sy = fma(w[q].y, fj[q], sy);
+        }
+    }
+    return make_double2(sx, sy);
+}
+
+// b, 1/rho, sigma = 0, r = b, p = 0 (update_igr_rhs!, IGR.jl:117-157, and
+// update_sigma!, :163-180), and the partial sums of r^2.  The trace is
+// D_x rho + D_y v1 as the reference writes it (flux_x[1] + flux_y[2]).
+__global__ __launch_bounds__(kBlock) void k_igr_init(const IgrArgs a)
+{
+    const int64_t lb = remapped_block();
+    if (lb >= a.nblk) return;  // whole block
+    const int64_t i = lb * kBlock + threadIdx.x;
+    double rr = 0.0;
+    if (i < a.N) {
+        double gxr = 0.0, gx1 = 0.0, gx2 = 0.0, gy1 = 0.0, gy2 = 0.0;
+        for (int s = 0; s < a.k; ++s) {
+            const int64_t q = (int64_t)s * a.N + i;
+            const double2 w = a.wxy[q];
+            double uj[4];
+            load_state<4>(a.u, a.idx[q], uj);
+            const double v1 = uj[1] / uj[0], v2 = uj[2] / uj[0];
+            gxr = fma(w.x, uj[0], gxr);
+            gx1 = fma(w.x, v1, gx1);
+            gx2 = fma(w.x, v2, gx2);
+            gy1 = fma(w.y, v1, gy1);
+            gy2 = fma(w.y, v2, gy2);
+        }
+        const double trace = gxr + gy1;
+        const double tsq = gx1 * gx1 + 2.0 * gy1 * gx2 + gy2 * gy2;
+        const double bi = a.alpha * (trace * trace + tsq);
+        double ui[4];
+        load_state<4>(a.u, i, ui);
+        a.rinv[i] = 1.0 / ui[0];
+        a.b[i] = bi;
+        a.x[i] = 0.0;
+        a.r[i] = bi;
+        a.p[i] = 0.0;
+        rr = bi * bi;
+    }
+    rr = block_sum1(rr);
+    if (threadIdx.x == 0) a.part[lb] = rr;
+}
+
+// The CG scalars from the partials (one block): mode 0 after the start
+// (residual, tolerance sqrt(eps) |r0|), 1 after the dot (alpha), 2 after
+// the norm (next residual, iteration count, convergence).
+__global__ __launch_bounds__(kBlock) void k_igr_scalars(const IgrArgs a, int mode)
+{
+    IgrScalars* sc = a.sc;
+    if (mode != 0 && sc->done) return;
+    double acc = 0.0;
+    for (int64_t q = threadIdx.x; q < a.nblk; q += kBlock) acc += a.part[q];
+    acc = block_sum1(acc);
+    if (threadIdx.x != 0) return;
+    if (mode == 0) {
+        sc->res = sqrt(acc);
+        sc->tol = sqrt(DBL_EPSILON) * sc->res;
+        sc->prev = 1.0;
+        sc->iter = 0;
+        sc->done = (a.maxiter <= 0 || sc->res <= sc->tol) ? 1 : 0;
+    } else if (mode == 1) {
+        sc->acg = sc->res * sc->res / acc;
+    } else {
+        sc->prev = sc->res;
+        sc->res = sqrt(acc);
+        sc->iter += 1;
+        sc->done = (sc->iter >= a.maxiter || sc->res <= sc->tol) ? 1 : 0;
+    }
+}
+
+// p = r + beta p, beta = res^2 / prev^2.
+__global__ __launch_bounds__(kBlock) void k_igr_p(const IgrArgs a)
+{
+    if (a.sc->done) return;
+    const int64_t i = remapped_node();
+    if (i >= a.N) return;
+    const double res = a.sc->res, prev = a.sc->prev;
+    const double beta = res * res / (prev * prev);
+    a.p[i] = a.r[i] + beta * a.p[i];
+}
+
+// t = rho^-1 (Dx p, Dy p).
+template <int BT>
+__global__ __launch_bounds__(kBlock) void k_igr_t(const IgrArgs a)
+{
+    if (a.sc->done) return;
+    const int64_t i = remapped_node();
+    if (i >= a.N) return;
+    const double2 g = igr_grad<BT>(a, i, a.p);
+    const double ri = a.rinv[i];
+    a.t1[i] = ri * g.x;
+    a.t2[i] = ri * g.y;
+}
+
+// c = rho^-1 p - alpha (Dx t1 + Dy t2), and partial sums of p . c.
+template <int BT>
+__global__ __launch_bounds__(kBlock) void k_igr_c(const IgrArgs a)
+{
+    if (a.sc->done) return;
+    const int64_t lb = remapped_block();
+    if (lb >= a.nblk) return;  // whole block
+    const int64_t i = lb * kBlock + threadIdx.x;
+    double pc = 0.0;
+    if (i < a.N) {
+        const double2 g1 = igr_grad<BT>(a, i, a.t1);
+        const double2 g2 = igr_grad<BT>(a, i, a.t2);
+        const double ci = a.rinv[i] * a.p[i] - a.alpha * (g1.x + g2.y);
+        a.c[i] = ci;
+        pc = a.p[i] * ci;
+    }
+    pc = block_sum1(pc);
+    if (threadIdx.x == 0) a.part[lb] = pc;
+}
+
+// sigma += alpha p, r -= alpha c, and partial sums of r^2.
+__global__ __launch_bounds__(kBlock) void k_igr_x(const IgrArgs a)
+{
+    if (a.sc->done) return;
+    const int64_t lb = remapped_block();
+    if (lb >= a.nblk) return;  // whole block
+    const int64_t i = lb * kBlock + threadIdx.x;
+    double rr = 0.0;
+    if (i < a.N) {
+        const double al = a.sc->acg;
+        a.x[i] += al * a.p[i];
+        const double ri = a.r[i] - al * a.c[i];
+        a.r[i] = ri;
+        rr = ri * ri;
+    }
+    rr = block_sum1(rr);
+    if (threadIdx.x == 0) a.part[lb] = rr;
+}
+
+// du_m1 -= Dx sigma, du_m2 -= Dy sigma (the flux_igr terms, IGR.jl:225-237),
+// then the second boundary pass and the output of the rhs! or stage.
+template <int BT>
+__global__ __launch_bounds__(kBlock) void k_igr_apply(const IgrArgs g, const RhsArgs a)
+{
+    const int64_t i = remapped_node();
+    if (i >= g.N) return;
+    const double2 s = igr_grad<BT>(g, i, g.x);
+    double du[4];
+    load_state<4>(a.du, i, du);
+    du[1] -= s.x;
+    du[2] -= s.y;
+    mask_du<4>(a, i, du);
+    emit<4>(a, i, du);
+}
+
+static IgrArgs make_igr(mf_ctx* c, const double* u)
+{
+    IgrArgs g{};
+    g.N = c->N;
+    g.k = c->k;
+    g.nblk = (c->N + kBlock - 1) / kBlock;
+    g.idx = c->d_idx;
+    g.wxy = c->d_wxy;
+    g.alpha = c->igr_alpha;
+    g.maxiter = c->igr_maxiter;
+    g.u = u;
+    double* f = c->d_igr;
+    const int64_t N = c->N;
+    g.rinv = f;
+    g.b = f + N;
+    g.x = f + 2 * N;
+    g.r = f + 3 * N;
+    g.p = f + 4 * N;
+    g.c = f + 5 * N;
+    g.t1 = f + 6 * N;
+    g.t2 = f + 7 * N;
+    g.part = c->d_igr_part;
+    g.sc = c->d_igr_sc;
+    return g;
+}
+
 // ============================================================ launchers
 static RhsArgs make_args(mf_ctx* c, double* u, double* du)
 {
@@ -1603,6 +1831,27 @@ static void launch_transpose(mf_ctx* c, const RhsArgs& a, unsigned grid, int gt)
         launch_transpose_tl<TM, false>(c, a, grid, 0, gt);
 }
 
+// The IGR source on state u: CG for sigma, then the flux terms added to du
+// (written without output by the kernels before) and the output a.
+static int run_igr(mf_ctx* c, const double* u, const RhsArgs& a, unsigned grid)
+{
+    const IgrArgs g = make_igr(c, u);
+    const dim3 G(grid), B(kBlock);
+    launch_k(c, -1, k_igr_init, G, B, 0, g);
+    launch_k(c, -1, k_igr_scalars, dim3(1), B, 0, g, 0);
+    for (int it = 0; it < c->igr_maxiter; ++it) {
+        launch_k(c, -1, k_igr_p, G, B, 0, g);
+        launch_k(c, -1, k_igr_t<5>, G, B, 0, g);
+        launch_k(c, -1, k_igr_c<5>, G, B, 0, g);
+        launch_k(c, -1, k_igr_scalars, dim3(1), B, 0, g, 1);
+        launch_k(c, -1, k_igr_x, G, B, 0, g);
+        launch_k(c, -1, k_igr_scalars, dim3(1), B, 0, g, 2);
+    }
+    launch_k(c, -1, k_igr_apply<5>, G, B, 0, g, a);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
 int run_rhs(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const StageOut& st)
 {
     return run_rhs_parts(c, prec, d_u, d_du, dt_off, st, RHS_ALL);
@@ -1625,17 +1874,28 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
         c->src_rank[MF_SRC_TOMINEC] < c->src_rank[MF_SRC_VISCOSITY])
         return fail(c, MF_ERR_UNSUPPORTED,
                     "a Tominec source before the residual viscosity source (its du would need lap'(lap u) first)");
+    const bool igr = c->igr_alpha != 0.0;
+    if (igr && c->nvar != 4) return fail(c, MF_ERR_UNSUPPORTED, "the IGR source needs the Euler equations");
+    if (igr && !c->d_igr) return fail(c, MF_ERR_STATE, "IGR source: buffers not prepared");
+    if (igr && ((hv && c->src_rank[MF_SRC_IGR] < c->src_rank[MF_SRC_FLYER]) ||
+                (tom && c->src_rank[MF_SRC_IGR] < c->src_rank[MF_SRC_TOMINEC]) ||
+                (rv && c->src_rank[MF_SRC_IGR] < c->src_rank[MF_SRC_VISCOSITY])))
+        return fail(c, MF_ERR_UNSUPPORTED, "the IGR source must come after the other sources");
     const int hvk = (hv ? 1 : 0) | (tom ? 2 : 0);
     // first boundary pass on u
     // with the residual viscosity the first boundary pass rides on the sum pass
     const bool fused_bc = rv && c->visc_mode == MF_VISC_RESIDUAL && c->have_bc && !c->bc.multi;
     int rc = ((parts & RHS_BC) && !fused_bc) ? bc_state_pass(c, d_u, dt_off, 0, false) : MF_OK;
     if (rc) return rc;
-    RhsArgs a = make_args(c, d_u, d_du);
-    a.st_kind = st.kind;
-    a.dt = st.dt;
-    a.uprev = st.uprev;
-    a.out = st.out;
+    // with the IGR source the other kernels leave du (masked) for its last
+    // kernel, which writes the output
+    RhsArgs out_args = make_args(c, d_u, d_du);
+    out_args.st_kind = st.kind;
+    out_args.dt = st.dt;
+    out_args.uprev = st.uprev;
+    out_args.out = st.out;
+    RhsArgs a = out_args;
+    if (igr) a.st_kind = ST_NONE;
     const int64_t nblk = (c->N + kBlock - 1) / kBlock;
     const unsigned grid = xcd_grid(nblk);
     if (rv || tom) {
@@ -1669,6 +1929,7 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
         }
     }
     MF_HIP(c, hipGetLastError());
+    if (igr && (parts & RHS_IGR) && (rc = run_igr(c, d_u, out_args, grid))) return rc;
     // second boundary pass on u: only the slip projection changes anything
     rc = (parts & RHS_BC) ? bc_state_pass(c, d_u, dt_off, 0, true) : MF_OK;
     return rc;

@@ -18,7 +18,7 @@ MF_EQ_ADVECTION_2D, MF_EQ_EULER_2D = 0, 1
 MF_BC_DIRICHLET, MF_BC_SLIP_WALL, MF_BC_DO_NOTHING = 0, 1, 2
 MF_FN_CONST, MF_FN_PULSE, MF_FN_WAVE, MF_FN_GRADIENT, MF_FN_TABLE = 0, 1, 2, 3, 4
 MF_OP_FLUX, MF_OP_HYPERVISCOSITY, MF_OP_LAPLACIAN = 0, 1, 2
-MF_SRC_FLYER, MF_SRC_TOMINEC, MF_SRC_VISCOSITY = 1, 2, 3
+MF_SRC_FLYER, MF_SRC_TOMINEC, MF_SRC_VISCOSITY, MF_SRC_IGR = 1, 2, 3, 4
 MF_VISC_NONE, MF_VISC_RESIDUAL, MF_VISC_UPWIND = 0, 1, 2
 MF_PREC_F64, MF_PREC_SPLIT32, MF_PREC_F64_REF = 0, 1, 2
 MF_BUF_U, MF_BUF_DU, MF_BUF_AUX0, MF_BUF_AUX1 = 0, 1, 2, 3
@@ -58,6 +58,8 @@ _SIGS = {
     "mf_set_source_hyperviscosity": [_ctx, C.c_double],
     "mf_set_source_hyperviscosity_tominec": [_ctx, C.c_double],
     "mf_set_source_order": [_ctx, C.POINTER(C.c_int32), C.c_int],
+    "mf_set_source_igr": [_ctx, C.c_double, C.c_int],
+    "mf_get_igr_sigma": [_ctx, _dp],
     "mf_set_source_viscosity": [_ctx, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int],
     "mf_state_upload": [_ctx, C.c_int, _dpp],
     "mf_state_download": [_ctx, C.c_int, _dpp],
@@ -224,6 +226,14 @@ class Context:
 
     def set_source_hyperviscosity_tominec(self, gamma):
         self._chk(self._L.mf_set_source_hyperviscosity_tominec(self.h, gamma))
+
+    def set_source_igr(self, alpha, maxiter=20):
+        self._chk(self._L.mf_set_source_igr(self.h, alpha, maxiter))
+
+    def igr_sigma(self):
+        out = np.empty(self.N)
+        self._chk(self._L.mf_get_igr_sigma(self.h, dptr(out)))
+        return out
 
     def set_source_order(self, kinds):
         k = np.ascontiguousarray(kinds, np.int32)

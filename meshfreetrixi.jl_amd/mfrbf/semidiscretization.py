@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _lib as L
 from .equations import CompressibleEulerEquations2D, LinearScalarAdvectionEquation2D
-from .sources import (SourceHyperviscosityFlyer, SourceHyperviscosityTominec,
+from .sources import (SourceHyperviscosityFlyer, SourceHyperviscosityTominec, SourceIGR,
                       SourceResidualViscosityTominec, SourceTerms, SourceUpwindViscosityTominec)
 
 PRECISIONS = {"f64": L.MF_PREC_F64, "split32": L.MF_PREC_SPLIT32, "f64_ref": L.MF_PREC_F64_REF}
@@ -92,9 +92,16 @@ class SemidiscretizationHyperbolic:
         self.visc_source = None
         self.hv_source = None
         self.tominec_source = None
+        self.igr_source = None
         order = []
         for s in self.source_terms.values():
-            if isinstance(s, SourceHyperviscosityTominec):
+            if isinstance(s, SourceIGR):
+                if self.igr_source is not None:
+                    raise NotImplementedError("more than one IGR source")
+                self.ctx.set_source_igr(s.alpha, s.maxiter)
+                self.igr_source = s
+                order.append(L.MF_SRC_IGR)
+            elif isinstance(s, SourceHyperviscosityTominec):
                 if self.tominec_source is not None:
                     raise NotImplementedError("more than one Tominec hyperviscosity source")
                 # lap = sum(compute_flux_operator(solver, domain, 2)), hyperviscosity.jl:111-113

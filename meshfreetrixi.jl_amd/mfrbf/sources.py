@@ -75,7 +75,17 @@ class SourceHyperviscosityTominec:
 
 
 class SourceIGR:
-    """Information geometric regularization (sources/IGR.jl): not provided."""
+    """Information geometric regularization (sources/IGR.jl:17-239): sigma
+    from `maxiter` conjugate-gradient iterations on
+    (1/rho - alpha (Dx diag(1/rho) Dx + Dy diag(1/rho) Dy)) sigma =
+    alpha (trace^2 + trace_sq), then du_m1 -= Dx sigma, du_m2 -= Dy sigma.
+    The reference passes linear_solver = cg and calls it as
+    linear_solver(sigma, A, b; maxiter = 20) (IGR.jl:183), which
+    IterativeSolvers.jl provides as cg!(sigma, A, b); that is what runs here.
+    Euler equations; it must be the last source in SourceTerms."""
 
-    def __init__(self, *a, **kw):
-        raise NotImplementedError("SourceIGR is not provided by the HIP engine")
+    def __init__(self, solver, equations, domain, alpha=1.0, linear_solver="cg", maxiter=20):
+        if linear_solver not in ("cg", "cg!"):
+            raise NotImplementedError("SourceIGR: only the conjugate-gradient solver is provided")
+        self.alpha = float(alpha)
+        self.maxiter = int(maxiter)

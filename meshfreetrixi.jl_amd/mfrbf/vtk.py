@@ -13,6 +13,7 @@ Point data follow the reference's write2vtk! methods (write2vtk.jl:303-351):
   source approx_du.  The residual source's eps_uw, eps_rv and residual are
   intermediates of the device kernel that are not kept, so they are not
   written;
+- SourceIGR: sigma;
 - always index (1-based, as the reference's eachelement) and the field data
   time and solver_version.
 The state is read back from the device; the files are written on the host.
@@ -23,7 +24,7 @@ import subprocess
 import numpy as np
 
 from .equations import CompressibleEulerEquations2D
-from .sources import SourceResidualViscosityTominec, SourceUpwindViscosityTominec
+from .sources import SourceIGR, SourceResidualViscosityTominec, SourceUpwindViscosityTominec
 
 _TYPES = {np.dtype(np.float64): "Float64", np.dtype(np.float32): "Float32",
           np.dtype(np.int64): "Int64", np.dtype(np.int32): "Int32",
@@ -168,6 +169,8 @@ def _system_data(system, u, semi):
         out += [("eps", eps), ("eps_scalar", code)]
         if isinstance(system, SourceResidualViscosityTominec):
             out.append(("approx_du", semi.ctx.approx_du().T))
+    elif isinstance(system, SourceIGR):
+        out.append(("sigma", semi.ctx.igr_sigma()))
     return out
 
 

@@ -932,6 +932,143 @@ static void or_src_tominec(const or_rhs_cfg* c, const double* u, double* du, dou
     }
 }
 
+/* ======================================================================
+ * IGR source, sources/IGR.jl.  Euler equations; state SoA.
+ * ==================================================================== */
+/* update_igr_rhs! (IGR.jl:117-157): primitive variables (Trixi cons2prim:
+ * v = rho_v / rho), their derivatives D_d prim by mul_by! (one CSC product
+ * per field, column order), then per node
+ *   trace    = flux_x[1] + flux_y[2]           (D_x rho + D_y v1, as written)
+ *   trace_sq = flux_x[2]^2 + 2 flux_y[2] flux_x[3] + flux_y[3]^2
+ *   b        = alpha (trace * trace + trace_sq)
+ * and rho_inv = 1 / rho (update_sigma!, :176-178). */
+void or_igr_rhs(const or_rhs_cfg* c, const double* u, double* b, double* rinv)
+{
+    const int64_t N = c->N;
+    double* v1 = (double*)malloc(sizeof(double) * N);
+    double* v2 = (double*)malloc(sizeof(double) * N);
+    double* f = (double*)calloc((size_t)5 * N, sizeof(double));
+    for (int64_t i = 0; i < N; ++i) {
+        v1[i] = u[N + i] / u[i];
+        v2[i] = u[2 * N + i] / u[i];
+    }
+    double* dxr = f;
+    double* dxv1 = f + N;
+    double* dxv2 = f + 2 * N;
+    double* dyv1 = f + 3 * N;
+    double* dyv2 = f + 4 * N;
+    or_csc_mul(N, c->colptr, c->rowval, c->dx, u, dxr, 1.0, 1);
+    or_csc_mul(N, c->colptr, c->rowval, c->dx, v1, dxv1, 1.0, 1);
+    or_csc_mul(N, c->colptr, c->rowval, c->dx, v2, dxv2, 1.0, 1);
+    or_csc_mul(N, c->colptr, c->rowval, c->dy, v1, dyv1, 1.0, 1);
+    or_csc_mul(N, c->colptr, c->rowval, c->dy, v2, dyv2, 1.0, 1);
+    for (int64_t i = 0; i < N; ++i) {
+        double trace = 0.0;
+        trace += dxr[i];
+        trace += dyv1[i];
+        const double tsq = dxv1[i] * dxv1[i] + 2.0 * dyv1[i] * dxv2[i] + dyv2[i] * dyv2[i];
+        b[i] = c->igr_alpha * (trace * trace + tsq);
+        rinv[i] = 1.0 / u[i];
+    }
+    free(v1);
+    free(v2);
+    free(f);
+}
+
+/* IGRCompositeMatrix mul! (IGR.jl:50-66):
+ *   tmp1 = Dx v; tmp2 = Dy v; tmp1 = rinv .* tmp1; tmp2 = rinv .* tmp2;
+ *   tmp3 = Dx tmp1; tmp1 = Dy tmp2; y = rinv .* v; y -= alpha .* (tmp3 + tmp1) */
+void or_igr_apply(const or_rhs_cfg* c, const double* rinv, const double* v, double* y)
+{
+    const int64_t N = c->N;
+    double* t1 = (double*)malloc(sizeof(double) * N);
+    double* t2 = (double*)malloc(sizeof(double) * N);
+    double* t3 = (double*)malloc(sizeof(double) * N);
+    or_csc_mul(N, c->colptr, c->rowval, c->dx, v, t1, 1.0, 1);
+    or_csc_mul(N, c->colptr, c->rowval, c->dy, v, t2, 1.0, 1);
+    for (int64_t i = 0; i < N; ++i) {
+        t1[i] = rinv[i] * t1[i];
+        t2[i] = rinv[i] * t2[i];
+    }
+    or_csc_mul(N, c->colptr, c->rowval, c->dx, t1, t3, 1.0, 1);
+    or_csc_mul(N, c->colptr, c->rowval, c->dy, t2, t1, 1.0, 1);
+    for (int64_t i = 0; i < N; ++i) {
+        y[i] = rinv[i] * v[i];
+        y[i] -= c->igr_alpha * (t3[i] + t1[i]);
+    }
+    free(t1);
+    free(t2);
+    free(t3);
+}
+
+static double or_norm2(int64_t n, const double* r)
+{
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += r[i] * r[i];
+    return sqrt(s);
+}
+
+/* Conjugate gradients as IterativeSolvers.jl's cg! (cg.jl, CGIterable with
+ * the identity preconditioner; published algorithm, package version not
+ * pinned by the reference's Project.toml): r = b - A x, residual = |r|,
+ * tol = max(reltol |r|, 0) with reltol = sqrt(eps); per iteration, unless
+ * residual <= tol or maxiter is reached:
+ *   beta = residual^2 / prev^2 (prev = 1 at start); u = r + beta u;
+ *   c = A u; alpha = residual^2 / dot(u, c); x += alpha u; r -= alpha c;
+ *   prev = residual; residual = |r|.
+ * The norms and dot products are summed in index order here (BLAS nrm2 and
+ * dot order: parity unpinned at rounding level). */
+int or_igr_cg(const or_rhs_cfg* c, const double* rinv, const double* b, double* x, int maxiter)
+{
+    const int64_t N = c->N;
+    double* r = (double*)malloc(sizeof(double) * N);
+    double* uu = (double*)calloc((size_t)N, sizeof(double));
+    double* cc = (double*)malloc(sizeof(double) * N);
+    or_igr_apply(c, rinv, x, cc);
+    for (int64_t i = 0; i < N; ++i) r[i] = b[i] - cc[i];
+    double residual = or_norm2(N, r), prev = 1.0;
+    const double tol = sqrt(DBL_EPSILON) * residual;
+    int it = 0;
+    while (it < maxiter && !(residual <= tol)) {
+        const double beta = residual * residual / (prev * prev);
+        for (int64_t i = 0; i < N; ++i) uu[i] = r[i] + beta * uu[i];
+        or_igr_apply(c, rinv, uu, cc);
+        double d = 0.0;
+        for (int64_t i = 0; i < N; ++i) d += uu[i] * cc[i];
+        const double a = residual * residual / d;
+        for (int64_t i = 0; i < N; ++i) {
+            x[i] += a * uu[i];
+            r[i] -= a * cc[i];
+        }
+        prev = residual;
+        residual = or_norm2(N, r);
+        ++it;
+    }
+    free(r);
+    free(uu);
+    free(cc);
+    return it;
+}
+
+/* The source (IGR.jl:212-239): sigma = 0, solve, then for each dimension
+ * du += D_d (flux_igr(sigma) * -1) (mul_by_accum!(D_d, -1)); flux_igr puts
+ * sigma in component 2 (x) or 3 (y), the others are zero. */
+static void or_src_igr(const or_rhs_cfg* c, const double* u, double* du)
+{
+    const int64_t N = c->N;
+    double* b = (double*)malloc(sizeof(double) * N);
+    double* rinv = (double*)malloc(sizeof(double) * N);
+    double* sigma = (double*)calloc((size_t)N, sizeof(double));
+    or_igr_rhs(c, u, b, rinv);
+    or_igr_cg(c, rinv, b, sigma, c->igr_maxiter);
+    or_csc_mul(N, c->colptr, c->rowval, c->dx, sigma, du + N, -1.0, 0);
+    or_csc_mul(N, c->colptr, c->rowval, c->dy, sigma, du + 2 * N, -1.0, 0);
+    if (c->igr_sigma_out) memcpy(c->igr_sigma_out, sigma, sizeof(double) * N);
+    free(b);
+    free(rinv);
+    free(sigma);
+}
+
 /* residual / upwind viscosity source, hyperviscosity.jl:351-409 */
 static void or_src_rv(const or_rhs_cfg* c, const double* u, double* du, double* eps,
                       int8_t* eps_c, double* scratch)
@@ -992,15 +1129,17 @@ int or_rhs(const or_rhs_cfg* c, double t, double* u, double* du, double* eps_out
         if (!eps) eps = own_e;
         if (!eps_c) eps_c = own_c;
     }
-    int order[3] = {0, 0, 0};
+    int order[4] = {0, 0, 0, 0};
     if (c->src_order[0]) {
-        for (int q = 0; q < 3; ++q) order[q] = c->src_order[q];
+        for (int q = 0; q < 4; ++q) order[q] = c->src_order[q];
     } else {
         order[0] = c->hv_first ? OR_SRC_FLYER : OR_SRC_VISCOSITY;
         order[1] = c->hv_first ? OR_SRC_VISCOSITY : OR_SRC_FLYER;
         order[2] = OR_SRC_TOMINEC;
+        order[3] = OR_SRC_IGR;
     }
-    for (int q = 0; q < 3 && order[q]; ++q) {
+    for (int q = 0; q < 4 && order[q]; ++q) {
+        if (order[q] == OR_SRC_IGR && c->igr_alpha != 0.0 && c->eq == OR_EQ_EULER) or_src_igr(c, u, du);
         if (order[q] == OR_SRC_FLYER && c->hv) or_src_flyer(c, u, du);
         if (order[q] == OR_SRC_TOMINEC && c->lap) or_src_tominec(c, u, du, rv_scr + 3 * N);
         if (order[q] == OR_SRC_VISCOSITY && c->rv_mode != OR_RV_NONE) or_src_rv(c, u, du, eps, eps_c, rv_scr);

@@ -61,6 +61,7 @@ enum { OR_RV_NONE = 0, OR_RV_RESIDUAL = 1, OR_RV_UPWIND = 2 };
 #define OR_SRC_FLYER 1
 #define OR_SRC_TOMINEC 2
 #define OR_SRC_VISCOSITY 3
+#define OR_SRC_IGR 4
 
 typedef struct {
     int64_t N;
@@ -96,8 +97,14 @@ typedef struct {
     const double* lap;
     double tom_gamma;
     /* sources in keyword order (OR_SRC_*), 0-terminated; src_order[0] = 0:
-     * Flyer and the viscosity source ordered by hv_first, then Tominec */
-    int src_order[3];
+     * Flyer and the viscosity source ordered by hv_first, then Tominec,
+     * then IGR */
+    int src_order[4];
+    /* IGR source (Euler): alpha, CG iterations; igr_alpha = 0: none.
+     * igr_sigma_out (N doubles, may be NULL) receives sigma. */
+    double igr_alpha;
+    int igr_maxiter;
+    double* igr_sigma_out;
 } or_rhs_cfg;
 
 void or_flux(int eq, const double* adv_a, double gamma, int orientation, const double* u,
@@ -108,6 +115,15 @@ void or_bc_pass(const or_rhs_cfg* c, double t, double* u, double* du);
 int or_rhs(const or_rhs_cfg* c, double t, double* u, double* du, double* eps_out,
            int8_t* eps_c_out, double* scratch);
 int64_t or_rhs_scratch_len(const or_rhs_cfg* c);
+
+/* IGR pieces (sources/IGR.jl).  rinv = 1 / rho.  or_igr_rhs: b and rinv
+ * from the state (update_igr_rhs!, :117-157, and :176-178);
+ * or_igr_apply: y = (rinv - alpha (Dx rinv Dx + Dy rinv Dy)) v (the
+ * IGRCompositeMatrix mul!, :50-66); or_igr_cg: IterativeSolvers.jl's cg!
+ * from x (in/out), returns the iterations done. */
+void or_igr_rhs(const or_rhs_cfg* c, const double* u, double* b, double* rinv);
+void or_igr_apply(const or_rhs_cfg* c, const double* rinv, const double* v, double* y);
+int or_igr_cg(const or_rhs_cfg* c, const double* rinv, const double* b, double* x, int maxiter);
 
 /* residual viscosity pieces (hyperviscosity.jl) */
 void or_upwind_visc(const or_rhs_cfg* c, const double* u, double* eps_uw);

@@ -18,7 +18,7 @@ BC_DIRICHLET, BC_SLIP, BC_NOTHING = 0, 1, 2
 FN_CONST, FN_PULSE, FN_WAVE, FN_GRADIENT, FN_TABLE = 0, 1, 2, 3, 4
 RV_NONE, RV_RESIDUAL, RV_UPWIND = 0, 1, 2
 VAR_DENSITY, VAR_PRESSURE = 0, 1
-SRC_FLYER, SRC_TOMINEC, SRC_VISCOSITY = 1, 2, 3
+SRC_FLYER, SRC_TOMINEC, SRC_VISCOSITY, SRC_IGR = 1, 2, 3, 4
 
 
 def build():
@@ -58,7 +58,8 @@ class RhsCfg(C.Structure):
         ("tag_fn", _i32p), ("tag_fn_params", _dp), ("bidx", _i32p),
         ("bnormal", _dp), ("btable", _dp),
         ("approx_du", _dp), ("success_iter", C.c_int),
-        ("lap", _dp), ("tom_gamma", C.c_double), ("src_order", C.c_int * 3),
+        ("lap", _dp), ("tom_gamma", C.c_double), ("src_order", C.c_int * 4),
+        ("igr_alpha", C.c_double), ("igr_maxiter", C.c_int), ("igr_sigma_out", _dp),
     ]
 
 
@@ -92,6 +93,10 @@ def _declare(L):
     L.or_rhs.argtypes = [C.POINTER(RhsCfg), C.c_double, _dp, _dp, _dp, _i8p, _dp]
     L.or_rhs_scratch_len.argtypes = [C.POINTER(RhsCfg)]
     L.or_rhs_scratch_len.restype = C.c_int64
+    L.or_igr_rhs.argtypes = [C.POINTER(RhsCfg), _dp, _dp, _dp]
+    L.or_igr_apply.argtypes = [C.POINTER(RhsCfg), _dp, _dp, _dp]
+    L.or_igr_cg.argtypes = [C.POINTER(RhsCfg), _dp, _dp, _dp, C.c_int]
+    L.or_igr_cg.restype = C.c_int
     L.or_rv_norms.argtypes = [C.c_int64, C.c_int, _dp, _dp, _dp]
     L.or_upwind_visc.argtypes = [C.POINTER(RhsCfg), _dp, _dp]
     L.or_update_visc.argtypes = [C.c_int64, _dp, _dp, C.c_int, _dp, _i8p]
@@ -243,9 +248,12 @@ class Problem:
 
     def __init__(self, x, y, ptr, idx, wx, wy, eq, nvar, gamma=1.4, adv=(1.0, 1.0),
                  boundary=(), hv=None, hv_gamma=0.0, rv_mode=RV_NONE, c_rv=1.0, c_uw=1.0,
-                 dx_avg=0.0, hv_first=True, lap=None, tom_gamma=0.0, order=None):
-        """lap, tom_gamma: Tominec source; order: the sources in keyword order
-        (SRC_* values; default Flyer and viscosity by hv_first, then Tominec)."""
+                 dx_avg=0.0, hv_first=True, lap=None, tom_gamma=0.0, order=None, igr_alpha=0.0,
+                 igr_maxiter=20):
+        """lap, tom_gamma: Tominec source; igr_alpha, igr_maxiter: IGR source
+        (sigma of the last rhs! in self.igr_sigma); order: the sources in
+        keyword order (SRC_* values; default Flyer and viscosity by hv_first,
+        then Tominec, then IGR)."""
         self.N = N = len(x)
         self.x = np.ascontiguousarray(x, dtype=np.float64)
         self.y = np.ascontiguousarray(y, dtype=np.float64)
@@ -297,10 +305,30 @@ class Problem:
         c.success_iter = 0
         c.lap = d(self.lap)
         c.tom_gamma = tom_gamma
+        self.igr_sigma = np.zeros(N)
+        c.igr_alpha, c.igr_maxiter, c.igr_sigma_out = igr_alpha, igr_maxiter, d(self.igr_sigma)
         for q, k in enumerate(order or []):
             c.src_order[q] = k
         self.cfg = c
         self.scratch = np.zeros(lib().or_rhs_scratch_len(C.byref(c)) + 3 * nvar * N)
+
+    def igr_rhs(self, u):
+        """(b, rinv) of the IGR system for state u (nvar, N)."""
+        b, rinv = np.zeros(self.N), np.zeros(self.N)
+        lib().or_igr_rhs(C.byref(self.cfg), d(np.ascontiguousarray(u, dtype=np.float64)), d(b), d(rinv))
+        return b, rinv
+
+    def igr_apply(self, rinv, v):
+        y = np.zeros(self.N)
+        lib().or_igr_apply(C.byref(self.cfg), d(np.ascontiguousarray(rinv)), d(np.ascontiguousarray(v)), d(y))
+        return y
+
+    def igr_cg(self, rinv, b, x0=None, maxiter=20):
+        """(x, iterations) of cg! from x0 (zeros by default)."""
+        x = np.zeros(self.N) if x0 is None else np.array(x0, dtype=np.float64)
+        it = lib().or_igr_cg(C.byref(self.cfg), d(np.ascontiguousarray(rinv)), d(np.ascontiguousarray(b)), d(x),
+                             maxiter)
+        return x, it
 
     def rhs(self, u, t, success_iter=None, approx_du=None):
         """u: (nvar, N) array, mutated at boundary nodes like the reference.

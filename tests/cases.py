@@ -12,11 +12,12 @@ class Case:
 
 
 def build(spec, side=None, use_gpu=True, weights="oracle", hv=None, visc=None, bc=("dirichlet", "nothing"),
-          knn="grid", node_order=0, tom=None, order=None):
+          knn="grid", node_order=0, tom=None, order=None, igr=None, igr_maxiter=20):
     """weights: 'oracle' installs the oracle's weights on the GPU (so the
     rhs! comparison isolates the stencil kernels), 'gpu' keeps the GPU's own.
-    hv = (k, c): Flyer source; tom = c: Tominec source; order: the sources in
-    keyword order, names 'flyer', 'tominec', 'visc' (default: that order)."""
+    hv = (k, c): Flyer source; tom = c: Tominec source; igr = alpha: IGR
+    source; order: the sources in keyword order, names 'flyer', 'tominec',
+    'visc', 'igr' (default: that order)."""
     c = Case()
     c.spec = spec
     side = side or spec.side
@@ -70,8 +71,8 @@ def build(spec, side=None, use_gpu=True, weights="oracle", hv=None, visc=None, b
         c.lap_w = (lx + ly).reshape(N, -1)
         c.tom_gamma = tom * c.dx_min ** 4.5
     names = {"flyer": (O.SRC_FLYER, L.MF_SRC_FLYER), "tominec": (O.SRC_TOMINEC, L.MF_SRC_TOMINEC),
-             "visc": (O.SRC_VISCOSITY, L.MF_SRC_VISCOSITY)}
-    order = list(order or ["flyer", "tominec", "visc"])
+             "visc": (O.SRC_VISCOSITY, L.MF_SRC_VISCOSITY), "igr": (O.SRC_IGR, L.MF_SRC_IGR)}
+    order = list(order or ["flyer", "tominec", "visc", "igr"])
     hv_first = order.index("flyer") < order.index("visc") if "visc" in order and "flyer" in order else True
     c.visc = visc
     rv_mode = O.RV_NONE
@@ -81,7 +82,7 @@ def build(spec, side=None, use_gpu=True, weights="oracle", hv=None, visc=None, b
                        boundary=[(ko, nodes, nrm, fn, params, None) for ((kg, ko), nodes, nrm) in c.bc],
                        hv=c.hv_w, hv_gamma=c.hv_gamma, rv_mode=rv_mode, c_rv=1.0, c_uw=1.0,
                        dx_avg=c.dx_avg, hv_first=hv_first, lap=c.lap_w, tom_gamma=c.tom_gamma,
-                       order=[names[n][0] for n in order])
+                       order=[names[n][0] for n in order], igr_alpha=igr or 0.0, igr_maxiter=igr_maxiter)
     if not use_gpu:
         return c
     ctx = L.Context(0)
@@ -117,5 +118,7 @@ def build(spec, side=None, use_gpu=True, weights="oracle", hv=None, visc=None, b
     if visc is not None:
         mode = {"residual": L.MF_VISC_RESIDUAL, "upwind": L.MF_VISC_UPWIND}[visc]
         ctx.set_source_viscosity(mode, 1.0, 1.0, c.dx_avg, getattr(spec, "approx_order", 3), hv_first)
+    if igr is not None:
+        ctx.set_source_igr(igr, igr_maxiter)
     ctx.set_source_order([names[n][1] for n in order])
     return c

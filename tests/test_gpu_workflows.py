@@ -43,6 +43,8 @@ def _cyl_semi(visc=None, host_dirichlet=False):
     elif visc == "tominec_upwind":  # test/upwind_viscosity_test.jl:50-52
         src = mfrbf.SourceTerms(hv=mfrbf.SourceHyperviscosityTominec(solver, eqs, domain, c=1.0),
                                 rv=mfrbf.SourceUpwindViscosityTominec(solver, eqs, domain))
+    elif visc == "igr":
+        src = mfrbf.SourceTerms(igr=mfrbf.SourceIGR(solver, eqs, domain, alpha=domain.pd.dx_min ** 2))
     elif visc == "flyer_tominec":  # test/hyperviscosity_test.jl:42-45
         src = mfrbf.SourceTerms(hv=mfrbf.SourceHyperviscosityFlyer(solver, eqs, domain, k=2, c=1.0),
                                 hv2=mfrbf.SourceHyperviscosityTominec(solver, eqs, domain, c=1.0))
@@ -71,6 +73,8 @@ def _cyl_oracle(domain, visc=None):
         kw.update(lap=(lx + ly).reshape(N, k), tom_gamma=pd.dx_min ** 4.5)
     if visc == "tominec_upwind":
         kw.update(order=[O.SRC_TOMINEC, O.SRC_VISCOSITY])
+    if visc == "igr":
+        kw.update(igr_alpha=pd.dx_min ** 2, order=[O.SRC_IGR])
     if visc == "flyer_tominec":
         hx, hy, rc = O.weights(x, y, O.ell_ptr(N, k), idx, 5, 3, 4)
         kw.update(hv=(hx + hy).reshape(N, k), hv_gamma=pd.dx_min ** 4, order=[O.SRC_FLYER, O.SRC_TOMINEC])
@@ -79,7 +83,8 @@ def _cyl_oracle(domain, visc=None):
 
 
 @pytest.mark.parametrize("visc,host_dirichlet", [(None, False), ("upwind", False), (None, True),
-                                                 ("tominec_upwind", False), ("flyer_tominec", False)])
+                                                 ("tominec_upwind", False), ("flyer_tominec", False),
+                                                 ("igr", False)])
 def test_host_api_cyl_slip_walls(mf, visc, host_dirichlet):
     """Trixi.rhs! on the reference's Medusa cloud (2,154 nodes, five boundary
     groups): Dirichlet inlet, do-nothing outlet, three slip walls; with the
@@ -130,6 +135,11 @@ def test_trixi2vtk_cyl(mf, tmp_path):
     mfrbf.trixi2vtk(u, semi, 0.5, iter=1, output_directory=str(tmp_path), prefix="sol")
     pvd = open(os.path.join(str(tmp_path), "sol_CompressibleEulerEquations2D_1.pvd")).read()
     assert pvd.count("<DataSet") == 2 and 'timestep="0.5"' in pvd
+    # the IGR source writes sigma
+    semi2, _ = _cyl_semi("igr")
+    mfrbf.rhs_(du, u.copy(), semi2, 0.0)
+    _, pd2, _ = vtk.read_vtu(mfrbf.trixi2vtk(u, semi2, 0.0, output_directory=str(tmp_path), prefix="igr"))
+    assert np.array_equal(pd2["sigma"], semi2.ctx.igr_sigma()) and np.abs(pd2["sigma"]).max() > 0.0
 
 
 def test_limiter_vs_oracle(mf):

@@ -488,3 +488,82 @@ def test_source_order_flyer_tominec_upwind():
     assert np.abs(base + parts - ref).max() < 1e-12 * scale
     # and the Tominec part is not negligible here
     assert np.abs(du([T], fly=False, rv=False) - base).max() > 1e-6 * scale
+
+
+def _igr_case(side=24):
+    spec = configs.c2()
+    x, y, _, _, _ = O.cloud_square(side, spec.seed)
+    N, k = len(x), 30
+    idx, d2 = O.knn_grid(x, y, k)
+    ptr = O.ell_ptr(N, k)
+    wx, wy, rc = O.weights(x, y, ptr, idx, 5, 3, 1)
+    assert rc == 0
+    Dx = sp.csr_matrix((wx, idx.ravel(), np.arange(N + 1) * k), shape=(N, N))
+    Dy = sp.csr_matrix((wy, idx.ravel(), np.arange(N + 1) * k), shape=(N, N))
+    dx_min, _ = O.dx_stats(d2[:, 1])
+    return x, y, ptr, idx, wx, wy, Dx, Dy, dx_min
+
+
+def test_igr_rhs_closed_form():
+    """update_igr_rhs! (IGR.jl:117-157) on a state with constant density and
+    a linear velocity field: the derivative weights are exact on linear
+    functions, so trace = D_x rho + D_y v1 = b and trace_sq = a^2 + 2 b c + d^2
+    for v = (a x + b y, c x + d y) at every node."""
+    x, y, ptr, idx, wx, wy, Dx, Dy, _ = _igr_case()
+    a_, b_, c_, d_ = 0.3, -0.7, 1.1, 0.45
+    rho = 1.3 + 0 * x
+    u = np.stack([rho, rho * (a_ * x + b_ * y), rho * (c_ * x + d_ * y), 2.5 + 0 * x])
+    alpha = 0.25
+    p = O.Problem(x, y, ptr, idx, wx, wy, O.EQ_EULER, 4, igr_alpha=alpha)
+    b, rinv = p.igr_rhs(u)
+    expect = alpha * (b_ * b_ + a_ * a_ + 2 * b_ * c_ + d_ * d_)
+    assert np.abs(b - expect).max() < 1e-8 * abs(expect)
+    assert np.array_equal(rinv, 1.0 / rho)
+
+
+def test_igr_operator_and_cg():
+    """The IGRCompositeMatrix product (IGR.jl:50-66) against the explicit
+    sparse matrix diag(1/rho) - alpha (Dx diag(1/rho) Dx + Dy diag(1/rho) Dy),
+    and cg! against a direct solve where CG converges (alpha = dx^2; the
+    reference's default alpha = 1 makes the operator indefinite and CG does
+    not converge), with the early exit for b = 0."""
+    x, y, ptr, idx, wx, wy, Dx, Dy, dx_min = _igr_case()
+    N = len(x)
+    rng = np.random.default_rng(4)
+    rho = 1.0 + 0.2 * np.sin(2 * np.pi * (x + y))
+    u = np.stack([rho, rho * (1.0 + 0.1 * x), rho * (1.0 - 0.2 * y * y), 2.5 + rho])
+    alpha = dx_min ** 2
+    p = O.Problem(x, y, ptr, idx, wx, wy, O.EQ_EULER, 4, igr_alpha=alpha)
+    b, rinv = p.igr_rhs(u)
+    R = sp.diags(rinv)
+    A = (R - alpha * (Dx @ R @ Dx + Dy @ R @ Dy)).toarray()
+    v = rng.standard_normal(N)
+    assert rel_inf(p.igr_apply(rinv, v), A @ v) < 1e-12
+    xs = np.linalg.solve(A, b)
+    x60, it = p.igr_cg(rinv, b, maxiter=60)
+    assert it == 60 or np.linalg.norm(b - A @ x60) <= np.sqrt(np.finfo(float).eps) * np.linalg.norm(b) * 1.01
+    assert rel_inf(x60, xs) < 1e-5
+    x5, it5 = p.igr_cg(rinv, b, maxiter=5)
+    assert it5 == 5 and rel_inf(x5, xs) > rel_inf(x60, xs)
+    z, it0 = p.igr_cg(rinv, 0 * b, maxiter=20)
+    assert it0 == 0 and np.all(z == 0.0)
+
+
+def test_igr_source_terms_in_du():
+    """The source adds -Dx sigma to du_m1 and -Dy sigma to du_m2 only
+    (flux_igr, IGR.jl:188-239); sigma is the CG solution from 0."""
+    x, y, ptr, idx, wx, wy, Dx, Dy, dx_min = _igr_case()
+    rho = 1.0 + 0.2 * np.sin(2 * np.pi * (x + y))
+    u = np.stack([rho, rho, rho, 2.5 + rho])
+    alpha = 20 * dx_min ** 2
+    plain = O.Problem(x, y, ptr, idx, wx, wy, O.EQ_EULER, 4)
+    igr = O.Problem(x, y, ptr, idx, wx, wy, O.EQ_EULER, 4, igr_alpha=alpha, igr_maxiter=20)
+    du0, _, _ = plain.rhs(u.copy(), 0.0)
+    du1, _, _ = igr.rhs(u.copy(), 0.0)
+    b, rinv = igr.igr_rhs(u)
+    sig, _ = igr.igr_cg(rinv, b, maxiter=20)
+    assert np.array_equal(igr.igr_sigma, sig)
+    d = du1 - du0
+    assert np.all(d[0] == 0.0) and np.all(d[3] == 0.0)
+    assert np.abs(d[1] + Dx @ sig).max() < 1e-12 * np.abs(Dx @ sig).max()
+    assert np.abs(d[2] + Dy @ sig).max() < 1e-12 * np.abs(Dy @ sig).max()

@@ -159,6 +159,25 @@ def tominec(L, ctx, info, spec, t_now, steps):
     return res
 
 
+def igr(L, ctx, info, spec, t_now, steps):
+    """The same rhs! with the IGR source added last (alpha = dx_min^2, 20
+    conjugate-gradient iterations): rhs! time per precision.  Each CG
+    iteration is two stencil passes over scalar fields, two block-partial
+    reductions and three vector updates (seven launches)."""
+    ctx.set_source_igr(info["dx_min"] ** 2, 20)
+    ctx.set_source_order([L.MF_SRC_VISCOSITY, L.MF_SRC_IGR])
+    res = {"alpha": info["dx_min"] ** 2, "cg_iterations": 20}
+    try:
+        for name, prec in (("f64_ref", L.MF_PREC_F64_REF), ("f64_fma", L.MF_PREC_F64),
+                           ("split32", L.MF_PREC_SPLIT32)):
+            ctx.time_rhs(t_now, prec, 5)
+            ms, _ = ctx.time_rhs(t_now, prec, steps)
+            res[name] = {"value": info["N"] / (ms * 1e-3), "ms_per_step": ms}
+    finally:
+        ctx.set_source_igr(0.0, 20)
+    return res
+
+
 def cpu_baseline(L, ctx, info, spec, t_now, budget_s):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -408,6 +427,8 @@ def main():
         out["ensemble"] = ensemble(L, configs, 50)
         log("Tominec hyperviscosity after the residual viscosity, %d nodes" % N)
         out["tominec"] = tominec(L, ctx, info, spec, t_now, max(200, args.steps // 4))
+        log("IGR source after the residual viscosity, %d nodes" % N)
+        out["igr"] = igr(L, ctx, info, spec, t_now, 50)
         ctx.time_rhs(t_now, L.MF_PREC_F64_REF, 3)
     if not args.no_cpu:
         log("cpu baseline (oracle, %.0f s budget)" % args.cpu_seconds)

@@ -1349,9 +1349,10 @@ __device__ __forceinline__ double block_sum1(double v)
     return w[0];
 }
 
-// Sum over the stencil of node i of (Dx, Dy) times the scalar field f.
+// Sums over the stencil of node i of Dx times the field fx and Dy times the
+// field fy (each stencil entry's weights read once).
 template <int BT>
-__device__ __forceinline__ double2 igr_grad(const IgrArgs& a, int64_t i, const double* __restrict__ f)
+__device__ __forceinline__ double2 igr_grad2(const IgrArgs& a, int64_t i, const double* fx, const double* fy)
 {
     double sx = 0.0, sy = 0.0;
 #pragma unroll 1
@@ -1364,17 +1365,27 @@ __device__ __forceinline__ double2 igr_grad(const IgrArgs& a, int64_t i, const d
             j[q] = a.idx[(int64_t)s * a.N + i];
             w[q] = a.wxy[(int64_t)s * a.N + i];
         }
-        double fj[BT];
+        double gx[BT], gy[BT];
 #pragma unroll
-        for (int q = 0; q < BT; ++q) fj[q] = f[j[q]];
+        for (int q = 0; q < BT; ++q) {
+            gx[q] = fx[j[q]];
+            gy[q] = fx == fy ? gx[q] : fy[j[q]];
+        }
 #pragma unroll
         for (int q = 0; q < BT; ++q) {
             if (s0 + q >= a.k) continue;
-            sx = fma(w[q].x, fj[q], sx);
-            sy = fma(w[q].y, fj[q], sy);
+            sx = fma(w[q].x, gx[q], sx);
+            sy = fma(w[q].y, gy[q], sy);
         }
     }
     return make_double2(sx, sy);
+}
+
+// (Dx f, Dy f) at node i.
+template <int BT>
+__device__ __forceinline__ double2 igr_grad(const IgrArgs& a, int64_t i, const double* __restrict__ f)
+{
+    return igr_grad2<BT>(a, i, f, f);
 }
 
 // b, 1/rho, sigma = 0, r = b, p = 0 (update_igr_rhs!, IGR.jl:117-157, and
@@ -1477,9 +1488,8 @@ __global__ __launch_bounds__(kBlock) void k_igr_c(const IgrArgs a)
     const int64_t i = lb * kBlock + threadIdx.x;
     double pc = 0.0;
     if (i < a.N) {
-        const double2 g1 = igr_grad<BT>(a, i, a.t1);
-        const double2 g2 = igr_grad<BT>(a, i, a.t2);
-        const double ci = a.rinv[i] * a.p[i] - a.alpha * (g1.x + g2.y);
+        const double2 g = igr_grad2<BT>(a, i, a.t1, a.t2);  // (Dx t1, Dy t2)
+        const double ci = a.rinv[i] * a.p[i] - a.alpha * (g.x + g.y);
         a.c[i] = ci;
         pc = a.p[i] * ci;
     }

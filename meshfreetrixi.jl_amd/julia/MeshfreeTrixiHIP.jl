@@ -34,7 +34,7 @@ const MF_EQ_ADVECTION_2D, MF_EQ_EULER_2D = Cint(0), Cint(1)
 const MF_BC_DIRICHLET, MF_BC_SLIP_WALL, MF_BC_DO_NOTHING = Cint(0), Cint(1), Cint(2)
 const MF_FN_TABLE = Cint(4)
 const MF_OP_FLUX, MF_OP_HYPERVISCOSITY, MF_OP_LAPLACIAN = Cint(0), Cint(1), Cint(2)
-const MF_SRC_FLYER, MF_SRC_TOMINEC, MF_SRC_VISCOSITY = Int32(1), Int32(2), Int32(3)
+const MF_SRC_FLYER, MF_SRC_TOMINEC, MF_SRC_VISCOSITY, MF_SRC_IGR = Int32(1), Int32(2), Int32(3), Int32(4)
 const MF_VISC_NONE, MF_VISC_RESIDUAL, MF_VISC_UPWIND = Cint(0), Cint(1), Cint(2)
 const MF_PREC_F64, MF_PREC_SPLIT32 = Cint(0), Cint(1)
 
@@ -125,7 +125,12 @@ function install!(e::RBFFDEngineHIP, domain, equations, boundary_conditions, sou
     hv_first = false
     order = Int32[]  # the SourceTerms keyword order (generic_sources.jl:7-18)
     for s in values(source_terms)
-        if s isa SourceHyperviscosityTominec
+        if s isa SourceIGR
+            # sigma by cg! with maxiter = 20 (IGR.jl:183)
+            check(e, ccall((:mf_set_source_igr, libmfrbf), Cint, (Ptr{Cvoid}, Cdouble, Cint),
+                           e.ctx, s.cache.alpha, 20))
+            push!(order, MF_SRC_IGR)
+        elseif s isa SourceHyperviscosityTominec
             # lap = sum(compute_flux_operator(solver, domain, 2)) (hyperviscosity.jl:111-113)
             basis = solver.basis
             check(e, ccall((:mf_compute_operator, libmfrbf), Cint,

@@ -322,6 +322,43 @@ def test_set_tuning_rejects_bad_values(mf):
         c.ctx.set_tuning(9, 4)
 
 
+def test_api_rejects_bad_input(mf):
+    """Argument and call-order errors come back as status codes with a
+    message, before any kernel runs on bad data."""
+    def code(fn):
+        with pytest.raises(L.MfError) as e:
+            fn()
+        return e.value.code
+
+    x, y, _, _, _ = O.cloud_square(12, 5)
+    ctx = L.Context(0)
+    assert code(lambda: ctx.knn(8)) == L.MF_ERR_STATE  # no points yet
+    assert code(lambda: ctx.set_points(x[:1], y[:1])) == L.MF_ERR_ARGUMENT
+    ctx.set_equations(L.MF_EQ_ADVECTION_2D, 1.0, 1.0)
+    ctx.set_points(x, y)
+    assert code(lambda: ctx.knn(65)) == L.MF_ERR_ARGUMENT
+    assert code(lambda: ctx.compute_operator(5, 3, 1, L.MF_OP_FLUX, want=False)) == L.MF_ERR_STATE
+    bad = np.zeros((len(x), 15), np.int32)
+    bad[3, 4] = len(x)
+    assert code(lambda: ctx.set_stencil(bad)) == L.MF_ERR_ARGUMENT
+    ctx.knn(15)
+    u = np.ones((1, len(x)))
+    assert code(lambda: ctx.rhs_host(0.0, u, np.zeros_like(u))) == L.MF_ERR_STATE  # no flux operator
+    assert code(lambda: ctx.set_source_hyperviscosity_tominec(1.0)) == L.MF_ERR_STATE  # no lap operator
+    assert code(lambda: ctx.set_source_igr(1.0, 20)) == L.MF_ERR_UNSUPPORTED  # advection
+    assert code(lambda: ctx.set_source_order([L.MF_SRC_FLYER, L.MF_SRC_FLYER])) == L.MF_ERR_ARGUMENT
+    ctx.compute_operator(5, 3, 1, L.MF_OP_FLUX, want=False)
+    assert code(lambda: ctx.rhs_host(0.0, u, np.zeros_like(u), 7)) == L.MF_ERR_ARGUMENT  # precision
+    assert code(lambda: ctx.ssprk33(0.0, 0.01, 2, resume=True)) == L.MF_ERR_STATE
+    assert code(lambda: ctx.ssprk33(0.0, -0.01, 2)) == L.MF_ERR_ARGUMENT
+    assert code(lambda: ctx.compute_operator(5, 9, 1, L.MF_OP_FLUX, want=False)) in (L.MF_ERR_ARGUMENT,
+                                                                                    L.MF_ERR_UNSUPPORTED)
+    # the context still works after the errors
+    ctx.upload(L.MF_BUF_U, u)
+    ctx.rhs(0.0)
+    assert np.all(np.isfinite(ctx.download(L.MF_BUF_DU)))
+
+
 @pytest.mark.parametrize("k", [13, 25])
 def test_rhs_generic_stencil_size(mf, k):
     """Stencil sizes without a compiled specialisation take the generic

@@ -178,6 +178,26 @@ def igr(L, ctx, info, spec, t_now, steps):
     return res
 
 
+def stepping(L, ctx, info, steps):
+    """SSPRK33 steps at this workload (graph-replayed, FP64 reference order):
+    plain, with the history callback, and with the history callback and the
+    Zhang-Shu stage limiter on density and pressure (three limiter passes per
+    step), so the callbacks' cost per step is the difference."""
+    dt = info["dt"]
+    out = {}
+    ctx.set_stage_limiter([1e-6, 1e-6], [L.MF_VAR_DENSITY, L.MF_VAR_PRESSURE])
+    for name, flags in (("plain", 0), ("history", L.MF_STEP_HISTORY),
+                        ("history_limiter", L.MF_STEP_HISTORY | L.MF_STEP_LIMITER)):
+        ctx.upload(L.MF_BUF_U, info["u0"])
+        ctx.ssprk33(0.0, dt, 16, L.MF_PREC_F64_REF, flags | L.MF_STEP_GRAPH)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.ssprk33(0.0, dt, steps, L.MF_PREC_F64_REF, flags | L.MF_STEP_GRAPH, resume=True)
+        ctx.synchronize()
+        out[name + "_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / steps
+    return out
+
+
 def cpu_baseline(L, ctx, info, spec, t_now, budget_s):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -436,6 +456,10 @@ def main():
         out["cpu_baseline"] = cb
         out["rel_err_vs_oracle"] = {name: float(np.abs(d - du_cpu).max() / np.abs(du_cpu).max())
                                     for name, d in dus.items()}
+    if not args.no_extras:
+        # last: it advances the state and the history the comparisons above use
+        log("SSPRK33 steps with the history callback and the stage limiter, %d nodes" % N)
+        out["stepping"] = stepping(L, ctx, info, 64)
     print(json.dumps(out), flush=True)
 
 

@@ -238,9 +238,10 @@ int ensure_state(mf_ctx* c)
         if (!c->d_hist && (rc = dalloc(c, &c->d_hist, nv * (size_t)(c->approx_order + 1)))) return rc;
         if (!c->d_partials) {
             // reduction passes: the sum pass one node per thread (up to 2048
-            // blocks), the max pass up to 256 blocks of 4 nodes per thread
+            // blocks), the max pass up to kBlock blocks of 4 nodes per thread
+            // (the stencil kernels load one max partial per thread)
             c->n_psum = (int)std::min<int64_t>(2048, (c->N + 255) / 256);
-            c->n_pmax = (int)std::min<int64_t>(256, (c->N + 1023) / 1024);
+            c->n_pmax = (int)std::min<int64_t>(kBlock, (c->N + 1023) / 1024);
             if ((rc = dalloc(c, &c->d_partials, 4 * (size_t)(c->n_psum + c->n_pmax)))) return rc;
         }
     }

@@ -240,22 +240,16 @@ constexpr int kSeqViscosity = 1, kSeqTominec = 2, kSeqFlyer = 3;
 // With the Tominec source it is phase A too and stores L_i = sum_j lap_ij u_j
 // for the transposed pass du_i -= gamma sum_r lap_ri L_r (hyperviscosity.jl:
 // 111-134, the factors of H = lap' lap).
-// Residual-viscosity norms: every block reduces the max pass's partials
-// (same bits everywhere); block 0 publishes them.
-// Residual-viscosity norms in the stencil kernels: each thread loads its
-// share of the max pass's per-block partials at kernel start (the loads
-// overlap the halo staging and the stencil loop) and the block reduces them
-// just before the viscosity is needed; every block gets the same bits.
+// Residual-viscosity norms in the stencil kernels: thread b loads the max
+// pass's partial b at kernel start (at most kBlock partials) and keeps it
+// in registers untouched, so the load overlaps the halo staging and the
+// stencil loop instead of holding them up; the block reduces the partials
+// just before the viscosity is needed.  Every block gets the same bits.
 __device__ __forceinline__ void prefetch_max_partials(const RhsArgs& a, double* pm)
 {
-    for (int b = threadIdx.x; b < a.nparts; b += kBlock) {
-        double c[4];
+    if ((int)threadIdx.x < a.nparts)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) c[v] = a.pmax[4 * b + v];
-        if (lex_less(pm, c))
-#pragma unroll
-            for (int v = 0; v < 4; ++v) pm[v] = c[v];
-    }
+        for (int v = 0; v < 4; ++v) pm[v] = a.pmax[4 * threadIdx.x + v];
 }
 
 __device__ __forceinline__ void finish_norms(const RhsArgs& a, double* pm, double* ninf)

@@ -162,8 +162,9 @@ def tominec(L, ctx, info, spec, t_now, steps):
 def igr(L, ctx, info, spec, t_now, steps):
     """The same rhs! with the IGR source added last (alpha = dx_min^2, 20
     conjugate-gradient iterations): rhs! time per precision.  Each CG
-    iteration is two stencil passes over scalar fields, two block-partial
-    reductions and three vector updates (seven launches)."""
+    iteration is two stencil passes over scalar fields and two vector
+    updates, four launches; the dot products are per-block partials that the
+    next launch's blocks sum."""
     ctx.set_source_igr(info["dx_min"] ** 2, 20)
     ctx.set_source_order([L.MF_SRC_VISCOSITY, L.MF_SRC_IGR])
     res = {"alpha": info["dx_min"] ** 2, "cg_iterations": 20}

@@ -34,8 +34,10 @@ struct StepScalars {
 
 // Conjugate-gradient scalars of the IGR source, in device memory.
 struct IgrScalars {
-    double res, prev, tol, acg;
-    int iter, done;
+    double res[2];  // residual norm of CG iteration it in slot it & 1
+    double tol;     // sqrt(eps) |b|
+    int done;       // the current iteration is past convergence
+    int pad;
 };
 
 // Term table of the k-th derivative of r^p along one coordinate s:
@@ -146,7 +148,7 @@ struct mf_ctx {
     float* d_du32 = nullptr;  // split-FP32 partial du between the two viscosity phases
     double* d_L = nullptr;    // lap u between the phases (Tominec); FP32 in split-FP32
     double* d_igr = nullptr;  // IGR: 8 fields of N (1/rho, b, sigma, r, p, c, t1, t2)
-    double* d_igr_part = nullptr;          // per-block partial sums
+    double* d_igr_part = nullptr;          // per-block partial sums: r.r, then p.c
     mf::IgrScalars* d_igr_sc = nullptr;  // CG scalars on the device
     double* d_eps = nullptr;
     int8_t* d_epsc = nullptr;

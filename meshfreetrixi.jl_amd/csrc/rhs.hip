@@ -1319,8 +1319,9 @@ __global__ void k_fill(double* p, int64_t n, double v)
 //   (rho^-1 - alpha (Dx rho^-1 Dx + Dy rho^-1 Dy)) sigma = b,
 //   b = alpha (trace^2 + trace_sq) from the primitive-variable derivatives,
 // then du_m1 -= Dx sigma, du_m2 -= Dy sigma.  Scalar node fields; every
-// reduction is per block (fixed node ranges) into partials, which one block
-// sums in a fixed order, so every run gives the same bits.
+// reduction is per block (fixed node ranges) into partials, which the next
+// kernel's blocks each sum in the same fixed order, so every block, and
+// every run, gets the same bits.  Four launches per CG iteration.
 struct IgrArgs {
     int64_t N;
     int k;
@@ -1331,7 +1332,8 @@ struct IgrArgs {
     int maxiter;
     const double* __restrict__ u;
     double *rinv, *b, *x, *r, *p, *c, *t1, *t2;
-    double* part;
+    double* part_rr;  // per-block partial sums of r . r
+    double* part_pc;  // and of p . c
     IgrScalars* sc;
 };
 
@@ -1341,6 +1343,14 @@ __device__ __forceinline__ double block_sum1(double v)
     double w[4] = {v, 0.0, 0.0, 0.0};
     block_sum4(w);
     return w[0];
+}
+
+// Sum of the per-block partials in a fixed order (every block the same bits).
+__device__ __forceinline__ double sum_partials(const double* part, int64_t n)
+{
+    double acc = 0.0;
+    for (int64_t q = threadIdx.x; q < n; q += kBlock) acc += part[q];
+    return block_sum1(acc);
 }
 
 // Sums over the stencil of node i of Dx times the field fx and Dy times the
@@ -1418,43 +1428,29 @@ __global__ __launch_bounds__(kBlock) void k_igr_init(const IgrArgs a)
         rr = bi * bi;
     }
     rr = block_sum1(rr);
-    if (threadIdx.x == 0) a.part[lb] = rr;
+    if (threadIdx.x == 0) a.part_rr[lb] = rr;
 }
 
-// The CG scalars from the partials (one block): mode 0 after the start
-// (residual, tolerance sqrt(eps) |r0|), 1 after the dot (alpha), 2 after
-// the norm (next residual, iteration count, convergence).
-__global__ __launch_bounds__(kBlock) void k_igr_scalars(const IgrArgs a, int mode)
+// Start of CG iteration `it` (IterativeSolvers' cg! iterate): the residual
+// norm res = |r| from the partials, the tolerance sqrt(eps) |r0| at it = 0,
+// stop once res <= tol (the host launches at most maxiter iterations);
+// else p = r + beta p with beta = res^2 / prev^2 (prev = 1 at it = 0).
+// Block 0 publishes res (slot it & 1, where the next iteration finds it as
+// prev), the tolerance and the stop flag for this iteration's kernels.
+__global__ __launch_bounds__(kBlock) void k_igr_p(const IgrArgs a, int it)
 {
-    IgrScalars* sc = a.sc;
-    if (mode != 0 && sc->done) return;
-    double acc = 0.0;
-    for (int64_t q = threadIdx.x; q < a.nblk; q += kBlock) acc += a.part[q];
-    acc = block_sum1(acc);
-    if (threadIdx.x != 0) return;
-    if (mode == 0) {
-        sc->res = sqrt(acc);
-        sc->tol = sqrt(DBL_EPSILON) * sc->res;
-        sc->prev = 1.0;
-        sc->iter = 0;
-        sc->done = (a.maxiter <= 0 || sc->res <= sc->tol) ? 1 : 0;
-    } else if (mode == 1) {
-        sc->acg = sc->res * sc->res / acc;
-    } else {
-        sc->prev = sc->res;
-        sc->res = sqrt(acc);
-        sc->iter += 1;
-        sc->done = (sc->iter >= a.maxiter || sc->res <= sc->tol) ? 1 : 0;
+    const double res = sqrt(sum_partials(a.part_rr, a.nblk));
+    const double tol = it == 0 ? sqrt(DBL_EPSILON) * res : a.sc->tol;
+    const double prev = it == 0 ? 1.0 : a.sc->res[(it - 1) & 1];
+    const bool done = res <= tol;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.sc->res[it & 1] = res;
+        a.sc->tol = tol;
+        a.sc->done = done ? 1 : 0;
     }
-}
-
-// p = r + beta p, beta = res^2 / prev^2.
-__global__ __launch_bounds__(kBlock) void k_igr_p(const IgrArgs a)
-{
-    if (a.sc->done) return;
+    if (done) return;
     const int64_t i = remapped_node();
     if (i >= a.N) return;
-    const double res = a.sc->res, prev = a.sc->prev;
     const double beta = res * res / (prev * prev);
     a.p[i] = a.r[i] + beta * a.p[i];
 }
@@ -1488,26 +1484,29 @@ __global__ __launch_bounds__(kBlock) void k_igr_c(const IgrArgs a)
         pc = a.p[i] * ci;
     }
     pc = block_sum1(pc);
-    if (threadIdx.x == 0) a.part[lb] = pc;
+    if (threadIdx.x == 0) a.part_pc[lb] = pc;
 }
 
-// sigma += alpha p, r -= alpha c, and partial sums of r^2.
-__global__ __launch_bounds__(kBlock) void k_igr_x(const IgrArgs a)
+// alpha = res^2 / (p . c) from the partials, sigma += alpha p,
+// r -= alpha c, and partial sums of r^2.
+__global__ __launch_bounds__(kBlock) void k_igr_x(const IgrArgs a, int it)
 {
     if (a.sc->done) return;
     const int64_t lb = remapped_block();
     if (lb >= a.nblk) return;  // whole block
+    const double pc = sum_partials(a.part_pc, a.nblk);
+    const double res = a.sc->res[it & 1];
     const int64_t i = lb * kBlock + threadIdx.x;
     double rr = 0.0;
     if (i < a.N) {
-        const double al = a.sc->acg;
+        const double al = res * res / pc;
         a.x[i] += al * a.p[i];
         const double ri = a.r[i] - al * a.c[i];
         a.r[i] = ri;
         rr = ri * ri;
     }
     rr = block_sum1(rr);
-    if (threadIdx.x == 0) a.part[lb] = rr;
+    if (threadIdx.x == 0) a.part_rr[lb] = rr;
 }
 
 // du_m1 -= Dx sigma, du_m2 -= Dy sigma (the flux_igr terms, IGR.jl:225-237),
@@ -1547,7 +1546,8 @@ static IgrArgs make_igr(mf_ctx* c, const double* u)
     g.c = f + 5 * N;
     g.t1 = f + 6 * N;
     g.t2 = f + 7 * N;
-    g.part = c->d_igr_part;
+    g.part_rr = c->d_igr_part;
+    g.part_pc = c->d_igr_part + g.nblk;
     g.sc = c->d_igr_sc;
     return g;
 }
@@ -1842,14 +1842,11 @@ static int run_igr(mf_ctx* c, const double* u, const RhsArgs& a, unsigned grid)
     const IgrArgs g = make_igr(c, u);
     const dim3 G(grid), B(kBlock);
     launch_k(c, -1, k_igr_init, G, B, 0, g);
-    launch_k(c, -1, k_igr_scalars, dim3(1), B, 0, g, 0);
     for (int it = 0; it < c->igr_maxiter; ++it) {
-        launch_k(c, -1, k_igr_p, G, B, 0, g);
+        launch_k(c, -1, k_igr_p, G, B, 0, g, it);
         launch_k(c, -1, k_igr_t<5>, G, B, 0, g);
         launch_k(c, -1, k_igr_c<5>, G, B, 0, g);
-        launch_k(c, -1, k_igr_scalars, dim3(1), B, 0, g, 1);
-        launch_k(c, -1, k_igr_x, G, B, 0, g);
-        launch_k(c, -1, k_igr_scalars, dim3(1), B, 0, g, 2);
+        launch_k(c, -1, k_igr_x, G, B, 0, g, it);
     }
     launch_k(c, -1, k_igr_apply<5>, G, B, 0, g, a);
     MF_HIP(c, hipGetLastError());

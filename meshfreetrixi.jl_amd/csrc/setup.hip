@@ -458,40 +458,61 @@ __device__ double pow_k(double s, int k)
     return hi;
 }
 
-__device__ __forceinline__ void argmax_wave(double& v, int& idx)
+// Maximum over the wavefront of one double per lane (an exact selection,
+// so any combination order gives the same bits), returned to every lane:
+// a DPP max-scan within each row of 16 lanes (row_shr 1, 2, 4, 8), then
+// across rows (row_bcast 15, 31), which leaves the maximum in lane 63.
+// Lanes the shifted source does not reach take -1, below every |x|; every
+// lane of the wavefront must be active.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_shift(double v)
 {
-    for (int o = 32; o > 0; o >>= 1) {
-        const double v2 = __shfl_xor(v, o);
-        const int i2 = __shfl_xor(idx, o);
-        if (v2 > v || (v2 == v && i2 < idx)) {
-            v = v2;
-            idx = i2;
+    constexpr int kNeg1Hi = (int)0xbff00000;  // -1.0 = 0xbff00000'00000000
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(kNeg1Hi, __double2hiint(v), CTRL, ROW_MASK, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+    v = fmax(v, dpp_shift<0x111, 0xf>(v));  // row_shr:1
+    v = fmax(v, dpp_shift<0x112, 0xf>(v));  // row_shr:2
+    v = fmax(v, dpp_shift<0x114, 0xf>(v));  // row_shr:4
+    v = fmax(v, dpp_shift<0x118, 0xf>(v));  // row_shr:8
+    v = fmax(v, dpp_shift<0x142, 0xa>(v));  // row_bcast:15, rows 1 and 3
+    v = fmax(v, dpp_shift<0x143, 0xc>(v));  // row_bcast:31, rows 2 and 3
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                            __builtin_amdgcn_readlane(__double2loint(v), 63));
+}
+
+// The rank-1 and rank-2 updates of the factorization spread the flattened
+// upper triangle (packed column by column, so element f of the triangle is
+// A[f]) over the wavefront: lane l takes f = l + 64 t.  Its row and column
+// for each t are the same at every pivot step, so they are decoded once per
+// stencil into a register table, two 8-bit (row, column) pairs per word.
+// Each element gets exactly the reference's arithmetic, so the spread
+// changes no bits.
+constexpr int kTriSlots = (kMaxStencilM * (kMaxStencilM - 1) / 2 + 63) / 64;  // t per lane
+
+__device__ __forceinline__ void tri_table(int lane, uint32_t (&dec)[(kTriSlots + 1) / 2])
+{
+    int i = lane, j = 0;  // position of f = lane
+    while (i > j) {
+        i -= j + 1;
+        ++j;
+    }
+#pragma unroll
+    for (int t = 0; t < kTriSlots; ++t) {
+        const uint32_t pair = (uint32_t)i | ((uint32_t)j << 8);
+        if (t & 1)
+            dec[t >> 1] |= pair << 16;
+        else
+            dec[t >> 1] = pair;
+        i += 64;  // f + 64
+        while (i > j) {
+            i -= j + 1;
+            ++j;
         }
-    }
-}
-
-// Position (i, j), 0 <= i <= j, of the f-th element of an upper triangle
-// stored column by column (column j holds j + 1 elements), and the step to
-// the element 64 further on.  The rank-1 and rank-2 updates of the
-// factorization are spread over the wavefront in this order; each element
-// gets exactly the reference's arithmetic, so the order changes no bits.
-__device__ __forceinline__ void tri_pos(int f, int* i, int* j)
-{
-    int c = 0;
-    while (f > c) {
-        f -= c + 1;
-        ++c;
-    }
-    *i = f;
-    *j = c;
-}
-
-__device__ __forceinline__ void tri_next64(int* i, int* j)
-{
-    *i += 64;
-    while (*i > *j) {
-        *i -= *j + 1;
-        ++*j;
     }
 }
 
@@ -572,20 +593,23 @@ __global__ __launch_bounds__(64) void k_weights(const WeightsArgs a)
     const double alpha = (1.0 + sqrt(17.0)) / 8.0;
     int info = 0;
     int k = m;
-    int ti0, tj0;  // this lane's first element of the flattened upper triangle
-    tri_pos(lane, &ti0, &tj0);
+    uint32_t dec[(kTriSlots + 1) / 2];
+    tri_table(lane, dec);
     double* W = X;  // reused: 2 x ld scratch for the 2x2-pivot multipliers
     while (k >= 1) {
+        // the table is opaque to the compiler from step to step, so it is
+        // decoded where used instead of held decoded across the whole loop
+#pragma unroll
+        for (int q = 0; q < (kTriSlots + 1) / 2; ++q) asm volatile("" : "+v"(dec[q]));
         int kstep = 1;
         const double absakk = fabs(AA(k, k));
         double colmax = 0.0;
         int imax = 0;
         if (k > 1) {
-            double v = (lane < k - 1) ? fabs(AA(lane + 1, k)) : -1.0;
-            int id = (lane < k - 1) ? lane + 1 : 0x7fffffff;
-            argmax_wave(v, id);
-            colmax = v;
-            imax = id;
+            // IDAMAX: the largest magnitude, the first row holding it
+            const double v = (lane < k - 1) ? fabs(AA(lane + 1, k)) : -1.0;
+            colmax = wave_max(v);
+            imax = __ffsll((unsigned long long)__ballot(v == colmax));
         }
         int kp;
         if (fmax(absakk, colmax) == 0.0 || isnan(absakk)) {
@@ -600,8 +624,7 @@ __global__ __launch_bounds__(64) void k_weights(const WeightsArgs a)
                 double v = -1.0;
                 if (jj > imax && jj <= k) v = fabs(AA(imax, jj));
                 else if (jj < imax) v = fabs(AA(jj, imax));
-                for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-                const double rowmax = v;
+                const double rowmax = wave_max(v);
                 if (absakk >= alpha * colmax * (colmax / rowmax)) {
                     kp = k;
                 } else if (fabs(AA(imax, imax)) >= alpha * rowmax) {
@@ -639,15 +662,21 @@ __global__ __launch_bounds__(64) void k_weights(const WeightsArgs a)
             if (kstep == 1) {
                 // A(1:k-1, 1:k-1) += x x' * (-1/d), x = A(1:k-1, k) (DSYR, upper)
                 const double r1 = 1.0 / AA(k, k);
-                const int T = (k - 1) * k / 2;
-                int ti = ti0, tj = tj0;
-                for (int f = lane; f < T; f += 64) {
-                    const double xj = AA(tj + 1, k);
-                    if (xj != 0.0) {
+                const int T = (k - 1) * k / 2;  // also where column k starts
+                const double* xk = A + T;
+#pragma unroll
+                for (int t = 0; t < kTriSlots; ++t) {
+                    const int f = lane + 64 * t;
+                    if (64 * t >= T) break;
+                    if (f < T) {
+                        const uint32_t d = dec[t >> 1] >> ((t & 1) * 16);
+                        const int ti = d & 0xff, tj = (d >> 8) & 0xff;
+                        // DSYR skips a zero multiplier: the element is stored back unchanged
+                        const double xj = xk[tj], xi = xk[ti], aij = A[f];
                         const double temp = -r1 * xj;
-                        AA(ti + 1, tj + 1) = AA(ti + 1, tj + 1) + AA(ti + 1, k) * temp;
+                        const double upd = aij + xi * temp;
+                        A[f] = xj != 0.0 ? upd : aij;
                     }
-                    tri_next64(&ti, &tj);
                 }
                 __syncthreads();
                 const int i = lane + 1;
@@ -665,12 +694,18 @@ __global__ __launch_bounds__(64) void k_weights(const WeightsArgs a)
                 }
                 __syncthreads();
                 // A(1:k-2, 1:k-2) -= [A(:,k) A(:,k-1)] [wk wkm1]' (upper)
-                const int T = (k - 2) * (k - 1) / 2;
-                int ti = ti0, tj = tj0;
-                for (int f = lane; f < T; f += 64) {
-                    const int ii = ti + 1, jc = tj + 1;
-                    AA(ii, jc) = AA(ii, jc) - AA(ii, k) * W[ld + jc - 1] - AA(ii, k - 1) * W[jc - 1];
-                    tri_next64(&ti, &tj);
+                const int T = (k - 2) * (k - 1) / 2;  // also where column k-1 starts
+                const double* xkm1 = A + T;
+                const double* xk = A + (k - 1) * k / 2;
+#pragma unroll
+                for (int t = 0; t < kTriSlots; ++t) {
+                    const int f = lane + 64 * t;
+                    if (64 * t >= T) break;
+                    if (f < T) {
+                        const uint32_t d = dec[t >> 1] >> ((t & 1) * 16);
+                        const int ti = d & 0xff, tj = (d >> 8) & 0xff;
+                        A[f] = A[f] - xk[ti] * W[ld + tj] - xkm1[ti] * W[tj];
+                    }
                 }
                 __syncthreads();
                 if (j <= k - 2) {

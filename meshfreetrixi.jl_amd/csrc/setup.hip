@@ -485,6 +485,36 @@ __device__ __forceinline__ double wave_max(double v)
                             __builtin_amdgcn_readlane(__double2loint(v), 63));
 }
 
+// Lane l's value of v, to every lane (l uniform).
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// Interchange of rows p and q of B held one row per lane (p, q uniform).
+__device__ __forceinline__ void lanes_swap(double& b0, double& b1, int lane, int p, int q)
+{
+    const double p0 = lane_value(b0, p), p1 = lane_value(b1, p);
+    const double q0 = lane_value(b0, q), q1 = lane_value(b1, q);
+    if (lane == p) {
+        b0 = q0;
+        b1 = q1;
+    } else if (lane == q) {
+        b0 = p0;
+        b1 = p1;
+    }
+}
+
+// sum_{i < n} b[i] x[i], added strictly in order i = 0, 1, ... (DGEMV 'T').
+__device__ __forceinline__ double dot_rows(const double* b, const double* x, int n)
+{
+    double t = 0.0;
+#pragma unroll 8
+    for (int i = 0; i < n; ++i) t = t + b[i] * x[i];
+    return t;
+}
+
 // The rank-1 and rank-2 updates of the factorization spread the flattened
 // upper triangle (packed column by column, so element f of the triangle is
 // A[f]) over the wavefront: lane l takes f = l + 64 t.  Its row and column
@@ -727,76 +757,74 @@ __global__ __launch_bounds__(64) void k_weights(const WeightsArgs a)
     }
     if (info && lane == 0) atomicOr(a.status, 1);
     // ---- DSYTRS, UPLO = 'U', two right-hand sides
+    // First loop (U D solve, k = m down to 1): lane i holds row i + 1 of B in
+    // registers; the pivot rows reach every lane by readlane, interchanges
+    // are selects between two lanes.
+    double b0 = 0.0, b1 = 0.0;
+    if (lane < m) {
+        b0 = BB(lane + 1, 0);
+        b1 = BB(lane + 1, 1);
+    }
     k = m;
     while (k >= 1) {
-        if (ipiv[k - 1] > 0) {
-            const int kp = ipiv[k - 1];
-            if (kp != k && lane < 2) {
-                const double t = BB(k, lane);
-                BB(k, lane) = BB(kp, lane);
-                BB(kp, lane) = t;
+        const int K = k - 1;
+        const int pv = __builtin_amdgcn_readfirstlane(ipiv[K]);
+        if (pv > 0) {
+            if (pv != k) lanes_swap(b0, b1, lane, K, pv - 1);
+            const double bk0 = lane_value(b0, K), bk1 = lane_value(b1, K);
+            if (lane < K) {
+                const double aik = AA(lane + 1, k);
+                if (bk0 != 0.0) b0 = b0 + aik * (-1.0 * bk0);
+                if (bk1 != 0.0) b1 = b1 + aik * (-1.0 * bk1);
             }
-            __syncthreads();
-            const int i = lane + 1;
-            const double b0 = BB(k, 0), b1 = BB(k, 1);
-            if (i <= k - 1) {
-                if (b0 != 0.0) BB(i, 0) = BB(i, 0) + AA(i, k) * (-1.0 * b0);
-                if (b1 != 0.0) BB(i, 1) = BB(i, 1) + AA(i, k) * (-1.0 * b1);
+            const double r = 1.0 / AA(k, k);
+            if (lane == K) {
+                b0 = r * b0;
+                b1 = r * b1;
             }
-            __syncthreads();
-            if (lane < 2) {
-                const double r = 1.0 / AA(k, k);
-                BB(k, lane) = r * BB(k, lane);
-            }
-            __syncthreads();
             k -= 1;
         } else {
-            const int kp = -ipiv[k - 1];
-            if (kp != k - 1 && lane < 2) {
-                const double t = BB(k - 1, lane);
-                BB(k - 1, lane) = BB(kp, lane);
-                BB(kp, lane) = t;
+            const int kp = -pv;
+            if (kp != k - 1) lanes_swap(b0, b1, lane, K - 1, kp - 1);
+            const double bk0 = lane_value(b0, K), bk1 = lane_value(b1, K);
+            const double ck0 = lane_value(b0, K - 1), ck1 = lane_value(b1, K - 1);
+            if (lane < K - 1) {
+                const double aik = AA(lane + 1, k), aikm1 = AA(lane + 1, k - 1);
+                if (bk0 != 0.0) b0 = b0 + aik * (-1.0 * bk0);
+                if (bk1 != 0.0) b1 = b1 + aik * (-1.0 * bk1);
+                if (ck0 != 0.0) b0 = b0 + aikm1 * (-1.0 * ck0);
+                if (ck1 != 0.0) b1 = b1 + aikm1 * (-1.0 * ck1);
             }
-            __syncthreads();
-            const int i = lane + 1;
-            const double b0 = BB(k, 0), b1 = BB(k, 1);
-            const double c0v = BB(k - 1, 0), c1v = BB(k - 1, 1);
-            if (i <= k - 2) {
-                if (b0 != 0.0) BB(i, 0) = BB(i, 0) + AA(i, k) * (-1.0 * b0);
-                if (b1 != 0.0) BB(i, 1) = BB(i, 1) + AA(i, k) * (-1.0 * b1);
-                if (c0v != 0.0) BB(i, 0) = BB(i, 0) + AA(i, k - 1) * (-1.0 * c0v);
-                if (c1v != 0.0) BB(i, 1) = BB(i, 1) + AA(i, k - 1) * (-1.0 * c1v);
+            const double akm1k = AA(k - 1, k);
+            const double akm1 = AA(k - 1, k - 1) / akm1k;
+            const double ak = AA(k, k) / akm1k;
+            const double denom = akm1 * ak - 1.0;
+            if (lane == K - 1) {
+                b0 = (ak * (ck0 / akm1k) - bk0 / akm1k) / denom;
+                b1 = (ak * (ck1 / akm1k) - bk1 / akm1k) / denom;
+            } else if (lane == K) {
+                b0 = (akm1 * (bk0 / akm1k) - ck0 / akm1k) / denom;
+                b1 = (akm1 * (bk1 / akm1k) - ck1 / akm1k) / denom;
             }
-            __syncthreads();
-            if (lane < 2) {
-                const double akm1k = AA(k - 1, k);
-                const double akm1 = AA(k - 1, k - 1) / akm1k;
-                const double ak = AA(k, k) / akm1k;
-                const double denom = akm1 * ak - 1.0;
-                const double bkm1 = BB(k - 1, lane) / akm1k;
-                const double bk = BB(k, lane) / akm1k;
-                BB(k - 1, lane) = (ak * bkm1 - bk) / denom;
-                BB(k, lane) = (akm1 * bk - bkm1) / denom;
-            }
-            __syncthreads();
             k -= 2;
         }
     }
+    if (lane < m) {
+        BB(lane + 1, 0) = b0;
+        BB(lane + 1, 1) = b1;
+    }
+    __syncthreads();
+    // Second loop (U' solve, k = 1 up to m): B(k) -= A(1:k-1, k)' B(1:k-1)
+    // (DGEMV 'T', summed in row order), then the interchange; lane c < 2
+    // takes right-hand side c.
     k = 1;
     while (k <= m) {
         const bool one = ipiv[k - 1] > 0;
         if (lane < 2) {
-            double temp = 0.0;
-#pragma unroll 8
-            for (int i = 1; i <= k - 1; ++i) temp = temp + BB(i, lane) * AA(i, k);
-            const double nk = BB(k, lane) + -1.0 * temp;
+            const double* bc = &BB(1, lane);
+            const double nk = BB(k, lane) + -1.0 * dot_rows(bc, &AA(1, k), k - 1);
             double nk1 = 0.0;
-            if (!one) {
-                double temp2 = 0.0;
-#pragma unroll 8
-                for (int i = 1; i <= k - 1; ++i) temp2 = temp2 + BB(i, lane) * AA(i, k + 1);
-                nk1 = BB(k + 1, lane) + -1.0 * temp2;
-            }
+            if (!one) nk1 = BB(k + 1, lane) + -1.0 * dot_rows(bc, &AA(1, k + 1), k - 1);
             BB(k, lane) = nk;
             if (!one) BB(k + 1, lane) = nk1;
             const int kp = one ? ipiv[k - 1] : -ipiv[k - 1];

@@ -245,7 +245,8 @@ int ensure_state(mf_ctx* c)
             if ((rc = dalloc(c, &c->d_partials, 4 * (size_t)(c->n_psum + c->n_pmax)))) return rc;
         }
     }
-    if (!c->d_lim && (rc = dalloc(c, &c->d_lim, nv))) return rc;
+    // limiter scratch: nvar N blended values, then N flag bytes
+    if (!c->d_lim && (rc = dalloc(c, &c->d_lim, nv + (size_t)(c->N + 7) / 8))) return rc;
     return MF_OK;
 }
 

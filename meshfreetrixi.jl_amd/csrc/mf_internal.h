@@ -164,7 +164,7 @@ struct mf_ctx {
     int lim_n = 0;
     double lim_thr[2] = {0, 0};
     int lim_var[2] = {0, 0};
-    double* d_lim = nullptr;  // 2 nvar N scratch
+    double* d_lim = nullptr;  // limiter scratch: nvar N values, then N flag bytes
 
     // graph of one SSPRK33 step
     // graphs of one step and of kGraphSteps consecutive steps (fewer launches)

@@ -1250,24 +1250,45 @@ __device__ __forceinline__ double limiter_variable(int var, double gamma, const 
     return (gamma - 1.0) * (s[3] - 0.5 * (s[1] * s[1] + s[2] * s[2]) / s[0]);
 }
 
+// Stencil entries per load batch of the minimum: the batch's index loads,
+// then its neighbour loads, are all in flight together.
+constexpr int kLimBatch = 10;
+
 template <int V>
 __global__ __launch_bounds__(kBlock) void k_limiter_blend(int64_t N, int K, const int32_t* idx,
-                                                          const double* u, double* loc, double thr,
-                                                          int var, double gamma)
+                                                          const double* u, double* loc, uint8_t* flag,
+                                                          double thr, int var, double gamma)
 {
     const int64_t i = remapped_node();
     if (i >= N) return;
+    // the limited variable of every stencil node: density alone needs only
+    // the first component of each neighbour
+    const bool dens = var == MF_VAR_DENSITY || V == 1;
     double vmin = INFINITY;
-    for (int s = 0; s < K; ++s) {
-        double uj[V];
-        load_state<V>(u, idx[(int64_t)s * N + i], uj);
-        const double val = limiter_variable<V>(var, gamma, uj);
-        vmin = (val < vmin) ? val : vmin;
-    }
-    double o[V];
+#pragma unroll 1
+    for (int s0 = 0; s0 < K; s0 += kLimBatch) {
+        int32_t j[kLimBatch];
 #pragma unroll
-    for (int v = 0; v < V; ++v) o[v] = 0.0;
+        for (int b = 0; b < kLimBatch; ++b) j[b] = (s0 + b < K) ? idx[(int64_t)(s0 + b) * N + i] : (int32_t)i;
+        double val[kLimBatch];
+        if (dens) {
+#pragma unroll
+            for (int b = 0; b < kLimBatch; ++b) val[b] = u[(int64_t)j[b] * V];
+        } else {
+#pragma unroll
+            for (int b = 0; b < kLimBatch; ++b) {
+                double uj[V];
+                load_state<V>(u, j[b], uj);
+                val[b] = limiter_variable<V>(var, gamma, uj);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < kLimBatch; ++b)
+            if (s0 + b < K) vmin = (val[b] < vmin) ? val[b] : vmin;
+    }
+    uint8_t f = 0;
     if (vmin < thr) {
+        double o[V];
         double um[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) um[v] = 0.0;
@@ -1285,21 +1306,27 @@ __global__ __launch_bounds__(kBlock) void k_limiter_blend(int64_t N, int K, cons
         load_state<V>(u, i, ui);
 #pragma unroll
         for (int v = 0; v < V; ++v) o[v] = fma(theta, ui[v], (1.0 - theta) * um[v]);
+        // only non-zero rows are copied back (a row of zeros stays unwritten)
+        bool nz = false;
+#pragma unroll
+        for (int v = 0; v < V; ++v) nz |= (o[v] != 0.0);
+        if (nz) {
+            store_state<V>(loc, i, o);
+            f = 1;
+        }
     }
-    store_state<V>(loc, i, o);
+    flag[i] = f;
 }
 
 template <int V>
-__global__ __launch_bounds__(kBlock) void k_limiter_commit(int64_t N, const double* loc, double* u)
+__global__ __launch_bounds__(kBlock) void k_limiter_commit(int64_t N, const double* loc, const uint8_t* flag,
+                                                           double* u)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+    if (i >= N || !flag[i]) return;
     double o[V];
     load_state<V>(loc, i, o);
-    bool nz = false;
-#pragma unroll
-    for (int v = 0; v < V; ++v) nz |= (o[v] != 0.0);
-    if (nz) store_state<V>(u, i, o);
+    store_state<V>(u, i, o);
 }
 
 __global__ void k_axpby(int64_t n, double a, const double* x, double b, double* y)
@@ -1967,16 +1994,18 @@ int run_limiter(mf_ctx* c, double* d_u, double thr, int var)
     const int64_t nblk = (c->N + kBlock - 1) / kBlock;
     const unsigned grid = xcd_grid(nblk);
     const double gam = c->p0;
+    // blended rows, then one flag byte per node (the buffer's tail)
+    uint8_t* flag = reinterpret_cast<uint8_t*>(c->d_lim + (int64_t)c->nvar * c->N);
     if (c->nvar == 4) {
         hipLaunchKernelGGL(k_limiter_blend<4>, dim3(grid), dim3(kBlock), 0, c->stream, c->N, c->k,
-                           c->d_idx, d_u, c->d_lim, thr, var, gam);
+                           c->d_idx, d_u, c->d_lim, flag, thr, var, gam);
         hipLaunchKernelGGL(k_limiter_commit<4>, dim3((unsigned)nblk), dim3(kBlock), 0, c->stream,
-                           c->N, c->d_lim, d_u);
+                           c->N, c->d_lim, flag, d_u);
     } else {
         hipLaunchKernelGGL(k_limiter_blend<1>, dim3(grid), dim3(kBlock), 0, c->stream, c->N, c->k,
-                           c->d_idx, d_u, c->d_lim, thr, var, gam);
+                           c->d_idx, d_u, c->d_lim, flag, thr, var, gam);
         hipLaunchKernelGGL(k_limiter_commit<1>, dim3((unsigned)nblk), dim3(kBlock), 0, c->stream,
-                           c->N, c->d_lim, d_u);
+                           c->N, c->d_lim, flag, d_u);
     }
     MF_HIP(c, hipGetLastError());
     return MF_OK;

@@ -417,6 +417,54 @@ def test_vjp_matches_finite_differences():
     assert abs(lhs - rhs) < 1e-6 * abs(rhs)
 
 
+def _euler_jvp(c, u, v, gamma=1.4):
+    """Forward-mode J v of du = -(Dx F1(u) + Dy F2(u)) for the Euler flux,
+    by the chain rule through the textbook flux (v1 = m1/rho, v2 = m2/rho,
+    p = (gamma-1)(E - (m1 v1 + m2 v2)/2), F1 = (m1, m1 v1 + p, m2 v1,
+    (E + p) v1), F2 likewise): a derivation independent of the oracle's VJP."""
+    rho, m1, m2, E = u
+    dr, d1, d2, dE = v
+    v1, v2 = m1 / rho, m2 / rho
+    dv1, dv2 = (d1 - v1 * dr) / rho, (d2 - v2 * dr) / rho
+    p = (gamma - 1.0) * (E - 0.5 * (m1 * v1 + m2 * v2))
+    dp = (gamma - 1.0) * (dE - 0.5 * (d1 * v1 + m1 * dv1 + d2 * v2 + m2 * dv2))
+    dF1 = np.stack([d1, d1 * v1 + m1 * dv1 + dp, d2 * v1 + m2 * dv1, (dE + dp) * v1 + (E + p) * dv1])
+    dF2 = np.stack([d2, d1 * v2 + m1 * dv2, d2 * v2 + m2 * dv2 + dp, (dE + dp) * v2 + (E + p) * dv2])
+    return -(np.einsum("is,vis->vi", c.wx, dF1[:, c.idx]) + np.einsum("is,vis->vi", c.wy, dF2[:, c.idx]))
+
+
+@pytest.mark.parametrize("bc", [("nothing", "nothing"), ("dirichlet", "nothing")])
+def test_vjp_matches_forward_mode(bc):
+    """Dot-product test <lam, J v> = <J' lam, v> at rounding level: the
+    oracle's VJP against the forward-mode J v above.  A Dirichlet node's
+    state is overwritten by the boundary value (so it does not depend on u)
+    and its du is zeroed, so v and J v vanish there."""
+    c = cases.build(configs.c2(), side=20, use_gpu=False, bc=bc)
+    rng = np.random.default_rng(7)
+    lam = rng.standard_normal(c.u0.shape)
+    v = rng.standard_normal(c.u0.shape)
+    ub = c.prob.vjp(c.u0.copy(), lam, 0.0)
+    ubc = c.u0.copy()
+    c.prob.rhs(ubc, 0.0)  # applies the first boundary pass to ubc in place
+    fixed = c.bc[0][1] if bc[0] == "dirichlet" else np.zeros(0, np.int64)
+    vv = v.copy()
+    vv[:, fixed] = 0.0
+    jv = _euler_jvp(c, ubc, vv)
+    jv[:, fixed] = 0.0
+    lhs, rhs = float((lam * jv).sum()), float((ub * v).sum())
+    scale = float(np.abs(lam * jv).sum())
+    assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs, scale)
+    # and node by node: the VJP is the transpose of the same linear map
+    e = np.zeros_like(v)
+    q = int(rng.integers(c.N))
+    e[1, q] = 1.0
+    ee = e.copy()
+    ee[:, fixed] = 0.0
+    jq = _euler_jvp(c, ubc, ee)
+    jq[:, fixed] = 0.0
+    assert abs(float((lam * jq).sum()) - ub[1, q]) <= 1e-12 * float(np.abs(lam * jq).sum() + 1e-300)
+
+
 def test_tominec_hyperviscosity_matches_explicit_product():
     """SourceHyperviscosityTominec (hyperviscosity.jl:80-134): the reference
     builds H = lap' * lap explicitly (lap = Dxx + Dyy) and adds -gamma H u.

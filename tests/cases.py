@@ -122,3 +122,19 @@ def build(spec, side=None, use_gpu=True, weights="oracle", hv=None, visc=None, b
         ctx.set_source_igr(igr, igr_maxiter)
     ctx.set_source_order([names[n][1] for n in order])
     return c
+
+
+def euler_jvp(c, u, v, gamma=1.4):
+    """Forward-mode J v of du = -(Dx F1(u) + Dy F2(u)) for the Euler flux,
+    by the chain rule through the textbook flux (v1 = m1/rho, v2 = m2/rho,
+    p = (gamma-1)(E - (m1 v1 + m2 v2)/2), F1 = (m1, m1 v1 + p, m2 v1,
+    (E + p) v1), F2 likewise): a derivation independent of the oracle's VJP."""
+    rho, m1, m2, E = u
+    dr, d1, d2, dE = v
+    v1, v2 = m1 / rho, m2 / rho
+    dv1, dv2 = (d1 - v1 * dr) / rho, (d2 - v2 * dr) / rho
+    p = (gamma - 1.0) * (E - 0.5 * (m1 * v1 + m2 * v2))
+    dp = (gamma - 1.0) * (dE - 0.5 * (d1 * v1 + m1 * dv1 + d2 * v2 + m2 * dv2))
+    dF1 = np.stack([d1, d1 * v1 + m1 * dv1 + dp, d2 * v1 + m2 * dv1, (dE + dp) * v1 + (E + p) * dv1])
+    dF2 = np.stack([d2, d1 * v2 + m1 * dv2, d2 * v2 + m2 * dv2 + dp, (dE + dp) * v2 + (E + p) * dv2])
+    return -(np.einsum("is,vis->vi", c.wx, dF1[:, c.idx]) + np.einsum("is,vis->vi", c.wy, dF2[:, c.idx]))

@@ -417,22 +417,6 @@ def test_vjp_matches_finite_differences():
     assert abs(lhs - rhs) < 1e-6 * abs(rhs)
 
 
-def _euler_jvp(c, u, v, gamma=1.4):
-    """Forward-mode J v of du = -(Dx F1(u) + Dy F2(u)) for the Euler flux,
-    by the chain rule through the textbook flux (v1 = m1/rho, v2 = m2/rho,
-    p = (gamma-1)(E - (m1 v1 + m2 v2)/2), F1 = (m1, m1 v1 + p, m2 v1,
-    (E + p) v1), F2 likewise): a derivation independent of the oracle's VJP."""
-    rho, m1, m2, E = u
-    dr, d1, d2, dE = v
-    v1, v2 = m1 / rho, m2 / rho
-    dv1, dv2 = (d1 - v1 * dr) / rho, (d2 - v2 * dr) / rho
-    p = (gamma - 1.0) * (E - 0.5 * (m1 * v1 + m2 * v2))
-    dp = (gamma - 1.0) * (dE - 0.5 * (d1 * v1 + m1 * dv1 + d2 * v2 + m2 * dv2))
-    dF1 = np.stack([d1, d1 * v1 + m1 * dv1 + dp, d2 * v1 + m2 * dv1, (dE + dp) * v1 + (E + p) * dv1])
-    dF2 = np.stack([d2, d1 * v2 + m1 * dv2, d2 * v2 + m2 * dv2 + dp, (dE + dp) * v2 + (E + p) * dv2])
-    return -(np.einsum("is,vis->vi", c.wx, dF1[:, c.idx]) + np.einsum("is,vis->vi", c.wy, dF2[:, c.idx]))
-
-
 @pytest.mark.parametrize("bc", [("nothing", "nothing"), ("dirichlet", "nothing")])
 def test_vjp_matches_forward_mode(bc):
     """Dot-product test <lam, J v> = <J' lam, v> at rounding level: the
@@ -449,7 +433,7 @@ def test_vjp_matches_forward_mode(bc):
     fixed = c.bc[0][1] if bc[0] == "dirichlet" else np.zeros(0, np.int64)
     vv = v.copy()
     vv[:, fixed] = 0.0
-    jv = _euler_jvp(c, ubc, vv)
+    jv = cases.euler_jvp(c, ubc, vv)
     jv[:, fixed] = 0.0
     lhs, rhs = float((lam * jv).sum()), float((ub * v).sum())
     scale = float(np.abs(lam * jv).sum())
@@ -460,7 +444,7 @@ def test_vjp_matches_forward_mode(bc):
     e[1, q] = 1.0
     ee = e.copy()
     ee[:, fixed] = 0.0
-    jq = _euler_jvp(c, ubc, ee)
+    jq = cases.euler_jvp(c, ubc, ee)
     jq[:, fixed] = 0.0
     assert abs(float((lam * jq).sum()) - ub[1, q]) <= 1e-12 * float(np.abs(lam * jq).sum() + 1e-300)
 

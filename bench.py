@@ -333,6 +333,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip adjoint and batched weights")
+    ap.add_argument("--weights-only", action="store_true",
+                    help="time only the batched weights (1,048,576 ragged stencils) and print their JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one GPU); "
@@ -341,6 +343,10 @@ def main():
     import mfrbf  # noqa: F401
     from mfrbf import _lib as L
     from mfrbf import configs, tune
+
+    if args.weights_only:
+        print(json.dumps(weights_batch(L, configs, 1024)), flush=True)
+        return
 
     spec = configs.c2(args.side)
     log("setup: %d x %d cloud, k=%d" % (args.side, args.side, spec.k))

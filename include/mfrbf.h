@@ -113,7 +113,8 @@ int mf_synchronize(mf_ctx* ctx);
 
 /* --------------------------------------------------------------- cloud */
 /* Node coordinates.  Replaces the `points` of PointData
- * (domains/PointCloudDomain/geometry_primatives.jl:297-304, :322-339). */
+ * (domains/PointCloudDomain/geometry_primatives.jl:297-304, :322-339).
+ * MF_ERR_ARGUMENT for fewer than 2 points or a non-finite coordinate. */
 int mf_set_points(mf_ctx* ctx, const double* x, const double* y, int64_t n_points);
 
 /* k nearest neighbours of every node (self first), ordered by squared

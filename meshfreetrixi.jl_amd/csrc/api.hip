@@ -362,6 +362,10 @@ int mf_set_points(mf_ctx* c, const double* x, const double* y, int64_t n)
     if (rc) return rc;
     if (!x || !y || n < 2) return fail(c, MF_ERR_ARGUMENT, "mf_set_points: need >= 2 points");
     if (n > (int64_t)INT32_MAX) return fail(c, MF_ERR_ARGUMENT, "mf_set_points: too many points");
+    // the cell grid and the node order are built from the bounding box
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(x[i]) || !std::isfinite(y[i]))
+            return fail(c, MF_ERR_ARGUMENT, "mf_set_points: non-finite coordinate");
     MF_HIP(c, hipStreamSynchronize(c->stream));
     // a new cloud (or a new order) invalidates everything built on the old one
     free_state(c);

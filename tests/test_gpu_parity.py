@@ -359,6 +359,13 @@ def test_api_rejects_bad_input(mf):
     ctx = L.Context(0)
     assert code(lambda: ctx.knn(8)) == L.MF_ERR_STATE  # no points yet
     assert code(lambda: ctx.set_points(x[:1], y[:1])) == L.MF_ERR_ARGUMENT
+    xn = x.copy()
+    xn[5] = np.nan
+    assert code(lambda: ctx.set_points(xn, y)) == L.MF_ERR_ARGUMENT
+    xn[5] = np.inf
+    assert code(lambda: ctx.set_points(xn, y)) == L.MF_ERR_ARGUMENT
+    ctx.set_points(x[:20], y[:20])
+    assert code(lambda: ctx.knn(21)) == L.MF_ERR_ARGUMENT  # k > N
     ctx.set_equations(L.MF_EQ_ADVECTION_2D, 1.0, 1.0)
     ctx.set_points(x, y)
     assert code(lambda: ctx.knn(65)) == L.MF_ERR_ARGUMENT

@@ -164,6 +164,31 @@ def test_limiter_vs_oracle(mf):
         assert rel_inf(ug, uo) < 1e-14
 
 
+@pytest.mark.parametrize("k", [15, 13])
+def test_limiter_scalar_vs_oracle(mf, k):
+    """The limiter on a scalar state (advection, C1 stencils; the variable
+    is the state itself), with a stencil size that is not a multiple of the
+    10-entry load batch."""
+    _say("zhang-shu limiter scalar k=%d" % k)
+    c = cases.build(configs.c1(), side=40)
+    if k != c.k:
+        idx, _ = O.knn_grid(c.x, c.y, k)
+        c.ctx.set_stencil(idx)
+        c.idx, c.k = idx, k
+    u = c.u0.copy()
+    rng = np.random.default_rng(4)
+    bad = rng.choice(c.N, 40, replace=False)
+    u[0, bad] = -0.2
+    ptr = O.ell_ptr(c.N, c.k)
+    ug = u.copy()
+    c.ctx.upload(L.MF_BUF_U, ug)
+    c.ctx.limiter_zhang_shu(0.0, L.MF_VAR_DENSITY)
+    ug = c.ctx.download(L.MF_BUF_U)
+    uo = O.limiter_zhang_shu(u.copy(), ptr, c.idx, 0.0, L.MF_VAR_DENSITY)
+    assert np.any(uo != u)
+    assert rel_inf(ug, uo) < 1e-14
+
+
 def test_ssprk33_stage_limiter_vs_oracle(mf):
     _say("ssprk33 with stage limiter")
     c = cases.build(configs.c2(), side=40)

@@ -1139,41 +1139,60 @@ __global__ void k_advance_time(StepScalars* sc, double dt)
 // HistoryCallback (history.jl:207-268).  Single thread: shift the time
 // history, count the points in use and solve the transposed Vandermonde
 // system for the derivative weights (partial-pivoting LU, as `A' \ b`).
-__global__ void k_history_weights(StepScalars* sc, int P, int init)
+// One thread.  The step scalars are read once into registers, the
+// shifted history and the weights are computed there, and the results are
+// stored once: every global access in a dependent chain would cost a
+// memory round trip.
+__global__ __launch_bounds__(64) void k_history_weights(StepScalars* sc, int P, int init)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (init) {
-        for (int i = 0; i < kMaxHist; ++i) sc->time_hist[i] = 0.0;
-        sc->hist_head = 0;
-    } else {
-        sc->hist_head = (sc->hist_head + P - 1) % P;
-    }
-    for (int i = P - 1; i >= 1; --i) sc->time_hist[i] = sc->time_hist[i - 1];
-    sc->time_hist[0] = sc->t;
+    double th[kMaxHist];
+#pragma unroll
+    for (int i = 0; i < kMaxHist; ++i) th[i] = init ? 0.0 : sc->time_hist[i];
+    const int head = init ? 0 : (sc->hist_head + P - 1) % P;
+    const double tnow = sc->t;
     const int si = sc->success_iter;
-    sc->hist_si = si;
+#pragma unroll
+    for (int i = kMaxHist - 1; i >= 1; --i)
+        if (i < P) th[i] = th[i - 1];
+    th[0] = tnow;
     const int q = (si + 1 < P) ? si + 1 : P;
+#pragma unroll
+    for (int i = 0; i < kMaxHist; ++i) sc->time_hist[i] = th[i];
+    sc->hist_head = head;
+    sc->hist_si = si;
     sc->q = (si > 0) ? q : 0;
     if (si <= 0) return;
-    double A[kMaxHist * kMaxHist], b[kMaxHist], ts[kMaxHist];
+    // Vandermonde transpose at the scaled times, A(kk, i) = ts_i^kk, and
+    // its LU solve with partial pivoting.  Every index is static (loops run
+    // to kMaxHist, guarded by q), so the arrays stay in registers.
+    constexpr int H = kMaxHist;
+    double A[H][H], b[H], ts[H];
     double mx = 0.0;
-    for (int i = 0; i < q; ++i) mx = fmax(mx, fabs(sc->time_hist[i]));
+#pragma unroll
+    for (int i = 0; i < H; ++i)
+        if (i < q) mx = fmax(mx, fabs(th[i]));
     const double scale = 1.0 / mx;
-    for (int i = 0; i < q; ++i) ts[i] = sc->time_hist[i] * scale;
+#pragma unroll
+    for (int i = 0; i < H; ++i) ts[i] = i < q ? th[i] * scale : 0.0;
     const double te = ts[0];
-    for (int kk = 0; kk < q; ++kk) {
-        for (int i = 0; i < q; ++i) {
+#pragma unroll
+    for (int kk = 0; kk < H; ++kk) {
+#pragma unroll
+        for (int i = 0; i < H; ++i) {
             // ts^kk, nearly correctly rounded (double-double)
             double hi = 1.0, lo = 0.0;
+#pragma unroll
             for (int r = 0; r < kk; ++r) {
                 const double ph = hi * ts[i];
                 const double pl = fma(hi, ts[i], -ph) + lo * ts[i];
                 hi = ph + pl;
                 lo = pl - (hi - ph);
             }
-            A[kk + i * q] = hi;
+            A[kk][i] = hi;
         }
         double hi = 1.0, lo = 0.0;
+#pragma unroll
         for (int r = 0; r + 1 < kk; ++r) {
             const double ph = hi * te;
             const double pl = fma(hi, te, -ph) + lo * te;
@@ -1182,41 +1201,72 @@ __global__ void k_history_weights(StepScalars* sc, int P, int init)
         }
         b[kk] = (kk == 0) ? 0.0 : (double)kk * hi;
     }
-    int piv[kMaxHist];
-    for (int j = 0; j < q; ++j) {
+    int piv[H];
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+        piv[j] = j;
+        if (j >= q) continue;
         int p = j;
-        double pm = fabs(A[j + j * q]);
-        for (int i = j + 1; i < q; ++i)
-            if (fabs(A[i + j * q]) > pm) {
-                pm = fabs(A[i + j * q]);
+        double pm = fabs(A[j][j]);
+#pragma unroll
+        for (int i = j + 1; i < H; ++i)
+            if (i < q && fabs(A[i][j]) > pm) {
+                pm = fabs(A[i][j]);
                 p = i;
             }
         piv[j] = p;
-        if (p != j)
-            for (int c2 = 0; c2 < q; ++c2) {
-                const double tmp = A[j + c2 * q];
-                A[j + c2 * q] = A[p + c2 * q];
-                A[p + c2 * q] = tmp;
+        if (p != j) {
+#pragma unroll
+            for (int c2 = 0; c2 < H; ++c2) {
+                const double ajc = A[j][c2];
+                double apc = ajc;
+#pragma unroll
+                for (int r = j + 1; r < H; ++r)
+                    if (r == p) apc = A[r][c2];
+                A[j][c2] = apc;
+#pragma unroll
+                for (int r = j + 1; r < H; ++r)
+                    if (r == p) A[r][c2] = ajc;
             }
-        const double r = 1.0 / A[j + j * q];
-        for (int i = j + 1; i < q; ++i) A[i + j * q] = A[i + j * q] * r;
-        for (int c2 = j + 1; c2 < q; ++c2)
-            for (int i = j + 1; i < q; ++i)
-                A[i + c2 * q] = __dsub_rn(A[i + c2 * q], __dmul_rn(A[i + j * q], A[j + c2 * q]));
-    }
-    for (int j = 0; j < q; ++j)
-        if (piv[j] != j) {
-            const double tmp = b[j];
-            b[j] = b[piv[j]];
-            b[piv[j]] = tmp;
         }
-    for (int j = 0; j < q; ++j)
-        for (int i = j + 1; i < q; ++i) b[i] = __dsub_rn(b[i], __dmul_rn(A[i + j * q], b[j]));
-    for (int j = q - 1; j >= 0; --j) {
-        b[j] = b[j] / A[j + j * q];
-        for (int i = 0; i < j; ++i) b[i] = __dsub_rn(b[i], __dmul_rn(A[i + j * q], b[j]));
+        const double r = 1.0 / A[j][j];
+#pragma unroll
+        for (int i = j + 1; i < H; ++i)
+            if (i < q) A[i][j] = A[i][j] * r;
+#pragma unroll
+        for (int c2 = j + 1; c2 < H; ++c2)
+#pragma unroll
+            for (int i = j + 1; i < H; ++i)
+                if (c2 < q && i < q) A[i][c2] = __dsub_rn(A[i][c2], __dmul_rn(A[i][j], A[j][c2]));
     }
-    for (int i = 0; i < q; ++i) sc->tw[i] = scale * b[i];
+#pragma unroll
+    for (int j = 0; j < H; ++j)
+        if (j < q && piv[j] != j) {
+            const double bj = b[j];
+            double bp = bj;
+#pragma unroll
+            for (int r = j + 1; r < H; ++r)
+                if (r == piv[j]) bp = b[r];
+            b[j] = bp;
+#pragma unroll
+            for (int r = j + 1; r < H; ++r)
+                if (r == piv[j]) b[r] = bj;
+        }
+#pragma unroll
+    for (int j = 0; j < H; ++j)
+#pragma unroll
+        for (int i = j + 1; i < H; ++i)
+            if (j < q && i < q) b[i] = __dsub_rn(b[i], __dmul_rn(A[i][j], b[j]));
+#pragma unroll
+    for (int j = H - 1; j >= 0; --j) {
+        if (j >= q) continue;
+        b[j] = b[j] / A[j][j];
+#pragma unroll
+        for (int i = 0; i < j; ++i) b[i] = __dsub_rn(b[i], __dmul_rn(A[i][j], b[j]));
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i)
+        if (i < q) sc->tw[i] = scale * b[i];
 }
 
 // Snapshot u into the ring slot of the most recent entry and rebuild

@@ -1113,8 +1113,11 @@ static int ssprk_step(mf_ctx* c, double dt, int prec, bool limiter, bool history
     s1.dt = dt;
     s1.out = c->d_S;
     if ((rc = run_rhs(c, prec, U, c->d_buf[MF_BUF_DU], dt, s1))) return rc;
-    if ((rc = run_advance_time(c, dt))) return rc;
-    if (history && (rc = run_history(c, U, 0))) return rc;
+    if (history) {
+        if ((rc = run_history(c, U, 0, dt))) return rc;  // also advances the time
+    } else if ((rc = run_advance_time(c, dt))) {
+        return rc;
+    }
     return MF_OK;
 }
 

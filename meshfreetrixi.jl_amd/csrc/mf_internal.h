@@ -295,7 +295,9 @@ enum RhsPart { RHS_BC = 1, RHS_REDUCE = 2, RHS_FLUX = 4, RHS_TRANSPOSE = 8, RHS_
 int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const StageOut& st,
                   int parts);
 int run_set_time(mf_ctx* c, double t, int success_iter);
-int run_history(mf_ctx* c, const double* d_u, int init);
+// History callback; with advance_dt != 0 it first ends the step (t += dt,
+// success_iter += 1) in the same launch.
+int run_history(mf_ctx* c, const double* d_u, int init, double advance_dt = 0.0);
 int run_advance_time(mf_ctx* c, double dt);
 int run_limiter(mf_ctx* c, double* d_u, double thr, int var);
 int run_axpby(mf_ctx* c, int64_t n, double a, const double* x, double b, double* y);

@@ -1142,16 +1142,23 @@ __global__ void k_advance_time(StepScalars* sc, double dt)
 // One thread.  The step scalars are read once into registers, the
 // shifted history and the weights are computed there, and the results are
 // stored once: every global access in a dependent chain would cost a
-// memory round trip.
-__global__ __launch_bounds__(64) void k_history_weights(StepScalars* sc, int P, int init)
+// memory round trip.  With `advance` set it first ends the step as
+// k_advance_time does (t += dt, success_iter += 1), saving that launch.
+__global__ __launch_bounds__(64) void k_history_weights(StepScalars* sc, int P, int init, int advance, double dt)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double th[kMaxHist];
 #pragma unroll
     for (int i = 0; i < kMaxHist; ++i) th[i] = init ? 0.0 : sc->time_hist[i];
     const int head = init ? 0 : (sc->hist_head + P - 1) % P;
-    const double tnow = sc->t;
-    const int si = sc->success_iter;
+    double tnow = sc->t;
+    int si = sc->success_iter;
+    if (advance) {
+        tnow = tnow + dt;
+        si = si + 1;
+        sc->t = tnow;
+        sc->success_iter = si;
+    }
 #pragma unroll
     for (int i = kMaxHist - 1; i >= 1; --i)
         if (i < P) th[i] = th[i - 1];
@@ -2028,10 +2035,11 @@ int run_advance_time(mf_ctx* c, double dt)
     return MF_OK;
 }
 
-int run_history(mf_ctx* c, const double* d_u, int init)
+int run_history(mf_ctx* c, const double* d_u, int init, double advance_dt)
 {
     const int P = c->approx_order + 1;
-    hipLaunchKernelGGL(k_history_weights, dim3(1), dim3(1), 0, c->stream, c->d_sc, P, init);
+    hipLaunchKernelGGL(k_history_weights, dim3(1), dim3(1), 0, c->stream, c->d_sc, P, init,
+                       advance_dt != 0.0 ? 1 : 0, advance_dt);
     const int64_t NV = c->N * c->nvar;
     hipLaunchKernelGGL(k_history_apply, dim3((unsigned)((NV + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                        c->stream, NV, P, d_u, c->d_hist, c->d_approx, c->d_sc);

@@ -377,11 +377,6 @@ __device__ __forceinline__ int64_t stage_halo_a(const RhsArgs& a, double* lds)
     }
 }
 
-// Batches of BT stencil entries: every load of a batch (indices, weights,
-// then the gathered states) is issued before any of it is used, so BT
-// gathers are in flight per thread.  Entries past K read a valid node with
-// weight 0.  TL: neighbour states come from the LDS halo, addressed by the
-// 16-bit halo positions la (same slot-major layout as idx).
 // Batches of BT stencil entries: every load of a batch (positions,
 // weights, then the gathered states) is issued before any of it is used,
 // so BT gathers are in flight per thread.  Entries past K read a valid node
@@ -1311,6 +1306,45 @@ __device__ __forceinline__ double limiter_variable(int var, double gamma, const 
 // then its neighbour loads, are all in flight together.
 constexpr int kLimBatch = 10;
 
+// Node i with stencil minimum vmin below the threshold: the theta-blend with
+// the stencil mean (in stencil order) into loc, and the row's flag byte.
+template <int V>
+__device__ __forceinline__ void limiter_finish(int64_t N, int K, const int32_t* idx, const double* u,
+                                               double* loc, uint8_t* flag, double thr, int var, double gamma,
+                                               int64_t i, double vmin)
+{
+    uint8_t f = 0;
+    if (vmin < thr) {
+        double o[V];
+        double um[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) um[v] = 0.0;
+        for (int s = 0; s < K; ++s) {
+            double uj[V];
+            load_state<V>(u, idx[(int64_t)s * N + i], uj);
+#pragma unroll
+            for (int v = 0; v < V; ++v) um[v] = um[v] + uj[v];
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) um[v] = um[v] / (double)K;
+        const double vmean = limiter_variable<V>(var, gamma, um);
+        const double theta = (vmean - thr) / (vmean - vmin);
+        double ui[V];
+        load_state<V>(u, i, ui);
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[v] = fma(theta, ui[v], (1.0 - theta) * um[v]);
+        // only non-zero rows are copied back (a row of zeros stays unwritten)
+        bool nz = false;
+#pragma unroll
+        for (int v = 0; v < V; ++v) nz |= (o[v] != 0.0);
+        if (nz) {
+            store_state<V>(loc, i, o);
+            f = 1;
+        }
+    }
+    flag[i] = f;
+}
+
 template <int V>
 __global__ __launch_bounds__(kBlock) void k_limiter_blend(int64_t N, int K, const int32_t* idx,
                                                           const double* u, double* loc, uint8_t* flag,
@@ -1343,36 +1377,50 @@ __global__ __launch_bounds__(kBlock) void k_limiter_blend(int64_t N, int K, cons
         for (int b = 0; b < kLimBatch; ++b)
             if (s0 + b < K) vmin = (val[b] < vmin) ? val[b] : vmin;
     }
-    uint8_t f = 0;
-    if (vmin < thr) {
-        double o[V];
-        double um[V];
-#pragma unroll
-        for (int v = 0; v < V; ++v) um[v] = 0.0;
-        for (int s = 0; s < K; ++s) {
+    limiter_finish<V>(N, K, idx, u, loc, flag, thr, var, gamma, i, vmin);
+}
+
+// The same with the tile's stencil halo in LDS (the tiles of the stencil
+// kernel): the limited variable of every halo node is evaluated once into
+// LDS and the minimum reads it through the 16-bit halo positions la.  Same
+// values, same order, same bits.
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_limiter_blend_tl(int64_t N, int K, const int32_t* idx,
+                                                             const uint16_t* la, int64_t n_tiles,
+                                                             const int64_t* ha_off, const int32_t* ha_ids,
+                                                             const double* u, double* loc, uint8_t* flag,
+                                                             double thr, int var, double gamma)
+{
+    extern __shared__ double lvar[];
+    const int64_t t = remapped_block();
+    if (t >= n_tiles) return;  // whole block
+    const bool dens = var == MF_VAR_DENSITY || V == 1;
+    const int64_t h0 = ha_off[t];
+    const int nh = (int)(ha_off[t + 1] - h0);
+    for (int q = threadIdx.x; q < nh; q += kBlock) {
+        const int32_t j = ha_ids[h0 + q];
+        if (dens) {
+            lvar[q] = u[(int64_t)j * V];
+        } else {
             double uj[V];
-            load_state<V>(u, idx[(int64_t)s * N + i], uj);
-#pragma unroll
-            for (int v = 0; v < V; ++v) um[v] = um[v] + uj[v];
-        }
-#pragma unroll
-        for (int v = 0; v < V; ++v) um[v] = um[v] / (double)K;
-        const double vmean = limiter_variable<V>(var, gamma, um);
-        const double theta = (vmean - thr) / (vmean - vmin);
-        double ui[V];
-        load_state<V>(u, i, ui);
-#pragma unroll
-        for (int v = 0; v < V; ++v) o[v] = fma(theta, ui[v], (1.0 - theta) * um[v]);
-        // only non-zero rows are copied back (a row of zeros stays unwritten)
-        bool nz = false;
-#pragma unroll
-        for (int v = 0; v < V; ++v) nz |= (o[v] != 0.0);
-        if (nz) {
-            store_state<V>(loc, i, o);
-            f = 1;
+            load_state<V>(u, j, uj);
+            lvar[q] = limiter_variable<V>(var, gamma, uj);
         }
     }
-    flag[i] = f;
+    __syncthreads();
+    const int64_t i = t * kBlock + threadIdx.x;
+    if (i >= N) return;
+    double vmin = INFINITY;
+#pragma unroll 1
+    for (int s0 = 0; s0 < K; s0 += kLimBatch) {
+        double val[kLimBatch];
+#pragma unroll
+        for (int b = 0; b < kLimBatch; ++b) val[b] = lvar[(s0 + b < K) ? la[(int64_t)(s0 + b) * N + i] : 0];
+#pragma unroll
+        for (int b = 0; b < kLimBatch; ++b)
+            if (s0 + b < K) vmin = (val[b] < vmin) ? val[b] : vmin;
+    }
+    limiter_finish<V>(N, K, idx, u, loc, flag, thr, var, gamma, i, vmin);
 }
 
 template <int V>
@@ -2054,7 +2102,23 @@ int run_limiter(mf_ctx* c, double* d_u, double thr, int var)
     const double gam = c->p0;
     // blended rows, then one flag byte per node (the buffer's tail)
     uint8_t* flag = reinterpret_cast<uint8_t*>(c->d_lim + (int64_t)c->nvar * c->N);
-    if (c->nvar == 4) {
+    if (c->tiles_a && c->d_la) {
+        // the stencil kernel's tiles: one LDS double per halo entry
+        const size_t lds = sizeof(double) * (size_t)c->ha_max;
+        const unsigned tg = xcd_grid(c->n_tiles);
+        if (c->nvar == 4)
+            hipLaunchKernelGGL(k_limiter_blend_tl<4>, dim3(tg), dim3(kBlock), lds, c->stream, c->N, c->k, c->d_idx,
+                               c->d_la, c->n_tiles, c->d_ha_off, c->d_ha_ids, d_u, c->d_lim, flag, thr, var, gam);
+        else
+            hipLaunchKernelGGL(k_limiter_blend_tl<1>, dim3(tg), dim3(kBlock), lds, c->stream, c->N, c->k, c->d_idx,
+                               c->d_la, c->n_tiles, c->d_ha_off, c->d_ha_ids, d_u, c->d_lim, flag, thr, var, gam);
+        if (c->nvar == 4)
+            hipLaunchKernelGGL(k_limiter_commit<4>, dim3((unsigned)nblk), dim3(kBlock), 0, c->stream, c->N,
+                               c->d_lim, flag, d_u);
+        else
+            hipLaunchKernelGGL(k_limiter_commit<1>, dim3((unsigned)nblk), dim3(kBlock), 0, c->stream, c->N,
+                               c->d_lim, flag, d_u);
+    } else if (c->nvar == 4) {
         hipLaunchKernelGGL(k_limiter_blend<4>, dim3(grid), dim3(kBlock), 0, c->stream, c->N, c->k,
                            c->d_idx, d_u, c->d_lim, flag, thr, var, gam);
         hipLaunchKernelGGL(k_limiter_commit<4>, dim3((unsigned)nblk), dim3(kBlock), 0, c->stream,

@@ -154,14 +154,19 @@ def test_limiter_vs_oracle(mf):
     u[3, bad[30:]] = 0.2 * u[3, bad[30:]]  # low pressure
     ptr = O.ell_ptr(c.N, c.k)
     for thr, var in ((0.5, L.MF_VAR_DENSITY), (0.5, L.MF_VAR_PRESSURE)):
-        ug = u.copy()
-        c.ctx.upload(L.MF_BUF_U, ug)
-        c.ctx.limiter_zhang_shu(thr, var)
-        ug = c.ctx.download(L.MF_BUF_U)
         uo = O.limiter_zhang_shu(u.copy(), ptr, c.idx, thr, var)
         changed = np.any(uo != u, axis=0)
         assert changed.sum() > 0
-        assert rel_inf(ug, uo) < 1e-14
+        out = []
+        for tiles in (0, 1):  # global gathers; the stencil kernel's LDS tiles (built by a rhs! call)
+            c.ctx.set_tuning(L.MF_TUNE_TILES, tiles)
+            c.ctx.rhs_host(0.0, c.u0.copy(), np.zeros_like(c.u0))
+            assert c.ctx.tile_info()["tiles_a"] == tiles
+            c.ctx.upload(L.MF_BUF_U, u.copy())
+            c.ctx.limiter_zhang_shu(thr, var)
+            out.append(c.ctx.download(L.MF_BUF_U))
+            assert rel_inf(out[-1], uo) < 1e-14
+        assert np.array_equal(out[0], out[1])
 
 
 @pytest.mark.parametrize("k", [15, 13])

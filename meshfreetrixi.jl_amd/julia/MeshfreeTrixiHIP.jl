@@ -220,9 +220,10 @@ function Trixi.limiter_zhang_shu!(u, threshold::Real, variable, domain::PointClo
     return nothing
 end
 
-# --- adjoint (no counterpart in the reference): ChainRulesCore rrule of rhs!
-# is provided by MeshfreeTrixiHIPChainRulesExt when ChainRulesCore is loaded;
-# it calls mf_rhs_vjp with lambda in buffer MF_BUF_AUX0 and reads MF_BUF_AUX1.
+# --- adjoint (no counterpart in the reference): ubar = J' lambda of rhs!
+# through mf_rhs_vjp (lambda in buffer MF_BUF_AUX0, ubar read from
+# MF_BUF_AUX1).  The ChainRulesCore rrule built on it is in
+# MeshfreeTrixiHIPChainRules.jl beside this file.
 function rhs_vjp!(ubar, u, lambda, t, e::RBFFDEngineHIP)
     check(e, ccall((:mf_state_upload, libmfrbf), Cint, (Ptr{Cvoid}, Cint, Ptr{Ptr{Cdouble}}),
                    e.ctx, 0, fieldptrs(u)))

@@ -1467,6 +1467,12 @@ struct IgrArgs {
     double* part_rr;  // per-block partial sums of r . r
     double* part_pc;  // and of p . c
     IgrScalars* sc;
+    // the stencil kernel's LDS tiles (tile t = nodes t*kBlock ..; halo ids
+    // and 16-bit halo positions of every stencil entry)
+    int64_t n_tiles;
+    const int64_t* __restrict__ ha_off;
+    const int32_t* __restrict__ ha_ids;
+    const uint16_t* __restrict__ la;
 };
 
 // Block-wide sum of one value per thread (every thread gets it).
@@ -1522,6 +1528,61 @@ template <int BT>
 __device__ __forceinline__ double2 igr_grad(const IgrArgs& a, int64_t i, const double* __restrict__ f)
 {
     return igr_grad2<BT>(a, i, f, f);
+}
+
+// The same sums with the tile's halo values of the gathered field(s) in
+// LDS (one double, or (fx, fy) as one 16-byte record, per halo entry),
+// addressed by the 16-bit halo positions: same values in the same order.
+template <int BT, bool TWO>
+__device__ __forceinline__ double2 igr_grad_tl(const IgrArgs& a, int64_t i, const double* lds)
+{
+    double sx = 0.0, sy = 0.0;
+#pragma unroll 1
+    for (int s0 = 0; s0 < a.k; s0 += BT) {
+        int pos[BT];
+        double2 w[BT];
+#pragma unroll
+        for (int q = 0; q < BT; ++q) {
+            const int s = s0 + q < a.k ? s0 + q : s0;
+            pos[q] = a.la[(int64_t)s * a.N + i];
+            w[q] = a.wxy[(int64_t)s * a.N + i];
+        }
+        double gx[BT], gy[BT];
+#pragma unroll
+        for (int q = 0; q < BT; ++q) {
+            if constexpr (TWO) {
+                const double2 v = reinterpret_cast<const double2*>(lds)[pos[q]];
+                gx[q] = v.x;
+                gy[q] = v.y;
+            } else {
+                gx[q] = gy[q] = lds[pos[q]];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < BT; ++q) {
+            if (s0 + q >= a.k) continue;
+            sx = fma(w[q].x, gx[q], sx);
+            sy = fma(w[q].y, gy[q], sy);
+        }
+    }
+    return make_double2(sx, sy);
+}
+
+// Stages the halo of tile t: f0 (and f1 beside it with TWO) of every halo node.
+template <bool TWO>
+__device__ __forceinline__ void igr_stage(const IgrArgs& a, int64_t t, const double* f0, const double* f1,
+                                          double* lds)
+{
+    const int64_t h0 = a.ha_off[t];
+    const int nh = (int)(a.ha_off[t + 1] - h0);
+    for (int q = threadIdx.x; q < nh; q += kBlock) {
+        const int32_t j = a.ha_ids[h0 + q];
+        if constexpr (TWO)
+            reinterpret_cast<double2*>(lds)[q] = make_double2(f0[j], f1[j]);
+        else
+            lds[q] = f0[j];
+    }
+    __syncthreads();
 }
 
 // b, 1/rho, sigma = 0, r = b, p = 0 (update_igr_rhs!, IGR.jl:117-157, and
@@ -1600,6 +1661,23 @@ __global__ __launch_bounds__(kBlock) void k_igr_t(const IgrArgs a)
     a.t2[i] = ri * g.y;
 }
 
+// k_igr_t over the LDS tiles.
+template <int BT>
+__global__ __launch_bounds__(kBlock) void k_igr_t_tl(const IgrArgs a)
+{
+    extern __shared__ double lds[];
+    if (a.sc->done) return;
+    const int64_t t = remapped_block();
+    if (t >= a.n_tiles) return;  // whole block
+    igr_stage<false>(a, t, a.p, nullptr, lds);
+    const int64_t i = t * kBlock + threadIdx.x;
+    if (i >= a.N) return;
+    const double2 g = igr_grad_tl<BT, false>(a, i, lds);
+    const double ri = a.rinv[i];
+    a.t1[i] = ri * g.x;
+    a.t2[i] = ri * g.y;
+}
+
 // c = rho^-1 p - alpha (Dx t1 + Dy t2), and partial sums of p . c.
 template <int BT>
 __global__ __launch_bounds__(kBlock) void k_igr_c(const IgrArgs a)
@@ -1611,6 +1689,27 @@ __global__ __launch_bounds__(kBlock) void k_igr_c(const IgrArgs a)
     double pc = 0.0;
     if (i < a.N) {
         const double2 g = igr_grad2<BT>(a, i, a.t1, a.t2);  // (Dx t1, Dy t2)
+        const double ci = a.rinv[i] * a.p[i] - a.alpha * (g.x + g.y);
+        a.c[i] = ci;
+        pc = a.p[i] * ci;
+    }
+    pc = block_sum1(pc);
+    if (threadIdx.x == 0) a.part_pc[lb] = pc;
+}
+
+// k_igr_c over the LDS tiles (tile t is block t of k_igr_c).
+template <int BT>
+__global__ __launch_bounds__(kBlock) void k_igr_c_tl(const IgrArgs a)
+{
+    extern __shared__ double lds[];
+    if (a.sc->done) return;
+    const int64_t lb = remapped_block();
+    if (lb >= a.nblk) return;  // whole block
+    igr_stage<true>(a, lb, a.t1, a.t2, lds);
+    const int64_t i = lb * kBlock + threadIdx.x;
+    double pc = 0.0;
+    if (i < a.N) {
+        const double2 g = igr_grad_tl<BT, true>(a, i, lds);  // (Dx t1, Dy t2)
         const double ci = a.rinv[i] * a.p[i] - a.alpha * (g.x + g.y);
         a.c[i] = ci;
         pc = a.p[i] * ci;
@@ -1680,6 +1779,10 @@ static IgrArgs make_igr(mf_ctx* c, const double* u)
     g.t2 = f + 7 * N;
     g.part_rr = c->d_igr_part;
     g.part_pc = c->d_igr_part + g.nblk;
+    g.n_tiles = c->n_tiles;
+    g.ha_off = c->d_ha_off;
+    g.ha_ids = c->d_ha_ids;
+    g.la = c->d_la;
     g.sc = c->d_igr_sc;
     return g;
 }
@@ -1974,10 +2077,18 @@ static int run_igr(mf_ctx* c, const double* u, const RhsArgs& a, unsigned grid)
     const IgrArgs g = make_igr(c, u);
     const dim3 G(grid), B(kBlock);
     launch_k(c, -1, k_igr_init, G, B, 0, g);
+    // the stencil kernel's tiles, when built (one tile per block of nodes)
+    const bool tl = c->tiles_a && c->d_la && c->n_tiles == g.nblk;
+    const size_t lds1 = sizeof(double) * (size_t)c->ha_max, lds2 = 2 * lds1;
     for (int it = 0; it < c->igr_maxiter; ++it) {
         launch_k(c, -1, k_igr_p, G, B, 0, g, it);
-        launch_k(c, -1, k_igr_t<5>, G, B, 0, g);
-        launch_k(c, -1, k_igr_c<5>, G, B, 0, g);
+        if (tl) {
+            launch_k(c, -1, k_igr_t_tl<5>, G, B, lds1, g);
+            launch_k(c, -1, k_igr_c_tl<5>, G, B, lds2, g);
+        } else {
+            launch_k(c, -1, k_igr_t<5>, G, B, 0, g);
+            launch_k(c, -1, k_igr_c<5>, G, B, 0, g);
+        }
         launch_k(c, -1, k_igr_x, G, B, 0, g, it);
     }
     launch_k(c, -1, k_igr_apply<5>, G, B, 0, g, a);

@@ -43,6 +43,26 @@ def test_rhs_igr(mf, prec, tol):
     assert rel_inf(du, duo) < tol
 
 
+def test_igr_tiles_change_no_result(mf):
+    """The CG operator passes over the stencil kernel's LDS tiles (halo
+    values staged once, 16-bit positions) give the bits of the global
+    gathers: sigma and du, in every precision."""
+    spec = configs.c2()
+    c0 = cases.build(spec, side=64, use_gpu=False)
+    c = cases.build(spec, side=64, igr=c0.dx_min ** 2)
+    for prec in (L.MF_PREC_F64, L.MF_PREC_F64_REF, L.MF_PREC_SPLIT32):
+        out = []
+        for tiles in (1, 0):
+            _say("igr tiles=%d prec=%d" % (tiles, prec))
+            c.ctx.set_tuning(L.MF_TUNE_TILES, tiles)
+            du = _gpu_rhs(c, prec)
+            assert c.ctx.tile_info()["tiles_a"] == tiles
+            out.append((du, c.ctx.igr_sigma()))
+        c.ctx.set_tuning(L.MF_TUNE_TILES, 1)
+        assert np.array_equal(out[0][1], out[1][1]) and np.abs(out[0][1]).max() > 0.0
+        assert np.array_equal(out[0][0], out[1][0])
+
+
 def test_rhs_igr_indefinite(mf):
     """alpha = 5 dx^2: the operator is indefinite and the CG residual grows
     (15x over 20 iterations), yet the iterates stay stable to rounding, and

@@ -56,23 +56,28 @@ def shape_key(n, k, eq, visc, prec, device_name=""):
     return "%s|n2^%d|k%d|eq%d|visc%d|%s" % (device_name, bucket, k, eq, visc, PREC_NAMES[prec])
 
 
-def sweep(ctx, t, prec, k, calls=50):
+def sweep(ctx, t, prec, k, calls=50, rounds=3):
     """Times every variant of one precision; returns {"flux": {b: ms},
-    "transpose": {b: ms}} (ms per launch of the kernel alone)."""
+    "transpose": {b: ms}} (ms per launch of the kernel alone).  Each
+    variant is timed in `rounds` rounds, the pairing and order rotated
+    between rounds, and keeps its minimum: a slow outlier in one round
+    does not decide the pick."""
     res = {"flux": {}, "transpose": {}}
     fb = FLUX_BATCHES.get(k, (0,))
     tb = TRANSPOSE_BATCHES
-    for i in range(max(len(fb), len(tb))):
-        f = fb[i % len(fb)]
-        b = tb[i % len(tb)]
-        ctx.set_tuning(L.MF_TUNE_FLUX_BATCH, f, prec)
-        ctx.set_tuning(L.MF_TUNE_TRANSPOSE_BATCH, b, prec)
-        ctx.time_rhs(t, prec, 3)  # warm the new variant
-        _, parts = ctx.time_rhs(t, prec, calls)
-        if f not in res["flux"] or parts[0] < res["flux"][f]:
-            res["flux"][f] = float(parts[0])
-        if parts[1] > 0 and (b not in res["transpose"] or parts[1] < res["transpose"][b]):
-            res["transpose"][b] = float(parts[1])
+    n = max(len(fb), len(tb))
+    for r in range(rounds):
+        for i in range(n):
+            f = fb[(i + r) % len(fb)]
+            b = tb[(i + 2 * r) % len(tb)]
+            ctx.set_tuning(L.MF_TUNE_FLUX_BATCH, f, prec)
+            ctx.set_tuning(L.MF_TUNE_TRANSPOSE_BATCH, b, prec)
+            ctx.time_rhs(t, prec, 3)  # warm the new variant
+            _, parts = ctx.time_rhs(t, prec, calls)
+            if f not in res["flux"] or parts[0] < res["flux"][f]:
+                res["flux"][f] = float(parts[0])
+            if parts[1] > 0 and (b not in res["transpose"] or parts[1] < res["transpose"][b]):
+                res["transpose"][b] = float(parts[1])
     return res
 
 

@@ -11,7 +11,8 @@ reference's operation order (the mode within 1e-12 of the reference).
 Printed as one JSON line, with:
   roofline      the dominant kernel's algorithmic bytes / its event-timed
                 duration vs 8 TB/s (whole-rhs! fraction reported beside it)
-  cpu_baseline  the FP64 CPU oracle timed on this host (1 thread)
+  cpu_baseline  the FP64 CPU oracle timed on this host with OpenMP (one
+                thread per core of the host's share), 1-thread rate beside it
   f64_fma       the fused-multiply-add FP64 rhs! (faster, within 5e-12)
   split32       the split-FP32 rhs! rate, speed-up and error vs FP64
   adjoint       reverse-sweep / forward time of 100 SSPRK33 steps (65,536 nodes)
@@ -199,6 +200,17 @@ def stepping(L, ctx, info, steps):
     return out
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(L, ctx, info, spec, t_now, budget_s):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -211,21 +223,37 @@ def cpu_baseline(L, ctx, info, spec, t_now, budget_s):
                      boundary=[(O.BC_DIRICHLET, info["inflow"], None, O.FN_WAVE, prm, None),
                                (O.BC_NOTHING, info["outflow"], None, None, None, None)],
                      rv_mode=O.RV_RESIDUAL, c_rv=1.0, c_uw=1.0, dx_avg=info["dx_avg"])
-    u = ctx.download(L.MF_BUF_U)
     prob.approx_du[:] = ctx.approx_du().ravel()
     prob.cfg.success_iter = 3
-    calls, t0 = 0, time.perf_counter()
-    du = None
-    while True:
+    u = ctx.download(L.MF_BUF_U)
+    du, _, _ = prob.rhs(u.copy(), t_now)  # warm
+    ts, t0 = [], time.perf_counter()
+    while len(ts) < 20 and time.perf_counter() - t0 < budget_s:
         uu = u.copy()
+        c0 = time.perf_counter()
         du, _, _ = prob.rhs(uu, t_now)
-        calls += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or calls >= 20:
-            break
-    return dict(value=N * calls / el, unit="node-evals/s", cores=1, kind="port",
-                sample="%d FP64 rhs! calls of the full workload (%d nodes) by the C oracle, 1 thread, %.1f s"
-                % (calls, N, el)), du
+        ts.append(time.perf_counter() - c0)
+    cpu = _cpu_model()
+    single = dict(value=N / float(np.median(ts)), unit="node-evals/s", cores=1, kind="port",
+                  sample="median of %d FP64 rhs! calls of the full workload (%d nodes) by the C oracle, "
+                  "1 thread, %s" % (len(ts), N, cpu))
+    # the same rhs! with OpenMP over the host's share of cores (OMP_NUM_THREADS
+    # where set): fluxes, row gathers and column dots split over threads
+    nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    dp = prob.rhs_par(u.copy(), t_now, nth)  # warm: threads started, pages touched
+    ts, t0 = [], time.perf_counter()
+    while len(ts) < 50 and time.perf_counter() - t0 < budget_s:
+        uu = u.copy()
+        c0 = time.perf_counter()
+        dp = prob.rhs_par(uu, t_now, nth)
+        ts.append(time.perf_counter() - c0)
+    par_err = float(np.abs(dp - du).max() / np.abs(du).max())
+    if par_err > 1e-10:
+        raise RuntimeError("OpenMP oracle rhs! differs from the serial one: %.3e" % par_err)
+    return dict(value=N / float(np.median(ts)), unit="node-evals/s", cores=nth, kind="port",
+                sample="median of %d FP64 rhs! calls of the full workload (%d nodes) by the C oracle with "
+                "OpenMP, %d threads, %s; rel err vs 1 thread %.1e" % (len(ts), N, nth, cpu, par_err),
+                single_thread=single), du
 
 
 def adjoint_ratio(L, configs):

@@ -9,6 +9,7 @@
  * tree (NearestNeighbors, LAPACK, SparseArrays, Trixi, OrdinaryDiffEq) is
  * restated from their published algorithms; each such place says so.
  */
+#include <omp.h>
 #include "mf_oracle.h"
 
 #include <float.h>
@@ -1147,6 +1148,141 @@ int or_rhs(const or_rhs_cfg* c, double t, double* u, double* du, double* eps_out
     or_bc_pass(c, t, u, du); /* second pass, :422-425 */
     free(own_e);
     free(own_c);
+    return 0;
+}
+
+/* ======================================================================
+ * Multi-threaded rhs! for the CPU baseline timing only (not a parity
+ * reference): or_rhs for the Euler equations with the residual or upwind
+ * viscosity and strong boundary conditions, with the forward products as
+ * row gathers over the row-form stencil (rptr, ridx, rwx, rwy: the ELL or
+ * ragged input of the weights) and the transposed products as column dots
+ * over the CSC, each split over OpenMP threads.  Row sums and the mean run
+ * in a different order from or_rhs, so the two agree to rounding, not bit
+ * for bit.  Returns -1 for configurations it does not cover.  scratch:
+ * 17 N doubles.
+ * ==================================================================== */
+static inline int or_lex_less(const double* a, const double* b)
+{
+    for (int v = 0; v < 4; ++v) {
+        if (a[v] < b[v]) return 1;
+        if (a[v] > b[v]) return 0;
+    }
+    return 0;
+}
+
+int or_rhs_par(const or_rhs_cfg* c, const int64_t* rptr, const int32_t* ridx, const double* rwx,
+               const double* rwy, double t, double* u, double* du, double* scratch, int nthreads)
+{
+    if (c->eq != OR_EQ_EULER || c->nvar != 4 || c->hv || c->lap || c->igr_alpha != 0.0) return -1;
+    const int64_t N = c->N;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    memset(du, 0, sizeof(double) * 4 * N);
+    or_bc_pass(c, t, u, du);
+    double* F = scratch;         /* F1 in 4 N, then F2 in 4 N */
+    double* eps = scratch + 8 * N;
+    double* G = scratch + 9 * N; /* eps Dx u in 4 N, then eps Dy u in 4 N */
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < N; ++e) {
+        double ue[4], f1[4], f2[4];
+        for (int v = 0; v < 4; ++v) ue[v] = u[v * N + e];
+        or_flux(c->eq, c->adv_a, c->gamma, 0, ue, f1);
+        or_flux(c->eq, c->adv_a, c->gamma, 1, ue, f2);
+        for (int v = 0; v < 4; ++v) {
+            F[v * N + e] = f1[v];
+            F[(4 + v) * N + e] = f2[v];
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < N; ++e)
+        for (int v = 0; v < 4; ++v) {
+            double a = 0.0;
+            for (int64_t q = rptr[e]; q < rptr[e + 1]; ++q) a = a + rwx[q] * (F[v * N + ridx[q]] * -1.0);
+            for (int64_t q = rptr[e]; q < rptr[e + 1]; ++q) a = a + rwy[q] * (F[(4 + v) * N + ridx[q]] * -1.0);
+            du[v * N + e] = a;
+        }
+    if (c->rv_mode != OR_RV_NONE) {
+        const double gam = c->gamma, h = c->dx_avg;
+        double mean[4] = {0.0, 0.0, 0.0, 0.0}, ninf[4] = {1.0, 1.0, 1.0, 1.0};
+        if (c->rv_mode == OR_RV_RESIDUAL) {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : s0, s1, s2, s3)
+            for (int64_t e = 0; e < N; ++e) {
+                s0 += u[e];
+                s1 += u[N + e];
+                s2 += u[2 * N + e];
+                s3 += u[3 * N + e];
+            }
+            mean[0] = s0 / (double)(4 * N);
+            mean[1] = s1 / (double)(4 * N);
+            mean[2] = s2 / (double)(4 * N);
+            mean[3] = s3 / (double)(4 * N);
+            double best[4] = {-1.0, -1.0, -1.0, -1.0};
+#pragma omp parallel
+            {
+                double mine[4] = {-1.0, -1.0, -1.0, -1.0};
+#pragma omp for schedule(static)
+                for (int64_t e = 0; e < N; ++e) {
+                    double cur[4];
+                    for (int v = 0; v < 4; ++v) cur[v] = fabs(u[v * N + e] - mean[v]);
+                    if (or_lex_less(mine, cur))
+                        for (int v = 0; v < 4; ++v) mine[v] = cur[v];
+                }
+#pragma omp critical
+                if (or_lex_less(best, mine))
+                    for (int v = 0; v < 4; ++v) best[v] = mine[v];
+            }
+            for (int v = 0; v < 4; ++v) ninf[v] = (best[v] == 0.0) ? DBL_EPSILON : best[v];
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t e = 0; e < N; ++e) {
+            /* update_upwind_visc! (as or_upwind_visc) */
+            const double rho = u[e], m1 = u[N + e], m2 = u[2 * N + e], E = u[3 * N + e];
+            const double v1 = m1 / rho, v2 = m2 / rho;
+            const double p = (gam - 1.0) * fma(-0.5, fma(m1, v1, m2 * v2), E);
+            const double speed = sqrt(v1 * v1 + v2 * v2);
+            double sound = 0.0;
+            if (!(p < 0.0 || rho < 0.0)) sound = sqrt(gam * p / rho);
+            const double eps_uw = c->c_uw * 0.5 * h * (speed + sound);
+            if (c->rv_mode == OR_RV_UPWIND) {
+                eps[e] = eps_uw;
+                continue;
+            }
+            /* update_residual_visc! and update_visc! (as or_residual_visc, or_update_visc) */
+            double mr = 0.0;
+            for (int v = 0; v < 4; ++v) {
+                const double r = fabs(c->approx_du[v * N + e] - du[v * N + e]) / ninf[v];
+                mr = (v == 0) ? r : or_jmax(mr, r);
+            }
+            const double eps_rv = 0.5 * c->c_rv * (h * h) * mr;
+            if (isnan(eps_rv) || isinf(eps_rv) || c->success_iter == 0)
+                eps[e] = (isnan(eps_uw) || isinf(eps_uw)) ? DBL_EPSILON : eps_uw;
+            else
+                eps[e] = (eps_rv < eps_uw) ? eps_rv : eps_uw;
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t e = 0; e < N; ++e)
+            for (int v = 0; v < 4; ++v) {
+                double gx = 0.0, gy = 0.0;
+                for (int64_t q = rptr[e]; q < rptr[e + 1]; ++q) {
+                    gx = gx + rwx[q] * u[v * N + ridx[q]];
+                    gy = gy + rwy[q] * u[v * N + ridx[q]];
+                }
+                G[v * N + e] = eps[e] * gx;
+                G[(4 + v) * N + e] = eps[e] * gy;
+            }
+#pragma omp parallel for schedule(static)
+        for (int64_t col = 0; col < N; ++col)
+            for (int v = 0; v < 4; ++v) {
+                double tx = 0.0, ty = 0.0;
+                for (int64_t j = c->colptr[col]; j < c->colptr[col + 1]; ++j) {
+                    tx = tx + c->dx[j] * G[v * N + c->rowval[j]];
+                    ty = ty + c->dy[j] * G[(4 + v) * N + c->rowval[j]];
+                }
+                du[v * N + col] = (du[v * N + col] + tx * -1.0) + ty * -1.0;
+            }
+    }
+    or_bc_pass(c, t, u, du);
     return 0;
 }
 

@@ -145,6 +145,13 @@ void or_limiter_zhang_shu(int64_t N, int nvar, double gamma, const int64_t* ptr,
                           const int32_t* idx, double threshold, int variable, double* u,
                           double* scratch);
 
+/* Multi-threaded (OpenMP) rhs! for the CPU baseline timing: Euler with
+ * the residual or upwind viscosity and strong BCs only (-1 otherwise);
+ * row-form stencil rptr/ridx/rwx/rwy; agrees with or_rhs to rounding.
+ * scratch: 17 N doubles; nthreads <= 0 keeps OpenMP's default. */
+int or_rhs_par(const or_rhs_cfg* c, const int64_t* rptr, const int32_t* ridx, const double* rwx,
+               const double* rwy, double t, double* u, double* du, double* scratch, int nthreads);
+
 /* SSPRK33 with FSAL, history callback and optional stage limiter */
 typedef struct {
     int approx_order; /* 0 = no history callback */

@@ -91,6 +91,8 @@ def _declare(L):
     L.or_bc_value.argtypes = [C.c_int, _dp, C.c_int, C.c_double, C.c_double, C.c_double,
                               C.c_double, _dp]
     L.or_rhs.argtypes = [C.POINTER(RhsCfg), C.c_double, _dp, _dp, _dp, _i8p, _dp]
+    L.or_rhs_par.argtypes = [C.POINTER(RhsCfg), _i64p, _i32p, _dp, _dp, C.c_double, _dp, _dp, _dp,
+                             C.c_int]
     L.or_rhs_scratch_len.argtypes = [C.POINTER(RhsCfg)]
     L.or_rhs_scratch_len.restype = C.c_int64
     L.or_igr_rhs.argtypes = [C.POINTER(RhsCfg), _dp, _dp, _dp]
@@ -260,6 +262,9 @@ class Problem:
         self.csc = CSC(N, ptr, idx)
         self.dx = self.csc.values(wx)
         self.dy = self.csc.values(wy)
+        # the row form (the weights' own ELL/ragged order), for rhs_par
+        self.rwx = np.ascontiguousarray(np.asarray(wx, dtype=np.float64).ravel())
+        self.rwy = np.ascontiguousarray(np.asarray(wy, dtype=np.float64).ravel())
         self.hv = self.csc.values(hv) if hv is not None else None
         self.lap = self.csc.values(lap) if lap is not None else None
         self.nvar = nvar
@@ -343,6 +348,20 @@ class Problem:
         eps_c = np.zeros(self.N, np.int8)
         lib().or_rhs(C.byref(self.cfg), t, d(u), d(du), d(eps), _p(eps_c, _i8p), d(self.scratch))
         return du, eps, eps_c
+
+    def rhs_par(self, u, t, nthreads=0):
+        """The CPU baseline's multi-threaded rhs! (OpenMP, or_rhs_par): du
+        for the Euler equations with the viscosity sources only; agrees with
+        rhs to rounding.  Mutates u at boundary nodes like rhs."""
+        assert u.flags.c_contiguous and u.dtype == np.float64
+        du = np.zeros_like(u)
+        if not hasattr(self, "_par_scratch"):
+            self._par_scratch = np.zeros(17 * self.N)
+        rc = lib().or_rhs_par(C.byref(self.cfg), i64(self.csc.ptr), i32(self.csc.idx), d(self.rwx),
+                              d(self.rwy), t, d(u), d(du), d(self._par_scratch), int(nthreads))
+        if rc != 0:
+            raise ValueError("rhs_par covers the Euler equations with viscosity sources only")
+        return du
 
     def vjp(self, u, lam, t):
         ub = np.zeros_like(u)

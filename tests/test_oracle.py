@@ -449,6 +449,30 @@ def test_vjp_matches_forward_mode(bc):
     assert abs(float((lam * jq).sum()) - ub[1, q]) <= 1e-12 * float(np.abs(lam * jq).sum() + 1e-300)
 
 
+@pytest.mark.parametrize("visc", ["residual", "upwind"])
+def test_rhs_par_matches_serial(visc):
+    """The CPU baseline's OpenMP rhs! (row gathers and column dots split over
+    threads) against the serial oracle: the same du and the same boundary
+    mutation of u, to rounding (its sums run in another order)."""
+    c = cases.build(configs.c2(), side=40, use_gpu=False, visc=visc)
+    u1 = c.u0 * 1.0001
+    th, sh, ad = np.zeros(4), np.zeros(4 * c.u0.size), np.zeros(c.u0.size)
+    O.history_update(3, 0, 0.0, c.u0.ravel().copy(), th, sh, ad)
+    O.history_update(3, 1, 0.001, u1.ravel().copy(), th, sh, ad)
+    c.prob.approx_du[:] = ad
+    for si in (0, 1):  # 0: the upwind eps only; 1: the residual eps where finite
+        c.prob.cfg.success_iter = si
+        us, up = c.u0.copy(), c.u0.copy()
+        ds, _, _ = c.prob.rhs(us, 0.001)
+        dp = c.prob.rhs_par(up, 0.001, nthreads=4)
+        assert np.array_equal(us, up)  # the first boundary pass
+        assert rel_inf(dp, ds) < 1e-12
+        assert np.abs(ds).max() > 0.0
+    hv = cases.build(configs.c2(), side=16, use_gpu=False, hv=(2, 1.0))
+    with pytest.raises(ValueError):
+        hv.prob.rhs_par(hv.u0.copy(), 0.0)
+
+
 def test_tominec_hyperviscosity_matches_explicit_product():
     """SourceHyperviscosityTominec (hyperviscosity.jl:80-134): the reference
     builds H = lap' * lap explicitly (lap = Dxx + Dyy) and adds -gamma H u.

@@ -145,6 +145,58 @@ def test_rhs_euler_viscosity_f64(mf, visc):
     assert rel_inf(du, duo) < TOL64
 
 
+def _rv_history(c, u1):
+    """Two history snapshots (c.u0 at t = 0, u1 at t = 0.001) on both sides,
+    so that eps_rv is active."""
+    th, sh, ad = np.zeros(4), np.zeros(4 * c.u0.size), np.zeros(c.u0.size)
+    O.history_update(3, 0, 0.0, c.u0.ravel().copy(), th, sh, ad)
+    O.history_update(3, 1, 0.001, u1.ravel().copy(), th, sh, ad)
+    c.prob.approx_du[:] = ad
+    c.prob.cfg.success_iter = 1
+    c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+    c.ctx.history_update(0.0, 0)
+    c.ctx.upload(L.MF_BUF_U, np.ascontiguousarray(u1))
+    c.ctx.history_update(0.001, 1)
+
+
+@pytest.mark.parametrize("shape", ["spike", "spike_last", "tie", "flat"])
+def test_rhs_residual_norm_extremes(mf, shape):
+    """The lexicographic maximum of |u - mean| (the residual viscosity's
+    normalisation) on states built so that it sits at one node (a density
+    spike, also in the last, partial block), at two nodes of different
+    blocks with equal density (decided by the momentum), or is tied
+    everywhere in the first component (constant density).  eps and du
+    against the oracle."""
+    _say("rhs residual viscosity norm extremes: %s" % shape)
+    c = cases.build(configs.c2(), side=72, visc="residual")  # N = 5184: several ranges, a partial last one
+    N = c.u0.shape[1]
+    u = c.u0.copy()
+    i, j = N // 2 + 3, N // 9
+    if shape == "spike":
+        u[0, i] *= 1.5
+    elif shape == "spike_last":
+        u[0, N - 2] *= 1.5
+    elif shape == "tie":
+        u[0, i] = u[0, j] = 1.5 * u[0].max()
+        u[1, i] += 0.01
+        u[1, j] += 0.02
+    else:
+        u[0, :] = 1.0
+    u[3] = np.maximum(u[3], 0.5 * (u[1] ** 2 + u[2] ** 2) / u[0] + 0.5)  # positive pressure
+    c.u0 = np.ascontiguousarray(u)
+    _rv_history(c, c.u0 * 1.0001)
+    for prec in (L.MF_PREC_F64_REF, L.MF_PREC_F64, L.MF_PREC_SPLIT32):
+        uu = c.u0.copy()
+        du = np.zeros_like(uu)
+        c.ctx.rhs_host(0.001, uu, du, prec)
+        duo, eps, epsc = c.prob.rhs(c.u0.copy(), 0.001)
+        geps, gcode = c.ctx.viscosity()
+        _say("  prec=%d rel err %.2e eps err %.2e" % (prec, rel_inf(du, duo), rel_inf(geps, eps)))
+        assert np.array_equal(gcode, epsc)
+        assert rel_inf(geps, eps) < (1e-12 if prec != L.MF_PREC_SPLIT32 else 1e-6)
+        assert rel_inf(du, duo) < (1e-12 if prec != L.MF_PREC_SPLIT32 else TOL32)
+
+
 def test_rhs_split32_error(mf):
     _say("rhs euler split32")
     c = cases.build(configs.c2(), side=96)

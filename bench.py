@@ -16,7 +16,9 @@ Printed as one JSON line, with:
   f64_fma       the fused-multiply-add FP64 rhs! (faster, within 5e-12)
   split32       the split-FP32 rhs! rate, speed-up and error vs FP64
   adjoint       reverse-sweep / forward time of 100 SSPRK33 steps (65,536 nodes)
-  weights       batched weight generation for 1,048,576 ragged stencils
+  cold_cache    the same rhs! after a flush of the L2 and last-level caches
+  weights       batched weight generation for 1,048,576 ragged stencils,
+                with the CPU oracle on a 32,768-stencil sample beside it
   ensemble      64 clouds (5.7M nodes), 50 split-FP32 SSPRK33 steps each
 """
 import argparse
@@ -325,7 +327,7 @@ def ensemble(L, configs, steps):
                 node_evals_per_s_one_at_a_time=evals / wall_s, max_density_error_vs_exact=err)
 
 
-def weights_batch(L, configs, side):
+def weights_batch(L, configs, side, cpu_sample=0):
     from mfrbf.clouds import cloud_square
 
     sizes = configs.c3_stencils(side)
@@ -348,8 +350,44 @@ def weights_batch(L, configs, side):
     ms = 1e3 * (time.perf_counter() - t0) / reps
     m = sizes + 10
     flops = float(np.sum(m ** 3 / 3.0 + 6.0 * m ** 2))
-    return dict(stencils=N, ms=ms, gflop=flops / 1e9, tflops=flops / (ms * 1e-3) / 1e12,
-                frac_fp64_peak=flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS)
+    out = dict(stencils=N, ms=ms, gflop=flops / 1e9, tflops=flops / (ms * 1e-3) / 1e12,
+               frac_fp64_peak=flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS)
+    if cpu_sample:
+        out["cpu_baseline"] = weights_cpu(x, y, ptr, ridx, ctx.weights_ragged_get(), cpu_sample)
+    return out
+
+
+def weights_cpu(x, y, ptr, ridx, gpu_w, sample):
+    """The C oracle's weights on the first `sample` stencils of the same
+    batch (their ragged sizes as drawn): median of 3 timed passes on 1
+    thread and on the host's share of cores (OpenMP), in stencils/s; the
+    GPU's weights on that sample against the oracle's."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    p = ptr[:sample + 1]
+    nnz = int(p[-1])
+    ridx = ridx[:nnz]
+    nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    res = {}
+    for name, threads in (("single_thread", None), ("openmp", nth)):
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            wx, wy, rc = O.weights(x, y, p, ridx, 5, 3, 1, nthreads=threads)
+            ts.append(time.perf_counter() - t0)
+            if rc:
+                raise RuntimeError("oracle weights: singular stencil %d" % (rc - 1))
+        res[name] = dict(value=sample / float(np.median(ts)), unit="stencils/s", cores=threads or 1,
+                         kind="port")
+    err = max(float(np.abs(gpu_w[0][:nnz] - wx).max() / np.abs(wx).max()),
+              float(np.abs(gpu_w[1][:nnz] - wy).max() / np.abs(wy).max()))
+    res["sample"] = ("the first %d of the batch's stencils, FP64 C oracle, median of 3 passes each, %s"
+                     % (sample, _cpu_model()))
+    res["gpu_rel_err"] = err
+    if err > 1e-9:
+        raise RuntimeError("GPU weights differ from the oracle's: %.3e" % err)
+    return res
 
 
 def main():
@@ -429,6 +467,11 @@ def main():
     ctx.set_tuning(L.MF_TUNE_TILES, 1)
     ctx.time_rhs(t_now, L.MF_PREC_F64_REF, 3)
     log("without LDS tiles: rhs %.4f ms; stencil %.4f ms, transpose %.4f ms" % (nt_ms, nt_parts[0], nt_parts[1]))
+    # each call after a 512 MiB read-modify-write that evicts the L2 and
+    # last-level caches (median of 20 eager calls)
+    cold = {name: ctx.time_rhs_cold(t_now, prec, 20)
+            for name, prec in (("f64_ref", L.MF_PREC_F64_REF), ("split32", L.MF_PREC_SPLIT32))}
+    log("cold caches: f64_ref %.4f ms, split32 %.4f ms" % (cold["f64_ref"], cold["split32"]))
     dus = {}
     for name, prec in (("f64_ref", L.MF_PREC_F64_REF), ("f64_fma", L.MF_PREC_F64), ("split32", L.MF_PREC_SPLIT32)):
         ctx.rhs(t_now, prec)
@@ -467,6 +510,8 @@ def main():
         "split32": {"value": N / (s_ms * 1e-3), "ms_per_step": s_ms, "speedup_vs_f64": rhs_ms / s_ms,
                     "speedup_vs_f64_fma": f_ms / s_ms, "rel_err_vs_f64": s_err,
                     "kernels_ms": {"stencil": s_parts[0], "transpose": s_parts[1]}},
+        "cold_cache": {"ms_per_step": cold, "node_evals_per_s": {q: N / (v * 1e-3) for q, v in cold.items()},
+                       "flush": "512 MiB read-modify-write before each of 20 eager calls, median"},
         "lds_tiles": tinfo,
         "f64_global_gathers": {"ms_per_step": nt_ms,
                                "kernels_ms": {"stencil": nt_parts[0], "transpose": nt_parts[1]}},
@@ -476,8 +521,8 @@ def main():
     if not args.no_extras:
         log("adjoint: 100 SSPRK33 steps, 65,536 nodes")
         out["adjoint"] = adjoint_ratio(L, configs)
-        log("batched weights: 1,048,576 ragged stencils")
-        out["weights"] = weights_batch(L, configs, 1024)
+        log("batched weights: 1,048,576 ragged stencils (and the CPU oracle on a sample)")
+        out["weights"] = weights_batch(L, configs, 1024, 0 if args.no_cpu else 32768)
         log("ensemble: 64 clouds, 50 SSPRK33 steps each (split-FP32, graphs, concurrent streams)")
         out["ensemble"] = ensemble(L, configs, 50)
         log("Tominec hyperviscosity after the residual viscosity, %d nodes" % N)

@@ -328,6 +328,13 @@ int mf_tile_info(mf_ctx* ctx, int64_t* info);
 int mf_time_rhs(mf_ctx* ctx, double t, int precision, int n_calls, double* rhs_ms,
                 double* kernel_ms);
 
+/* Median device time (ms) of one eager rhs! on MF_BUF_U over n_calls calls,
+ * each after a read-modify-write of a 512 MiB buffer (allocated by the first
+ * call, kept until mf_ctx_destroy) that leaves none of the call's data in
+ * the L2 or last-level caches: the cold-cache counterpart of mf_time_rhs.
+ * No counterpart in the reference; for performance accounting. */
+int mf_time_rhs_cold(mf_ctx* ctx, double t, int precision, int n_calls, double* rhs_ms);
+
 #ifdef __cplusplus
 }
 #endif

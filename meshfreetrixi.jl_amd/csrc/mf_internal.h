@@ -155,6 +155,7 @@ struct mf_ctx {
     double* d_approx = nullptr;
     double* d_hist = nullptr;
     double* d_partials = nullptr;  // 4 n_psum sums, then 4 n_pmax lexicographic maxima
+    double* d_flush = nullptr;     // kFlushDoubles, allocated by the first mf_time_rhs_cold
     int n_psum = 0, n_pmax = 0;
     mf::StepScalars* d_sc = nullptr;
     int host_success_iter = 0;
@@ -275,6 +276,9 @@ void free_tiles(mf_ctx* c);
 void free_tiles_b(mf_ctx* c);
 int launch_gather_nodes_d(mf_ctx* c, const double* d_src, double* d_dst);
 int launch_copy_buffer(mf_ctx* c, const double* d_src, double* d_dst, int64_t n);
+int launch_flush(mf_ctx* c, double* d_buf, int64_t n);
+// cache-flush buffer of mf_time_rhs_cold: 512 MiB, twice the 256 MB last-level cache
+constexpr size_t kFlushDoubles = (size_t)1 << 26;
 int launch_gather_nodes_i8(mf_ctx* c, const int8_t* d_src, int8_t* d_dst);
 int launch_sort_stencil(mf_ctx* c);
 int launch_pack_sorted(mf_ctx* c, int target, const double* d_wx, const double* d_wy);

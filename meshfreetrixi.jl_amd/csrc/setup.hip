@@ -1301,6 +1301,21 @@ int launch_copy_buffer(mf_ctx* c, const double* d_src, double* d_dst, int64_t n)
     return MF_OK;
 }
 
+// read-modify-write of a buffer larger than the last-level cache, so that a
+// call timed after it finds none of its own data cached
+__global__ void k_flush(int64_t n, double* p)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) p[q] = p[q] * 0.5 + 1.0;
+}
+
+int launch_flush(mf_ctx* c, double* d_buf, int64_t n)
+{
+    hipLaunchKernelGGL(k_flush, dim3(4096), dim3(256), 0, c->stream, n, d_buf);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
 // per-node arrays back to the caller's order: dst[u] = src[inv[u]]
 template <class T>
 __global__ void k_gather_nodes(int64_t N, const int32_t* inv, const T* src, T* dst)

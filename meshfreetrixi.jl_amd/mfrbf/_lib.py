@@ -75,6 +75,7 @@ _SIGS = {
     "mf_rhs_vjp": [_ctx, C.c_double],
     "mf_ssprk33_vjp": [_ctx, C.c_double, C.c_double, C.c_int, C.c_int, _dp],
     "mf_time_rhs": [_ctx, C.c_double, C.c_int, C.c_int, _dp, _dp],
+    "mf_time_rhs_cold": [_ctx, C.c_double, C.c_int, C.c_int, _dp],
     "mf_set_tuning": [_ctx, C.c_int, C.c_int, C.c_int],
     "mf_node_order": [_ctx, _i32p],
     "mf_tile_info": [_ctx, _i64p],
@@ -317,6 +318,12 @@ class Context:
         out = np.zeros(self.N, np.int32)
         self._chk(self._L.mf_node_order(self.h, i32ptr(out)))
         return out
+
+    def time_rhs_cold(self, t, precision, n_calls):
+        """Median ms of one rhs! after a cache flush (mf_time_rhs_cold)."""
+        a = C.c_double()
+        self._chk(self._L.mf_time_rhs_cold(self.h, t, precision, n_calls, C.byref(a)))
+        return a.value
 
     def time_rhs(self, t, precision, n_calls):
         """(ms per rhs!, [ms of stencil kernel, transposed gather, reductions, boundary])."""

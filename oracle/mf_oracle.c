@@ -660,6 +660,28 @@ int or_weights(const double* x, const double* y, int64_t N, const int64_t* ptr,
     return rc;
 }
 
+/* or_weights over chunks of 64 stencils on OpenMP threads: each stencil is
+ * computed by the same code as in or_weights, so the output is the same
+ * bits.  Returns 0, or 1 + the first singular stencil's index. */
+int or_weights_par(const double* x, const double* y, int64_t N, const int64_t* ptr,
+                   const int32_t* idx, int phs_p, int poly_deg, int deriv_k, double* wx,
+                   double* wy, int nthreads)
+{
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    const int64_t nch = (N + 63) / 64;
+    int64_t first = 0;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t ch = 0; ch < nch; ++ch) {
+        const int64_t e0 = ch * 64, n = (N - e0 < 64) ? N - e0 : 64;
+        const int r = or_weights(x, y, n, ptr + e0, idx, phs_p, poly_deg, deriv_k, wx, wy);
+        if (r) {
+#pragma omp critical
+            if (first == 0 || e0 + r < first) first = e0 + r;
+        }
+    }
+    return (int)first;
+}
+
 /* ======================================================================
  * Sparse operators.  sparse(rows, cols, vals) (compute_operators.jl:443-452)
  * gives CSC sorted by column then row; the products below restate

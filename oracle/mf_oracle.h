@@ -42,6 +42,11 @@ int or_npoly(int deg);
 int or_weights(const double* x, const double* y, int64_t N, const int64_t* ptr,
                const int32_t* idx, int phs_p, int poly_deg, int deriv_k, double* wx,
                double* wy);
+/* or_weights on OpenMP threads (chunks of 64 stencils, the same bits);
+ * nthreads <= 0 keeps OpenMP's default */
+int or_weights_par(const double* x, const double* y, int64_t N, const int64_t* ptr,
+                   const int32_t* idx, int phs_p, int poly_deg, int deriv_k, double* wx,
+                   double* wy, int nthreads);
 double or_phs_deriv(int p, int k, int dir, double x, double y);
 
 /* ------------------------------------------------------------ sparse CSC */

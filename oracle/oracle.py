@@ -82,6 +82,8 @@ def _declare(L):
     L.or_bk_solve.argtypes = [C.c_int, _dp, C.c_int, _ip, C.c_int, _dp, C.c_int]
     L.or_weights.argtypes = [_dp, _dp, C.c_int64, _i64p, _i32p, C.c_int, C.c_int, C.c_int,
                              _dp, _dp]
+    L.or_weights_par.argtypes = [_dp, _dp, C.c_int64, _i64p, _i32p, C.c_int, C.c_int, C.c_int,
+                                 _dp, _dp, C.c_int]
     L.or_phs_deriv.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
     L.or_phs_deriv.restype = C.c_double
     L.or_csc_build.argtypes = [C.c_int64, _i64p, _i32p, _i64p, _i32p, _i64p]
@@ -191,15 +193,21 @@ def phs_deriv(p, k, direction, x, y):
     return lib().or_phs_deriv(p, k, direction, x, y)
 
 
-def weights(x, y, ptr, idx, phs_p, poly_deg, deriv_k=1):
-    """Ragged stencils: stencil e = idx[ptr[e]:ptr[e+1]] (centre first)."""
+def weights(x, y, ptr, idx, phs_p, poly_deg, deriv_k=1, nthreads=None):
+    """Ragged stencils: stencil e = idx[ptr[e]:ptr[e+1]] (centre first).
+    nthreads: None runs the serial loop; otherwise OpenMP over chunks of 64
+    stencils (0 = OpenMP's default thread count), the same bits."""
     ptr = np.ascontiguousarray(ptr, dtype=np.int64)
     idx = np.ascontiguousarray(idx, dtype=np.int32).ravel()
     n = len(ptr) - 1
     wx = np.zeros(len(idx))
     wy = np.zeros(len(idx))
-    rc = lib().or_weights(d(np.ascontiguousarray(x)), d(np.ascontiguousarray(y)), n, i64(ptr),
-                          i32(idx), phs_p, poly_deg, deriv_k, d(wx), d(wy))
+    xs, ys = np.ascontiguousarray(x, dtype=np.float64), np.ascontiguousarray(y, dtype=np.float64)
+    if nthreads is None:
+        rc = lib().or_weights(d(xs), d(ys), n, i64(ptr), i32(idx), phs_p, poly_deg, deriv_k, d(wx), d(wy))
+    else:
+        rc = lib().or_weights_par(d(xs), d(ys), n, i64(ptr), i32(idx), phs_p, poly_deg, deriv_k, d(wx),
+                                  d(wy), int(nthreads))
     return wx, wy, rc
 
 

@@ -389,6 +389,26 @@ def test_rhs_batch_variants_bitwise(mf, prec):
         assert np.array_equal(du, ref), key
 
 
+def test_timing_entry_points(mf):
+    """mf_time_rhs and mf_time_rhs_cold return positive times and leave the
+    state and du of an ordinary rhs! unchanged; n_calls < 1 is rejected."""
+    _say("timing entry points")
+    c = _rhs_residual_case(configs.c2(), 40)
+    c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+    c.ctx.rhs(0.001, L.MF_PREC_F64)
+    du0 = c.ctx.download(L.MF_BUF_DU)
+    warm, parts = c.ctx.time_rhs(0.001, L.MF_PREC_F64, 5)
+    cold = c.ctx.time_rhs_cold(0.001, L.MF_PREC_F64, 3)
+    _say("  warm %.4f ms cold %.4f ms" % (warm, cold))
+    assert 0.0 < warm < 100.0 and 0.0 < cold < 100.0 and np.all(np.isfinite(parts))
+    c.ctx.rhs(0.001, L.MF_PREC_F64)
+    assert np.array_equal(c.ctx.download(L.MF_BUF_DU), du0)
+    for f in (c.ctx.time_rhs_cold, c.ctx.time_rhs):
+        with pytest.raises(L.MfError) as e:
+            f(0.001, L.MF_PREC_F64, 0)
+        assert e.value.code == L.MF_ERR_ARGUMENT
+
+
 def test_set_tuning_rejects_bad_values(mf):
     c = cases.build(configs.c1(), side=16)
     with pytest.raises(L.MfError):

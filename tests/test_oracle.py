@@ -178,6 +178,21 @@ def test_weights_reproduce_polynomial_derivatives(p, deg, k):
     assert rel_inf(wx[e], w) < 1e-10
 
 
+def test_weights_par_matches_serial():
+    """The CPU baseline's OpenMP weights (chunks of 64 ragged stencils) give
+    the serial loop's bits."""
+    x, y, _, _, _ = O.cloud_square(30, 20261203)
+    N = len(x)
+    idx, _ = O.knn_grid(x, y, 45)
+    sizes = np.where(np.arange(N) % 7 == 0, 45, 30)  # ragged, two chunk-crossing sizes
+    ptr = np.r_[0, np.cumsum(sizes)].astype(np.int64)
+    ridx = np.concatenate([idx[e, :sizes[e]] for e in range(N)])
+    ws = O.weights(x, y, ptr, ridx, 5, 3, 1)
+    wp = O.weights(x, y, ptr, ridx, 5, 3, 1, nthreads=3)
+    assert ws[2] == 0 and wp[2] == 0
+    assert np.array_equal(ws[0], wp[0]) and np.array_equal(ws[1], wp[1])
+
+
 def test_hyperviscosity_weights_annihilate_cubics():
     x, y, _, _, _ = O.cloud_square(40, 20261202)
     N, k = len(x), 30

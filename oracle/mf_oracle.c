@@ -667,10 +667,10 @@ int or_weights_par(const double* x, const double* y, int64_t N, const int64_t* p
                    const int32_t* idx, int phs_p, int poly_deg, int deriv_k, double* wx,
                    double* wy, int nthreads)
 {
-    if (nthreads > 0) omp_set_num_threads(nthreads);
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
     const int64_t nch = (N + 63) / 64;
     int64_t first = 0;
-#pragma omp parallel for schedule(dynamic, 4)
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nt)
     for (int64_t ch = 0; ch < nch; ++ch) {
         const int64_t e0 = ch * 64, n = (N - e0 < 64) ? N - e0 : 64;
         const int r = or_weights(x, y, n, ptr + e0, idx, phs_p, poly_deg, deriv_k, wx, wy);
@@ -1198,13 +1198,13 @@ int or_rhs_par(const or_rhs_cfg* c, const int64_t* rptr, const int32_t* ridx, co
 {
     if (c->eq != OR_EQ_EULER || c->nvar != 4 || c->hv || c->lap || c->igr_alpha != 0.0) return -1;
     const int64_t N = c->N;
-    if (nthreads > 0) omp_set_num_threads(nthreads);
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
     memset(du, 0, sizeof(double) * 4 * N);
     or_bc_pass(c, t, u, du);
     double* F = scratch;         /* F1 in 4 N, then F2 in 4 N */
     double* eps = scratch + 8 * N;
     double* G = scratch + 9 * N; /* eps Dx u in 4 N, then eps Dy u in 4 N */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) num_threads(nt)
     for (int64_t e = 0; e < N; ++e) {
         double ue[4], f1[4], f2[4];
         for (int v = 0; v < 4; ++v) ue[v] = u[v * N + e];
@@ -1215,7 +1215,7 @@ int or_rhs_par(const or_rhs_cfg* c, const int64_t* rptr, const int32_t* ridx, co
             F[(4 + v) * N + e] = f2[v];
         }
     }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) num_threads(nt)
     for (int64_t e = 0; e < N; ++e)
         for (int v = 0; v < 4; ++v) {
             double a = 0.0;
@@ -1228,7 +1228,7 @@ int or_rhs_par(const or_rhs_cfg* c, const int64_t* rptr, const int32_t* ridx, co
         double mean[4] = {0.0, 0.0, 0.0, 0.0}, ninf[4] = {1.0, 1.0, 1.0, 1.0};
         if (c->rv_mode == OR_RV_RESIDUAL) {
             double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-#pragma omp parallel for schedule(static) reduction(+ : s0, s1, s2, s3)
+#pragma omp parallel for schedule(static) num_threads(nt) reduction(+ : s0, s1, s2, s3)
             for (int64_t e = 0; e < N; ++e) {
                 s0 += u[e];
                 s1 += u[N + e];
@@ -1240,7 +1240,7 @@ int or_rhs_par(const or_rhs_cfg* c, const int64_t* rptr, const int32_t* ridx, co
             mean[2] = s2 / (double)(4 * N);
             mean[3] = s3 / (double)(4 * N);
             double best[4] = {-1.0, -1.0, -1.0, -1.0};
-#pragma omp parallel
+#pragma omp parallel num_threads(nt)
             {
                 double mine[4] = {-1.0, -1.0, -1.0, -1.0};
 #pragma omp for schedule(static)
@@ -1256,7 +1256,7 @@ int or_rhs_par(const or_rhs_cfg* c, const int64_t* rptr, const int32_t* ridx, co
             }
             for (int v = 0; v < 4; ++v) ninf[v] = (best[v] == 0.0) ? DBL_EPSILON : best[v];
         }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) num_threads(nt)
         for (int64_t e = 0; e < N; ++e) {
             /* update_upwind_visc! (as or_upwind_visc) */
             const double rho = u[e], m1 = u[N + e], m2 = u[2 * N + e], E = u[3 * N + e];
@@ -1282,7 +1282,7 @@ int or_rhs_par(const or_rhs_cfg* c, const int64_t* rptr, const int32_t* ridx, co
             else
                 eps[e] = (eps_rv < eps_uw) ? eps_rv : eps_uw;
         }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) num_threads(nt)
         for (int64_t e = 0; e < N; ++e)
             for (int v = 0; v < 4; ++v) {
                 double gx = 0.0, gy = 0.0;
@@ -1293,7 +1293,7 @@ int or_rhs_par(const or_rhs_cfg* c, const int64_t* rptr, const int32_t* ridx, co
                 G[v * N + e] = eps[e] * gx;
                 G[(4 + v) * N + e] = eps[e] * gy;
             }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) num_threads(nt)
         for (int64_t col = 0; col < N; ++col)
             for (int v = 0; v < 4; ++v) {
                 double tx = 0.0, ty = 0.0;

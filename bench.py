@@ -16,6 +16,7 @@ Printed as one JSON line, with:
   f64_fma       the fused-multiply-add FP64 rhs! (faster, within 5e-12)
   split32       the split-FP32 rhs! rate, speed-up and error vs FP64
   adjoint       reverse-sweep / forward time of 100 SSPRK33 steps (65,536 nodes)
+  c1_advection  200 SSPRK33 steps of advection on 4,096 nodes, GPU and CPU oracle
   cold_cache    the same rhs! after a flush of the L2 and last-level caches
   weights       batched weight generation for 1,048,576 ragged stencils,
                 with the CPU oracle on a 32,768-stencil sample beside it
@@ -327,6 +328,66 @@ def ensemble(L, configs, steps):
                 node_evals_per_s_one_at_a_time=evals / wall_s, max_density_error_vs_exact=err)
 
 
+def advection_c1(L, configs, with_cpu):
+    """C1, the reference's own CPU-sized case: linear advection of a Gaussian
+    pulse on 4,096 nodes (r^3 + degree 2, 15-point stencils), 200 SSPRK33
+    steps from one captured graph; with the C oracle's 200 steps on the same
+    weights timed beside it and the final states compared."""
+    from mfrbf.clouds import cloud_square
+    from mfrbf.configs import advection_dt
+    from mfrbf.equations import LinearScalarAdvectionEquation2D
+
+    spec = configs.c1()
+    x, y, tag, _, _ = cloud_square(spec.side, spec.seed)
+    N = len(x)
+    ctx = L.Context(0)
+    ctx.set_equations(L.MF_EQ_ADVECTION_2D, spec.adv[0], spec.adv[1])
+    ctx.set_points(x, y)
+    idx, _, dx_min, _ = ctx.knn(spec.k)
+    wx, wy = ctx.compute_operator(spec.p, spec.deg, 1, L.MF_OP_FLUX, want=True, k=spec.k)
+    inflow = np.nonzero(tag == 1)[0].astype(np.int32)
+    outflow = np.nonzero(tag == 2)[0].astype(np.int32)
+    prm = spec.ic.params()
+    ctx.set_boundary([0, len(inflow), len(inflow) + len(outflow)], [L.MF_BC_DIRICHLET, L.MF_BC_DO_NOTHING],
+                     [spec.ic.code, spec.ic.code], np.concatenate([prm, prm]),
+                     np.concatenate([inflow, outflow]), np.zeros((len(inflow) + len(outflow), 2)))
+    u0 = np.ascontiguousarray(np.asarray(spec.ic(np.c_[x, y], 0.0, LinearScalarAdvectionEquation2D(spec.adv)),
+                                         dtype=np.float64).reshape(1, N))
+    dt, n = advection_dt(spec, dx_min), spec.nsteps
+    ctx.upload(L.MF_BUF_U, u0)
+    ctx.ssprk33(0.0, dt, n, flags=L.MF_STEP_GRAPH)  # warm: capture
+    ts = []
+    for _ in range(5):
+        ctx.upload(L.MF_BUF_U, u0)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.ssprk33(0.0, dt, n, flags=L.MF_STEP_GRAPH)
+        ctx.synchronize()
+        ts.append(time.perf_counter() - t0)
+    ug = ctx.download(L.MF_BUF_U)
+    ms = 1e3 * float(np.median(ts))
+    out = dict(nodes=N, steps=n, ms=ms, node_evals_per_s=3 * n * N / (ms * 1e-3),
+               timing="median of 5 runs of 200 graph-replayed steps, host clock around a synchronized call")
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        prob = O.Problem(x, y, O.ell_ptr(N, spec.k), idx, wx, wy, O.EQ_ADVECTION, 1, adv=spec.adv,
+                         boundary=[(O.BC_DIRICHLET, inflow, None, spec.ic.code, prm, None),
+                                   (O.BC_NOTHING, outflow, None, None, None, None)])
+        uo = u0.copy()
+        t0 = time.perf_counter()
+        prob.ssprk33(uo, 0.0, dt, n)
+        el = time.perf_counter() - t0
+        err = float(np.abs(ug - uo).max() / np.abs(uo).max())
+        if err > 1e-12:
+            raise RuntimeError("C1 final state differs from the oracle's: %.3e" % err)
+        out["cpu_baseline"] = dict(value=3 * n * N / el, unit="node-evals/s", cores=1, kind="port",
+                                   sample="the same 200 steps by the C oracle, 1 thread, %.2f s, %s"
+                                   % (el, _cpu_model()))
+        out["rel_err_vs_oracle"] = err
+    return out
+
+
 def weights_batch(L, configs, side, cpu_sample=0):
     from mfrbf.clouds import cloud_square
 
@@ -521,6 +582,8 @@ def main():
     if not args.no_extras:
         log("adjoint: 100 SSPRK33 steps, 65,536 nodes")
         out["adjoint"] = adjoint_ratio(L, configs)
+        log("C1: advection, 4,096 nodes, 200 SSPRK33 steps")
+        out["c1_advection"] = advection_c1(L, configs, not args.no_cpu)
         log("batched weights: 1,048,576 ragged stencils (and the CPU oracle on a sample)")
         out["weights"] = weights_batch(L, configs, 1024, 0 if args.no_cpu else 32768)
         log("ensemble: 64 clouds, 50 SSPRK33 steps each (split-FP32, graphs, concurrent streams)")

@@ -1175,8 +1175,10 @@ int or_rhs(const or_rhs_cfg* c, double t, double* u, double* du, double* eps_out
 
 /* ======================================================================
  * Multi-threaded rhs! for the CPU baseline timing only (not a parity
- * reference): or_rhs for the Euler equations with the residual or upwind
- * viscosity and strong boundary conditions, with the forward products as
+ * reference): the steps of or_rhs (pointcloudsolver/rbfsolver.jl:397-428;
+ * sources/hyperviscosity.jl:246-349 for eps, :382-409 for its apply) for
+ * the Euler equations with the residual or upwind viscosity and strong
+ * boundary conditions, with the forward products as
  * row gathers over the row-form stencil (rptr, ridx, rwx, rwy: the ELL or
  * ragged input of the weights) and the transposed products as column dots
  * over the CSC, each split over OpenMP threads.  Row sums and the mean run

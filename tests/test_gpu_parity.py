@@ -1,12 +1,14 @@
 """GPU parity: libmfrbf (through the C ABI) against the CPU oracle on the same
 seeded inputs.  Tolerances: integer outputs bit-exact; FP64 rhs! 1e-12
 norm-wise relative (max|d| / max|ref|); split-FP32 rhs! 1e-6."""
+import os
+
 import numpy as np
 import pytest
 
 import cases
 import oracle as O
-from conftest import rel_inf
+from conftest import GOLDEN, rel_inf
 from mfrbf import _lib as L
 from mfrbf import configs
 
@@ -100,6 +102,33 @@ def test_rhs_advection_f64(mf):
     u, du, uo, duo, _, _ = _rhs_pair(c, t=0.05)
     assert rel_inf(du, duo) < TOL64
     assert rel_inf(u, uo) < 1e-15
+
+
+def test_c1_against_fixture(mf):
+    """C1 through the device path (kNN, weights, rhs!, 200 graph-replayed
+    SSPRK33 steps) against the committed oracle fixture: stencil indices
+    bit for bit; weights 1e-9; du and the final state 1e-10 (the weights
+    differ from the fixture's in the last bits)."""
+    _say("C1 against the committed fixture")
+    ref = np.load(os.path.join(GOLDEN, "c1_advection.npz"))
+    spec = configs.c1()
+    ctx = L.Context(0)
+    ctx.set_points(ref["x"], ref["y"])
+    idx, _, _, _ = ctx.knn(spec.k)
+    assert np.array_equal(idx, ref["idx"])
+    wx, wy = ctx.compute_operator(spec.p, spec.deg, 1, L.MF_OP_FLUX, want=True, k=spec.k)
+    assert rel_inf(wx, ref["wx"]) < 1e-9 and rel_inf(wy, ref["wy"]) < 1e-9
+    c = cases.build(spec, weights="gpu")  # the same cloud; device weights, oracle stencils
+    assert np.array_equal(c.x, ref["x"]) and np.array_equal(c.idx, ref["idx"])
+    u = ref["u0"].copy()
+    du = np.zeros_like(u)
+    c.ctx.rhs_host(0.0, u, du, L.MF_PREC_F64)
+    assert rel_inf(du, ref["du0"]) < 1e-10
+    c.ctx.upload(L.MF_BUF_U, ref["u0"].copy())
+    c.ctx.ssprk33(0.0, float(ref["dt"]), int(ref["nsteps"]), flags=L.MF_STEP_GRAPH)
+    ug = c.ctx.download(L.MF_BUF_U)
+    _say("  final state rel err %.2e" % rel_inf(ug, ref["u_final"]))
+    assert rel_inf(ug, ref["u_final"]) < 1e-10
 
 
 def test_rhs_euler_f64(mf):

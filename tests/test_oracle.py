@@ -1,6 +1,7 @@
 """Pins of the CPU oracle (no GPU): the reference's own known-answer and
 identity tests, library routines, and what the mathematics fixes."""
 import os
+import sys
 
 import numpy as np
 import pytest
@@ -638,3 +639,19 @@ def test_igr_source_terms_in_du():
     assert np.all(d[0] == 0.0) and np.all(d[3] == 0.0)
     assert np.abs(d[1] + Dx @ sig).max() < 1e-12 * np.abs(Dx @ sig).max()
     assert np.abs(d[2] + Dy @ sig).max() < 1e-12 * np.abs(Dy @ sig).max()
+
+
+def test_c1_fixture_reproduced():
+    """The oracle still computes the committed C1 fixture (cloud, stencils,
+    weights, du at t = 0, the state after 200 SSPRK33 steps) bit for bit."""
+    sys.path.insert(0, os.path.join(GOLDEN))
+    import make_c1_fixture
+
+    ref = np.load(os.path.join(GOLDEN, "c1_advection.npz"))
+    now = make_c1_fixture.compute()
+    assert sorted(ref.files) == sorted(now)
+    for name in ref.files:
+        assert np.array_equal(ref[name], now[name]), name
+    # the pulse has moved and stayed bounded
+    assert 0.5 < np.abs(ref["u_final"]).max() < 1.01
+

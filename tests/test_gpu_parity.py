@@ -131,6 +131,33 @@ def test_c1_against_fixture(mf):
     assert rel_inf(ug, ref["u_final"]) < 1e-10
 
 
+def test_euler_rv_against_fixture(mf):
+    """Three SSPRK33 steps of Euler with the residual viscosity and the
+    history callback (32 x 32 nodes) against the committed oracle fixture:
+    the state to 1e-12; approx_du and eps of an rhs! at the new state to
+    1e-10 and 1e-9, codes exactly.  approx_du is a time difference with
+    weights of order 1/dt, which scales the 1e-15 differences of the states
+    by about 1e3, and eps's residual |approx_du - du| cancels again: on the
+    CPU, noise of 2e-15 on u and 2e-12 on approx_du moves eps by 1.4e-11
+    and flips no code."""
+    _say("Euler + residual viscosity against the committed fixture")
+    ref = np.load(os.path.join(GOLDEN, "euler_rv_1024.npz"))
+    c = cases.build(configs.c2(), side=32, visc="residual")
+    assert np.array_equal(c.u0, ref["u0"])
+    dt, n = float(ref["dt"]), int(ref["nsteps"])
+    c.ctx.upload(L.MF_BUF_U, ref["u0"].copy())
+    c.ctx.ssprk33(0.0, dt, n, flags=L.MF_STEP_HISTORY | L.MF_STEP_GRAPH)
+    ug = c.ctx.download(L.MF_BUF_U)
+    assert rel_inf(ug, ref["u_final"]) < 1e-12
+    assert rel_inf(c.ctx.approx_du(), ref["approx_du"]) < 1e-10
+    c.ctx.rhs(n * dt, L.MF_PREC_F64_REF)
+    geps, gcode = c.ctx.viscosity()
+    _say("  approx_du rel err %.2e, eps rel err %.2e" % (rel_inf(c.ctx.approx_du(), ref["approx_du"]),
+                                                         rel_inf(geps, ref["eps"])))
+    assert np.array_equal(gcode, ref["eps_code"])
+    assert rel_inf(geps, ref["eps"]) < 1e-9
+
+
 def test_rhs_euler_f64(mf):
     _say("rhs euler 64x64 k=30")
     c = cases.build(configs.c2(), side=64)

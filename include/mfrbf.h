@@ -222,7 +222,8 @@ int mf_set_source_order(mf_ctx* ctx, const int32_t* kinds, int n);
  * alpha = 0 removes the source. */
 int mf_set_source_igr(mf_ctx* ctx, double alpha, int maxiter);
 /* sigma of the last rhs! with the IGR source (n_points doubles, the
- * caller's order); replaces SourceIGR's cache.sigma (IGR.jl:97, :344-349). */
+ * caller's order); replaces SourceIGR's cache.sigma (IGR.jl:113), which
+ * update_sigma! fills (IGR.jl:170-190). */
 int mf_get_igr_sigma(mf_ctx* ctx, double* sigma_out);
 
 /* --------------------------------------------------------------- state */
@@ -260,7 +261,9 @@ int mf_get_viscosity(mf_ctx* ctx, double* eps, int8_t* eps_c);
 /* --------------------------------------------------- step/stage hooks */
 /* HistoryCallback affect! for the viscosity source: shift the solution
  * history with u = MF_BUF_U at time t and rebuild approx_du
- * (callbacks_step/history.jl:178-245).  success_iter = integrator.success_iter. */
+ * (modify_cache!, callbacks_step/history.jl:91-103; shift_soln_history!,
+ * update_approx_du!, time_deriv_weights!, :105-152).  success_iter =
+ * integrator.success_iter. */
 int mf_history_update(mf_ctx* ctx, double t, int success_iter);
 int mf_get_approx_du(mf_ctx* ctx, double* const* fields);
 

@@ -1131,7 +1131,7 @@ __global__ void k_advance_time(StepScalars* sc, double dt)
     }
 }
 
-// HistoryCallback (history.jl:207-268).  Single thread: shift the time
+// HistoryCallback (history.jl:91-152).  Single thread: shift the time
 // history, count the points in use and solve the transposed Vandermonde
 // system for the derivative weights (partial-pivoting LU, as `A' \ b`).
 // One thread.  The step scalars are read once into registers, the
@@ -1272,7 +1272,7 @@ __global__ __launch_bounds__(64) void k_history_weights(StepScalars* sc, int P, 
 }
 
 // Snapshot u into the ring slot of the most recent entry and rebuild
-// approx_du = sum_i w_i u^(i) (history.jl:221-245).
+// approx_du = sum_i w_i u^(i) (history.jl:113-129).
 __global__ __launch_bounds__(kBlock) void k_history_apply(int64_t NV, int P, const double* u,
                                                           double* hist, double* approx,
                                                           const StepScalars* sc)

@@ -23,6 +23,7 @@ using MeshfreeTrixi
 using MeshfreeTrixi: AbstractRBFEngine, PointCloudSolver, PointCloudDomain, SourceTerms,
                      SourceHyperviscosityFlyer, SourceResidualViscosityTominec,
                      SourceUpwindViscosityTominec, BoundaryConditionDoNothing, FluxZero
+using SparseArrays
 using StructArrays
 using Trixi
 
@@ -36,13 +37,20 @@ const MF_FN_TABLE = Cint(4)
 const MF_OP_FLUX, MF_OP_HYPERVISCOSITY, MF_OP_LAPLACIAN = Cint(0), Cint(1), Cint(2)
 const MF_SRC_FLYER, MF_SRC_TOMINEC, MF_SRC_VISCOSITY, MF_SRC_IGR = Int32(1), Int32(2), Int32(3), Int32(4)
 const MF_VISC_NONE, MF_VISC_RESIDUAL, MF_VISC_UPWIND = Cint(0), Cint(1), Cint(2)
-const MF_PREC_F64, MF_PREC_SPLIT32 = Cint(0), Cint(1)
+const MF_PREC_F64, MF_PREC_SPLIT32, MF_PREC_F64_REF = Cint(0), Cint(1), Cint(2)
+
+const PRECISIONS = Dict(:f64 => MF_PREC_F64_REF, :f64_fma => MF_PREC_F64, :split32 => MF_PREC_SPLIT32)
 
 """
     RBFFDEngineHIP(; device = 0, precision = :f64)
 
-RBF-FD backend on a HIP device; `precision = :split32` selects FP32 stencil
-sums on centred differences (FP64 state).
+RBF-FD backend on a HIP device.  `precision = :f64` runs the stencil sums in
+the reference's operation order (SparseArrays' CSC `mul!`: ascending column,
+rounded products), which reproduces the reference's rhs! bit for bit where
+no transcendental enters; `:f64_fma` uses fused multiply-adds in stencil
+order (faster by about 1%, about 1e-12 relative from the reference at
+262,144 nodes); `:split32` runs FP32 stencil sums on centred differences
+(FP64 state, about 5e-7 relative).
 """
 mutable struct RBFFDEngineHIP <: AbstractRBFEngine
     ctx::Ptr{Cvoid}
@@ -53,7 +61,8 @@ mutable struct RBFFDEngineHIP <: AbstractRBFEngine
         h = Ref{Ptr{Cvoid}}(C_NULL)
         rc = ccall((:mf_ctx_create, libmfrbf), Cint, (Cint, Ref{Ptr{Cvoid}}), device, h)
         rc == MF_OK || error("mf_ctx_create failed ($rc)")
-        e = new(h[], Cint(device), precision === :split32 ? MF_PREC_SPLIT32 : MF_PREC_F64, false)
+        haskey(PRECISIONS, precision) || error("RBFFDEngineHIP: precision must be :f64, :f64_fma or :split32")
+        e = new(h[], Cint(device), PRECISIONS[precision], false)
         finalizer(e -> ccall((:mf_ctx_destroy, libmfrbf), Cint, (Ptr{Cvoid},), e.ctx), e)
         return e
     end
@@ -109,8 +118,17 @@ function install!(e::RBFFDEngineHIP, domain, equations, boundary_conditions, sou
     nodes = Int32[]; normals = Float64[]
     for (key, bc) in zip(keys(boundary_conditions), boundary_conditions)
         bd = domain.boundary_tags[key]
-        push!(kinds, bc isa BoundaryConditionDirichlet ? MF_BC_DIRICHLET :
-                     bc isa BoundaryConditionDoNothing ? MF_BC_DO_NOTHING : MF_BC_SLIP_WALL)
+        # the kinds calc_single_boundary_flux! can apply (PointCloudBCs.jl:49-115);
+        # anything else has no method there and raises here
+        if bc isa BoundaryConditionDirichlet
+            push!(kinds, MF_BC_DIRICHLET)
+        elseif bc isa BoundaryConditionDoNothing
+            push!(kinds, MF_BC_DO_NOTHING)
+        elseif bc === Trixi.boundary_condition_slip_wall
+            push!(kinds, MF_BC_SLIP_WALL)
+        else
+            throw(ArgumentError("RBFFDEngineHIP: unsupported boundary condition $(typeof(bc)) on :$key"))
+        end
         push!(fns, MF_FN_TABLE)          # values uploaded per call (any Julia function)
         append!(params, zeros(8))
         append!(nodes, Int32.(bd.idx .- 1))
@@ -126,7 +144,7 @@ function install!(e::RBFFDEngineHIP, domain, equations, boundary_conditions, sou
     order = Int32[]  # the SourceTerms keyword order (generic_sources.jl:7-18)
     for s in values(source_terms)
         if s isa SourceIGR
-            # sigma by cg! with maxiter = 20 (IGR.jl:183)
+            # sigma by cg! with maxiter = 20 (IGR.jl:189)
             check(e, ccall((:mf_set_source_igr, libmfrbf), Cint, (Ptr{Cvoid}, Cdouble, Cint),
                            e.ctx, s.cache.alpha, 20))
             push!(order, MF_SRC_IGR)
@@ -141,11 +159,13 @@ function install!(e::RBFFDEngineHIP, domain, equations, boundary_conditions, sou
                            e.ctx, s.cache.gamma))
             push!(order, MF_SRC_TOMINEC)
         elseif s isa SourceHyperviscosityFlyer
-            basis = solver.basis
-            check(e, ccall((:mf_compute_operator, libmfrbf), Cint,
-                           (Ptr{Cvoid}, Cint, Cint, Cint, Cint, Ptr{Cdouble}, Ptr{Cdouble}),
-                           e.ctx, basis.approximation_type.Nrbf, basis.N, 2 * s.cache.k, MF_OP_HYPERVISCOSITY,
-                           C_NULL, C_NULL))
+            # the cache holds the assembled operator sum(compute_flux_operator(solver,
+            # domain, 2k)) and gamma (hyperviscosity.jl:33-50), not k: upload its
+            # entries in stencil order
+            H = stencil_values(s.cache.hv_differentiation_matrix, domain.pd)
+            check(e, ccall((:mf_set_operator, libmfrbf), Cint,
+                           (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{Cdouble}),
+                           e.ctx, MF_OP_HYPERVISCOSITY, H, C_NULL))
             check(e, ccall((:mf_set_source_hyperviscosity, libmfrbf), Cint, (Ptr{Cvoid}, Cdouble),
                            e.ctx, s.cache.gamma))
             hv_first = true
@@ -193,18 +213,47 @@ function Trixi.rhs!(du, u, t, domain, equations, initial_condition, boundary_con
     return nothing
 end
 
-# --- HistoryCallback hook (replaces callbacks_step/history.jl:207-219)
-function MeshfreeTrixi.modify_cache!(source::SourceResidualViscosityTominec, u, t, approx_order,
-                                     integrator)
-    semi = integrator.p
+# Entries of an assembled N x N operator (SparseMatrixCSC, the reference's
+# compute_flux_operator output, compute_operators.jl:443-452) in stencil
+# order: row i, slot j holds A[i, neighbors[i][j]]; an (nv, N) matrix whose
+# column i is node i's row, the layout mf_set_operator reads.
+function stencil_values(A, pd)
+    At = sparse(transpose(A))  # column i of At = row i of A
+    cols, vals = rowvals(At), nonzeros(At)
+    W = zeros(Float64, pd.num_neighbors, pd.num_points)
+    for i in 1:pd.num_points
+        for (j, q) in enumerate(pd.neighbors[i])
+            for r in nzrange(At, i)
+                if cols[r] == q
+                    W[j, i] = vals[r]
+                    break
+                end
+            end
+        end
+    end
+    return W
+end
+
+# --- HistoryCallback hook: the reference's update_history! loops over the
+# sources and calls modify_cache! on each (callbacks_step/history.jl:77-103).
+# For a semidiscretization on this engine the history and approx_du live on
+# the device, so this more specific method takes the place of that loop for
+# the HIP solver only; the reference's modify_cache! methods, and every
+# other engine, stay untouched.
+const HIPSemi = Trixi.SemidiscretizationHyperbolic{<:Any, <:Any, <:Any, <:Any, <:Any,
+                                                   <:PointCloudSolver{2, <:Any, <:Any, RBFFDEngineHIP}}
+
+function MeshfreeTrixi.update_history!(semi::HIPSemi, u, t, approx_order, integrator)
     e = semi.solver.engine
-    e isa RBFFDEngineHIP || return invoke(MeshfreeTrixi.modify_cache!,
-                                          Tuple{SourceResidualViscosityTominec, Any, Any, Any, Any},
-                                          source, u, t, approx_order, integrator)
-    check(e, ccall((:mf_state_upload, libmfrbf), Cint, (Ptr{Cvoid}, Cint, Ptr{Ptr{Cdouble}}),
-                   e.ctx, 0, fieldptrs(u)))
-    check(e, ccall((:mf_history_update, libmfrbf), Cint, (Ptr{Cvoid}, Cdouble, Cint),
-                   e.ctx, t, integrator.success_iter))
+    for source in values(semi.source_terms)
+        source isa SourceResidualViscosityTominec || continue
+        source.cache.success_iter .= integrator.success_iter  # as modify_cache!, history.jl:96
+        check(e, ccall((:mf_state_upload, libmfrbf), Cint, (Ptr{Cvoid}, Cint, Ptr{Ptr{Cdouble}}),
+                       e.ctx, 0, fieldptrs(u)))
+        check(e, ccall((:mf_history_update, libmfrbf), Cint, (Ptr{Cvoid}, Cdouble, Cint),
+                       e.ctx, t, integrator.success_iter))
+    end
+    return nothing
 end
 
 # --- stage limiter (replaces callbacks_stage/positivity_zhang_shu_point2d.jl:22-83)

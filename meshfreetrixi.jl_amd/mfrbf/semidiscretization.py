@@ -169,7 +169,7 @@ def rhs_(du, u, semi, t, precision="f64"):
 
 # ---------------------------------------------------------- callbacks
 class HistoryCallback:
-    """HistoryCallback(; approx_order) (history.jl:159-165)."""
+    """HistoryCallback(; approx_order) (history.jl:43-51)."""
 
     def __init__(self, approx_order):
         self.approx_order = int(approx_order)

@@ -80,7 +80,7 @@ class SourceIGR:
     (1/rho - alpha (Dx diag(1/rho) Dx + Dy diag(1/rho) Dy)) sigma =
     alpha (trace^2 + trace_sq), then du_m1 -= Dx sigma, du_m2 -= Dy sigma.
     The reference passes linear_solver = cg and calls it as
-    linear_solver(sigma, A, b; maxiter = 20) (IGR.jl:183), which
+    linear_solver(sigma, A, b; maxiter = 20) (IGR.jl:189), which
     IterativeSolvers.jl provides as cg!(sigma, A, b); that is what runs here.
     Euler equations; it must be the last source in SourceTerms."""
 

@@ -1311,11 +1311,11 @@ int or_rhs_par(const or_rhs_cfg* c, const int64_t* rptr, const int32_t* ridx, co
 }
 
 /* ======================================================================
- * HistoryCallback, callbacks_step/history.jl:207-268.
+ * HistoryCallback, callbacks_step/history.jl:91-152.
  * time_hist[0] is the most recent time; sol_hist holds approx_order+1
  * snapshots of nvals values each, slot 0 the most recent.
  * ==================================================================== */
-/* time_deriv_weights!, history.jl:247-268: w = scale * (A' \ b) with the
+/* time_deriv_weights!, history.jl:131-152: w = scale * (A' \ b) with the
  * Vandermonde A[:, k] = t_.^(k-1) at t_ = t*scale, b[k] = (k-1) t_1^(k-2).
  * The dense solve restates LAPACK DGETRF/DGETRS (partial pivoting). */
 void or_time_deriv_weights(int q, const double* t, double* w)
@@ -1370,7 +1370,7 @@ void or_time_deriv_weights(int q, const double* t, double* w)
     for (int i = 0; i < q; ++i) w[i] = scale * b[i];
 }
 
-/* modify_cache!(::SourceResidualViscosityTominec), history.jl:207-245:
+/* modify_cache!(::SourceResidualViscosityTominec), history.jl:91-129:
  * shift_soln_history! then update_approx_du! (the accumulation sits in
  * the @muladd block, hence fma). */
 void or_history_update(int64_t nvals, int approx_order, int success_iter, double t,
@@ -1448,7 +1448,7 @@ void or_limiter_zhang_shu(int64_t N, int nvar, double gamma, const int64_t* ptr,
  *   u2 = (3 u_n + u1 + dt f(u1)) / 4      at t + dt
  *   u_{n+1} = (u_n + 2 u2 + 2 dt f(u2))/3 at t + dt/2; f(u_{n+1}) at t + dt
  * with a stage limiter after each stage, success_iter += 1 per step, and
- * the HistoryCallback (history.jl:159-191) after each step and at init.
+ * the HistoryCallback (history.jl:43-75) after each step and at init.
  * fsal must hold f(u_n) on entry when *success_iter_io > 0 (a restart);
  * on a fresh start (*success_iter_io == -1) it is computed here.
  * ==================================================================== */

@@ -12,12 +12,13 @@ class Case:
 
 
 def build(spec, side=None, use_gpu=True, weights="oracle", hv=None, visc=None, bc=("dirichlet", "nothing"),
-          knn="grid", node_order=0, tom=None, order=None, igr=None, igr_maxiter=20):
+          knn="grid", node_order=0, tom=None, order=None, igr=None, igr_maxiter=20, c_rv=1.0, c_uw=1.0):
     """weights: 'oracle' installs the oracle's weights on the GPU (so the
     rhs! comparison isolates the stencil kernels), 'gpu' keeps the GPU's own.
     hv = (k, c): Flyer source; tom = c: Tominec source; igr = alpha: IGR
     source; order: the sources in keyword order, names 'flyer', 'tominec',
-    'visc', 'igr' (default: that order)."""
+    'visc', 'igr' (default: that order); c_rv, c_uw: the viscosity
+    source's coefficients."""
     c = Case()
     c.spec = spec
     side = side or spec.side
@@ -80,7 +81,7 @@ def build(spec, side=None, use_gpu=True, weights="oracle", hv=None, visc=None, b
         rv_mode = {"residual": O.RV_RESIDUAL, "upwind": O.RV_UPWIND}[visc]
     c.prob = O.Problem(x, y, ptr, idx, wx, wy, c.eq, c.nvar, gamma=spec.gamma, adv=spec.adv,
                        boundary=[(ko, nodes, nrm, fn, params, None) for ((kg, ko), nodes, nrm) in c.bc],
-                       hv=c.hv_w, hv_gamma=c.hv_gamma, rv_mode=rv_mode, c_rv=1.0, c_uw=1.0,
+                       hv=c.hv_w, hv_gamma=c.hv_gamma, rv_mode=rv_mode, c_rv=c_rv, c_uw=c_uw,
                        dx_avg=c.dx_avg, hv_first=hv_first, lap=c.lap_w, tom_gamma=c.tom_gamma,
                        order=[names[n][0] for n in order], igr_alpha=igr or 0.0, igr_maxiter=igr_maxiter)
     if not use_gpu:
@@ -117,7 +118,7 @@ def build(spec, side=None, use_gpu=True, weights="oracle", hv=None, visc=None, b
         ctx.set_source_hyperviscosity_tominec(c.tom_gamma)
     if visc is not None:
         mode = {"residual": L.MF_VISC_RESIDUAL, "upwind": L.MF_VISC_UPWIND}[visc]
-        ctx.set_source_viscosity(mode, 1.0, 1.0, c.dx_avg, getattr(spec, "approx_order", 3), hv_first)
+        ctx.set_source_viscosity(mode, c_rv, c_uw, c.dx_avg, getattr(spec, "approx_order", 3), hv_first)
     if igr is not None:
         ctx.set_source_igr(igr, igr_maxiter)
     ctx.set_source_order([names[n][1] for n in order])

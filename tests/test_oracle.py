@@ -357,6 +357,151 @@ def test_update_visc_branches():
     assert list(code) == [1, 1, 2, 1, 1] and eps[2] == np.finfo(float).eps
 
 
+# Known answers for the residual-viscosity indicator of a whole rhs!
+# (hyperviscosity.jl:289-349, auxiliary/mpi.jl:40-52, 71-82), computed here
+# from the formulas with numpy and compared with the oracle's eps.  Each
+# test also evaluates the plausible mistakes (h for h^2, dx_min for dx_avg,
+# mean over N instead of 4N, component-wise max, du after the second
+# boundary pass, u before the first) and checks that every one of them
+# would miss the oracle by far more than the tolerance.
+def _euler_flux_np(u, gamma=1.4):
+    rho, m1, m2, E = u
+    v1, v2 = m1 / rho, m2 / rho
+    p = (gamma - 1.0) * (E - 0.5 * (m1 * v1 + m2 * v2))
+    f1 = np.stack([m1, m1 * v1 + p, m2 * v1, (E + p) * v1])
+    f2 = np.stack([m2, m1 * v2, m2 * v2 + p, (E + p) * v2])
+    return f1, f2, v1, v2, p
+
+
+def _eps_rv_np(c, u_seen, du_seen, approx_du, h, c_rv, mean_div=4, lex=True):
+    N = u_seen.shape[1]
+    mean = u_seen.sum(axis=1) / (mean_div * N)
+    dev = np.abs(u_seen - mean[:, None])
+    if lex:
+        j = max(range(N), key=lambda i: tuple(dev[:, i]))
+        ninf = dev[:, j].copy()
+    else:
+        ninf = dev.max(axis=1)
+    ninf[ninf == 0.0] = np.finfo(float).eps
+    return 0.5 * c_rv * h * h * (np.abs(approx_du - du_seen) / ninf[:, None]).max(axis=0)
+
+
+def test_residual_viscosity_constant_state_closed_form():
+    """u = c at every node (no boundary conditions): the flux divergence
+    vanishes to rounding, the mean is c/4 (sum over all 4N scalars), every
+    node ties at |c - c/4| = 3|c|/4, so n_v = 3|c_v|/4 and
+    eps_rv,i = c_rv/2 * dx_avg^2 * max_v |approx_du_v,i| / (3|c_v|/4)."""
+    c = cases.build(configs.c2(), side=14, use_gpu=False, bc=("nothing", "nothing"))
+    N = c.N
+    const = np.array([1.0, 0.5, -0.25, 2.5])
+    u = np.ascontiguousarray(np.repeat(const[:, None], N, axis=1))
+    rng = np.random.default_rng(3)
+    A = rng.uniform(0.5, 1.5, (4, N)) * rng.choice([-1.0, 1.0], (4, N))
+    c_rv, h = 0.7, c.dx_avg
+    assert abs(c.dx_min / c.dx_avg - 1.0) > 0.05  # the two spacings are distinguishable
+    prob = O.Problem(c.x, c.y, c.ptr, c.idx, c.wx, c.wy, O.EQ_EULER, 4, rv_mode=O.RV_RESIDUAL, c_rv=c_rv,
+                     c_uw=1e6, dx_avg=h)
+    _, eps, code = prob.rhs(u.copy(), 0.0, success_iter=2, approx_du=A)
+    ninf = 0.75 * np.abs(const)
+    expect = 0.5 * c_rv * h * h * (np.abs(A) / ninf[:, None]).max(axis=0)
+    assert np.all(code == 0)
+    assert rel_inf(eps, expect) < 1e-10
+    wrong = {
+        "h": 0.5 * c_rv * h * (np.abs(A) / ninf[:, None]).max(axis=0),
+        "dx_min": 0.5 * c_rv * c.dx_min ** 2 * (np.abs(A) / ninf[:, None]).max(axis=0),
+        "mean/N": _eps_rv_np(c, u, np.zeros_like(u), A, h, c_rv, mean_div=1),
+        "c_rv twice": c_rv * h * h * (np.abs(A) / ninf[:, None]).max(axis=0),
+    }
+    for name, w in wrong.items():
+        assert rel_inf(w, expect) > 1e-3, name
+
+
+def _rv_case(c_uw):
+    """Euler on 16 x 16 nodes, Dirichlet density-wave inflow and
+    do-nothing outflow, a seeded non-smooth state and approx_du."""
+    c = cases.build(configs.c2(), side=16, use_gpu=False, visc="residual", c_rv=0.8, c_uw=c_uw)
+    N = c.N
+    rng = np.random.default_rng(11)
+    r = rng.uniform(0.0, 1.0, (4, N))
+    u = np.ascontiguousarray(np.stack([1.0 + 0.2 * r[0], -3.0 + 0.5 * r[1], -0.2 + 0.1 * r[2], 6.5 + 0.2 * r[3]]))
+    inflow = c.bc[0][1]
+    # the first boundary pass sets the inflow nodes to the density wave at t = 0
+    xb, yb = c.x[inflow], c.y[inflow]
+    rho_b = 1.0 + 0.2 * np.sin(2.0 * np.pi * (xb + yb))
+    u_seen = u.copy()
+    u_seen[:, inflow] = np.stack([rho_b, rho_b, rho_b, 1.0 / 0.4 + rho_b])
+    f1, f2, v1, v2, p = _euler_flux_np(u_seen)
+    du_seen = -(np.einsum("is,vis->vi", c.wx, f1[:, c.idx]) + np.einsum("is,vis->vi", c.wy, f2[:, c.idx]))
+    A = du_seen + rng.uniform(0.5, 1.5, (4, N)) * rng.choice([-1.0, 1.0], (4, N))
+    eps_uw = c_uw * 0.5 * c.dx_avg * (np.sqrt(v1 * v1 + v2 * v2) + np.sqrt(1.4 * p / u_seen[0]))
+    return c, u, u_seen, du_seen, A, eps_uw, inflow
+
+
+def test_residual_viscosity_known_answer_with_boundary():
+    """eps_rv on a non-smooth state with Dirichlet inflow nodes: the mean and
+    the lexicographic max over the state after the first boundary pass, the
+    residual against du before the second pass (the flux divergence, not the
+    zeroed Dirichlet rows).  eps_uw is made large so that eps = eps_rv."""
+    c, u, u_seen, du_seen, A, _, inflow = _rv_case(1e6)
+    _, eps, code = c.prob.rhs(u.copy(), 0.0, success_iter=1, approx_du=A)
+    h = c.dx_avg
+    expect = _eps_rv_np(c, u_seen, du_seen, A, h, 0.8)
+    assert np.all(code == 0)
+    assert rel_inf(eps, expect) < 1e-10
+    du_post = du_seen.copy()
+    du_post[:, inflow] = 0.0
+    f1, f2, _, _, _ = _euler_flux_np(u)
+    du_pre = -(np.einsum("is,vis->vi", c.wx, f1[:, c.idx]) + np.einsum("is,vis->vi", c.wy, f2[:, c.idx]))
+    dev = np.abs(u_seen - (u_seen.sum(1) / (4 * c.N))[:, None])
+    assert np.argmax(dev[0]) != np.argmax(dev[1])  # the lexicographic and component-wise maxima differ
+    wrong = {
+        "h": expect / h,
+        "dx_min": expect * (c.dx_min / h) ** 2,
+        "mean/N": _eps_rv_np(c, u_seen, du_seen, A, h, 0.8, mean_div=1),
+        "component-wise max": _eps_rv_np(c, u_seen, du_seen, A, h, 0.8, lex=False),
+        "du after the second boundary pass": _eps_rv_np(c, u_seen, du_post, A, h, 0.8),
+        "u before the first boundary pass": _eps_rv_np(c, u, du_pre, A, h, 0.8),
+    }
+    for name, w in wrong.items():
+        assert rel_inf(w, expect) > 1e-4, name
+
+
+def test_residual_viscosity_branches_known_answer():
+    """eps = min(eps_rv, eps_uw) with code 0 where eps_rv < eps_uw, 1 else
+    (update_visc!, hyperviscosity.jl:331-349), with c_uw chosen so that both
+    branches occur; success_iter = 0 gives eps_uw with code 1 everywhere."""
+    c, u, u_seen, du_seen, A, _, _ = _rv_case(1.0)
+    rv = _eps_rv_np(c, u_seen, du_seen, A, c.dx_avg, 0.8)
+    _, _, _, _, _, uw1, _ = _rv_case(1.0)
+    c_uw = float(np.median(rv / uw1))
+    c, u, u_seen, du_seen, A, uw, _ = _rv_case(c_uw)
+    _, eps, code = c.prob.rhs(u.copy(), 0.0, success_iter=4, approx_du=A)
+    expect_code = np.where(rv < uw, 0, 1)
+    clear = np.abs(rv / uw - 1.0) > 1e-9
+    assert 0.2 < np.mean(expect_code) < 0.8
+    assert np.array_equal(code[clear], expect_code[clear])
+    assert rel_inf(eps, np.minimum(rv, uw)) < 1e-10
+    _, eps0, code0 = c.prob.rhs(u.copy(), 0.0, success_iter=0, approx_du=A)
+    assert np.all(code0 == 1) and rel_inf(eps0, uw) < 1e-13
+
+
+def test_host_viscosity_spacing_is_dx_avg():
+    """The host API passes PointData's dx_avg (the mean nearest-neighbour
+    distance) as h of both viscosity sources and dx_min to the
+    hyperviscosity sources (hyperviscosity.jl:47, :116, :280, :324)."""
+    from mfrbf.domain import PointData
+    from mfrbf.sources import (SourceHyperviscosityFlyer, SourceHyperviscosityTominec,
+                               SourceResidualViscosityTominec, SourceUpwindViscosityTominec)
+
+    class Dom:
+        pd = PointData(np.zeros((2, 2)), np.zeros((2, 1), np.int32), 0.01, 0.03)
+
+    assert SourceResidualViscosityTominec(None, None, Dom).h == 0.03
+    assert SourceUpwindViscosityTominec(None, None, Dom).h == 0.03
+    assert SourceHyperviscosityFlyer(None, None, Dom, k=2, c=3.0).gamma == 3.0 * 0.01 ** 4
+    assert SourceHyperviscosityTominec(None, None, Dom, c=2.0).gamma == 2.0 * 0.01 ** 4.5
+
+
 # ----------------------------------------------------------------- history
 def test_history_closed_form():
     """test/history_test.jl:1-32: linear data gives derivative 1 (order 5)."""
@@ -401,6 +546,56 @@ def test_ssprk33_is_third_order_taylor_for_linear_systems():
     t2 = dt * (Lop @ t1) / 2
     t3 = dt * (Lop @ t2) / 3
     assert rel_inf(u[0], v + t1 + t2 + t3) < 1e-13
+
+
+def test_ssprk33_history_interleave_hand_stages():
+    """Three SSPRK33 steps with the HistoryCallback (approx_order 3) and the
+    residual viscosity, stage by stage from single rhs! calls: the callback
+    runs at init (success_iter 0: approx_du = 0) and after each step with
+    success_iter 1, 2, 3; approx_du is then the 2-, 3- and 4-point backward
+    difference (1, -1)/dt, (3/2, -2, 1/2)/dt, (11/6, -3, 3/2, -1/3)/dt
+    (history.jl:91-152); the stages of step n see the success_iter and
+    approx_du of the callback after step n-1; the FSAL evaluation at the
+    end of a step runs before that step's callback and is not redone after
+    it (u_modified!(integrator, false), history.jl:73)."""
+    c = cases.build(configs.c2(), side=14, use_gpu=False, visc="residual")
+    u0 = c.u0.copy()
+    dt = c.dt
+    u = u0.copy()
+    c.prob.ssprk33(u, 0.0, dt, 3, approx_order=3)
+    got_u, got_ad = u, c.prob.approx_du.reshape(u0.shape).copy()
+
+    bdf = {2: [1.0, -1.0], 3: [1.5, -2.0, 0.5], 4: [11.0 / 6.0, -3.0, 1.5, -1.0 / 3.0]}
+
+    def run(redo_fsal):
+        f = lambda v, t, si, ad: c.prob.rhs(v, t, success_iter=si, approx_du=ad)[0]  # noqa: E731
+        hist = []  # most recent first
+        un = u0.copy()
+        ad = np.zeros_like(un)
+        hist.insert(0, un.copy())
+        k = f(un, 0.0, 0, ad)
+        t = 0.0
+        for n in range(3):
+            si = n
+            a = un + dt * k
+            k = f(a, t + dt, si, ad)
+            b = (3.0 * un + (a + dt * k)) / 4.0
+            k = f(b, t + dt / 2.0, si, ad)
+            un = (un + 2.0 * b + 2.0 * dt * k) / 3.0
+            k = f(un, t + dt, si, ad)  # FSAL, before the callback
+            t = t + dt
+            hist.insert(0, un.copy())
+            q = min(n + 2, 4)
+            ad = sum(w * h for w, h in zip(bdf[q], hist[:q])) / dt
+            if redo_fsal:
+                k = f(un, t, n + 1, ad)
+        return un, ad
+
+    u_ref, ad_ref = run(False)
+    assert rel_inf(got_u, u_ref) < 1e-13
+    assert rel_inf(got_ad, ad_ref) < 1e-8  # backward differences cancel about 4 digits
+    u_bad, _ = run(True)
+    assert rel_inf(u_bad, u_ref) > 1e-9  # the interleave is visible in the state
 
 
 # ----------------------------------------------------------------- limiter
@@ -623,7 +818,7 @@ def test_igr_operator_and_cg():
 
 def test_igr_source_terms_in_du():
     """The source adds -Dx sigma to du_m1 and -Dy sigma to du_m2 only
-    (flux_igr, IGR.jl:188-239); sigma is the CG solution from 0."""
+    (flux_igr, IGR.jl:193-239); sigma is the CG solution from 0."""
     x, y, ptr, idx, wx, wy, Dx, Dy, dx_min = _igr_case()
     rho = 1.0 + 0.2 * np.sin(2 * np.pi * (x + y))
     u = np.stack([rho, rho, rho, 2.5 + rho])

@@ -83,21 +83,22 @@ def rhs_bytes(N, k, V=4, visc=True):
     return flux
 
 
-def kernel_bytes(N, k, ti, V=4, passes=1):
+def kernel_bytes(N, k, ti, V=4):
     """Bytes the stencil kernel (phase A) and the transposed gather (phase B)
-    must move in the format they use, each array once (gathers of the halo
-    not counted).  Phase A reads u, the stencil positions, (Dx, Dy) and
+    must move in the format they use, each array counted once (gathers of
+    the halo not counted; the reference-order kernel reads its stencil
+    positions once for the x pass and again for the y pass, but the array is
+    counted once).  Phase A reads u, the stencil positions, (Dx, Dy) and
     approx_du and writes du, G, eps and its code; phase B reads the
     transposed positions and (Dx, Dy) (N k entries, padding not counted), G
     and du and writes du.  With LDS tiles the positions are 16-bit and each
     tile's halo list (int32) and offsets are read once; without, the
-    positions are int32 node indices.  passes: times the stencil positions
-    are read (2 in the reference-order kernel: x pass, then y pass)."""
+    positions are int32 node indices."""
     if ti["tiles_a"]:
-        a = N * (V * 8 + 2 * k * passes + 16 * k + V * 8 + V * 8 + 2 * V * 8 + 9) + 4 * ti["halo_a_total"] \
+        a = N * (V * 8 + 2 * k + 16 * k + V * 8 + V * 8 + 2 * V * 8 + 9) + 4 * ti["halo_a_total"] \
             + 8 * (ti["n_tiles"] + 1)
     else:
-        a = N * (V * 8 + 4 * k * passes + 16 * k + V * 8 + V * 8 + 2 * V * 8 + 9)
+        a = N * (V * 8 + 4 * k + 16 * k + V * 8 + V * 8 + 2 * V * 8 + 9)
     if ti["tiles_b"]:
         b = N * k * (2 + 16) + N * (2 * V * 8 + V * 8 + V * 8 + 1) + 4 * ti["halo_b_total"] \
             + ti["n_tiles"] * (256 + 4 * (8 + 4) + 8)
@@ -149,7 +150,7 @@ def tominec(L, ctx, info, spec, t_now, steps):
             ctx.time_rhs(t_now, prec, 20)
             ms, parts = ctx.time_rhs(t_now, prec, steps)
             ti = ctx.tile_info()
-            ba, bb = kernel_bytes(N, k, ti, passes=2 if name == "f64_ref" else 1)
+            ba, bb = kernel_bytes(N, k, ti)
             ba += N * k * 8 + N * V * 8
             bb += N * k * 8 + N * V * 8
             res[name] = {"value": N / (ms * 1e-3), "ms_per_step": ms,
@@ -451,6 +452,153 @@ def weights_cpu(x, y, ptr, ridx, gpu_w, sample):
     return res
 
 
+# ------------------------------------------------------------ rocprofv3 summaries
+# The profiles/ summaries are made by running rocprofv3 over this script
+# and then this script's --profile-summary mode over the CSV files it wrote
+# (the commands below, from the repository root on one MI355X, REPO = the
+# repository root).
+PROFILE_COMMANDS = {
+    "traffic": [
+        "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 200 rocprofv3 --pmc FETCH_SIZE --output-format csv "
+        "-d $REPO/gpurun_out/prof_fetch -o fetch -- python3 $REPO/bench.py --steps 10 --warmup 2 --no-extras --no-cpu",
+        "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 200 rocprofv3 --pmc WRITE_SIZE --output-format csv "
+        "-d $REPO/gpurun_out/prof_write -o write -- python3 $REPO/bench.py --steps 10 --warmup 2 --no-extras --no-cpu",
+        "python3 bench.py --profile-summary traffic gpurun_out/prof_fetch gpurun_out/prof_write",
+    ],
+    "trace": [
+        "cd /tmp && export TMPDIR=/tmp && timeout -k 10 300 rocprofv3 --kernel-trace --stats --output-format csv "
+        "-d $REPO/gpurun_out/prof_trace -o trace -- python3 $REPO/bench.py --steps 200 --warmup 10 --no-extras --no-cpu",
+        "python3 bench.py --profile-summary trace gpurun_out/prof_trace",
+    ],
+    "weights": [
+        "cd /tmp && export TMPDIR=/tmp && timeout -k 10 200 rocprofv3 --kernel-trace --stats --output-format csv "
+        "-d $REPO/gpurun_out/wprof_trace -o trace -- python3 $REPO/bench.py --weights-only",
+        "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 120 rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_SALU "
+        "SQ_INSTS_LDS SQ_WAVES SQ_WAVE_CYCLES SQ_INSTS_BRANCH SQ_INSTS_VALU_MFMA_F64 SQ_BUSY_CYCLES "
+        "--output-format csv -d $REPO/gpurun_out/wprof_pmc1 -o p -- python3 $REPO/bench.py --weights-only",
+        "python3 bench.py --profile-summary weights gpurun_out/wprof_trace gpurun_out/wprof_pmc1",
+    ],
+}
+PROFILE_OUT = {"traffic": "rhs_euler_rv_262144_hbm_traffic", "trace": "rhs_euler_rv_262144_kernel_trace",
+               "weights": "weights_c3_1048576_counters"}
+
+
+def _csv_rows(d, suffix):
+    import csv
+    import glob
+
+    files = sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True))
+    if not files:
+        raise SystemExit("no *%s under %s" % (suffix, d))
+    rows = []
+    for fn in files:
+        with open(fn, newline="") as f:
+            rows += list(csv.DictReader(f))
+    return rows
+
+
+def _counter_medians(d):
+    """{kernel: {counter: (median value per dispatch, dispatches)}} from the
+    counter-collection CSVs under d."""
+    per = {}
+    for r in _csv_rows(d, "counter_collection.csv"):
+        key = (r["Kernel_Name"], r.get("Dispatch_Id", r.get("Correlation_Id")))
+        per.setdefault(r["Kernel_Name"], {}).setdefault(r["Counter_Name"], {})
+        per[r["Kernel_Name"]][r["Counter_Name"]][key] = \
+            per[r["Kernel_Name"]][r["Counter_Name"]].get(key, 0.0) + float(r["Counter_Value"])
+    return {k: {c: (float(np.median(list(v.values()))), len(v)) for c, v in cs.items()} for k, cs in per.items()}
+
+
+def _short(name, n=90):
+    return name if len(name) <= n else name[:n]
+
+
+def _trace_table(d, top=24):
+    rows = _csv_rows(d, "kernel_stats.csv")
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    out = ["| kernel | calls | average ns | min ns | max ns | % |", "|---|---:|---:|---:|---:|---:|"]
+    for r in rows[:top]:
+        out.append("| `%s` | %s | %.0f | %s | %s | %.2f |" % (_short(r["Name"]), r["Calls"], float(r["AverageNs"]),
+                                                             r["MinNs"], r["MaxNs"], float(r["Percentage"])))
+    return out, rows
+
+
+def profile_summary(kind, dirs):
+    """Writes profiles/<summary>.md (and .json for the traffic) from the
+    rocprofv3 CSV output of the PROFILE_COMMANDS of `kind`."""
+    stem = os.path.join(ROOT, "profiles", PROFILE_OUT[kind])
+    cmds = "\n".join(PROFILE_COMMANDS[kind])
+    if kind == "traffic":
+        fetch, write = _counter_medians(dirs[0]), _counter_medians(dirs[1])
+        # both calibration kernels move 16 bytes per double of a 262,144 x 4 state
+        true_kib = 262144 * 4 * 8 / 1024.0
+        cal = {}
+        for name, tag in (("k_copy_buffer", "copy"), ("k_gather_probe", "gather")):
+            kf = [k for k in fetch if name in k]
+            kw = [k for k in write if name in k]
+            if kf and kw:
+                cal[tag] = (fetch[kf[0]]["FETCH_SIZE"][0], write[kw[0]]["WRITE_SIZE"][0])
+        if "copy" not in cal:
+            raise SystemExit("calibration kernel k_copy_buffer not in the profile")
+        fs, ws = true_kib / cal["copy"][0], true_kib / cal["copy"][1]
+        tab = {"fetch_scale": fs, "write_scale": ws, "calibration_kernel": "k_copy_buffer",
+               "calibration_true_kib": true_kib, "kernels": {}}
+        if "gather" in cal:
+            tab["check_kernel"] = "k_gather_probe"
+            tab["check_scaled_over_true"] = {"fetch": cal["gather"][0] * fs / true_kib,
+                                             "write": cal["gather"][1] * ws / true_kib}
+        lines = ["| kernel | dispatches | FETCH_SIZE KiB (median) | WRITE_SIZE KiB (median) | scaled MB per dispatch |",
+                 "|---|---:|---:|---:|---:|"]
+        for k in sorted(set(fetch) & set(write)):
+            fk, nd = fetch[k]["FETCH_SIZE"]
+            wk, _ = write[k]["WRITE_SIZE"]
+            b = 1024.0 * (fk * fs + wk * ws)
+            tab["kernels"][k] = {"fetch_kib": fk, "write_kib": wk, "bytes": b}
+            lines.append("| `%s` | %d | %.1f | %.1f | %.1f |" % (_short(k), nd, fk, wk, b / 1e6))
+        with open(stem + ".json", "w") as f:
+            json.dump(tab, f, indent=1)
+        chk = ""
+        if "check_scaled_over_true" in tab:
+            chk = ("The check kernel `k_gather_probe` (mf_bandwidth_probe kind 1: each wavefront gathers "
+                   "scattered doubles within 2 KiB windows, the access shape of a stencil gather, %.0f KiB read "
+                   "and written) then measures %.3f of its true reads and %.3f of its true writes after the "
+                   "same scaling.\n\n" % (true_kib, tab["check_scaled_over_true"]["fetch"],
+                                            tab["check_scaled_over_true"]["write"]))
+        text = ("# HBM traffic per dispatch (FETCH_SIZE, WRITE_SIZE): Euler + RV rhs!, 262,144 nodes, k = 30\n\n"
+                "Commands (from the repository root, on one MI355X):\n\n```\n%s\n```\n\n"
+                "Calibration: `k_copy_buffer` (mf_bandwidth_probe kind 0, a plain copy) reads and writes exactly "
+                "%.0f KiB; it reads FETCH_SIZE %.1f KiB and WRITE_SIZE %.1f KiB, so FETCH_SIZE is scaled by %.3f "
+                "and WRITE_SIZE by %.3f.  %s%s\n" % (cmds, true_kib, cal["copy"][0], cal["copy"][1], fs, ws, chk,
+                                                   "\n".join(lines)))
+    elif kind == "trace":
+        lines, _ = _trace_table(dirs[0])
+        text = ("# Kernel trace: Euler + residual viscosity rhs!, 262,144 nodes, k = 30\n\n"
+                "Commands (from the repository root, on one MI355X):\n\n```\n%s\n```\n\n"
+                "Top kernels by total time.  The run includes the autotuner's sweep (every load-batch variant "
+                "inside whole rhs! calls) and the FP64, split-FP32 and reference-order timing loops, so each "
+                "kernel's average is over whole-rhs! sequences.\n\n%s\n" % (cmds, "\n".join(lines)))
+    else:  # weights
+        lines, rows = _trace_table(dirs[0], top=8)
+        cnt = _counter_medians(dirs[1])
+        names = ["SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_BRANCH", "SQ_INSTS_VALU_MFMA_F64",
+                 "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES"]
+        tl = ["| kernel | dispatches | waves | " + " | ".join(n[3:] + " per wave" for n in names[:6]) + " |",
+              "|---|---:|---:|" + "---:|" * 6]
+        for k, cs in sorted(cnt.items()):
+            if "weights" not in k or "SQ_WAVES" not in cs:
+                continue
+            w = cs["SQ_WAVES"][0]
+            tl.append("| `%s` | %d | %.0f | " % (_short(k), cs["SQ_WAVES"][1], w) +
+                      " | ".join("%.0f" % (cs[n][0] / w) if n in cs else "-" for n in names[:6]) + " |")
+        text = ("# Batched weights (C3): 1,048,576 ragged stencils (n = 30, 8%% with n in 40..50), PHS r^5 + "
+                "degree 3\n\nCommands (from the repository root, on one MI355X):\n\n```\n%s\n```\n\n"
+                "## Kernel trace\n\n%s\n\n## Instruction counters per wavefront (one wavefront per stencil; "
+                "medians over dispatches)\n\n%s\n" % (cmds, "\n".join(lines), "\n".join(tl)))
+    with open(stem + ".md", "w") as f:
+        f.write(text)
+    print(stem + ".md")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -462,7 +610,13 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip adjoint and batched weights")
     ap.add_argument("--weights-only", action="store_true",
                     help="time only the batched weights (1,048,576 ragged stencils) and print their JSON")
+    ap.add_argument("--profile-summary", nargs="+", metavar=("KIND", "DIR"),
+                    help="write profiles/ summaries from rocprofv3 CSV directories: traffic FETCH_DIR WRITE_DIR, "
+                         "trace TRACE_DIR, or weights TRACE_DIR PMC_DIR (no GPU needed)")
     args = ap.parse_args()
+    if args.profile_summary:
+        profile_summary(args.profile_summary[0], args.profile_summary[1:])
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
@@ -483,8 +637,16 @@ def main():
     # a plain copy of known size (N * 4 doubles read and written, three times):
     # the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters of a
     # profiled run of this script
+    # and the windowed gather of the same size, the second calibration kernel
+    probe = {"copy": [], "gather": []}
     for _ in range(3):
         ctx.copy(L.MF_BUF_AUX0, L.MF_BUF_U)
+    for _ in range(5):
+        probe["copy"].append(ctx.bandwidth_probe(0, L.MF_BUF_AUX0, L.MF_BUF_U))
+        probe["gather"].append(ctx.bandwidth_probe(1, L.MF_BUF_AUX0, L.MF_BUF_U))
+    probe_bytes = 16 * N * 4
+    probe = {q: {"bytes": probe_bytes, "ms": float(np.min(v)), "gbs": probe_bytes / (np.min(v) * 1e-3) / 1e9}
+             for q, v in probe.items()}
     # three SSPRK33 steps with the history callback so the residual viscosity is live
     ctx.ssprk33(0.0, info["dt"], 3, flags=L.MF_STEP_HISTORY)
     t_now = 3 * info["dt"]
@@ -507,7 +669,7 @@ def main():
     wall = time.perf_counter() - w0
     value = N / (rhs_ms * 1e-3)
     tinfo = ctx.tile_info()
-    ba, bb = kernel_bytes(N, k, tinfo, passes=2)
+    ba, bb = kernel_bytes(N, k, tinfo)
     kernels = {"stencil": (parts[0], ba), "transpose": (parts[1], bb)}
     dom = max(kernels, key=lambda q: kernels[q][0])
     kms, kbytes = kernels[dom]
@@ -560,7 +722,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                      "traffic_kernel": traffic_kernel,
                      "kernel_ms": kms, "kernel_bytes": kbytes,
-                     "rhs_frac": rbytes / (rhs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "rhs_bytes": rbytes},
+                     "rhs_frac": rbytes / (rhs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "rhs_bytes": rbytes,
+                     "rhs_frac_cold": rbytes / (cold["f64_ref"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "probe_gbs": {q: v["gbs"] for q, v in probe.items()}},
         "kernels_ms": {"stencil": parts[0], "transpose": parts[1], "reductions": parts[2],
                        "boundary": parts[3]},
         "f64_fma": {"value": N / (f_ms * 1e-3), "ms_per_step": f_ms,

@@ -152,9 +152,15 @@ int mf_compute_operator(mf_ctx* ctx, int phs_power, int poly_degree, int deriv_o
 int mf_set_operator(mf_ctx* ctx, int target, const double* wx, const double* wy);
 
 /* Batched weights for ragged stencils over the cloud of mf_set_points:
- * stencil e = idx[ptr[e] .. ptr[e+1]), centre first.  The same per-stencil
- * computation as compute_operators.jl:422-441, for many stencils of uneven
- * size in one launch.  ptr/idx are uploaded once; mf_weights_ragged_run
+ * stencil e = idx[ptr[e] .. ptr[e+1]), centre first, 1 to 64 nodes each.
+ * The same per-stencil system as compute_operators.jl:422-441 (the
+ * reference's per-node loop, M \ rhs at :435), for many stencils of uneven
+ * size.  By default (MF_TUNE_WEIGHTS_SOLVER 0 or 2) the saddle systems are
+ * solved in their null-space form on the FP64 matrix cores, every stencil
+ * in one persistent launch (odd PHS power, polynomial degree <= 4, every
+ * stencil larger than the monomial count); otherwise, or with
+ * MF_TUNE_WEIGHTS_SOLVER 1, by the Bunch-Kaufman restatement, one launch
+ * per stencil size.  ptr/idx are uploaded once; mf_weights_ragged_run
  * recomputes on the device (timing loop), mf_weights_ragged_get copies
  * the result (layout like idx). */
 int mf_weights_ragged_set(mf_ctx* ctx, const int64_t* ptr, const int32_t* idx,
@@ -234,6 +240,14 @@ int mf_state_download(mf_ctx* ctx, int buf, double* const* fields);
  * while stepping on); a plain streaming copy of nvar * n_points doubles.
  * No counterpart in the reference (copyto! on host arrays there). */
 int mf_state_copy(mf_ctx* ctx, int dst_buf, int src_buf);
+/* A known-bytes kernel from buffer src_buf into dst_buf (nvar * n_points
+ * doubles, each read once and written once: 16 bytes per double): kind 0
+ * the streaming copy of mf_state_copy, kind 1 a gather whose wavefronts
+ * read scattered doubles within 2 KiB windows (the access shape of a
+ * stencil gather).  ms (may be NULL) receives the kernel's time from HIP
+ * events.  No counterpart in the reference; the calibration kernels of the
+ * HBM traffic counters and a practical bandwidth ceiling. */
+int mf_bandwidth_probe(mf_ctx* ctx, int kind, int dst_buf, int src_buf, double* ms);
 /* Device address of a context buffer: node-major, nvar doubles per node,
  * nodes in the context's internal order (see mf_node_order). */
 int mf_state_device_ptr(mf_ctx* ctx, int buf, double** dptr);
@@ -303,6 +317,11 @@ int mf_ssprk33_vjp(mf_ctx* ctx, double t0, double dt, int n_steps, int checkpoin
                                      (default), 1 the caller's order; precision is ignored */
 #define MF_TUNE_TILES 3           /* 1 (default): stage stencil halos in LDS where they fit;
                                      0: gather from global memory; precision is ignored */
+#define MF_TUNE_WEIGHTS_SOLVER 4  /* saddle-system solver of the weights: 0 (default) the
+                                     null-space form on the matrix cores for ragged batches
+                                     and Bunch-Kaufman (the reference's factorization) for
+                                     mf_compute_operator; 1 Bunch-Kaufman for both; 2 the
+                                     null-space form for both; precision is ignored */
 /* Selects a compiled kernel variant for one precision (MF_PREC_*, or -1 for
  * all).  A batch size that does not divide the stencil size falls back to
  * the default.  Results do not depend on it (the per-node operation order is

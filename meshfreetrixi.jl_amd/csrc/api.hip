@@ -525,9 +525,13 @@ int mf_compute_operator(mf_ctx* c, int phs_power, int poly_degree, int deriv_ord
     if ((rc = dalloc(c, &wx, n)) || (rc = dalloc(c, &wy, n))) return rc;
     if (!c->d_status && (rc = dalloc(c, &c->d_status, 1))) return rc;
     MF_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
-    if ((rc = launch_weights(c, nullptr, c->d_idx, c->k, c->N, c->k, phs_power, poly_degree, deriv_order,
-                             wx, wy, c->d_status)))
-        return rc;
+    if (c->weights_solver == 2 && (phs_power & 1) && poly_degree <= 4 && c->k > np)
+        rc = launch_weights_ns(c, nullptr, c->d_idx, c->k, c->N, c->k, phs_power, poly_degree, deriv_order, wx, wy,
+                               c->d_status);
+    else
+        rc = launch_weights(c, nullptr, c->d_idx, c->k, c->N, c->k, phs_power, poly_degree, deriv_order, wx, wy,
+                            c->d_status);
+    if (rc) return rc;
     if ((rc = alloc_op(c, target))) return rc;
     if ((rc = launch_pack_ops(c, target, wx, wy))) return rc;
     if ((rc = launch_pack_sorted(c, target, wx, wy))) return rc;
@@ -587,11 +591,12 @@ int mf_weights_ragged_set(mf_ctx* c, const int64_t* ptr, const int32_t* idx, int
     if (rc) return rc;
     if (!ptr || !idx || ns < 1) return fail(c, MF_ERR_ARGUMENT, "mf_weights_ragged_set: bad arguments");
     if (c->N == 0) return fail(c, MF_ERR_STATE, "mf_weights_ragged_set: no points");
-    int maxn = 0;
+    int maxn = 0, minn = 64;
     for (int64_t e = 0; e < ns; ++e) {
         const int64_t n = ptr[e + 1] - ptr[e];
         if (n < 1 || n > 64) return fail(c, MF_ERR_ARGUMENT, "mf_weights_ragged_set: stencil size out of range");
         maxn = std::max<int>(maxn, (int)n);
+        minn = std::min<int>(minn, (int)n);
     }
     const int64_t nnz = ptr[ns] - ptr[0];
     if (ptr[0] != 0) return fail(c, MF_ERR_ARGUMENT, "mf_weights_ragged_set: ptr[0] must be 0");
@@ -637,6 +642,7 @@ int mf_weights_ragged_set(mf_ctx* c, const int64_t* ptr, const int32_t* idx, int
     c->rn = ns;
     c->rnnz = nnz;
     c->rmaxn = maxn;
+    c->rminn = minn;
     return MF_OK;
 }
 
@@ -648,6 +654,10 @@ int mf_weights_ragged_run(mf_ctx* c, int phs_power, int poly_degree, int deriv_o
     if (phs_power < 1 || poly_degree < 0 || deriv_order < 1 || deriv_order > 8)
         return fail(c, MF_ERR_ARGUMENT, "mf_weights_ragged_run: bad basis");
     MF_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
+    const int np = (poly_degree + 1) * (poly_degree + 2) / 2;
+    if (c->weights_solver != 1 && (phs_power & 1) && poly_degree <= 4 && c->rminn > np)
+        return launch_weights_ns(c, c->d_rptr, c->d_ridx, 0, c->rn, c->rmaxn, phs_power, poly_degree, deriv_order,
+                                 c->d_rwx, c->d_rwy, c->d_status);
     for (size_t q = 0; q < c->rclass_n.size(); ++q) {
         const int64_t b0 = c->rclass_start[q], b1 = c->rclass_start[q + 1];
         if ((rc = launch_weights(c, c->d_rptr, c->d_ridx, 0, b1 - b0, c->rclass_n[q], phs_power, poly_degree,
@@ -918,6 +928,33 @@ int mf_state_copy(mf_ctx* c, int dst_buf, int src_buf)
     MF_HIP(c, hipStreamSynchronize(c->stream));
     if (dst_buf == MF_BUF_U) c->fsal_valid = false;
     return MF_OK;
+}
+
+int mf_bandwidth_probe(mf_ctx* c, int kind, int dst_buf, int src_buf, double* ms)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (dst_buf < 0 || dst_buf > 3 || src_buf < 0 || src_buf > 3 || dst_buf == src_buf || (kind != 0 && kind != 1))
+        return fail(c, MF_ERR_ARGUMENT, "mf_bandwidth_probe: bad arguments");
+    if ((rc = ensure_state(c))) return rc;
+    hipEvent_t e0, e1;
+    MF_HIP(c, hipEventCreate(&e0));
+    MF_HIP(c, hipEventCreate(&e1));
+    MF_HIP(c, hipEventRecord(e0, c->stream));
+    const int64_t n = c->N * c->nvar;
+    rc = kind == 0 ? launch_copy_buffer(c, c->d_buf[src_buf], c->d_buf[dst_buf], n)
+                   : launch_gather_probe(c, c->d_buf[src_buf], c->d_buf[dst_buf], n);
+    if (rc == MF_OK) {
+        MF_HIP(c, hipEventRecord(e1, c->stream));
+        MF_HIP(c, hipEventSynchronize(e1));
+        float t = 0.0f;
+        MF_HIP(c, hipEventElapsedTime(&t, e0, e1));
+        if (ms) *ms = t;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (dst_buf == MF_BUF_U) c->fsal_valid = false;
+    return rc;
 }
 
 int mf_node_order(mf_ctx* c, int32_t* order_out)
@@ -1208,6 +1245,10 @@ int mf_set_tuning(mf_ctx* c, int key, int precision, int value)
     } else if (key == MF_TUNE_NODE_ORDER) {
         if (value != 0 && value != 1) return fail(c, MF_ERR_ARGUMENT, "mf_set_tuning: node order must be 0 or 1");
         c->node_order = value;
+        return MF_OK;
+    } else if (key == MF_TUNE_WEIGHTS_SOLVER) {
+        if (value < 0 || value > 2) return fail(c, MF_ERR_ARGUMENT, "mf_set_tuning: weights solver must be 0, 1 or 2");
+        c->weights_solver = value;
         return MF_OK;
     } else if (key == MF_TUNE_TRANSPOSE_BATCH) {
         if (value != 0 && value != 2 && value != 4 && value != 8)

@@ -274,4 +274,56 @@ __device__ __forceinline__ void lds_load_unit32(const float* lds, int64_t q, int
     s[3] = a.w;
 }
 
+// ------------------------------------------------ RBF-FD weight helpers
+// shared by the Bunch-Kaufman weights (setup.hip) and the null-space weights
+// (weights_ns.hip)
+
+__device__ __forceinline__ double ipow(double v, int e)
+{
+    double r = 1.0;
+    for (int i = 0; i < e; ++i) r = r * v;
+    return r;
+}
+
+__device__ __forceinline__ void mono_exp(int t, int* ax, int* ay)
+{
+    int d = 0;
+    while (t >= d + 1) {
+        t -= d + 1;
+        ++d;
+    }
+    *ax = d - t;
+    *ay = t;
+}
+
+__device__ __forceinline__ double phs_eval(const PhsTable& tb, double s, double x, double y)
+{
+#pragma clang fp contract(off)
+    const double r = sqrt(x * x + y * y);
+    double val = 0.0;
+    for (int t = 0; t < tb.nt; ++t) {
+        if (tb.c[t] == 0.0) continue;
+        double term = tb.c[t] * ipow(s, tb.a[t]);
+        if (tb.b[t] >= 0)
+            term = term * ipow(r, tb.b[t]);
+        else
+            term = term / ipow(r, -tb.b[t]);
+        val = val + term;
+    }
+    return val;
+}
+
+// s^k nearly correctly rounded (double-double product).
+__device__ __forceinline__ double pow_k(double s, int k)
+{
+    double hi = 1.0, lo = 0.0;
+    for (int r = 0; r < k; ++r) {
+        const double ph = hi * s;
+        const double pl = fma(hi, s, -ph) + lo * s;
+        hi = ph + pl;
+        lo = pl - (hi - ph);
+    }
+    return hi;
+}
+
 }  // namespace mf

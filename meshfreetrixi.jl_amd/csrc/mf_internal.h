@@ -180,6 +180,7 @@ struct mf_ctx {
     // ragged weights batch
     int64_t rn = 0, rnnz = 0;
     int rmaxn = 0;
+    int rminn = 0;
     int64_t* d_rptr = nullptr;
     int32_t* d_ridx = nullptr;
     int32_t* d_rorder = nullptr;          // stencils grouped by size
@@ -226,6 +227,7 @@ struct mf_ctx {
     // per precision (MF_PREC_*); 0 = the default
     int tune_flux_batch[3] = {0, 0, 0};       // stencil entries per load batch
     int tune_transpose_batch[3] = {0, 0, 0};  // transposed entries per load batch
+    int weights_solver = 0;                   // MF_TUNE_WEIGHTS_SOLVER
 
     // when set, the rhs! kernels are dispatched with start/stop events:
     // probe[2 s], probe[2 s + 1] for slot s (see launch_k in rhs.hip)
@@ -261,6 +263,10 @@ int launch_knn(mf_ctx* c, int k, double* d_dist2);
 int launch_weights(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k,
                    int64_t n_stencils, int maxn, int p, int deg, int kd, double* d_wx,
                    double* d_wy, int* d_status, const int32_t* d_order = nullptr, int64_t base = 0);
+// the same weights by the null-space form on the matrix cores, every
+// stencil (ragged, any sizes) in one persistent launch
+int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k, int64_t n_stencils,
+                      int maxn, int p, int deg, int kd, double* d_wx, double* d_wy, int* d_status);
 int launch_pack_ops(mf_ctx* c, int target, const double* d_wx, const double* d_wy);
 int launch_build_transpose(mf_ctx* c);
 int launch_dx_stats(mf_ctx* c, const double* d_dist2, double* dx_min, double* dx_avg);
@@ -276,6 +282,7 @@ void free_tiles(mf_ctx* c);
 void free_tiles_b(mf_ctx* c);
 int launch_gather_nodes_d(mf_ctx* c, const double* d_src, double* d_dst);
 int launch_copy_buffer(mf_ctx* c, const double* d_src, double* d_dst, int64_t n);
+int launch_gather_probe(mf_ctx* c, const double* d_src, double* d_dst, int64_t n);
 int launch_flush(mf_ctx* c, double* d_buf, int64_t n);
 // cache-flush buffer of mf_time_rhs_cold: 512 MiB, twice the 256 MB last-level cache
 constexpr size_t kFlushDoubles = (size_t)1 << 26;

@@ -410,54 +410,6 @@ struct WeightsArgs {
     int* status;
 };
 
-__device__ __forceinline__ double ipow(double v, int e)
-{
-    double r = 1.0;
-    for (int i = 0; i < e; ++i) r = r * v;
-    return r;
-}
-
-__device__ __forceinline__ void mono_exp(int t, int* ax, int* ay)
-{
-    int d = 0;
-    while (t >= d + 1) {
-        t -= d + 1;
-        ++d;
-    }
-    *ax = d - t;
-    *ay = t;
-}
-
-__device__ double phs_eval(const PhsTable& tb, double s, double x, double y)
-{
-#pragma clang fp contract(off)
-    const double r = sqrt(x * x + y * y);
-    double val = 0.0;
-    for (int t = 0; t < tb.nt; ++t) {
-        if (tb.c[t] == 0.0) continue;
-        double term = tb.c[t] * ipow(s, tb.a[t]);
-        if (tb.b[t] >= 0)
-            term = term * ipow(r, tb.b[t]);
-        else
-            term = term / ipow(r, -tb.b[t]);
-        val = val + term;
-    }
-    return val;
-}
-
-// s^k nearly correctly rounded (double-double product).
-__device__ double pow_k(double s, int k)
-{
-    double hi = 1.0, lo = 0.0;
-    for (int r = 0; r < k; ++r) {
-        const double ph = hi * s;
-        const double pl = fma(hi, s, -ph) + lo * s;
-        hi = ph + pl;
-        lo = pl - (hi - ph);
-    }
-    return hi;
-}
-
 // Maximum over the wavefront of one double per lane (an exact selection,
 // so any combination order gives the same bits), returned to every lane:
 // a DPP max-scan within each row of 16 lanes (row_shr 1, 2, 4, 8), then
@@ -1297,6 +1249,30 @@ int launch_copy_buffer(mf_ctx* c, const double* d_src, double* d_dst, int64_t n)
 {
     const int64_t blocks = std::min<int64_t>(4096, (n / 4 + 255) / 256 + 1);
     hipLaunchKernelGGL(k_copy_buffer, dim3((unsigned)blocks), dim3(256), 0, c->stream, n, d_src, d_dst);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+// dst[i] = src[(i & ~255) | reverse8(i & 255)]: every element of src read
+// once and every element of dst written once (16 bytes per element, like the
+// copy), but each wavefront's loads are scattered over a 2 KiB window, the
+// access shape of a stencil gather.  A second known-bytes kernel for the
+// HBM traffic counters.
+__global__ void k_gather_probe(int64_t n, const double* __restrict__ src, double* __restrict__ dst)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t full = n & ~(int64_t)255;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        const int64_t r = q < full ? (q & ~(int64_t)255) | (int64_t)(__builtin_bitreverse32((uint32_t)(q & 255)) >> 24)
+                                   : q;
+        dst[q] = src[r];
+    }
+}
+
+int launch_gather_probe(mf_ctx* c, const double* d_src, double* d_dst, int64_t n)
+{
+    const int64_t blocks = std::min<int64_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(k_gather_probe, dim3((unsigned)blocks), dim3(256), 0, c->stream, n, d_src, d_dst);
     MF_HIP(c, hipGetLastError());
     return MF_OK;
 }

@@ -1,0 +1,488 @@
+// Batched RBF-FD weights by the null-space (projector) form of the saddle
+// system, with the dense products on FP64 matrix cores.
+//
+// Reference: compute_flux_operator, pointcloudsolver/compute_operators.jl:
+// 409-453 (first derivatives) and :549-594 (k-th derivatives): per stencil,
+// shift_stencil (:225-246), the blocks R = phi(|X_i - X_j|) and P (:191-223),
+// the right-hand sides L phi at the mirrored points and L p at the centre
+// (:248-263, :283-291, :345-384), M \ rhs with M = [R P; P' 0] (:435), the
+// scaling by s_d^k (:438-439, :579-580).  The reference factors M by
+// Bunch-Kaufman; this kernel solves the same system another way:
+//
+//   P = Q U            (Cholesky QR: G = P'P = U'U, Q = P U^-1, n x np)
+//   w = w_p + y,       w_p = Q U'^-1 c  (so that P'w = c)
+//   K y = f,           K = s Pi R Pi + Q Q'  (SPD),  f = s Pi (b - R w_p)
+//
+// with Pi = I - QQ' and s = (-1)^ceil(p/2), the sign that makes the
+// polyharmonic spline r^p conditionally positive definite on the null space
+// of P'.  K is formed as s R - W Q' - Q W' with X = R Q, Y = Q'X and
+// W = s (X - Q Y/2) - Q/2, and solved by Gauss-Jordan elimination without
+// pivoting (K is symmetric positive definite).  The products G, X, Y, W,
+// Q'h and the rank-2np update of K run as v_mfma_f64_16x16x4 tiles.
+//
+// One wavefront (workgroup) per stencil, n <= 64 nodes, np <= 16
+// monomials, n > np; stencils of any sizes within those bounds run in one
+// launch.  (A persistent grid that loops over stencils took twice the
+// registers: the loop-invariant LDS addresses of the unrolled phases stay
+// live across the loop.)
+#include <hip/hip_runtime.h>
+
+#include "mf_device.h"
+
+namespace mf {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kNsLq = 18;  // row stride (doubles) of the n x 16 LDS matrices
+
+struct NsArgs {
+    const double* x;
+    const double* y;
+    const int64_t* ptr;  // ragged stencils (or null: ELL)
+    const int32_t* ridx;
+    const int32_t* ell;  // ELL, slot-major (ptr null)
+    int ell_k;
+    int64_t N;
+    int64_t ns;          // stencils
+    int p, np, kd;
+    int rows;            // LDS rows of the n x 16 matrices (>= largest n, >= 16)
+    double kfact, sgn;
+    PhsTable tab;
+    double* wx;
+    double* wy;
+    int* status;
+};
+
+// D = A B + C for 16 x 4 by 4 x 16 FP64 tiles.  Lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16] and holds D[4 r + l / 16][l % 16] in element r.
+__device__ __forceinline__ d4 mfma4(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ double bcast(double v, int l)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+__device__ __forceinline__ double wave_absmax(double v)
+{
+    v = fabs(v);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+// phi(|X_i - X_j|) = r^p from the scaled coordinates in LDS (interleaved x, y)
+__device__ __forceinline__ double phs_r(const double* co, int i, int j, int p)
+{
+#pragma clang fp contract(off)
+    const double dx = co[2 * i] - co[2 * j], dy = co[2 * i + 1] - co[2 * j + 1];
+    return ipow(sqrt(dx * dx + dy * dy), p);
+}
+
+__global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double sm[];
+    const int lane = threadIdx.x;
+    const int q = lane >> 4, c = lane & 15;
+    const int LQ = kNsLq;
+    double* co = sm;                      // 64 x (x, y)
+    double* Qm = sm + 128;                // rows x LQ: P, then Q
+    double* Xm = Qm + a.rows * LQ;        // rows x LQ: X = R Q, then W
+    double* Zm = Xm + a.rows * LQ;        // 16 x LQ: G, L of G, Z, then K' tiles
+    double* col = Zm + 16 * LQ;           // 64: broadcast column
+    double* hv = col + 64;                // 128: h = b - X g (2 per node)
+    double* gs = hv + 128;                // 32: g (2 per monomial)
+    double* ss = gs + 32;                 // 32: s = Q'h (2 per monomial)
+    const int np = a.np;
+
+    {
+        const int64_t e = blockIdx.x;
+        int64_t off;
+        int n;
+        if (a.ptr) {
+            off = a.ptr[e];
+            n = (int)(a.ptr[e + 1] - off);
+        } else {
+            off = e;
+            n = a.ell_k;
+        }
+        auto sidx = [&](int j) -> int32_t { return a.ptr ? a.ridx[off + j] : a.ell[(int64_t)j * a.N + e]; };
+        const int T = (n + 15) >> 4;      // 16-row tiles
+        const int nk = (n + 3) >> 2;      // k-steps over the n nodes
+        const int npk = (np + 3) >> 2;    // k-steps over the monomials
+        bool ok = true;
+
+        // ---- shift_stencil (:225-246): centre on node 0, scale per dimension
+        const int32_t c0 = sidx(0);
+        double px = 0.0, py = 0.0;
+        if (lane < n) {
+            const int32_t qn = sidx(lane);
+            px = a.x[qn] - a.x[c0];
+            py = a.y[qn] - a.y[c0];
+        }
+        const double sx = 1.0 / wave_absmax(px), sy = 1.0 / wave_absmax(py);
+        const double xs = px * sx, ys = py * sy;
+        co[2 * lane] = xs;
+        co[2 * lane + 1] = ys;
+
+        // ---- P row of this node (poly_block, :211-223), into LDS
+        if (lane < a.rows) {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                double v = 0.0;
+                if (t < np && lane < n) {
+                    int ax, ay;
+                    mono_exp(t, &ax, &ay);
+                    v = ipow(xs, ax) * ipow(ys, ay);
+                }
+                Qm[lane * LQ + t] = v;
+            }
+        }
+        __syncthreads();
+
+        // ---- G = P'P on the matrix cores
+        {
+            d4 g = {0.0, 0.0, 0.0, 0.0};
+            for (int ks = 0; ks < nk; ++ks) {
+                const int k = 4 * ks + q;
+                const double v = k < n ? Qm[k * LQ + c] : 0.0;
+                g = mfma4(v, v, g);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Zm[(4 * r + q) * LQ + c] = g[r];
+        }
+        __syncthreads();
+
+        // ---- G = L L' (lane = row of G), L kept in Zm
+        {
+            double gr[16];
+#pragma unroll
+            for (int t = 0; t < 16; ++t) gr[t] = lane < 16 ? Zm[lane * LQ + t] : 0.0;
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                if (j < np) {
+                    const double d = bcast(gr[j], j);
+                    ok = ok && d > 0.0;
+                    const double rs = 1.0 / sqrt(d);
+                    const double l = lane >= j ? gr[j] * rs : 0.0;
+                    col[lane] = l;
+                    __syncthreads();
+#pragma unroll
+                    for (int cc = j + 1; cc < 16; ++cc) gr[cc] = fma(-l, col[cc], gr[cc]);
+                    gr[j] = l;
+                    __syncthreads();
+                }
+            }
+            if (lane < 16) {
+#pragma unroll
+                for (int t = 0; t < 16; ++t) Zm[lane * LQ + t] = t <= lane && lane < np ? gr[t] : 0.0;
+            }
+            __syncthreads();
+        }
+
+        // ---- Q = P L'^-1, row by row in place
+        if (lane < n) {
+            double qr[16];
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                qr[t] = 0.0;
+                if (t < np) {
+                    double aq = Qm[lane * LQ + t];
+#pragma unroll
+                    for (int s = 0; s < t; ++s) aq = fma(-qr[s], Zm[t * LQ + s], aq);
+                    qr[t] = aq / Zm[t * LQ + t];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 16; ++t) Qm[lane * LQ + t] = qr[t];
+        }
+        // g = L^-1 c, lane = monomial (c: L applied to the monomials at the
+        // centre, poly_linearoperator :283-291)
+        {
+            int ax, ay;
+            mono_exp(lane & 15, &ax, &ay);
+            double a0 = (lane < np && ax == a.kd && ay == 0) ? a.kfact : 0.0;
+            double a1 = (lane < np && ay == a.kd && ax == 0) ? a.kfact : 0.0;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                if (t < np) {
+                    const double d = Zm[t * LQ + t];
+                    const double g0 = bcast(a0, t) / d, g1 = bcast(a1, t) / d;
+                    if (lane == t) {
+                        a0 = g0;
+                        a1 = g1;
+                    } else if (lane > t && lane < 16) {
+                        const double l = Zm[lane * LQ + t];
+                        a0 = fma(-l, g0, a0);
+                        a1 = fma(-l, g1, a1);
+                    }
+                }
+            }
+            if (lane < 16) {
+                gs[2 * lane] = lane < np ? a0 : 0.0;
+                gs[2 * lane + 1] = lane < np ? a1 : 0.0;
+            }
+        }
+        __syncthreads();
+
+        // ---- X = R Q on the matrix cores
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+            if (rt < T) {
+                const int i = 16 * rt + c;
+                d4 acc = {0.0, 0.0, 0.0, 0.0};
+                for (int ks = 0; ks < nk; ++ks) {
+                    const int k = 4 * ks + q;
+                    const double av = (i < n && k < n) ? phs_r(co, i, k, a.p) : 0.0;
+                    const double bv = k < n ? Qm[k * LQ + c] : 0.0;
+                    acc = mfma4(av, bv, acc);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * rt + 4 * r + q;
+                    if (row < n) Xm[row * LQ + c] = acc[r];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- right-hand sides (mirror_stencil :248-263, rbf_linearoperator
+        //      :345-358): b at the mirrored points, centre at (eps, eps);
+        //      w_p = Q g; h = b - X g
+        double wp0 = 0.0, wp1 = 0.0, h0 = 0.0, h1 = 0.0;
+        if (lane < n) {
+            double mx = -xs, my = -ys;
+            if (lane == 0) mx = my = DBL_EPSILON;
+            const double b0 = phs_eval(a.tab, mx, mx, my);
+            const double b1 = phs_eval(a.tab, my, mx, my);
+            double x0 = 0.0, x1 = 0.0;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                if (t < np) {
+                    const double gt0 = gs[2 * t], gt1 = gs[2 * t + 1];
+                    const double xv = Xm[lane * LQ + t], qv = Qm[lane * LQ + t];
+                    wp0 = fma(qv, gt0, wp0);
+                    wp1 = fma(qv, gt1, wp1);
+                    x0 = fma(xv, gt0, x0);
+                    x1 = fma(xv, gt1, x1);
+                }
+            }
+            h0 = b0 - x0;
+            h1 = b1 - x1;
+        }
+        hv[2 * lane] = h0;
+        hv[2 * lane + 1] = h1;
+        __syncthreads();
+
+        // ---- s = Q'h (matrix cores; columns 0 and 1 of the tile)
+        {
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+            for (int ks = 0; ks < nk; ++ks) {
+                const int k = 4 * ks + q;
+                const double av = k < n ? Qm[k * LQ + c] : 0.0;
+                const double bv = (k < n && c < 2) ? hv[2 * k + c] : 0.0;
+                acc = mfma4(av, bv, acc);
+            }
+            if (c < 2) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) ss[2 * (4 * r + q) + c] = acc[r];
+            }
+        }
+        __syncthreads();
+
+        // f = sgn (h - Q s), lane = node
+        double f0 = 0.0, f1 = 0.0;
+        if (lane < n) {
+            double a0 = h0, a1 = h1;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                if (t < np) {
+                    const double qv = Qm[lane * LQ + t];
+                    a0 = fma(-qv, ss[2 * t], a0);
+                    a1 = fma(-qv, ss[2 * t + 1], a1);
+                }
+            }
+            f0 = a.sgn * a0;
+            f1 = a.sgn * a1;
+        }
+
+        // ---- Y = Q'X, Z = sgn Y / 2 + I / 2 (matrix cores)
+        {
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+            for (int ks = 0; ks < nk; ++ks) {
+                const int k = 4 * ks + q;
+                const double av = k < n ? Qm[k * LQ + c] : 0.0;
+                const double bv = k < n ? Xm[k * LQ + c] : 0.0;
+                acc = mfma4(av, bv, acc);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                Zm[(4 * r + q) * LQ + c] = 0.5 * a.sgn * acc[r] + (4 * r + q == c ? 0.5 : 0.0);
+        }
+        __syncthreads();
+
+        // ---- W = sgn X - Q Z over the X rows (matrix cores)
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+            if (rt < T) {
+                const int i = 16 * rt + c;
+                d4 acc;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * rt + 4 * r + q;
+                    acc[r] = row < n ? a.sgn * Xm[row * LQ + c] : 0.0;
+                }
+                for (int ks = 0; ks < npk; ++ks) {
+                    const int k = 4 * ks + q;
+                    const double av = i < n ? -Qm[i * LQ + k] : 0.0;
+                    const double bv = Zm[k * LQ + c];
+                    acc = mfma4(av, bv, acc);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * rt + 4 * r + q;
+                    if (row < n) Xm[row * LQ + c] = acc[r];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- K = sgn R - W Q' - Q W' tile by tile (matrix cores), each tile
+        //      staged through Zm into the rows of the lanes that own them
+        double kr[64];
+#pragma unroll
+        for (int t = 0; t < 64; ++t) kr[t] = 0.0;
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            if (it < T) {
+#pragma unroll
+                for (int jt = 0; jt < 4; ++jt) {
+                    if (jt < T) {
+                        d4 acc;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = 16 * it + 4 * r + q, cl = 16 * jt + c;
+                            acc[r] = (row < n && cl < n) ? a.sgn * phs_r(co, row, cl, a.p)
+                                                         : (row == cl ? 1.0 : 0.0);
+                        }
+                        const int ia = 16 * it + c, jb = 16 * jt + c;
+                        for (int ks = 0; ks < npk; ++ks) {  // -W Q'
+                            const int k = 4 * ks + q;
+                            const double av = ia < n ? -Xm[ia * LQ + k] : 0.0;
+                            const double bv = jb < n ? Qm[jb * LQ + k] : 0.0;
+                            acc = mfma4(av, bv, acc);
+                        }
+                        for (int ks = 0; ks < npk; ++ks) {  // -Q W'
+                            const int k = 4 * ks + q;
+                            const double av = ia < n ? -Qm[ia * LQ + k] : 0.0;
+                            const double bv = jb < n ? Xm[jb * LQ + k] : 0.0;
+                            acc = mfma4(av, bv, acc);
+                        }
+                        __syncthreads();
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) Zm[(4 * r + q) * LQ + c] = acc[r];
+                        __syncthreads();
+                        if ((lane >> 4) == it) {
+#pragma unroll
+                            for (int t = 0; t < 16; ++t) kr[16 * jt + t] = Zm[c * LQ + t];
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- K y = f by Gauss-Jordan elimination (lane = row).  Step j
+        //      eliminates column j from every other row; the trailing block
+        //      stays symmetric, so row j's entries right of j are column j's.
+        double piv = 1.0;
+#pragma unroll
+        for (int jc = 0; jc < 4; ++jc) {
+            if (jc < T) {
+#pragma unroll
+                for (int jj = 0; jj < 16; ++jj) {
+                    const int j = 16 * jc + jj;
+                    if (j < n) {
+                        col[lane] = kr[j];
+                        __syncthreads();
+                        const double pj = bcast(kr[j], j);
+                        const double fj0 = bcast(f0, j), fj1 = bcast(f1, j);
+                        ok = ok && pj > 0.0;
+                        const double m = lane == j ? 0.0 : kr[j] / pj;
+                        if (lane == j) piv = pj;
+#pragma unroll
+                        for (int cc = jc; cc < 4; ++cc) {
+                            if (cc < T) {
+#pragma unroll
+                                for (int ci = 0; ci < 16; ++ci) {
+                                    const int cl = 16 * cc + ci;
+                                    if (cl > j) kr[cl] = fma(-m, col[cl], kr[cl]);
+                                }
+                            }
+                        }
+                        f0 = fma(-m, fj0, f0);
+                        f1 = fma(-m, fj1, f1);
+                        __syncthreads();
+                    }
+                }
+            }
+        }
+
+        // ---- w = w_p + y, scaled back (:438-439, :579-580)
+        if (lane < n) {
+            const double w0 = wp0 + f0 / piv, w1 = wp1 + f1 / piv;
+            const double fx = pow_k(sx, a.kd), fy = pow_k(sy, a.kd);
+            const int64_t o = a.ptr ? off + lane : (int64_t)lane * a.N + e;
+            a.wx[o] = fx * w0;
+            a.wy[o] = fy * w1;
+        }
+        if (!ok && lane == 0) atomicOr(a.status, 2);
+        __syncthreads();
+    }
+}
+
+// LDS doubles of one wavefront for stencils of up to `rows` nodes.
+static size_t ns_lds_doubles(int rows) { return 128 + 2 * (size_t)rows * kNsLq + 16 * kNsLq + 64 + 128 + 32 + 32; }
+
+int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k, int64_t n_stencils,
+                      int maxn, int p, int deg, int kd, double* d_wx, double* d_wy, int* d_status)
+{
+    NsArgs a;
+    a.x = c->d_x;
+    a.y = c->d_y;
+    a.ptr = d_ptr;
+    a.ridx = d_ptr ? d_idx : nullptr;
+    a.ell = d_ptr ? nullptr : d_idx;
+    a.ell_k = ell_k;
+    a.N = c->N;
+    a.ns = n_stencils;
+    a.p = p;
+    a.np = (deg + 1) * (deg + 2) / 2;
+    a.kd = kd;
+    if (a.np > 16) return fail(c, MF_ERR_UNSUPPORTED, "null-space weights: polynomial degree above 4");
+    if (maxn > 64) return fail(c, MF_ERR_UNSUPPORTED, "null-space weights: stencil above 64 nodes");
+    if ((p & 1) == 0) return fail(c, MF_ERR_UNSUPPORTED, "null-space weights: even PHS power");
+    a.rows = maxn < 16 ? 16 : maxn;
+    double kf = 1.0;
+    for (int i = 2; i <= kd; ++i) kf *= i;
+    a.kfact = kf;
+    a.sgn = (((p + 1) / 2) & 1) ? -1.0 : 1.0;  // (-1)^ceil(p/2)
+    a.tab = phs_table(p, kd);
+    a.wx = d_wx;
+    a.wy = d_wy;
+    a.status = d_status;
+    const size_t lds = sizeof(double) * ns_lds_doubles(a.rows);
+    if (n_stencils > 0x7fffffff) return fail(c, MF_ERR_ARGUMENT, "null-space weights: too many stencils");
+    const int64_t grid = n_stencils;
+    hipLaunchKernelGGL(k_weights_ns, dim3((unsigned)grid), dim3(64), lds, c->stream, a);
+    MF_HIP(c, hipGetLastError());
+    return MF_OK;
+}
+
+}  // namespace mf

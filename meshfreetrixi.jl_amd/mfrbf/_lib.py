@@ -24,7 +24,7 @@ MF_PREC_F64, MF_PREC_SPLIT32, MF_PREC_F64_REF = 0, 1, 2
 MF_BUF_U, MF_BUF_DU, MF_BUF_AUX0, MF_BUF_AUX1 = 0, 1, 2, 3
 MF_VAR_DENSITY, MF_VAR_PRESSURE = 0, 1
 MF_STEP_GRAPH, MF_STEP_LIMITER, MF_STEP_HISTORY, MF_STEP_NOSYNC = 1, 2, 4, 8
-MF_TUNE_FLUX_BATCH, MF_TUNE_TRANSPOSE_BATCH, MF_TUNE_NODE_ORDER, MF_TUNE_TILES = 0, 1, 2, 3
+MF_TUNE_FLUX_BATCH, MF_TUNE_TRANSPOSE_BATCH, MF_TUNE_NODE_ORDER, MF_TUNE_TILES, MF_TUNE_WEIGHTS_SOLVER = 0, 1, 2, 3, 4
 
 
 class MfError(RuntimeError):
@@ -80,6 +80,7 @@ _SIGS = {
     "mf_node_order": [_ctx, _i32p],
     "mf_tile_info": [_ctx, _i64p],
     "mf_state_copy": [_ctx, C.c_int, C.c_int],
+    "mf_bandwidth_probe": [_ctx, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)],
 }
 
 _lib = None
@@ -304,6 +305,13 @@ class Context:
     def copy(self, dst_buf, src_buf):
         """Device-side copy of buffer src_buf into dst_buf."""
         self._chk(self._L.mf_state_copy(self.h, dst_buf, src_buf))
+
+    def bandwidth_probe(self, kind, dst_buf, src_buf):
+        """Time (ms) of a known-bytes kernel from src_buf into dst_buf: kind 0
+        the streaming copy, 1 the windowed gather (mf_bandwidth_probe)."""
+        ms = C.c_double(0.0)
+        self._chk(self._L.mf_bandwidth_probe(self.h, kind, dst_buf, src_buf, C.byref(ms)))
+        return ms.value
 
     def tile_info(self):
         """dict of the LDS tile state (mf_tile_info)."""

@@ -75,6 +75,32 @@ def test_knn_grid_large_vs_ckdtree():
     assert np.array_equal(ig, ii[:, :30])
 
 
+def _config_clouds():
+    out = [("C1", 64, configs.SEED0 + 1, 15), ("C2", 512, configs.SEED0 + 2, 30),
+           ("C3", 1024, configs.SEED0 + 3, 50), ("C4", 256, configs.SEED0 + 4, 30)]
+    for r, spec in enumerate(configs.c5()):
+        out.append(("C5-%d" % r, spec.side, spec.seed, spec.k))
+    return out
+
+
+def test_config_clouds_have_no_near_ties():
+    """Every configuration's cloud (C1-C4 and the 64 C5 clouds, C3 with its
+    largest stencil k = 50): along each node's k + 1 nearest neighbours
+    consecutive distances differ by more than 1e-12 of the distance (about
+    4,500 units in the last place; the closest pair, in C5 cloud 25, is
+    1.3e-10 apart), so the stencils, their order and their membership at k
+    do not depend on how a kNN search breaks ties or rounds distances; the k
+    nearest by scipy's cKDTree equal the oracle's grid search."""
+    for name, side, seed, k in _config_clouds():
+        x, y, _, _, _ = O.cloud_square(side, seed)
+        dd, ii = cKDTree(np.c_[x, y]).query(np.c_[x, y], k=k + 1, workers=-1)
+        rel_gap = (np.diff(dd, axis=1)[:, 1:] / dd[:, 2:]).min()
+        assert rel_gap > 1e-12, (name, float(rel_gap))
+        if side <= 512:
+            ig, _ = O.knn_grid(x, y, k)
+            assert np.array_equal(ig, ii[:, :k]), name
+
+
 def test_dx_stats():
     x, y, _, _, _ = O.cloud_square(64, 20261202)
     _, d2 = O.knn_brute(x, y, 2)

@@ -33,7 +33,11 @@ namespace mf {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int kNsLq = 18;  // row stride (doubles) of the n x 16 LDS matrices
+constexpr int kNsLz = 18;  // row stride (doubles) of the 16 x 16 LDS tile
+
+// Row stride of the n x np LDS matrices: odd, so that the 16 rows one MFMA
+// operand reads fall in distinct banks.
+constexpr int ns_lq(int np) { return np <= 11 ? 11 : 17; }
 
 struct NsArgs {
     const double* x;
@@ -45,7 +49,8 @@ struct NsArgs {
     int64_t N;
     int64_t ns;          // stencils
     int p, np, kd;
-    int rows;            // LDS rows of the n x 16 matrices (>= largest n, >= 16)
+    int rows;            // LDS rows of the n x np matrices (>= largest n, >= 16)
+    int lq;              // their row stride, ns_lq(np)
     double kfact, sgn;
     PhsTable tab;
     double* wx;
@@ -64,6 +69,26 @@ __device__ __forceinline__ double bcast(double v, int l)
 {
     return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
                             __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// 1/sqrt(d) and 1/d for d > 0: the hardware estimates (about 2^-29
+// relative) refined by two Newton steps each, about 1 ulp; cheaper than
+// the correctly rounded quotient and root, whose last bit is not needed here
+__device__ __forceinline__ double rsqrt_nr(double d)
+{
+    double y = __builtin_amdgcn_rsq(d);
+    const double h = 0.5 * d;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
+__device__ __forceinline__ double rcp_nr(double d)
+{
+    double y = __builtin_amdgcn_rcp(d);
+    y = y * fma(-d, y, 2.0);
+    y = y * fma(-d, y, 2.0);
+    return y;
 }
 
 __device__ __forceinline__ double wave_absmax(double v)
@@ -88,15 +113,16 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
     extern __shared__ double sm[];
     const int lane = threadIdx.x;
     const int q = lane >> 4, c = lane & 15;
-    const int LQ = kNsLq;
+    const int LQ = a.lq, LZ = kNsLz;
     double* co = sm;                      // 64 x (x, y)
-    double* Qm = sm + 128;                // rows x LQ: P, then Q
-    double* Xm = Qm + a.rows * LQ;        // rows x LQ: X = R Q, then W
-    double* Zm = Xm + a.rows * LQ;        // 16 x LQ: G, L of G, Z, then K' tiles
-    double* col = Zm + 16 * LQ;           // 64: broadcast column
+    double* Qm = sm + 128;                // rows x LQ: P, then Q (columns < np)
+    double* Xm = Qm + a.rows * LQ;        // rows x LQ: X = R Q, then W (columns < np)
+    double* Zm = Xm + a.rows * LQ;        // 16 x LZ: G, L of G, Z, then K tiles
+    double* col = Zm + 16 * LZ;           // 64: broadcast column (16-byte aligned)
     double* hv = col + 64;                // 128: h = b - X g (2 per node)
     double* gs = hv + 128;                // 32: g (2 per monomial)
     double* ss = gs + 32;                 // 32: s = Q'h (2 per monomial)
+    double* dg = ss + 32;                 // 16: 1 / L_jj of the Cholesky factor of G
     const int np = a.np;
 
     {
@@ -139,7 +165,7 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
                     mono_exp(t, &ax, &ay);
                     v = ipow(xs, ax) * ipow(ys, ay);
                 }
-                Qm[lane * LQ + t] = v;
+                if (t < LQ) Qm[lane * LQ + t] = v;
             }
         }
         __syncthreads();
@@ -149,11 +175,11 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
             d4 g = {0.0, 0.0, 0.0, 0.0};
             for (int ks = 0; ks < nk; ++ks) {
                 const int k = 4 * ks + q;
-                const double v = k < n ? Qm[k * LQ + c] : 0.0;
+                const double v = (k < n && c < np) ? Qm[k * LQ + c] : 0.0;
                 g = mfma4(v, v, g);
             }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) Zm[(4 * r + q) * LQ + c] = g[r];
+            for (int r = 0; r < 4; ++r) Zm[(4 * r + q) * LZ + c] = g[r];
         }
         __syncthreads();
 
@@ -161,14 +187,15 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
         {
             double gr[16];
 #pragma unroll
-            for (int t = 0; t < 16; ++t) gr[t] = lane < 16 ? Zm[lane * LQ + t] : 0.0;
+            for (int t = 0; t < 16; ++t) gr[t] = lane < 16 ? Zm[lane * LZ + t] : 0.0;
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
                 if (j < np) {
                     const double d = bcast(gr[j], j);
                     ok = ok && d > 0.0;
-                    const double rs = 1.0 / sqrt(d);
+                    const double rs = rsqrt_nr(d);
+                    if (lane == 0) dg[j] = rs;
                     const double l = lane >= j ? gr[j] * rs : 0.0;
                     col[lane] = l;
                     __syncthreads();
@@ -180,7 +207,7 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
             }
             if (lane < 16) {
 #pragma unroll
-                for (int t = 0; t < 16; ++t) Zm[lane * LQ + t] = t <= lane && lane < np ? gr[t] : 0.0;
+                for (int t = 0; t < 16; ++t) Zm[lane * LZ + t] = t <= lane && lane < np ? gr[t] : 0.0;
             }
             __syncthreads();
         }
@@ -194,12 +221,13 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
                 if (t < np) {
                     double aq = Qm[lane * LQ + t];
 #pragma unroll
-                    for (int s = 0; s < t; ++s) aq = fma(-qr[s], Zm[t * LQ + s], aq);
-                    qr[t] = aq / Zm[t * LQ + t];
+                    for (int s = 0; s < t; ++s) aq = fma(-qr[s], Zm[t * LZ + s], aq);
+                    qr[t] = aq * dg[t];
                 }
             }
 #pragma unroll
-            for (int t = 0; t < 16; ++t) Qm[lane * LQ + t] = qr[t];
+            for (int t = 0; t < 16; ++t)
+                if (t < np) Qm[lane * LQ + t] = qr[t];
         }
         // g = L^-1 c, lane = monomial (c: L applied to the monomials at the
         // centre, poly_linearoperator :283-291)
@@ -211,13 +239,13 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
 #pragma unroll
             for (int t = 0; t < 16; ++t) {
                 if (t < np) {
-                    const double d = Zm[t * LQ + t];
-                    const double g0 = bcast(a0, t) / d, g1 = bcast(a1, t) / d;
+                    const double rd = dg[t];
+                    const double g0 = bcast(a0, t) * rd, g1 = bcast(a1, t) * rd;
                     if (lane == t) {
                         a0 = g0;
                         a1 = g1;
                     } else if (lane > t && lane < 16) {
-                        const double l = Zm[lane * LQ + t];
+                        const double l = Zm[lane * LZ + t];
                         a0 = fma(-l, g0, a0);
                         a1 = fma(-l, g1, a1);
                     }
@@ -230,22 +258,53 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
         }
         __syncthreads();
 
-        // ---- X = R Q on the matrix cores
+        // ---- X = R Q on the matrix cores.  For n <= 32 the A-operand values
+        //      R[16 rt + c][4 ks + q] are kept: by the symmetry of R they are
+        //      the C-layout values R[16 it + 4 r + q][16 jt + c] the tiles of K
+        //      start from (rt = jt, ks = 4 it + r), which saves evaluating R twice.
+        double ra[16];
 #pragma unroll
-        for (int rt = 0; rt < 4; ++rt) {
-            if (rt < T) {
-                const int i = 16 * rt + c;
-                d4 acc = {0.0, 0.0, 0.0, 0.0};
-                for (int ks = 0; ks < nk; ++ks) {
-                    const int k = 4 * ks + q;
-                    const double av = (i < n && k < n) ? phs_r(co, i, k, a.p) : 0.0;
-                    const double bv = k < n ? Qm[k * LQ + c] : 0.0;
-                    acc = mfma4(av, bv, acc);
+        for (int t = 0; t < 16; ++t) ra[t] = 0.0;
+        if (T <= 2) {
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt) {
+                if (rt < T) {
+                    const int i = 16 * rt + c;
+                    d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int ks = 0; ks < 8; ++ks) {
+                        if (ks < nk) {
+                            const int k = 4 * ks + q;
+                            const double av = (i < n && k < n) ? phs_r(co, i, k, a.p) : 0.0;
+                            const double bv = (k < n && c < np) ? Qm[k * LQ + c] : 0.0;
+                            ra[8 * rt + ks] = av;
+                            acc = mfma4(av, bv, acc);
+                        }
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = 16 * rt + 4 * r + q;
+                        if (row < n && c < np) Xm[row * LQ + c] = acc[r];
+                    }
                 }
+            }
+        } else {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = 16 * rt + 4 * r + q;
-                    if (row < n) Xm[row * LQ + c] = acc[r];
+            for (int rt = 0; rt < 4; ++rt) {
+                if (rt < T) {
+                    const int i = 16 * rt + c;
+                    d4 acc = {0.0, 0.0, 0.0, 0.0};
+                    for (int ks = 0; ks < nk; ++ks) {
+                        const int k = 4 * ks + q;
+                        const double av = (i < n && k < n) ? phs_r(co, i, k, a.p) : 0.0;
+                        const double bv = (k < n && c < np) ? Qm[k * LQ + c] : 0.0;
+                        acc = mfma4(av, bv, acc);
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = 16 * rt + 4 * r + q;
+                        if (row < n && c < np) Xm[row * LQ + c] = acc[r];
+                    }
                 }
             }
         }
@@ -284,7 +343,7 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
             d4 acc = {0.0, 0.0, 0.0, 0.0};
             for (int ks = 0; ks < nk; ++ks) {
                 const int k = 4 * ks + q;
-                const double av = k < n ? Qm[k * LQ + c] : 0.0;
+                const double av = (k < n && c < np) ? Qm[k * LQ + c] : 0.0;
                 const double bv = (k < n && c < 2) ? hv[2 * k + c] : 0.0;
                 acc = mfma4(av, bv, acc);
             }
@@ -316,14 +375,14 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
             d4 acc = {0.0, 0.0, 0.0, 0.0};
             for (int ks = 0; ks < nk; ++ks) {
                 const int k = 4 * ks + q;
-                const double av = k < n ? Qm[k * LQ + c] : 0.0;
-                const double bv = k < n ? Xm[k * LQ + c] : 0.0;
+                const double av = (k < n && c < np) ? Qm[k * LQ + c] : 0.0;
+                const double bv = (k < n && c < np) ? Xm[k * LQ + c] : 0.0;
                 acc = mfma4(av, bv, acc);
             }
             __syncthreads();
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                Zm[(4 * r + q) * LQ + c] = 0.5 * a.sgn * acc[r] + (4 * r + q == c ? 0.5 : 0.0);
+                Zm[(4 * r + q) * LZ + c] = 0.5 * a.sgn * acc[r] + (4 * r + q == c ? 0.5 : 0.0);
         }
         __syncthreads();
 
@@ -336,18 +395,18 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = 16 * rt + 4 * r + q;
-                    acc[r] = row < n ? a.sgn * Xm[row * LQ + c] : 0.0;
+                    acc[r] = (row < n && c < np) ? a.sgn * Xm[row * LQ + c] : 0.0;
                 }
                 for (int ks = 0; ks < npk; ++ks) {
                     const int k = 4 * ks + q;
-                    const double av = i < n ? -Qm[i * LQ + k] : 0.0;
-                    const double bv = Zm[k * LQ + c];
+                    const double av = (i < n && k < np) ? -Qm[i * LQ + k] : 0.0;
+                    const double bv = Zm[k * LZ + c];
                     acc = mfma4(av, bv, acc);
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = 16 * rt + 4 * r + q;
-                    if (row < n) Xm[row * LQ + c] = acc[r];
+                    if (row < n && c < np) Xm[row * LQ + c] = acc[r];
                 }
             }
         }
@@ -368,29 +427,33 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const int row = 16 * it + 4 * r + q, cl = 16 * jt + c;
-                            acc[r] = (row < n && cl < n) ? a.sgn * phs_r(co, row, cl, a.p)
-                                                         : (row == cl ? 1.0 : 0.0);
+                            double rv;
+                            if (it < 2 && jt < 2)
+                                rv = T <= 2 ? ra[(8 * jt + 4 * it + r) & 15] : phs_r(co, row, cl, a.p);
+                            else
+                                rv = phs_r(co, row, cl, a.p);
+                            acc[r] = (row < n && cl < n) ? a.sgn * rv : (row == cl ? 1.0 : 0.0);
                         }
                         const int ia = 16 * it + c, jb = 16 * jt + c;
                         for (int ks = 0; ks < npk; ++ks) {  // -W Q'
                             const int k = 4 * ks + q;
-                            const double av = ia < n ? -Xm[ia * LQ + k] : 0.0;
-                            const double bv = jb < n ? Qm[jb * LQ + k] : 0.0;
+                            const double av = (ia < n && k < np) ? -Xm[ia * LQ + k] : 0.0;
+                            const double bv = (jb < n && k < np) ? Qm[jb * LQ + k] : 0.0;
                             acc = mfma4(av, bv, acc);
                         }
                         for (int ks = 0; ks < npk; ++ks) {  // -Q W'
                             const int k = 4 * ks + q;
-                            const double av = ia < n ? -Qm[ia * LQ + k] : 0.0;
-                            const double bv = jb < n ? Xm[jb * LQ + k] : 0.0;
+                            const double av = (ia < n && k < np) ? -Qm[ia * LQ + k] : 0.0;
+                            const double bv = (jb < n && k < np) ? Xm[jb * LQ + k] : 0.0;
                             acc = mfma4(av, bv, acc);
                         }
                         __syncthreads();
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) Zm[(4 * r + q) * LQ + c] = acc[r];
+                        for (int r = 0; r < 4; ++r) Zm[(4 * r + q) * LZ + c] = acc[r];
                         __syncthreads();
                         if ((lane >> 4) == it) {
 #pragma unroll
-                            for (int t = 0; t < 16; ++t) kr[16 * jt + t] = Zm[c * LQ + t];
+                            for (int t = 0; t < 16; ++t) kr[16 * jt + t] = Zm[c * LZ + t];
                         }
                     }
                 }
@@ -401,7 +464,7 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
         // ---- K y = f by Gauss-Jordan elimination (lane = row).  Step j
         //      eliminates column j from every other row; the trailing block
         //      stays symmetric, so row j's entries right of j are column j's.
-        double piv = 1.0;
+        double piv = 1.0;  // 1 / pivot of this lane's row
 #pragma unroll
         for (int jc = 0; jc < 4; ++jc) {
             if (jc < T) {
@@ -414,15 +477,20 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
                         const double pj = bcast(kr[j], j);
                         const double fj0 = bcast(f0, j), fj1 = bcast(f1, j);
                         ok = ok && pj > 0.0;
-                        const double m = lane == j ? 0.0 : kr[j] / pj;
-                        if (lane == j) piv = pj;
+                        const double ip = rcp_nr(pj);
+                        const double m = lane == j ? 0.0 : kr[j] * ip;
+                        if (lane == j) piv = ip;
 #pragma unroll
                         for (int cc = jc; cc < 4; ++cc) {
                             if (cc < T) {
 #pragma unroll
-                                for (int ci = 0; ci < 16; ++ci) {
+                                for (int ci = 0; ci < 16; ci += 2) {
                                     const int cl = 16 * cc + ci;
-                                    if (cl > j) kr[cl] = fma(-m, col[cl], kr[cl]);
+                                    if (cl + 1 > j) {
+                                        const double2 cv = *reinterpret_cast<const double2*>(col + cl);
+                                        if (cl > j) kr[cl] = fma(-m, cv.x, kr[cl]);
+                                        kr[cl + 1] = fma(-m, cv.y, kr[cl + 1]);
+                                    }
                                 }
                             }
                         }
@@ -436,7 +504,7 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
 
         // ---- w = w_p + y, scaled back (:438-439, :579-580)
         if (lane < n) {
-            const double w0 = wp0 + f0 / piv, w1 = wp1 + f1 / piv;
+            const double w0 = wp0 + f0 * piv, w1 = wp1 + f1 * piv;
             const double fx = pow_k(sx, a.kd), fy = pow_k(sy, a.kd);
             const int64_t o = a.ptr ? off + lane : (int64_t)lane * a.N + e;
             a.wx[o] = fx * w0;
@@ -448,7 +516,7 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
 }
 
 // LDS doubles of one wavefront for stencils of up to `rows` nodes.
-static size_t ns_lds_doubles(int rows) { return 128 + 2 * (size_t)rows * kNsLq + 16 * kNsLq + 64 + 128 + 32 + 32; }
+static size_t ns_lds_doubles(int rows, int lq) { return 128 + 2 * (size_t)rows * lq + 16 * kNsLz + 64 + 128 + 32 + 32 + 16; }
 
 int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k, int64_t n_stencils,
                       int maxn, int p, int deg, int kd, double* d_wx, double* d_wy, int* d_status)
@@ -477,7 +545,8 @@ int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int
     a.wx = d_wx;
     a.wy = d_wy;
     a.status = d_status;
-    const size_t lds = sizeof(double) * ns_lds_doubles(a.rows);
+    a.lq = ns_lq(a.np);
+    const size_t lds = sizeof(double) * ns_lds_doubles(a.rows, a.lq);
     if (n_stencils > 0x7fffffff) return fail(c, MF_ERR_ARGUMENT, "null-space weights: too many stencils");
     const int64_t grid = n_stencils;
     hipLaunchKernelGGL(k_weights_ns, dim3((unsigned)grid), dim3(64), lds, c->stream, a);

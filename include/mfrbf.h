@@ -126,6 +126,20 @@ int mf_set_points(mf_ctx* ctx, const double* x, const double* y, int64_t n_point
 int mf_knn(mf_ctx* ctx, int k, int32_t* idx_out, double* dist_out, double* dx_min,
            double* dx_avg);
 
+/* mf_knn without allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  No counterpart in
+ * the reference.  The same stencils as mf_knn (self first, ordered by
+ * squared distance, ties by index) for the points of mf_set_points, written
+ * to caller-owned DEVICE memory: d_idx_out (n_points*k int32, row per node,
+ * the caller's indices) and d_dist_out (Euclidean distances, same layout;
+ * may be NULL).  Unlike mf_knn it leaves the context's stencil unchanged.
+ * The search's intermediates live in a 9 MiB static device array that each
+ * GPU holds until the process ends: at most 262,144 points (and as many
+ * search-grid cells); larger clouds return MF_ERR_UNSUPPORTED.  Every call
+ * on one GPU shares that array, so calls that overlap on one GPU (from
+ * other threads or contexts) race. */
+int mf_knn_noalloc(mf_ctx* ctx, int k, int32_t* d_idx_out, double* d_dist_out);
+
 /* Uploads a stencil computed elsewhere (n_points x k, row per node, self
  * first); replaces PointData.neighbors (geometry_primatives.jl:299). */
 int mf_set_stencil(mf_ctx* ctx, const int32_t* idx, int64_t n_points, int k);

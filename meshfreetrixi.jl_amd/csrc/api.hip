@@ -452,6 +452,18 @@ int mf_knn(mf_ctx* c, int k, int32_t* idx_out, double* dist_out, double* dx_min,
     return MF_OK;
 }
 
+int mf_knn_noalloc(mf_ctx* c, int k, int32_t* d_idx_out, double* d_dist_out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->N == 0) return fail(c, MF_ERR_STATE, "mf_knn_noalloc: no points");
+    if (k < 2 || k > 64 || k > c->N) return fail(c, MF_ERR_ARGUMENT, "mf_knn_noalloc: need 2 <= k <= min(64, N)");
+    if (!d_idx_out) return fail(c, MF_ERR_ARGUMENT, "mf_knn_noalloc: null output");
+    if (c->N > knn_noalloc_max_nodes())
+        return fail(c, MF_ERR_UNSUPPORTED, "mf_knn_noalloc: more than 262,144 points, the size of its static scratch");
+    return launch_knn_noalloc(c, k, d_idx_out, d_dist_out);
+}
+
 int mf_set_stencil(mf_ctx* c, const int32_t* idx, int64_t n_points, int k)
 {
     int rc = check_ctx(c);

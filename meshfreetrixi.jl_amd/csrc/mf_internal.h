@@ -258,6 +258,10 @@ inline unsigned xcd_grid(int64_t nblocks) { return (unsigned)(((nblocks + 7) / 8
 
 // kernels / launchers (defined in the .hip files)
 int launch_knn(mf_ctx* c, int k, double* d_dist2);
+// the same search with its intermediates in a static device array (no
+// allocation): row per caller node, caller indices, distances
+int launch_knn_noalloc(mf_ctx* c, int k, int32_t* d_idx, double* d_dist);
+int64_t knn_noalloc_max_nodes();
 // n_stencils stencils: all of them (d_order null) or d_order[base + b],
 // b < n_stencils, each with at most maxn nodes (sizes the LDS)
 int launch_weights(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k,

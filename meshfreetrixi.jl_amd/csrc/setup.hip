@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <hipcub/hipcub.hpp>
+#include <mutex>
 
 #include "mf_device.h"
 
@@ -210,8 +211,10 @@ struct KnnArgs {
     const double2* xys;
     const int32_t* uid;  // internal -> caller index (ties compare caller indices)
     const int32_t* inv;  // caller -> internal index
-    int32_t* out_idx;    // slot-major, internal indices
-    double* out_d2;      // slot-major
+    int32_t* out_idx;    // slot-major, internal indices (rows = 0)
+    double* out_d2;      // slot-major squared distances (rows = 0)
+    int rows;            // 1: out_idx row per caller node with caller indices,
+                         //    out_d2 the distances (not squared), same layout
 };
 
 __global__ __launch_bounds__(64) void k_knn_query(const KnnArgs a)
@@ -269,6 +272,14 @@ __global__ __launch_bounds__(64) void k_knn_query(const KnnArgs a)
             dm *= (1.0 - 1e-9);
             if (dm > 0 && bd[(K - 1) * 64 + lane] < dm * dm) break;
         }
+    }
+    if (a.rows) {
+        const int64_t row = (int64_t)a.uid[q] * K;
+        for (int s = 0; s < K; ++s) {
+            a.out_idx[row + s] = bi[s * 64 + lane];
+            if (a.out_d2) a.out_d2[row + s] = sqrt(bd[s * 64 + lane]);
+        }
+        return;
     }
     for (int s = 0; s < K; ++s) {
         a.out_idx[(int64_t)s * a.N + q] = a.inv[bi[s * 64 + lane]];
@@ -328,6 +339,7 @@ int launch_knn(mf_ctx* c, int k, double* d_d2)
     a.inv = c->d_inv;
     a.out_idx = c->d_idx;
     a.out_d2 = d_d2;
+    a.rows = 0;
     const size_t lds = (size_t)64 * k * (sizeof(double) + sizeof(int32_t));
     hipLaunchKernelGGL(k_knn_query, dim3((unsigned)((N + 63) / 64)), dim3(64), lds, c->stream, a);
     MF_HIP(c, hipGetLastError());
@@ -339,6 +351,133 @@ int launch_knn(mf_ctx* c, int k, double* d_d2)
     dfree(cursor);
     dfree(order);
     dfree(xys);
+    return MF_OK;
+}
+
+// ------------------------------------------------ kNN without allocation
+// mf_knn_noalloc runs the same grid search with every intermediate (the
+// bounding box, the cell of each point, the cell counts, starts and fill
+// cursors, the points in cell order and their coordinates) in `scratch`, a
+// file-scope device array of fixed size allocated with the code object:
+// up to kKnnScratchNodes points and as many grid cells.  Host code uses
+// the array only through the address hipGetSymbolAddress returns for the
+// current device (knn_scratch_ptr); every call on a device shares its
+// copy, so calls that overlap on one device race.
+constexpr int64_t kKnnScratchNodes = 262144;
+constexpr int64_t kKnnScratchCells = 262144;
+constexpr size_t kKnnScratchBytes = 256 + 3 * kKnnScratchCells * sizeof(int32_t) +
+                                    kKnnScratchNodes * (2 * sizeof(int32_t) + sizeof(double2));
+static __device__ __align__(256) unsigned char scratch[kKnnScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static void* g_knn_scratch[kMaxDevices];
+static std::mutex g_knn_scratch_mu;
+
+int64_t knn_noalloc_max_nodes() { return kKnnScratchNodes; }
+
+// Device address of `scratch` on the current device (looked up once per device).
+static int knn_scratch_ptr(mf_ctx* c, unsigned char** out)
+{
+    if (c->device < 0 || c->device >= kMaxDevices) return fail(c, MF_ERR_ARGUMENT, "kNN scratch: device index");
+    std::lock_guard<std::mutex> lock(g_knn_scratch_mu);
+    if (!g_knn_scratch[c->device]) {
+        void* ptr = nullptr;
+        MF_HIP(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(c, MF_ERR_HIP, "kNN scratch: null symbol address");
+        g_knn_scratch[c->device] = ptr;
+    }
+    *out = static_cast<unsigned char*>(g_knn_scratch[c->device]);
+    return MF_OK;
+}
+
+__global__ void k_fill_i32(int64_t n, int32_t* p, int32_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Exclusive prefix sum of n counts in one workgroup of 1024 threads: each
+// thread sums a contiguous chunk, the chunk totals are scanned in LDS, and
+// each thread writes its chunk's prefixes.
+__global__ __launch_bounds__(1024) void k_scan_excl(int64_t n, const int32_t* in, int32_t* out)
+{
+    __shared__ int32_t tot[1024];
+    const int t = threadIdx.x;
+    const int64_t chunk = (n + 1023) / 1024;
+    const int64_t lo = (int64_t)t * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    int32_t sum = 0;
+    for (int64_t i = lo; i < hi; ++i) sum += in[i];
+    tot[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int32_t v = t >= o ? tot[t - o] : 0;
+        __syncthreads();
+        tot[t] += v;
+        __syncthreads();
+    }
+    int32_t run = tot[t] - sum;  // exclusive prefix of this chunk
+    for (int64_t i = lo; i < hi; ++i) {
+        out[i] = run;
+        run += in[i];
+    }
+}
+
+int launch_knn_noalloc(mf_ctx* c, int k, int32_t* d_idx, double* d_dist)
+{
+    const int64_t N = c->N;
+    unsigned char* base = nullptr;
+    int rc = knn_scratch_ptr(c, &base);
+    if (rc) return rc;
+    double* d_box = reinterpret_cast<double*>(base);
+    int32_t* count = reinterpret_cast<int32_t*>(base + 256);
+    int32_t* start = count + kKnnScratchCells;
+    int32_t* cursor = start + kKnnScratchCells;
+    int32_t* cell = cursor + kKnnScratchCells;
+    int32_t* order = cell + kKnnScratchNodes;
+    double2* xys = reinterpret_cast<double2*>(order + kKnnScratchNodes);
+    hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, c->stream, N, c->d_x, c->d_y, d_box);
+    double box[4];
+    MF_HIP(c, hipMemcpyAsync(box, d_box, sizeof box, hipMemcpyDeviceToHost, c->stream));
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    Grid g;
+    g.x0 = box[0];
+    g.y0 = box[2];
+    double W = box[1] - box[0], H = box[3] - box[2];
+    if (W <= 0) W = 1e-300;
+    if (H <= 0) H = 1e-300;
+    g.cs = sqrt(2.0 * W * H / (double)N);
+    if (!(g.cs > 0)) g.cs = fmax(W, H);
+    g.ncx = (int64_t)floor(W / g.cs) + 1;
+    g.ncy = (int64_t)floor(H / g.cs) + 1;
+    const int64_t ncell = g.ncx * g.ncy;
+    if (ncell > kKnnScratchCells)
+        return fail(c, MF_ERR_UNSUPPORTED, "mf_knn_noalloc: the cloud's search grid needs more cells than its "
+                                           "static scratch holds (262,144; an elongated cloud)");
+    const unsigned gN = (unsigned)((N + 255) / 256), gC = (unsigned)((ncell + 255) / 256);
+    hipLaunchKernelGGL(k_fill_i32, dim3(gC), dim3(256), 0, c->stream, ncell, count, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(gC), dim3(256), 0, c->stream, ncell, cursor, 0);
+    hipLaunchKernelGGL(k_cell_count, dim3(gN), dim3(256), 0, c->stream, N, c->d_x, c->d_y, g, cell, count);
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, c->stream, ncell, count, start);
+    hipLaunchKernelGGL(k_cell_scatter, dim3(gN), dim3(256), 0, c->stream, N, cell, start, cursor, order);
+    hipLaunchKernelGGL(k_cell_finish, dim3(gC), dim3(256), 0, c->stream, ncell, start, count, order,
+                       c->d_x, c->d_y, xys);
+    KnnArgs a;
+    a.N = N;
+    a.k = k;
+    a.g = g;
+    a.start = start;
+    a.count = count;
+    a.order = order;
+    a.xys = xys;
+    a.uid = c->d_uid;
+    a.inv = c->d_inv;
+    a.out_idx = d_idx;
+    a.out_d2 = d_dist;
+    a.rows = 1;
+    const size_t lds = (size_t)64 * k * (sizeof(double) + sizeof(int32_t));
+    hipLaunchKernelGGL(k_knn_query, dim3((unsigned)((N + 63) / 64)), dim3(64), lds, c->stream, a);
+    MF_HIP(c, hipGetLastError());
+    MF_HIP(c, hipStreamSynchronize(c->stream));
     return MF_OK;
 }
 

@@ -46,6 +46,7 @@ _SIGS = {
     "mf_synchronize": [_ctx],
     "mf_set_points": [_ctx, _dp, _dp, C.c_int64],
     "mf_knn": [_ctx, C.c_int, _i32p, _dp, _dp, _dp],
+    "mf_knn_noalloc": [_ctx, C.c_int, C.c_void_p, C.c_void_p],
     "mf_set_stencil": [_ctx, _i32p, C.c_int64, C.c_int],
     "mf_compute_operator": [_ctx, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp],
     "mf_set_operator": [_ctx, C.c_int, _dp, _dp],
@@ -175,6 +176,11 @@ class Context:
     def set_stencil(self, idx):
         idx = np.ascontiguousarray(idx, dtype=np.int32)
         self._chk(self._L.mf_set_stencil(self.h, i32ptr(idx), idx.shape[0], idx.shape[1]))
+
+    def knn_noalloc(self, k, d_idx, d_dist=None):
+        """mf_knn_noalloc: stencils into caller-owned device memory (d_idx,
+        d_dist are device addresses as integers; d_dist may be None)."""
+        self._chk(self._L.mf_knn_noalloc(self.h, k, C.c_void_p(d_idx), C.c_void_p(d_dist) if d_dist else None))
 
     def compute_operator(self, p, deg, deriv, target, want=True, k=None):
         wx = wy = None

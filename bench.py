@@ -286,11 +286,43 @@ def adjoint_ratio(L, configs):
 
 def ensemble(L, configs, steps):
     """C5: 64 Euler clouds (16k..262k nodes) with Flyer hyperviscosity,
-    split-FP32 rhs!, SSPRK33 steps replayed from each context's captured
-    graph.  All contexts are enqueued from this thread (each has its own
-    stream) before any is synchronized; the same steps one context at a
-    time are timed for comparison."""
+    split-FP32 rhs!, SSPRK33 steps.  Headline path: one grouped context
+    (mf_set_points_grouped) whose every launch covers all 64 clouds, the
+    steps replayed from its captured graph at the common time step.  For
+    comparison the same clouds as 64 contexts (one stream each, their own
+    time steps) enqueued from this thread before any is synchronized."""
+    from mfrbf.ensemble import Ensemble
+    from mfrbf.equations import CompressibleEulerEquations2D
+
     specs = configs.c5()
+    t0 = time.perf_counter()
+    ens = Ensemble(specs)
+    t_setup_g = time.perf_counter() - t0
+    g = ens.ctx
+    g.upload(L.MF_BUF_U, ens.u0)
+    g.ssprk33(0.0, ens.dt, 16, L.MF_PREC_SPLIT32, L.MF_STEP_GRAPH)  # capture, build the LDS tiles
+    g.upload(L.MF_BUF_U, ens.u0)
+    g.synchronize()
+    w0 = time.perf_counter()
+    g.ssprk33(0.0, ens.dt, steps, L.MF_PREC_SPLIT32, L.MF_STEP_GRAPH)
+    wall_g = time.perf_counter() - w0
+    ug = g.download(L.MF_BUF_U)
+    eq = CompressibleEulerEquations2D(specs[0].gamma)
+    err_g = 0.0
+    for r, spec in enumerate(specs):
+        lo, hi = ens.offsets[r], ens.offsets[r + 1]
+        ex = spec.ic(np.c_[ens.x[lo:hi], ens.y[lo:hi]], steps * ens.dt, eq)
+        err_g = max(err_g, float(np.abs(ug[0, lo:hi] - ex[0]).max()))
+    rhs_ms, rhs_parts = g.time_rhs(steps * ens.dt, L.MF_PREC_SPLIT32, 50)
+    total = ens.N
+    evals = total * (3 * steps + 1)  # + the first (FSAL) rhs!
+    out = dict(clouds=len(specs), nodes=total, steps=steps, dt=ens.dt, setup_s_grouped=t_setup_g,
+               ms_grouped=1e3 * wall_g, node_evals_per_s=evals / wall_g,
+               max_density_error_vs_exact=err_g,
+               rhs_grouped={"ms": rhs_ms, "node_evals_per_s": total / (rhs_ms * 1e-3),
+                            "kernels_ms": {"stencil": rhs_parts[0], "transpose": rhs_parts[1]}})
+    del ens, g
+    # the same clouds as 64 contexts on 64 streams
     t0 = time.perf_counter()
     clouds = []
     for spec in specs:
@@ -301,32 +333,20 @@ def ensemble(L, configs, steps):
     t_setup = time.perf_counter() - t0
     flags = L.MF_STEP_GRAPH | L.MF_STEP_NOSYNC
 
-    def run(concurrent):
+    def run():
         for _, ctx, info in clouds:
             ctx.upload(L.MF_BUF_U, info["u0"])
         w0 = time.perf_counter()
         for _, ctx, info in clouds:
-            ctx.ssprk33(0.0, info["dt"], steps, L.MF_PREC_SPLIT32, flags if concurrent else L.MF_STEP_GRAPH)
+            ctx.ssprk33(0.0, info["dt"], steps, L.MF_PREC_SPLIT32, flags)
         for _, ctx, _ in clouds:
             ctx.synchronize()
         return time.perf_counter() - w0
 
-    run(True)  # capture the graphs, build the LDS tiles
-    wall_c = run(True)
-    wall_s = run(False)
-    total = sum(info["N"] for _, _, info in clouds)
-    evals = sum(info["N"] * (3 * steps + 1) for _, _, info in clouds)  # + the first (FSAL) rhs!
-    # the density wave is an exact solution: report the largest density error at T
-    from mfrbf.equations import CompressibleEulerEquations2D
-    err = 0.0
-    for spec, ctx, info in clouds:
-        T = steps * info["dt"]
-        ex = spec.ic(np.c_[info["x"], info["y"]], T, CompressibleEulerEquations2D(spec.gamma))
-        u = ctx.download(L.MF_BUF_U)
-        err = max(err, float(np.abs(u[0] - ex[0]).max()))
-    return dict(clouds=len(clouds), nodes=total, steps=steps, setup_s=t_setup, ms_concurrent=1e3 * wall_c,
-                ms_one_at_a_time=1e3 * wall_s, node_evals_per_s=evals / wall_c,
-                node_evals_per_s_one_at_a_time=evals / wall_s, max_density_error_vs_exact=err)
+    run()  # capture the graphs, build the LDS tiles
+    wall_c = run()
+    out["contexts"] = dict(setup_s=t_setup, ms_concurrent=1e3 * wall_c, node_evals_per_s=evals / wall_c)
+    return out
 
 
 def advection_c1(L, configs, with_cpu):
@@ -750,7 +770,7 @@ def main():
         out["c1_advection"] = advection_c1(L, configs, not args.no_cpu)
         log("batched weights: 1,048,576 ragged stencils (and the CPU oracle on a sample)")
         out["weights"] = weights_batch(L, configs, 1024, 0 if args.no_cpu else 32768)
-        log("ensemble: 64 clouds, 50 SSPRK33 steps each (split-FP32, graphs, concurrent streams)")
+        log("ensemble: 64 clouds, 50 SSPRK33 steps each (split-FP32; one grouped context, and 64 contexts)")
         out["ensemble"] = ensemble(L, configs, 50)
         log("Tominec hyperviscosity after the residual viscosity, %d nodes" % N)
         out["tominec"] = tominec(L, ctx, info, spec, t_now, max(200, args.steps // 4))

@@ -117,6 +117,19 @@ int mf_synchronize(mf_ctx* ctx);
  * MF_ERR_ARGUMENT for fewer than 2 points or a non-finite coordinate. */
 int mf_set_points(mf_ctx* ctx, const double* x, const double* y, int64_t n_points);
 
+/* The same for a point set made of n_groups independent clouds (an
+ * ensemble of runs in one context): group g is points group_ptr[g] ..
+ * group_ptr[g+1] (group_ptr[0] = 0, group_ptr[n_groups] = n_points).  The
+ * internal node order keeps each group contiguous (Morton order inside the
+ * group), so every kernel launch of rhs! and of the SSPRK33 step covers all
+ * clouds while no LDS tile mixes more than two of them.  Stencils must stay
+ * inside their group (mf_set_stencil; mf_knn searches the whole set).  The
+ * residual viscosity and IGR sources, which reduce over all points, return
+ * MF_ERR_UNSUPPORTED at rhs! on a grouped set.  No counterpart in the
+ * reference (which runs one cloud per semidiscretization). */
+int mf_set_points_grouped(mf_ctx* ctx, const double* x, const double* y, int64_t n_points, int n_groups,
+                          const int64_t* group_ptr);
+
 /* k nearest neighbours of every node (self first), ordered by squared
  * distance, ties by index.  Replaces knn(KDTree(points), points, nv, true)
  * and the dx_min/dx_avg statistics of PointData(points, basis)

@@ -44,6 +44,7 @@ template int dalloc<int16_t>(mf_ctx*, int16_t**, size_t);
 template int dalloc<int8_t>(mf_ctx*, int8_t**, size_t);
 template int dalloc<uint8_t>(mf_ctx*, uint8_t**, size_t);
 template int dalloc<uint16_t>(mf_ctx*, uint16_t**, size_t);
+template int dalloc<uint64_t>(mf_ctx*, uint64_t**, size_t);
 template int dalloc<IgrScalars>(mf_ctx*, IgrScalars**, size_t);
 
 void dfree(void* p)
@@ -194,6 +195,12 @@ static void free_bc(mf_ctx* c)
     dfree(c->bc.btable);
     dfree(c->bc.flag);
     dfree(c->bc.entry);
+    dfree(c->bc.d_kind);
+    dfree(c->bc.d_kind_slip);
+    dfree(c->bc.d_fn);
+    dfree(c->bc.d_params);
+    c->bc.d_kind = c->bc.d_kind_slip = c->bc.d_fn = nullptr;
+    c->bc.d_params = nullptr;
     c->bc.bidx = c->bc.btag = c->bc.entry = nullptr;
     c->bc.bnormal = c->bc.btable = nullptr;
     c->bc.flag = nullptr;
@@ -357,7 +364,24 @@ int mf_synchronize(mf_ctx* c)
     return MF_OK;
 }
 
+static int set_points(mf_ctx* c, const double* x, const double* y, int64_t n, int n_groups,
+                      const int64_t* group_ptr);
+
 int mf_set_points(mf_ctx* c, const double* x, const double* y, int64_t n)
+{
+    return set_points(c, x, y, n, 1, nullptr);
+}
+
+int mf_set_points_grouped(mf_ctx* c, const double* x, const double* y, int64_t n, int n_groups,
+                          const int64_t* group_ptr)
+{
+    if (c && (n_groups < 1 || !group_ptr))
+        return fail(c, MF_ERR_ARGUMENT, "mf_set_points_grouped: need n_groups >= 1 and group_ptr");
+    return set_points(c, x, y, n, n_groups, group_ptr);
+}
+
+static int set_points(mf_ctx* c, const double* x, const double* y, int64_t n, int n_groups,
+                      const int64_t* group_ptr)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -367,6 +391,17 @@ int mf_set_points(mf_ctx* c, const double* x, const double* y, int64_t n)
     for (int64_t i = 0; i < n; ++i)
         if (!std::isfinite(x[i]) || !std::isfinite(y[i]))
             return fail(c, MF_ERR_ARGUMENT, "mf_set_points: non-finite coordinate");
+    std::vector<int32_t> group;
+    if (group_ptr) {
+        if (group_ptr[0] != 0 || group_ptr[n_groups] != n)
+            return fail(c, MF_ERR_ARGUMENT, "mf_set_points_grouped: group_ptr must run from 0 to n_points");
+        group.resize(n);
+        for (int g = 0; g < n_groups; ++g) {
+            if (group_ptr[g + 1] < group_ptr[g])
+                return fail(c, MF_ERR_ARGUMENT, "mf_set_points_grouped: group_ptr not monotone");
+            for (int64_t i = group_ptr[g]; i < group_ptr[g + 1]; ++i) group[i] = g;
+        }
+    }
     MF_HIP(c, hipStreamSynchronize(c->stream));
     // a new cloud (or a new order) invalidates everything built on the old one
     free_state(c);
@@ -385,13 +420,18 @@ int mf_set_points(mf_ctx* c, const double* x, const double* y, int64_t n)
             return rc;
     }
     double *xu = nullptr, *yu = nullptr;
+    int32_t* gu = nullptr;
     if ((rc = dalloc(c, &xu, n)) || (rc = dalloc(c, &yu, n))) return rc;
+    if (group_ptr && (rc = dalloc(c, &gu, n))) return rc;
     MF_HIP(c, hipMemcpyAsync(xu, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     MF_HIP(c, hipMemcpyAsync(yu, y, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    rc = launch_node_order(c, xu, yu);
+    if (gu) MF_HIP(c, hipMemcpyAsync(gu, group.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    rc = launch_node_order(c, xu, yu, gu);
     (void)hipStreamSynchronize(c->stream);
     dfree(xu);
     dfree(yu);
+    dfree(gu);
+    c->n_groups = group_ptr ? n_groups : 1;
     return rc;
 }
 
@@ -736,6 +776,10 @@ int mf_set_boundary(mf_ctx* c, int n_tags, const int32_t* tag_ptr, const int32_t
     b.any_slip = 0;
     b.multi = 0;
     if (n_tags == 0) return MF_OK;
+    b.tag_kind.assign(n_tags, 0);
+    b.tag_fn.assign(n_tags, 0);
+    b.tag_ptr.assign(n_tags + 1, 0);
+    b.tag_params.assign(8 * (size_t)n_tags, 0.0);
     if (!tag_ptr || !tag_kind || !node_idx) return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: null arrays");
     if (tag_ptr[0] != 0) return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: tag_ptr[0] must be 0");
     const int nb = tag_ptr[n_tags];
@@ -775,8 +819,19 @@ int mf_set_boundary(mf_ctx* c, int n_tags, const int32_t* tag_ptr, const int32_t
     b.tag_ptr[n_tags] = nb;
     b.nb = nb;
     if ((rc = dalloc(c, &b.bidx, nb)) || (rc = dalloc(c, &b.btag, nb)) || (rc = dalloc(c, &b.bnormal, 2 * (size_t)nb)) ||
-        (rc = dalloc(c, &b.btable, (size_t)nb * 4)) || (rc = dalloc(c, &b.flag, c->N)) || (rc = dalloc(c, &b.entry, c->N)))
+        (rc = dalloc(c, &b.btable, (size_t)nb * 4)) || (rc = dalloc(c, &b.flag, c->N)) || (rc = dalloc(c, &b.entry, c->N)) ||
+        (rc = dalloc(c, &b.d_kind, n_tags)) || (rc = dalloc(c, &b.d_kind_slip, n_tags)) ||
+        (rc = dalloc(c, &b.d_fn, n_tags)) || (rc = dalloc(c, &b.d_params, 8 * (size_t)n_tags)))
         return rc;
+    std::vector<int32_t> kind_slip(n_tags);
+    for (int g = 0; g < n_tags; ++g)
+        kind_slip[g] = b.tag_kind[g] == MF_BC_SLIP_WALL ? MF_BC_SLIP_WALL : MF_BC_DO_NOTHING;
+    MF_HIP(c, hipMemcpyAsync(b.d_kind, b.tag_kind.data(), sizeof(int32_t) * n_tags, hipMemcpyHostToDevice, c->stream));
+    MF_HIP(c, hipMemcpyAsync(b.d_kind_slip, kind_slip.data(), sizeof(int32_t) * n_tags, hipMemcpyHostToDevice,
+                             c->stream));
+    MF_HIP(c, hipMemcpyAsync(b.d_fn, b.tag_fn.data(), sizeof(int32_t) * n_tags, hipMemcpyHostToDevice, c->stream));
+    MF_HIP(c, hipMemcpyAsync(b.d_params, b.tag_params.data(), sizeof(double) * 8 * n_tags, hipMemcpyHostToDevice,
+                             c->stream));
     std::vector<double> nrm(2 * (size_t)std::max(nb, 1), 0.0);
     if (normals) std::copy(normals, normals + 2 * (size_t)nb, nrm.begin());
     MF_HIP(c, hipMemcpyAsync(b.bidx, bnode.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream));

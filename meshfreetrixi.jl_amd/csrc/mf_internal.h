@@ -11,7 +11,7 @@
 namespace mf {
 
 constexpr int kMaxVar = 4;
-constexpr int kMaxTags = 64;
+constexpr int kMaxTags = 4096;
 constexpr int kMaxHist = 8;
 constexpr int kSellC = 64;  // slice height of the transposed operator
 constexpr int kMaxStencilM = 64;  // saddle systems up to 64 x 64 (one wavefront)
@@ -61,10 +61,14 @@ struct BcDev {
     double* btable = nullptr;    // nvar nb
     uint8_t* flag = nullptr;     // N: bit0 Dirichlet, bit1 slip
     int32_t* entry = nullptr;    // N: boundary entry of the node, -1 if none
-    int tag_kind[kMaxTags];
-    int tag_fn[kMaxTags];
-    int tag_ptr[kMaxTags + 1];
-    double tag_params[kMaxTags * 8];
+    std::vector<int32_t> tag_kind;
+    std::vector<int32_t> tag_fn;
+    std::vector<int32_t> tag_ptr;
+    std::vector<double> tag_params;    // 8 per tag
+    int32_t* d_kind = nullptr;         // ntags: tag kinds
+    int32_t* d_kind_slip = nullptr;    // ntags: slip walls kept, other kinds as do-nothing
+    int32_t* d_fn = nullptr;           // ntags: value functions
+    double* d_params = nullptr;        // 8 ntags: value-function parameters
 };
 
 }  // namespace mf
@@ -85,6 +89,7 @@ struct mf_ctx {
     int32_t* d_inv = nullptr;
     std::vector<int32_t> h_inv;  // host copy of d_inv (boundary and stencil indices)
     int node_order = 0;          // MF_TUNE_NODE_ORDER value for the next mf_set_points
+    int n_groups = 1;            // independent clouds in the point set (mf_set_points_grouped)
 
     // stencil (ELL, slot-major: entry s of node i at s*N + i)
     int k = 0;
@@ -281,7 +286,9 @@ int launch_rows_to_ell_d(mf_ctx* c, const double* d_rows, double* d_ell, int k);
 int launch_soa_to_aos(mf_ctx* c, const double* d_soa, double* d_aos);
 int launch_aos_to_soa(mf_ctx* c, const double* d_aos, double* d_soa);
 int launch_build_didx(mf_ctx* c);
-int launch_node_order(mf_ctx* c, const double* d_xu, const double* d_yu);
+// d_group (caller order, may be null): group of each point; the internal
+// order keeps each group contiguous (Morton order within the group)
+int launch_node_order(mf_ctx* c, const double* d_xu, const double* d_yu, const int32_t* d_group = nullptr);
 void free_tiles(mf_ctx* c);
 void free_tiles_b(mf_ctx* c);
 int launch_gather_nodes_d(mf_ctx* c, const double* d_src, double* d_dst);

@@ -979,9 +979,9 @@ struct BcArgs {
     const StepScalars* sc;
     double dt_off;
     int homog;
-    int tag_kind[kMaxTags];
-    int tag_fn[kMaxTags];
-    double tag_params[kMaxTags * 8];
+    const int32_t* tag_kind;  // per tag, device tables
+    const int32_t* tag_fn;
+    const double* tag_params;  // 8 per tag
 };
 
 // The first-pass condition of boundary entry e on the state s of its node;
@@ -1893,11 +1893,9 @@ static BcArgs make_bc(mf_ctx* c, double dt_off, int homog)
     b.entry = c->bc.entry;
     b.dt_off = dt_off;
     b.homog = homog;
-    for (int g = 0; g < c->bc.ntags; ++g) {
-        b.tag_kind[g] = c->bc.tag_kind[g];
-        b.tag_fn[g] = c->bc.tag_fn[g];
-        for (int q = 0; q < 8; ++q) b.tag_params[8 * g + q] = c->bc.tag_params[8 * g + q];
-    }
+    b.tag_kind = c->bc.d_kind;
+    b.tag_fn = c->bc.d_fn;
+    b.tag_params = c->bc.d_params;
     return b;
 }
 
@@ -1925,9 +1923,7 @@ static int bc_state_pass(mf_ctx* c, double* u, double dt_off, int homog, bool sl
     if (!c->have_bc || c->bc.nb == 0) return MF_OK;
     if (slip_only && !c->bc.any_slip) return MF_OK;
     BcArgs b = make_bc(c, dt_off, homog);
-    if (slip_only)
-        for (int g = 0; g < c->bc.ntags; ++g)
-            if (b.tag_kind[g] != MF_BC_SLIP_WALL) b.tag_kind[g] = MF_BC_DO_NOTHING;
+    if (slip_only) b.tag_kind = c->bc.d_kind_slip;
     auto launch = [&](int e0, int e1) {
         if (e1 <= e0) return;
         b.e0 = e0;
@@ -2111,6 +2107,12 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
     const bool tom = c->have_lap && c->tom_gamma != 0.0;
     const bool rv = c->visc_mode != MF_VISC_NONE;
     if (rv && c->nvar != 4) return fail(c, MF_ERR_UNSUPPORTED, "viscosity sources need the Euler equations");
+    // the residual viscosity's norms and the IGR source's conjugate-gradient
+    // dot products are sums over the whole point set: on a grouped set they
+    // would couple the independent clouds
+    if (c->n_groups > 1 && (c->visc_mode == MF_VISC_RESIDUAL || c->igr_alpha != 0.0))
+        return fail(c, MF_ERR_UNSUPPORTED,
+                    "rhs!: the residual viscosity and IGR sources reduce over all points; not on grouped point sets");
     if (tom && c->nvar != 4) return fail(c, MF_ERR_UNSUPPORTED, "the Tominec source needs the Euler equations");
     if ((rv || tom) && !c->have_T) return fail(c, MF_ERR_STATE, "viscosity source without transposed operator");
     if (tom && !c->lap_T) return fail(c, MF_ERR_STATE, "Tominec source: transposed lap not prepared");

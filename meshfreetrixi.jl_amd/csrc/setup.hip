@@ -79,6 +79,21 @@ __global__ void k_morton(int64_t N, const double* x, const double* y, const doub
     iota[i] = (int32_t)i;
 }
 
+// Group-major Morton keys: (group << 32) | morton code of the point.
+__global__ void k_morton_grouped(int64_t N, const double* x, const double* y, const double* box,
+                                 const int32_t* group, uint64_t* code, int32_t* iota)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double ext = fmax(box[1] - box[0], box[3] - box[2]);
+    if (!(ext > 0.0)) ext = 1.0;
+    const double fx = (x[i] - box[0]) / ext * 65536.0, fy = (y[i] - box[2]) / ext * 65536.0;
+    const uint32_t qx = (uint32_t)fmin(fmax(fx, 0.0), 65535.0);
+    const uint32_t qy = (uint32_t)fmin(fmax(fy, 0.0), 65535.0);
+    code[i] = ((uint64_t)(uint32_t)group[i] << 32) | (uint64_t)(spread16(qx) | (spread16(qy) << 1));
+    iota[i] = (int32_t)i;
+}
+
 __global__ void k_iota(int64_t N, int32_t* v)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -96,12 +111,35 @@ __global__ void k_apply_order(int64_t N, const int32_t* uid, const double* xu, c
     inv[u] = (int32_t)p;
 }
 
-int launch_node_order(mf_ctx* c, const double* d_xu, const double* d_yu)
+int launch_node_order(mf_ctx* c, const double* d_xu, const double* d_yu, const int32_t* d_group)
 {
     const int64_t N = c->N;
     const unsigned g = (unsigned)((N + 255) / 256);
     int rc;
-    if (c->node_order == 1) {
+    if (c->node_order != 1 && d_group) {
+        double* box = nullptr;
+        uint64_t *code = nullptr, *code_s = nullptr;
+        int32_t* iota = nullptr;
+        if ((rc = dalloc(c, &box, 4)) || (rc = dalloc(c, &code, N)) || (rc = dalloc(c, &code_s, N)) ||
+            (rc = dalloc(c, &iota, N)))
+            return rc;
+        hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, c->stream, N, d_xu, d_yu, box);
+        hipLaunchKernelGGL(k_morton_grouped, dim3(g), dim3(256), 0, c->stream, N, d_xu, d_yu, box, d_group, code,
+                           iota);
+        size_t tb = 0;
+        MF_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, code, code_s, iota, c->d_uid, (int)N, 0, 64,
+                                                     c->stream));
+        void* tmp = nullptr;
+        MF_HIP(c, hipMalloc(&tmp, tb + 16));
+        MF_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, code, code_s, iota, c->d_uid, (int)N, 0, 64,
+                                                     c->stream));
+        MF_HIP(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(tmp);
+        dfree(box);
+        dfree(code);
+        dfree(code_s);
+        dfree(iota);
+    } else if (c->node_order == 1) {
         hipLaunchKernelGGL(k_iota, dim3(g), dim3(256), 0, c->stream, N, c->d_uid);
     } else {
         double* box = nullptr;

@@ -45,6 +45,7 @@ _SIGS = {
     "mf_ctx_destroy": [_ctx],
     "mf_synchronize": [_ctx],
     "mf_set_points": [_ctx, _dp, _dp, C.c_int64],
+    "mf_set_points_grouped": [_ctx, _dp, _dp, C.c_int64, C.c_int, C.POINTER(C.c_int64)],
     "mf_knn": [_ctx, C.c_int, _i32p, _dp, _dp, _dp],
     "mf_knn_noalloc": [_ctx, C.c_int, C.c_void_p, C.c_void_p],
     "mf_set_stencil": [_ctx, _i32p, C.c_int64, C.c_int],
@@ -164,6 +165,16 @@ class Context:
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.ascontiguousarray(y, dtype=np.float64)
         self._chk(self._L.mf_set_points(self.h, dptr(x), dptr(y), len(x)))
+        self.N = len(x)
+
+    def set_points_grouped(self, x, y, group_ptr):
+        """mf_set_points_grouped: the point set is len(group_ptr) - 1
+        independent clouds, cloud g = points group_ptr[g] .. group_ptr[g+1]."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        gp = np.ascontiguousarray(group_ptr, dtype=np.int64)
+        self._chk(self._L.mf_set_points_grouped(self.h, dptr(x), dptr(y), len(x), len(gp) - 1,
+                                                gp.ctypes.data_as(C.POINTER(C.c_int64))))
         self.N = len(x)
 
     def knn(self, k, want_idx=True, want_dist=False):

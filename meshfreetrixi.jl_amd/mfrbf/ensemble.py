@@ -1,0 +1,91 @@
+"""Ensembles of independent point clouds in one context.
+
+The reference runs one cloud per semidiscretization; an ensemble of
+independent runs there is a loop over semidiscretizations.  Here the clouds
+are concatenated into one grouped point set (mf_set_points_grouped), so that
+every kernel launch of rhs! and of the captured SSPRK33 step covers all of
+them:
+  - each cloud's stencils come from its own kNN search (a scratch context
+    per cloud size) and are offset into the concatenated numbering;
+  - the weights are computed once over the whole set (each stencil's
+    weights depend only on its own nodes);
+  - each cloud keeps its own boundary tags (Dirichlet values with its own
+    parameters) and its own Flyer hyperviscosity coefficient
+    gamma_r = c dx_min,r^(2k), folded into the rows of the operator;
+  - the clouds advance with one common time step, the smallest of their
+    CFL steps (a lockstep ensemble).
+"""
+import numpy as np
+
+from . import _lib as L
+from .clouds import cloud_square
+from .configs import euler_dt
+from .equations import CompressibleEulerEquations2D
+
+
+class Ensemble:
+    """Builds one grouped context from Euler specs (mfrbf.configs) with
+    Dirichlet inflow (the spec's analytic state) and do-nothing outflow, and
+    the Flyer hyperviscosity (spec.hv = dict(k, c)) when given."""
+
+    def __init__(self, specs, device=0):
+        self.specs = list(specs)
+        eq = CompressibleEulerEquations2D(self.specs[0].gamma)
+        scratch = L.Context(device)
+        xs, ys, idxs, u0s, tags = [], [], [], [], []
+        self.offsets = [0]
+        self.dx_min, self.dx_avg, dts = [], [], []
+        for spec in self.specs:
+            x, y, tag, nx, ny = cloud_square(spec.side, spec.seed)
+            scratch.set_points(x, y)
+            idx, _, dmin, davg = scratch.knn(spec.k)
+            u0 = np.asarray(spec.ic(np.c_[x, y], 0.0, eq))
+            xs.append(x)
+            ys.append(y)
+            idxs.append(idx + self.offsets[-1])
+            u0s.append(u0)
+            tags.append(tag)
+            self.dx_min.append(dmin)
+            self.dx_avg.append(davg)
+            dts.append(euler_dt(spec, u0, dmin))
+            self.offsets.append(self.offsets[-1] + len(x))
+        del scratch
+        self.x, self.y = np.concatenate(xs), np.concatenate(ys)
+        self.N = len(self.x)
+        self.u0 = np.ascontiguousarray(np.concatenate(u0s, axis=1))
+        self.dt = float(min(dts))
+        self.cloud_dt = dts
+        k = self.specs[0].k
+        ctx = L.Context(device)
+        self.ctx = ctx
+        ctx.set_equations(L.MF_EQ_EULER_2D, self.specs[0].gamma)
+        ctx.set_points_grouped(self.x, self.y, self.offsets)
+        ctx.set_stencil(np.ascontiguousarray(np.concatenate(idxs)))
+        spec0 = self.specs[0]
+        ctx.compute_operator(spec0.p, spec0.deg, 1, L.MF_OP_FLUX, want=False)
+        if spec0.hv is not None:
+            hk, hc = spec0.hv["k"], spec0.hv["c"]
+            hv, _ = ctx.compute_operator(spec0.p, spec0.deg, 2 * hk, L.MF_OP_HYPERVISCOSITY, want=True, k=k)
+            gam = np.concatenate([np.full(self.offsets[r + 1] - self.offsets[r], hc * self.dx_min[r] ** (2 * hk))
+                                  for r in range(len(self.specs))])
+            self.gamma = gam
+            ctx.set_operator(L.MF_OP_HYPERVISCOSITY, hv * gam[:, None])
+            ctx.set_source_hyperviscosity(1.0)
+        # boundary: per cloud an inflow tag (Dirichlet, the cloud's own
+        # analytic state) and an outflow tag (do nothing)
+        ptr, kinds, fns, params, nodes = [0], [], [], [], []
+        for r, spec in enumerate(self.specs):
+            tag = tags[r]
+            for code, kind in ((1, L.MF_BC_DIRICHLET), (2, L.MF_BC_DO_NOTHING)):
+                sel = np.nonzero(tag == code)[0] + self.offsets[r]
+                nodes.append(sel.astype(np.int32))
+                ptr.append(ptr[-1] + len(sel))
+                kinds.append(kind)
+                fns.append(spec.ic.code)
+                params.append(spec.ic.params())
+        ctx.set_boundary(ptr, kinds, fns, np.concatenate(params), np.concatenate(nodes),
+                         np.zeros((ptr[-1], 2)))
+
+    def cloud(self, r, u):
+        """Cloud r's part of a state (nvar, N)."""
+        return u[:, self.offsets[r]:self.offsets[r + 1]]

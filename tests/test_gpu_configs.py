@@ -6,10 +6,11 @@ against the CPU oracle on the same seeded inputs.
       them in the reference's operation order within 1e-12; 100 steps on
       128 x 128 nodes within 1e-12.
   C3  kNN (1024 x 1024, k = 50, the ragged stencils' prefixes) bit for bit.
-  C5  the ensemble path (contexts stepped concurrently from captured
-      graphs, split-FP32, Flyer hyperviscosity, 50 steps) on the smallest,
-      the largest and two more of the 64 clouds: each final state within
-      1e-6 of the oracle's FP64 steps and close to the exact travelling wave.
+  C5  the ensemble paths (contexts stepped concurrently from captured
+      graphs; one grouped context whose launches cover all clouds), split-
+      FP32, Flyer hyperviscosity, 50 steps, on the smallest, the largest and
+      two more of the 64 clouds: each final state within 1e-6 of the
+      oracle's FP64 steps and close to the exact travelling wave.
 """
 import numpy as np
 import pytest
@@ -134,3 +135,52 @@ def test_c5_ensemble_vs_oracle(mf):
         _say("  cloud %d (N=%d): vs oracle %.2e, density vs exact %.2e" % (r, c.N, err, err_ex))
         assert err < 1e-6
         assert err_ex < 1e-5
+
+
+@pytest.mark.timeout(600)
+def test_c5_grouped_ensemble_vs_oracle(mf):
+    """C5 as one grouped context (mf_set_points_grouped): four of its clouds
+    (the smallest, the largest and two between) concatenated, each with its
+    own stencils, boundary tags and Flyer coefficient, all advanced by every
+    launch of one captured SSPRK33 step (split-FP32, 50 steps, the common
+    time step).  Each cloud's final state is within 1e-6 of the oracle's
+    FP64 steps of that cloud alone, and its density within 1e-5 of the
+    exact travelling wave."""
+    from mfrbf.ensemble import Ensemble
+    from mfrbf.equations import CompressibleEulerEquations2D
+
+    specs = configs.c5()
+    pick = _c5_pick()
+    _say("grouped ensemble of clouds %s" % pick)
+    ens = Ensemble([specs[r] for r in pick])
+    steps = specs[0].nsteps
+    ens.ctx.upload(L.MF_BUF_U, ens.u0)
+    ens.ctx.ssprk33(0.0, ens.dt, steps, L.MF_PREC_SPLIT32, L.MF_STEP_GRAPH)
+    ug = ens.ctx.download(L.MF_BUF_U)
+    eq = CompressibleEulerEquations2D(1.4)
+    for q, r in enumerate(pick):
+        c = cases.build(specs[r], hv=(2, 1e-4), use_gpu=False)
+        uo = c.u0.copy()
+        c.prob.ssprk33(uo, 0.0, ens.dt, steps)
+        gr = ens.cloud(q, ug)
+        ex = specs[r].ic(np.c_[c.x, c.y], steps * ens.dt, eq)
+        err, err_ex = rel_inf(gr, uo), float(np.abs(gr[0] - ex[0]).max())
+        _say("  cloud %d (N=%d): vs oracle %.2e, density vs exact %.2e" % (r, c.N, err, err_ex))
+        assert err < 1e-6
+        assert err_ex < 1e-5
+
+
+def test_grouped_set_refuses_global_reductions(mf):
+    """The residual viscosity reduces over all points, which would couple the
+    clouds of a grouped set: rhs! refuses it there."""
+    from mfrbf.ensemble import Ensemble
+
+    specs = configs.c5(2)
+    for s in specs:
+        s.side = 24
+    ens = Ensemble(specs)
+    ens.ctx.set_source_viscosity(L.MF_VISC_RESIDUAL, 1.0, 1.0, ens.dx_avg[0], 3, True)
+    ens.ctx.upload(L.MF_BUF_U, ens.u0)
+    with pytest.raises(L.MfError) as e:
+        ens.ctx.rhs(0.0, L.MF_PREC_F64)
+    assert e.value.code == L.MF_ERR_UNSUPPORTED

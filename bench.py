@@ -658,15 +658,19 @@ def main():
     # the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters of a
     # profiled run of this script
     # and the windowed gather of the same size, the second calibration kernel
-    probe = {"copy": [], "gather": []}
+    # (both 16.8 MB per launch: launch latency weighs on them), and a
+    # read-modify-write streamed over 512 MiB, the device's practical
+    # streaming bandwidth
+    probe = {"copy": [], "gather": [], "stream": []}
     for _ in range(3):
         ctx.copy(L.MF_BUF_AUX0, L.MF_BUF_U)
     for _ in range(5):
         probe["copy"].append(ctx.bandwidth_probe(0, L.MF_BUF_AUX0, L.MF_BUF_U))
         probe["gather"].append(ctx.bandwidth_probe(1, L.MF_BUF_AUX0, L.MF_BUF_U))
-    probe_bytes = 16 * N * 4
-    probe = {q: {"bytes": probe_bytes, "ms": float(np.min(v)), "gbs": probe_bytes / (np.min(v) * 1e-3) / 1e9}
-             for q, v in probe.items()}
+        probe["stream"].append(ctx.bandwidth_probe(2, L.MF_BUF_AUX0, L.MF_BUF_U))
+    probe_bytes = {"copy": 16 * N * 4, "gather": 16 * N * 4, "stream": 2 * 8 * (1 << 26)}
+    probe = {q: {"bytes": probe_bytes[q], "ms": float(np.min(v)),
+                 "gbs": probe_bytes[q] / (np.min(v) * 1e-3) / 1e9} for q, v in probe.items()}
     # three SSPRK33 steps with the history callback so the residual viscosity is live
     ctx.ssprk33(0.0, info["dt"], 3, flags=L.MF_STEP_HISTORY)
     t_now = 3 * info["dt"]
@@ -744,7 +748,8 @@ def main():
                      "kernel_ms": kms, "kernel_bytes": kbytes,
                      "rhs_frac": rbytes / (rhs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "rhs_bytes": rbytes,
                      "rhs_frac_cold": rbytes / (cold["f64_ref"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "probe_gbs": {q: v["gbs"] for q, v in probe.items()}},
+                     "probe_gbs": {q: v["gbs"] for q, v in probe.items()},
+                     "frac_of_stream": achieved / probe["stream"]["gbs"]},
         "kernels_ms": {"stencil": parts[0], "transpose": parts[1], "reductions": parts[2],
                        "boundary": parts[3]},
         "f64_fma": {"value": N / (f_ms * 1e-3), "ms_per_step": f_ms,

@@ -271,9 +271,12 @@ int mf_state_copy(mf_ctx* ctx, int dst_buf, int src_buf);
  * doubles, each read once and written once: 16 bytes per double): kind 0
  * the streaming copy of mf_state_copy, kind 1 a gather whose wavefronts
  * read scattered doubles within 2 KiB windows (the access shape of a
- * stencil gather).  ms (may be NULL) receives the kernel's time from HIP
- * events.  No counterpart in the reference; the calibration kernels of the
- * HBM traffic counters and a practical bandwidth ceiling. */
+ * stencil gather); kind 2 ignores the buffers and streams a read-modify-
+ * write over a 512 MiB scratch array (1 GiB of HBM traffic, twice the
+ * last-level cache: the device's streaming bandwidth).  ms (may be NULL)
+ * receives the kernel's time from HIP events.  No counterpart in the
+ * reference; the calibration kernels of the HBM traffic counters and a
+ * practical bandwidth ceiling. */
 int mf_bandwidth_probe(mf_ctx* ctx, int kind, int dst_buf, int src_buf, double* ms);
 /* Device address of a context buffer: node-major, nvar doubles per node,
  * nodes in the context's internal order (see mf_node_order). */
@@ -298,6 +301,12 @@ int mf_rhs_host(mf_ctx* ctx, double t, int precision, double* const* u, double* 
 /* Viscosity of the last rhs! with a viscosity source: eps and the choice
  * code eps_c (0 residual, 1 upwind, 2 eps()), hyperviscosity.jl:331-349. */
 int mf_get_viscosity(mf_ctx* ctx, double* eps, int8_t* eps_c);
+
+/* Inspection (no counterpart in the reference, where these are locals of
+ * update_residual_visc!, hyperviscosity.jl:289-311): mean(u) and ninf, the
+ * lexicographic maximum of |u - mean| with zeros replaced by eps(), of the
+ * last rhs! with the residual viscosity.  Either pointer may be NULL. */
+int mf_get_viscosity_norms(mf_ctx* ctx, double* mean, double* ninf);
 
 /* --------------------------------------------------- step/stage hooks */
 /* HistoryCallback affect! for the viscosity source: shift the solution

@@ -244,10 +244,10 @@ int ensure_state(mf_ctx* c)
         }
         if (!c->d_hist && (rc = dalloc(c, &c->d_hist, nv * (size_t)(c->approx_order + 1)))) return rc;
         if (!c->d_partials) {
-            // reduction passes: the sum pass one node per thread (up to 2048
-            // blocks), the max pass up to kBlock blocks of 4 nodes per thread
-            // (the stencil kernels load one max partial per thread)
-            c->n_psum = (int)std::min<int64_t>(2048, (c->N + 255) / 256);
+            // reduction passes: the sum pass and the max pass up to kBlock
+            // blocks of 4 nodes per thread (the max pass loads one sum partial
+            // per thread, the stencil kernels one max partial per thread)
+            c->n_psum = (int)std::min<int64_t>(kBlock, (c->N + 1023) / 1024);
             c->n_pmax = (int)std::min<int64_t>(kBlock, (c->N + 1023) / 1024);
             if ((rc = dalloc(c, &c->d_partials, 4 * (size_t)(c->n_psum + c->n_pmax)))) return rc;
         }
@@ -1001,16 +1001,18 @@ int mf_bandwidth_probe(mf_ctx* c, int kind, int dst_buf, int src_buf, double* ms
 {
     int rc = check_ctx(c);
     if (rc) return rc;
-    if (dst_buf < 0 || dst_buf > 3 || src_buf < 0 || src_buf > 3 || dst_buf == src_buf || (kind != 0 && kind != 1))
+    if (dst_buf < 0 || dst_buf > 3 || src_buf < 0 || src_buf > 3 || dst_buf == src_buf || kind < 0 || kind > 2)
         return fail(c, MF_ERR_ARGUMENT, "mf_bandwidth_probe: bad arguments");
     if ((rc = ensure_state(c))) return rc;
+    if (kind == 2 && !c->d_flush && (rc = dalloc(c, &c->d_flush, kFlushDoubles))) return rc;
     hipEvent_t e0, e1;
     MF_HIP(c, hipEventCreate(&e0));
     MF_HIP(c, hipEventCreate(&e1));
     MF_HIP(c, hipEventRecord(e0, c->stream));
     const int64_t n = c->N * c->nvar;
-    rc = kind == 0 ? launch_copy_buffer(c, c->d_buf[src_buf], c->d_buf[dst_buf], n)
-                   : launch_gather_probe(c, c->d_buf[src_buf], c->d_buf[dst_buf], n);
+    rc = kind == 0   ? launch_copy_buffer(c, c->d_buf[src_buf], c->d_buf[dst_buf], n)
+         : kind == 1 ? launch_gather_probe(c, c->d_buf[src_buf], c->d_buf[dst_buf], n)
+                     : launch_flush(c, c->d_flush, kFlushDoubles);
     if (rc == MF_OK) {
         MF_HIP(c, hipEventRecord(e1, c->stream));
         MF_HIP(c, hipEventSynchronize(e1));
@@ -1130,6 +1132,22 @@ int mf_get_viscosity(mf_ctx* c, double* eps, int8_t* eps_c)
     }
     MF_HIP(c, hipStreamSynchronize(c->stream));
     dfree(tmp);
+    return MF_OK;
+}
+
+int mf_get_viscosity_norms(mf_ctx* c, double* mean, double* ninf)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->visc_mode != MF_VISC_RESIDUAL || !c->d_partials)
+        return fail(c, MF_ERR_STATE, "no residual viscosity evaluated");
+    StepScalars sc;
+    MF_HIP(c, hipStreamSynchronize(c->stream));
+    MF_HIP(c, hipMemcpy(&sc, c->d_sc, sizeof(StepScalars), hipMemcpyDeviceToHost));
+    for (int v = 0; v < 4; ++v) {
+        if (mean) mean[v] = sc.norms[v];
+        if (ninf) ninf[v] = sc.norms[4 + v];
+    }
     return MF_OK;
 }
 

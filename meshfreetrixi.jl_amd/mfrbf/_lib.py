@@ -69,6 +69,7 @@ _SIGS = {
     "mf_rhs": [_ctx, C.c_double, C.c_int],
     "mf_rhs_host": [_ctx, C.c_double, C.c_int, _dpp, _dpp],
     "mf_get_viscosity": [_ctx, _dp, _i8p],
+    "mf_get_viscosity_norms": [_ctx, _dp, _dp],
     "mf_history_update": [_ctx, C.c_double, C.c_int],
     "mf_get_approx_du": [_ctx, _dpp],
     "mf_limiter_zhang_shu": [_ctx, C.c_double, C.c_int],
@@ -289,6 +290,12 @@ class Context:
         self._chk(self._L.mf_get_viscosity(self.h, dptr(eps), code.ctypes.data_as(_i8p)))
         return eps, code
 
+    def viscosity_norms(self):
+        """(mean, ninf) of the last rhs! with the residual viscosity."""
+        mean, ninf = np.empty(4), np.empty(4)
+        self._chk(self._L.mf_get_viscosity_norms(self.h, dptr(mean), dptr(ninf)))
+        return mean, ninf
+
     def history_update(self, t, success_iter):
         self._chk(self._L.mf_history_update(self.h, t, success_iter))
 
@@ -325,7 +332,8 @@ class Context:
 
     def bandwidth_probe(self, kind, dst_buf, src_buf):
         """Time (ms) of a known-bytes kernel from src_buf into dst_buf: kind 0
-        the streaming copy, 1 the windowed gather (mf_bandwidth_probe)."""
+        the streaming copy, 1 the windowed gather; kind 2 a read-modify-write
+        over a 512 MiB scratch array (mf_bandwidth_probe)."""
         ms = C.c_double(0.0)
         self._chk(self._L.mf_bandwidth_probe(self.h, kind, dst_buf, src_buf, C.byref(ms)))
         return ms.value

@@ -253,6 +253,48 @@ def test_rhs_residual_norm_extremes(mf, shape):
         assert rel_inf(du, duo) < (1e-12 if prec != L.MF_PREC_SPLIT32 else TOL32)
 
 
+@pytest.mark.parametrize("shape", ["wave", "spike", "dip", "tie", "flat"])
+def test_rv_norms_vs_oracle(mf, shape):
+    """mean(u) (sum / 4N) and ninf, the lexicographic maximum of |u - mean|,
+    against the oracle's pairwise mean and brute-force maximum: at one node
+    (a density spike; a density dip, which |rho - mean| never ranks first,
+    since the mean is a quarter of the average density), at two nodes of
+    equal density that the momentum decides, and with the density flat (the
+    momentum decides among all nodes)."""
+    _say("residual viscosity norms: %s" % shape)
+    c = cases.build(configs.c2(), side=48, visc="residual", bc=("nothing", "nothing"))
+    u = c.u0.copy()
+    inner = np.nonzero(c.tag == 0)[0]
+    i, j = inner[len(inner) // 3], inner[2 * len(inner) // 3]
+    if shape == "spike":
+        u[0, i] *= 1.5
+    elif shape == "dip":
+        u[0, i] = 0.25 * u[0].min()
+    elif shape == "tie":
+        u[0, i] = u[0, j] = 1.5 * u[0].max()
+        u[1, i] += 0.01
+        u[1, j] += 0.02
+    elif shape == "flat":
+        u[0, :] = 1.0
+    u[3] = np.maximum(u[3], 0.5 * (u[1] ** 2 + u[2] ** 2) / u[0] + 0.5)  # positive pressure
+    c.u0 = np.ascontiguousarray(u)
+    _rv_history(c, c.u0 * 1.0001)
+    mean_o, ninf_o = O.rv_norms(c.u0)
+    for prec in (L.MF_PREC_F64_REF, L.MF_PREC_SPLIT32):
+        uu, du = c.u0.copy(), np.zeros_like(c.u0)
+        c.ctx.rhs_host(0.001, uu, du, prec)
+        assert np.array_equal(uu, c.u0)  # no boundary pass changed the state
+        mean, ninf = c.ctx.viscosity_norms()
+        _say("  prec=%d mean err %.2e, ninf err %.2e" % (prec, np.abs(mean - mean_o).max(),
+                                                         np.abs(ninf - ninf_o).max()))
+        assert np.abs(mean - mean_o).max() <= 1e-14 * np.abs(mean_o).max()
+        assert np.all(np.abs(ninf - ninf_o) <= 1e-13 * np.maximum(1.0, np.abs(mean_o)))
+        duo, eps, epsc = c.prob.rhs(c.u0.copy(), 0.001)
+        geps, gcode = c.ctx.viscosity()
+        assert np.array_equal(gcode, epsc)
+        assert rel_inf(geps, eps) < (1e-12 if prec != L.MF_PREC_SPLIT32 else 1e-6)
+
+
 def test_rhs_split32_error(mf):
     _say("rhs euler split32")
     c = cases.build(configs.c2(), side=96)

@@ -578,8 +578,8 @@ int mf_compute_operator(mf_ctx* c, int phs_power, int poly_degree, int deriv_ord
     if (!c->d_status && (rc = dalloc(c, &c->d_status, 1))) return rc;
     MF_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
     if (c->weights_solver == 2 && (phs_power & 1) && poly_degree <= 4 && c->k > np)
-        rc = launch_weights_ns(c, nullptr, c->d_idx, c->k, c->N, c->k, phs_power, poly_degree, deriv_order, wx, wy,
-                               c->d_status);
+        rc = launch_weights_ns(c, nullptr, c->d_idx, c->k, nullptr, c->N, c->k, phs_power, poly_degree, deriv_order,
+                               wx, wy, c->d_status);
     else
         rc = launch_weights(c, nullptr, c->d_idx, c->k, c->N, c->k, phs_power, poly_degree, deriv_order, wx, wy,
                             c->d_status);
@@ -707,9 +707,22 @@ int mf_weights_ragged_run(mf_ctx* c, int phs_power, int poly_degree, int deriv_o
         return fail(c, MF_ERR_ARGUMENT, "mf_weights_ragged_run: bad basis");
     MF_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
     const int np = (poly_degree + 1) * (poly_degree + 2) / 2;
-    if (c->weights_solver != 1 && (phs_power & 1) && poly_degree <= 4 && c->rminn > np)
-        return launch_weights_ns(c, c->d_rptr, c->d_ridx, 0, c->rn, c->rmaxn, phs_power, poly_degree, deriv_order,
-                                 c->d_rwx, c->d_rwy, c->d_status);
+    if (c->weights_solver != 1 && (phs_power & 1) && poly_degree <= 4 && c->rminn > np) {
+        // the stencils in size order (d_rorder): those of up to 32 nodes in
+        // the smaller variant, the rest in the larger
+        int64_t n_small = 0;
+        int maxn_small = 0;
+        for (size_t q = 0; q < c->rclass_n.size(); ++q)
+            if (c->rclass_n[q] <= 32) {
+                n_small = c->rclass_start[q + 1];
+                maxn_small = c->rclass_n[q];
+            }
+        if ((rc = launch_weights_ns(c, c->d_rptr, c->d_ridx, 0, c->d_rorder, n_small, maxn_small, phs_power,
+                                    poly_degree, deriv_order, c->d_rwx, c->d_rwy, c->d_status)))
+            return rc;
+        return launch_weights_ns(c, c->d_rptr, c->d_ridx, 0, c->d_rorder + n_small, c->rn - n_small, c->rmaxn,
+                                 phs_power, poly_degree, deriv_order, c->d_rwx, c->d_rwy, c->d_status);
+    }
     for (size_t q = 0; q < c->rclass_n.size(); ++q) {
         const int64_t b0 = c->rclass_start[q], b1 = c->rclass_start[q + 1];
         if ((rc = launch_weights(c, c->d_rptr, c->d_ridx, 0, b1 - b0, c->rclass_n[q], phs_power, poly_degree,

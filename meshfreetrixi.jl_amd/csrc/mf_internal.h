@@ -272,10 +272,12 @@ int64_t knn_noalloc_max_nodes();
 int launch_weights(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k,
                    int64_t n_stencils, int maxn, int p, int deg, int kd, double* d_wx,
                    double* d_wy, int* d_status, const int32_t* d_order = nullptr, int64_t base = 0);
-// the same weights by the null-space form on the matrix cores, every
-// stencil (ragged, any sizes) in one persistent launch
-int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k, int64_t n_stencils,
-                      int maxn, int p, int deg, int kd, double* d_wx, double* d_wy, int* d_status);
+// the same weights by the null-space form on the matrix cores: one
+// workgroup per stencil, stencil d_sid[b] (or b when d_sid is null) for
+// workgroup b of n_launch; maxn, the largest of their sizes, picks the
+// variant (up to 32 or up to 64 nodes)
+int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k, const int32_t* d_sid,
+                      int64_t n_launch, int maxn, int p, int deg, int kd, double* d_wx, double* d_wy, int* d_status);
 int launch_pack_ops(mf_ctx* c, int target, const double* d_wx, const double* d_wy);
 int launch_build_transpose(mf_ctx* c);
 int launch_dx_stats(mf_ctx* c, const double* d_dist2, double* dx_min, double* dx_avg);

@@ -20,11 +20,13 @@
 // pivoting (K is symmetric positive definite).  The products G, X, Y, W,
 // Q'h and the rank-2np update of K run as v_mfma_f64_16x16x4 tiles.
 //
-// One wavefront (workgroup) per stencil, n <= 64 nodes, np <= 16
-// monomials, n > np; stencils of any sizes within those bounds run in one
-// launch.  (A persistent grid that loops over stencils took twice the
-// registers: the loop-invariant LDS addresses of the unrolled phases stay
-// live across the loop.)
+// One wavefront (workgroup) per stencil, np <= 16 monomials, n > np.  The
+// kernel is built for up to MT 16-row tiles: MT = 2 (n <= 32: 128 VGPRs and
+// under 10 KiB of LDS, four wavefronts per SIMD) and MT = 4 (n <= 64: 192
+// VGPRs, two per SIMD); a ragged batch runs as one launch per variant over
+// the stencils of its size class.  (A persistent grid that loops over
+// stencils took twice the registers: the loop-invariant LDS addresses of
+// the unrolled phases stay live across the loop.)
 #include <hip/hip_runtime.h>
 
 #include "mf_device.h"
@@ -44,6 +46,7 @@ struct NsArgs {
     const double* y;
     const int64_t* ptr;  // ragged stencils (or null: ELL)
     const int32_t* ridx;
+    const int32_t* sid;  // stencil of each workgroup (null: workgroup b runs stencil b)
     const int32_t* ell;  // ELL, slot-major (ptr null)
     int ell_k;
     int64_t N;
@@ -107,26 +110,28 @@ __device__ __forceinline__ double phs_r(const double* co, int i, int j, int p)
     return ipow(sqrt(dx * dx + dy * dy), p);
 }
 
-__global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
+template <int MT>
+__global__ __launch_bounds__(64, MT == 2 ? 4 : 1) void k_weights_ns(const NsArgs a)
 {
 #pragma clang fp contract(off)
+    constexpr int NL = 16 * MT;  // largest stencil
     extern __shared__ double sm[];
     const int lane = threadIdx.x;
     const int q = lane >> 4, c = lane & 15;
     const int LQ = a.lq, LZ = kNsLz;
-    double* co = sm;                      // 64 x (x, y)
-    double* Qm = sm + 128;                // rows x LQ: P, then Q (columns < np)
+    double* co = sm;                      // NL x (x, y)
+    double* Qm = sm + 2 * NL;             // rows x LQ: P, then Q (columns < np)
     double* Xm = Qm + a.rows * LQ;        // rows x LQ: X = R Q, then W (columns < np)
     double* Zm = Xm + a.rows * LQ;        // 16 x LZ: G, L of G, Z, then K tiles
-    double* col = Zm + 16 * LZ;           // 64: broadcast column (16-byte aligned)
-    double* hv = col + 64;                // 128: h = b - X g (2 per node)
-    double* gs = hv + 128;                // 32: g (2 per monomial)
+    double* col = Zm + 16 * LZ;           // NL: broadcast column (16-byte aligned)
+    double* hv = col + NL;                // 2 NL: h = b - X g (2 per node)
+    double* gs = hv + 2 * NL;             // 32: g (2 per monomial)
     double* ss = gs + 32;                 // 32: s = Q'h (2 per monomial)
     double* dg = ss + 32;                 // 16: 1 / L_jj of the Cholesky factor of G
     const int np = a.np;
 
     {
-        const int64_t e = blockIdx.x;
+        const int64_t e = a.sid ? (int64_t)a.sid[blockIdx.x] : (int64_t)blockIdx.x;
         int64_t off;
         int n;
         if (a.ptr) {
@@ -152,8 +157,10 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
         }
         const double sx = 1.0 / wave_absmax(px), sy = 1.0 / wave_absmax(py);
         const double xs = px * sx, ys = py * sy;
-        co[2 * lane] = xs;
-        co[2 * lane + 1] = ys;
+        if (lane < NL) {
+            co[2 * lane] = xs;
+            co[2 * lane + 1] = ys;
+        }
 
         // ---- P row of this node (poly_block, :211-223), into LDS
         if (lane < a.rows) {
@@ -197,7 +204,7 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
                     const double rs = rsqrt_nr(d);
                     if (lane == 0) dg[j] = rs;
                     const double l = lane >= j ? gr[j] * rs : 0.0;
-                    col[lane] = l;
+                    if (lane < NL) col[lane] = l;
                     __syncthreads();
 #pragma unroll
                     for (int cc = j + 1; cc < 16; ++cc) gr[cc] = fma(-l, col[cc], gr[cc]);
@@ -265,7 +272,7 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
         double ra[16];
 #pragma unroll
         for (int t = 0; t < 16; ++t) ra[t] = 0.0;
-        if (T <= 2) {
+        if (MT == 2 || T <= 2) {
 #pragma unroll
             for (int rt = 0; rt < 2; ++rt) {
                 if (rt < T) {
@@ -290,7 +297,7 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
             }
         } else {
 #pragma unroll
-            for (int rt = 0; rt < 4; ++rt) {
+            for (int rt = 0; rt < MT; ++rt) {
                 if (rt < T) {
                     const int i = 16 * rt + c;
                     d4 acc = {0.0, 0.0, 0.0, 0.0};
@@ -334,8 +341,10 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
             h0 = b0 - x0;
             h1 = b1 - x1;
         }
-        hv[2 * lane] = h0;
-        hv[2 * lane + 1] = h1;
+        if (lane < NL) {
+            hv[2 * lane] = h0;
+            hv[2 * lane + 1] = h1;
+        }
         __syncthreads();
 
         // ---- s = Q'h (matrix cores; columns 0 and 1 of the tile)
@@ -388,7 +397,7 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
 
         // ---- W = sgn X - Q Z over the X rows (matrix cores)
 #pragma unroll
-        for (int rt = 0; rt < 4; ++rt) {
+        for (int rt = 0; rt < MT; ++rt) {
             if (rt < T) {
                 const int i = 16 * rt + c;
                 d4 acc;
@@ -414,14 +423,14 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
 
         // ---- K = sgn R - W Q' - Q W' tile by tile (matrix cores), each tile
         //      staged through Zm into the rows of the lanes that own them
-        double kr[64];
+        double kr[NL];
 #pragma unroll
-        for (int t = 0; t < 64; ++t) kr[t] = 0.0;
+        for (int t = 0; t < NL; ++t) kr[t] = 0.0;
 #pragma unroll
-        for (int it = 0; it < 4; ++it) {
+        for (int it = 0; it < MT; ++it) {
             if (it < T) {
 #pragma unroll
-                for (int jt = 0; jt < 4; ++jt) {
+                for (int jt = 0; jt < MT; ++jt) {
                     if (jt < T) {
                         d4 acc;
 #pragma unroll
@@ -429,7 +438,7 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
                             const int row = 16 * it + 4 * r + q, cl = 16 * jt + c;
                             double rv;
                             if (it < 2 && jt < 2)
-                                rv = T <= 2 ? ra[(8 * jt + 4 * it + r) & 15] : phs_r(co, row, cl, a.p);
+                                rv = (MT == 2 || T <= 2) ? ra[(8 * jt + 4 * it + r) & 15] : phs_r(co, row, cl, a.p);
                             else
                                 rv = phs_r(co, row, cl, a.p);
                             acc[r] = (row < n && cl < n) ? a.sgn * rv : (row == cl ? 1.0 : 0.0);
@@ -466,13 +475,13 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
         //      stays symmetric, so row j's entries right of j are column j's.
         double piv = 1.0;  // 1 / pivot of this lane's row
 #pragma unroll
-        for (int jc = 0; jc < 4; ++jc) {
+        for (int jc = 0; jc < MT; ++jc) {
             if (jc < T) {
 #pragma unroll
                 for (int jj = 0; jj < 16; ++jj) {
                     const int j = 16 * jc + jj;
                     if (j < n) {
-                        col[lane] = kr[j];
+                        if (lane < NL) col[lane] = kr[j];
                         __syncthreads();
                         const double pj = bcast(kr[j], j);
                         const double fj0 = bcast(f0, j), fj1 = bcast(f1, j);
@@ -481,7 +490,7 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
                         const double m = lane == j ? 0.0 : kr[j] * ip;
                         if (lane == j) piv = ip;
 #pragma unroll
-                        for (int cc = jc; cc < 4; ++cc) {
+                        for (int cc = jc; cc < MT; ++cc) {
                             if (cc < T) {
 #pragma unroll
                                 for (int ci = 0; ci < 16; ci += 2) {
@@ -515,27 +524,35 @@ __global__ __launch_bounds__(64) void k_weights_ns(const NsArgs a)
     }
 }
 
-// LDS doubles of one wavefront for stencils of up to `rows` nodes.
-static size_t ns_lds_doubles(int rows, int lq) { return 128 + 2 * (size_t)rows * lq + 16 * kNsLz + 64 + 128 + 32 + 32 + 16; }
-
-int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k, int64_t n_stencils,
-                      int maxn, int p, int deg, int kd, double* d_wx, double* d_wy, int* d_status)
+// LDS doubles of one wavefront of the MT variant for stencils of up to
+// `rows` nodes.
+static size_t ns_lds_doubles(int mt, int rows, int lq)
 {
+    const size_t nl = 16 * (size_t)mt;
+    return 2 * nl + 2 * (size_t)rows * lq + 16 * kNsLz + nl + 2 * nl + 32 + 32 + 16;
+}
+
+int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k, const int32_t* d_sid,
+                      int64_t n_launch, int maxn, int p, int deg, int kd, double* d_wx, double* d_wy, int* d_status)
+{
+    if (n_launch == 0) return MF_OK;
     NsArgs a;
     a.x = c->d_x;
     a.y = c->d_y;
     a.ptr = d_ptr;
     a.ridx = d_ptr ? d_idx : nullptr;
+    a.sid = d_sid;
     a.ell = d_ptr ? nullptr : d_idx;
     a.ell_k = ell_k;
     a.N = c->N;
-    a.ns = n_stencils;
+    a.ns = n_launch;
     a.p = p;
     a.np = (deg + 1) * (deg + 2) / 2;
     a.kd = kd;
     if (a.np > 16) return fail(c, MF_ERR_UNSUPPORTED, "null-space weights: polynomial degree above 4");
     if (maxn > 64) return fail(c, MF_ERR_UNSUPPORTED, "null-space weights: stencil above 64 nodes");
     if ((p & 1) == 0) return fail(c, MF_ERR_UNSUPPORTED, "null-space weights: even PHS power");
+    if (n_launch > 0x7fffffff) return fail(c, MF_ERR_ARGUMENT, "null-space weights: too many stencils");
     a.rows = maxn < 16 ? 16 : maxn;
     double kf = 1.0;
     for (int i = 2; i <= kd; ++i) kf *= i;
@@ -546,10 +563,12 @@ int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int
     a.wy = d_wy;
     a.status = d_status;
     a.lq = ns_lq(a.np);
-    const size_t lds = sizeof(double) * ns_lds_doubles(a.rows, a.lq);
-    if (n_stencils > 0x7fffffff) return fail(c, MF_ERR_ARGUMENT, "null-space weights: too many stencils");
-    const int64_t grid = n_stencils;
-    hipLaunchKernelGGL(k_weights_ns, dim3((unsigned)grid), dim3(64), lds, c->stream, a);
+    const int mt = maxn <= 32 ? 2 : 4;
+    const size_t lds = sizeof(double) * ns_lds_doubles(mt, a.rows, a.lq);
+    if (mt == 2)
+        hipLaunchKernelGGL(k_weights_ns<2>, dim3((unsigned)n_launch), dim3(64), lds, c->stream, a);
+    else
+        hipLaunchKernelGGL(k_weights_ns<4>, dim3((unsigned)n_launch), dim3(64), lds, c->stream, a);
     MF_HIP(c, hipGetLastError());
     return MF_OK;
 }

@@ -188,7 +188,8 @@ def stepping(L, ctx, info, steps):
     """SSPRK33 steps at this workload (graph-replayed, FP64 reference order):
     plain, with the history callback, and with the history callback and the
     Zhang-Shu stage limiter on density and pressure (three limiter passes per
-    step), so the callbacks' cost per step is the difference."""
+    step), so the callbacks' cost per step is the difference; and the cost of
+    one stage (a third of a step)."""
     dt = info["dt"]
     out = {}
     ctx.set_stage_limiter([1e-6, 1e-6], [L.MF_VAR_DENSITY, L.MF_VAR_PRESSURE])
@@ -201,6 +202,9 @@ def stepping(L, ctx, info, steps):
         ctx.ssprk33(0.0, dt, steps, L.MF_PREC_F64_REF, flags | L.MF_STEP_GRAPH, resume=True)
         ctx.synchronize()
         out[name + "_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / steps
+    # one stage = one rhs! with the stage update fused into its last kernel
+    # (plus, with the callbacks, their share of the step)
+    out["per_stage_ms"] = {name: out[name + "_ms_per_step"] / 3 for name in ("plain", "history", "history_limiter")}
     return out
 
 
@@ -284,17 +288,15 @@ def adjoint_ratio(L, configs):
                 ratio_reverse=(t_adj - t_fwd) / t_fwd, loss=loss)
 
 
-def ensemble(L, configs, steps):
-    """C5: 64 Euler clouds (16k..262k nodes) with Flyer hyperviscosity,
-    split-FP32 rhs!, SSPRK33 steps.  Headline path: one grouped context
-    (mf_set_points_grouped) whose every launch covers all 64 clouds, the
-    steps replayed from its captured graph at the common time step.  For
-    comparison the same clouds as 64 contexts (one stream each, their own
-    time steps) enqueued from this thread before any is synchronized."""
+def _ensemble_runs(L, specs, steps):
+    """One grouped context over the clouds (mf_set_points_grouped: every
+    launch covers all of them, the steps replayed from its captured graph at
+    the common time step), and for comparison the same clouds as separate
+    contexts (one stream each, their own time steps) enqueued from this
+    thread before any is synchronized."""
     from mfrbf.ensemble import Ensemble
     from mfrbf.equations import CompressibleEulerEquations2D
 
-    specs = configs.c5()
     t0 = time.perf_counter()
     ens = Ensemble(specs)
     t_setup_g = time.perf_counter() - t0
@@ -322,7 +324,6 @@ def ensemble(L, configs, steps):
                rhs_grouped={"ms": rhs_ms, "node_evals_per_s": total / (rhs_ms * 1e-3),
                             "kernels_ms": {"stencil": rhs_parts[0], "transpose": rhs_parts[1]}})
     del ens, g
-    # the same clouds as 64 contexts on 64 streams
     t0 = time.perf_counter()
     clouds = []
     for spec in specs:
@@ -346,6 +347,19 @@ def ensemble(L, configs, steps):
     run()  # capture the graphs, build the LDS tiles
     wall_c = run()
     out["contexts"] = dict(setup_s=t_setup, ms_concurrent=1e3 * wall_c, node_evals_per_s=evals / wall_c)
+    return out
+
+
+def ensemble(L, configs, steps):
+    """C5: 64 Euler clouds (16k..262k nodes) with Flyer hyperviscosity,
+    split-FP32 rhs!, SSPRK33 steps, as one grouped context and as 64
+    contexts; and the same physics on 256 small clouds (48 x 48 nodes each),
+    where the launches rather than the bytes set the time."""
+    out = _ensemble_runs(L, configs.c5(), steps)
+    small = configs.c5(256)
+    for r, sp in enumerate(small):
+        sp.name, sp.side = "small_%d" % r, 48
+    out["small_clouds"] = _ensemble_runs(L, small, steps)
     return out
 
 
@@ -487,7 +501,7 @@ PROFILE_COMMANDS = {
     ],
     "trace": [
         "cd /tmp && export TMPDIR=/tmp && timeout -k 10 300 rocprofv3 --kernel-trace --stats --output-format csv "
-        "-d $REPO/gpurun_out/prof_trace -o trace -- python3 $REPO/bench.py --steps 200 --warmup 10 --no-extras --no-cpu",
+        "-d $REPO/gpurun_out/prof_trace -o trace -- python3 $REPO/bench.py --steps 200 --warmup 10 --headline-only",
         "python3 bench.py --profile-summary trace gpurun_out/prof_trace",
     ],
     "weights": [
@@ -628,6 +642,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip adjoint and batched weights")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="time the headline mode's rhs! only and print its figures (for profiled runs)")
     ap.add_argument("--weights-only", action="store_true",
                     help="time only the batched weights (1,048,576 ragged stencils) and print their JSON")
     ap.add_argument("--profile-summary", nargs="+", metavar=("KIND", "DIR"),
@@ -703,6 +719,12 @@ def main():
                                                tinfo["tiles_b"] if dom == "transpose" else tinfo["tiles_a"])
     log("rhs %.4f ms (%.3e node-evals/s); stencil %.4f ms, transpose %.4f ms, reductions %.4f ms, bc %.4f ms"
         % (rhs_ms, value, parts[0], parts[1], parts[2], parts[3]))
+    if args.headline_only:
+        # for a profiled run: the headline mode's calls only
+        print(json.dumps({"metric": METRIC, "value": value, "unit": "node-evals/s", "ms_per_step": rhs_ms,
+                          "kernels_ms": {"stencil": parts[0], "transpose": parts[1], "reductions": parts[2]}}),
+              flush=True)
+        return
     log("timing %d rhs! (FP64 fused multiply-add, split-FP32)" % args.steps)
     f_ms, f_parts = ctx.time_rhs(t_now, L.MF_PREC_F64, args.steps)
     s_ms, s_parts = ctx.time_rhs(t_now, L.MF_PREC_SPLIT32, args.steps)

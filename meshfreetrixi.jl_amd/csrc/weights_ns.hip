@@ -102,15 +102,36 @@ __device__ __forceinline__ double wave_absmax(double v)
     return v;
 }
 
-// phi(|X_i - X_j|) = r^p from the scaled coordinates in LDS (interleaved x, y)
+// phi(|X_i - X_j|) = r^p from the scaled coordinates in LDS (interleaved x,
+// y).  r = sqrt(d2) as d2 y with y the Newton-refined 1/sqrt(d2), then one
+// Newton step on r itself (r += y/2 (d2 - r^2), residual by fma): within an
+// ulp, without the scaling and special-case steps of the library root (the
+// coordinates are scaled to [-1, 1], so 0 < d2 <= 8 off the diagonal);
+// r = 0 on the diagonal.  P > 0: the power at compile time (the products of
+// ipow, without its loop); P = 0: p at run time.
+template <int P>
 __device__ __forceinline__ double phs_r(const double* co, int i, int j, int p)
 {
 #pragma clang fp contract(off)
     const double dx = co[2 * i] - co[2 * j], dy = co[2 * i + 1] - co[2 * j + 1];
-    return ipow(sqrt(dx * dx + dy * dy), p);
+    const double d2 = dx * dx + dy * dy;
+    double r = 0.0;
+    if (d2 > 0.0) {
+        const double y = rsqrt_nr(d2);
+        const double r0 = d2 * y;
+        r = fma(0.5 * y, fma(-r0, r0, d2), r0);
+    }
+    if constexpr (P == 0) {
+        return ipow(r, p);
+    } else {
+        double v = r;
+#pragma unroll
+        for (int e = 1; e < P; ++e) v = v * r;
+        return v;
+    }
 }
 
-template <int MT>
+template <int MT, int P>
 __global__ __launch_bounds__(64, MT == 2 ? 4 : 1) void k_weights_ns(const NsArgs a)
 {
 #pragma clang fp contract(off)
@@ -282,7 +303,7 @@ __global__ __launch_bounds__(64, MT == 2 ? 4 : 1) void k_weights_ns(const NsArgs
                     for (int ks = 0; ks < 8; ++ks) {
                         if (ks < nk) {
                             const int k = 4 * ks + q;
-                            const double av = (i < n && k < n) ? phs_r(co, i, k, a.p) : 0.0;
+                            const double av = (i < n && k < n) ? phs_r<P>(co, i, k, a.p) : 0.0;
                             const double bv = (k < n && c < np) ? Qm[k * LQ + c] : 0.0;
                             ra[8 * rt + ks] = av;
                             acc = mfma4(av, bv, acc);
@@ -303,7 +324,7 @@ __global__ __launch_bounds__(64, MT == 2 ? 4 : 1) void k_weights_ns(const NsArgs
                     d4 acc = {0.0, 0.0, 0.0, 0.0};
                     for (int ks = 0; ks < nk; ++ks) {
                         const int k = 4 * ks + q;
-                        const double av = (i < n && k < n) ? phs_r(co, i, k, a.p) : 0.0;
+                        const double av = (i < n && k < n) ? phs_r<P>(co, i, k, a.p) : 0.0;
                         const double bv = (k < n && c < np) ? Qm[k * LQ + c] : 0.0;
                         acc = mfma4(av, bv, acc);
                     }
@@ -438,9 +459,9 @@ __global__ __launch_bounds__(64, MT == 2 ? 4 : 1) void k_weights_ns(const NsArgs
                             const int row = 16 * it + 4 * r + q, cl = 16 * jt + c;
                             double rv;
                             if (it < 2 && jt < 2)
-                                rv = (MT == 2 || T <= 2) ? ra[(8 * jt + 4 * it + r) & 15] : phs_r(co, row, cl, a.p);
+                                rv = (MT == 2 || T <= 2) ? ra[(8 * jt + 4 * it + r) & 15] : phs_r<P>(co, row, cl, a.p);
                             else
-                                rv = phs_r(co, row, cl, a.p);
+                                rv = phs_r<P>(co, row, cl, a.p);
                             acc[r] = (row < n && cl < n) ? a.sgn * rv : (row == cl ? 1.0 : 0.0);
                         }
                         const int ia = 16 * it + c, jb = 16 * jt + c;
@@ -565,10 +586,15 @@ int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int
     a.lq = ns_lq(a.np);
     const int mt = maxn <= 32 ? 2 : 4;
     const size_t lds = sizeof(double) * ns_lds_doubles(mt, a.rows, a.lq);
-    if (mt == 2)
-        hipLaunchKernelGGL(k_weights_ns<2>, dim3((unsigned)n_launch), dim3(64), lds, c->stream, a);
-    else
-        hipLaunchKernelGGL(k_weights_ns<4>, dim3((unsigned)n_launch), dim3(64), lds, c->stream, a);
+    // the configurations' PHS powers at compile time, others at run time
+    void (*kern)(NsArgs) = nullptr;
+    switch (p) {
+        case 3: kern = mt == 2 ? k_weights_ns<2, 3> : k_weights_ns<4, 3>; break;
+        case 5: kern = mt == 2 ? k_weights_ns<2, 5> : k_weights_ns<4, 5>; break;
+        case 7: kern = mt == 2 ? k_weights_ns<2, 7> : k_weights_ns<4, 7>; break;
+        default: kern = mt == 2 ? k_weights_ns<2, 0> : k_weights_ns<4, 0>; break;
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)n_launch), dim3(64), lds, c->stream, a);
     MF_HIP(c, hipGetLastError());
     return MF_OK;
 }

@@ -183,9 +183,10 @@ int mf_set_operator(mf_ctx* ctx, int target, const double* wx, const double* wy)
  * The same per-stencil system as compute_operators.jl:422-441 (the
  * reference's per-node loop, M \ rhs at :435), for many stencils of uneven
  * size.  By default (MF_TUNE_WEIGHTS_SOLVER 0 or 2) the saddle systems are
- * solved in their null-space form on the FP64 matrix cores, every stencil
- * in one persistent launch (odd PHS power, polynomial degree <= 4, every
- * stencil larger than the monomial count); otherwise, or with
+ * solved in their null-space form on the FP64 matrix cores, one workgroup
+ * per stencil in one launch per size variant (stencils of up to 32 nodes,
+ * and of 33 to 64; odd PHS power, polynomial degree <= 4, every stencil
+ * larger than the monomial count); otherwise, or with
  * MF_TUNE_WEIGHTS_SOLVER 1, by the Bunch-Kaufman restatement, one launch
  * per stencil size.  ptr/idx are uploaded once; mf_weights_ragged_run
  * recomputes on the device (timing loop), mf_weights_ragged_get copies

@@ -708,20 +708,24 @@ int mf_weights_ragged_run(mf_ctx* c, int phs_power, int poly_degree, int deriv_o
     MF_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
     const int np = (poly_degree + 1) * (poly_degree + 2) / 2;
     if (c->weights_solver != 1 && (phs_power & 1) && poly_degree <= 4 && c->rminn > np) {
-        // the stencils in size order (d_rorder): those of up to 32 nodes in
-        // the smaller variant, the rest in the larger
-        int64_t n_small = 0;
-        int maxn_small = 0;
-        for (size_t q = 0; q < c->rclass_n.size(); ++q)
-            if (c->rclass_n[q] <= 32) {
-                n_small = c->rclass_start[q + 1];
-                maxn_small = c->rclass_n[q];
-            }
-        if ((rc = launch_weights_ns(c, c->d_rptr, c->d_ridx, 0, c->d_rorder, n_small, maxn_small, phs_power,
-                                    poly_degree, deriv_order, c->d_rwx, c->d_rwy, c->d_status)))
-            return rc;
-        return launch_weights_ns(c, c->d_rptr, c->d_ridx, 0, c->d_rorder + n_small, c->rn - n_small, c->rmaxn,
-                                 phs_power, poly_degree, deriv_order, c->d_rwx, c->d_rwy, c->d_status);
+        // the stencils in size order (d_rorder), one launch per kernel
+        // variant: up to 32, 33 to 48 and 49 to 64 nodes
+        int64_t lo = 0;
+        for (int bound : {32, 48, 64}) {
+            int64_t hi = lo;
+            int maxn = 0;
+            for (size_t q = 0; q < c->rclass_n.size(); ++q)
+                if (c->rclass_n[q] <= bound) {
+                    hi = c->rclass_start[q + 1];
+                    maxn = c->rclass_n[q];
+                }
+            if (hi > lo && (rc = launch_weights_ns(c, c->d_rptr, c->d_ridx, 0, c->d_rorder + lo, hi - lo, maxn,
+                                                   phs_power, poly_degree, deriv_order, c->d_rwx, c->d_rwy,
+                                                   c->d_status)))
+                return rc;
+            lo = std::max(lo, hi);
+        }
+        return MF_OK;
     }
     for (size_t q = 0; q < c->rclass_n.size(); ++q) {
         const int64_t b0 = c->rclass_start[q], b1 = c->rclass_start[q + 1];

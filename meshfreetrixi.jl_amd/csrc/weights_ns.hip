@@ -22,9 +22,9 @@
 //
 // One wavefront (workgroup) per stencil, np <= 16 monomials, n > np.  The
 // kernel is built for up to MT 16-row tiles: MT = 2 (n <= 32: 128 VGPRs and
-// under 10 KiB of LDS, four wavefronts per SIMD) and MT = 4 (n <= 64: 192
-// VGPRs, two per SIMD); a ragged batch runs as one launch per variant over
-// the stencils of its size class.  (A persistent grid that loops over
+// under 10 KiB of LDS, four wavefronts per SIMD), MT = 3 (n <= 48: 160
+// VGPRs, three) and MT = 4 (n <= 64: 192 VGPRs, two); a ragged batch runs
+// as one launch per variant over the stencils of its size class.  (A persistent grid that loops over
 // stencils took twice the registers: the loop-invariant LDS addresses of
 // the unrolled phases stay live across the loop.)
 #include <hip/hip_runtime.h>
@@ -132,7 +132,7 @@ __device__ __forceinline__ double phs_r(const double* co, int i, int j, int p)
 }
 
 template <int MT, int P>
-__global__ __launch_bounds__(64, MT == 2 ? 4 : 1) void k_weights_ns(const NsArgs a)
+__global__ __launch_bounds__(64, MT == 2 ? 4 : (MT == 3 ? 3 : 1)) void k_weights_ns(const NsArgs a)
 {
 #pragma clang fp contract(off)
     constexpr int NL = 16 * MT;  // largest stencil
@@ -584,15 +584,16 @@ int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int
     a.wy = d_wy;
     a.status = d_status;
     a.lq = ns_lq(a.np);
-    const int mt = maxn <= 32 ? 2 : 4;
+    const int mt = maxn <= 32 ? 2 : (maxn <= 48 ? 3 : 4);
     const size_t lds = sizeof(double) * ns_lds_doubles(mt, a.rows, a.lq);
     // the configurations' PHS powers at compile time, others at run time
     void (*kern)(NsArgs) = nullptr;
+    auto pick = [&](auto k2, auto k3, auto k4) { kern = mt == 2 ? k2 : (mt == 3 ? k3 : k4); };
     switch (p) {
-        case 3: kern = mt == 2 ? k_weights_ns<2, 3> : k_weights_ns<4, 3>; break;
-        case 5: kern = mt == 2 ? k_weights_ns<2, 5> : k_weights_ns<4, 5>; break;
-        case 7: kern = mt == 2 ? k_weights_ns<2, 7> : k_weights_ns<4, 7>; break;
-        default: kern = mt == 2 ? k_weights_ns<2, 0> : k_weights_ns<4, 0>; break;
+        case 3: pick(k_weights_ns<2, 3>, k_weights_ns<3, 3>, k_weights_ns<4, 3>); break;
+        case 5: pick(k_weights_ns<2, 5>, k_weights_ns<3, 5>, k_weights_ns<4, 5>); break;
+        case 7: pick(k_weights_ns<2, 7>, k_weights_ns<3, 7>, k_weights_ns<4, 7>); break;
+        default: pick(k_weights_ns<2, 0>, k_weights_ns<3, 0>, k_weights_ns<4, 0>); break;
     }
     hipLaunchKernelGGL(kern, dim3((unsigned)n_launch), dim3(64), lds, c->stream, a);
     MF_HIP(c, hipGetLastError());

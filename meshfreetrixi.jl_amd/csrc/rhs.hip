@@ -264,13 +264,16 @@ __device__ __forceinline__ void finish_norms(const RhsArgs& a, double* pm, doubl
 
 
 // Halo record of a node in LDS: for the Euler equations (rho, m1, m2, E,
-// v1, v2, p, 0) with the velocity and pressure evaluated exactly as
-// flux_xy does (so the fluxes built from a record have the same bits),
-// which takes the divisions out of the per-entry work; otherwise the state.
+// v1, v2) with the velocity evaluated exactly as flux_xy does, which takes
+// the divisions out of the per-entry work; the pressure is formed from the
+// record per entry with flux_xy's expression (so the fluxes have the same
+// bits).  48 bytes, three 16-byte chunks: a stride that spreads the
+// scattered reads of a wavefront over all LDS banks, and a halo of 3/4 the
+// size of the 64-byte record with the pressure.  Otherwise the state.
 template <int V>
 constexpr int rec_len()
 {
-    return V == 4 ? 8 : V;
+    return V == 4 ? 6 : V;
 }
 
 template <int V>
@@ -285,8 +288,6 @@ __device__ __forceinline__ void make_record(const EqParams& e, const double* u, 
         r[3] = E;
         r[4] = v1;
         r[5] = v2;
-        r[6] = (e.gamma - 1.0) * fma(-0.5, fma(m1, v1, m2 * v2), E);
-        r[7] = 0.0;
     } else {
 #pragma unroll
         for (int v = 0; v < V; ++v) r[v] = u[v];
@@ -297,7 +298,8 @@ template <int V>
 __device__ __forceinline__ void flux_record(const EqParams& e, const double* r, double* f1, double* f2)
 {
     if constexpr (V == 4) {
-        const double m1 = r[1], m2 = r[2], E = r[3], v1 = r[4], v2 = r[5], p = r[6];
+        const double m1 = r[1], m2 = r[2], E = r[3], v1 = r[4], v2 = r[5];
+        const double p = (e.gamma - 1.0) * fma(-0.5, fma(m1, v1, m2 * v2), E);
         const double ep = E + p;
         f1[0] = m1;
         f1[1] = fma(m1, v1, p);
@@ -316,8 +318,14 @@ template <int V>
 __device__ __forceinline__ void load_record(const double* lds, int64_t l, double* r)
 {
     if constexpr (V == 4) {
-        lds_load_unit<2>(lds, l, 0, r);
-        lds_load_unit<2>(lds, l, 1, r + 4);
+        const double2* p = reinterpret_cast<const double2*>(lds) + 3 * l;
+        const double2 c0 = p[0], c1 = p[1], c2 = p[2];
+        r[0] = c0.x;
+        r[1] = c0.y;
+        r[2] = c1.x;
+        r[3] = c1.y;
+        r[4] = c2.x;
+        r[5] = c2.y;
     } else {
         load_state<V>(lds, l, r);
     }
@@ -361,8 +369,10 @@ __device__ __forceinline__ int64_t stage_halo_a(const RhsArgs& a, double* lds)
             load_state<V>(a.u, a.ha_ids[h0 + q], s);
             make_record<V>(a.eq, s, r);
             if constexpr (V == 4) {
-                lds_store_unit<2>(lds, q, 0, r);
-                lds_store_unit<2>(lds, q, 1, r + 4);
+                double2* p = reinterpret_cast<double2*>(lds) + 3 * q;
+                p[0] = make_double2(r[0], r[1]);
+                p[1] = make_double2(r[2], r[3]);
+                p[2] = make_double2(r[4], r[5]);
             } else {
                 store_state<V>(lds, q, r);
             }

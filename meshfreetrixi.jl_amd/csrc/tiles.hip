@@ -138,8 +138,8 @@ int build_tiles(mf_ctx* c, bool want_b, int rec_b)
                     }
                 }
             std::sort(ids.begin(), ids.end());
-            // phase A halo records: 8 doubles per node for Euler (state, v1, v2, p)
-            const size_t rec = (V == 4) ? 8 : (size_t)V;
+            // phase A halo records: 6 doubles per node for Euler (state, v1, v2)
+            const size_t rec = (V == 4) ? 6 : (size_t)V;
             if (ids.size() > 65535 || ids.size() * rec * sizeof(double) > kTileLdsBytes) fits = false;
             for (size_t q = 0; q < ids.size(); ++q) mark[ids[q]] = (int32_t)q;
             for (int s = 0; s < K; ++s)

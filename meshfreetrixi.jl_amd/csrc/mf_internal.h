@@ -26,10 +26,8 @@ struct StepScalars {
     int success_iter;
     int hist_head;  // ring slot of the most recent snapshot
     int q;          // number of history points in use
-    int ticket;     // last-block detection of the reductions
-    int ticket2;
     int hist_si;  // success_iter as last seen by the history callback
-    int pad[2];
+    int pad[3];
 };
 
 // Conjugate-gradient scalars of the IGR source, in device memory.

@@ -50,7 +50,6 @@ struct NsArgs {
     const int32_t* ell;  // ELL, slot-major (ptr null)
     int ell_k;
     int64_t N;
-    int64_t ns;          // stencils
     int p, np, kd;
     int rows;            // LDS rows of the n x np matrices (>= largest n, >= 16)
     int lq;              // their row stride, ns_lq(np)
@@ -566,7 +565,6 @@ int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int
     a.ell = d_ptr ? nullptr : d_idx;
     a.ell_k = ell_k;
     a.N = c->N;
-    a.ns = n_launch;
     a.p = p;
     a.np = (deg + 1) * (deg + 2) / 2;
     a.kd = kd;

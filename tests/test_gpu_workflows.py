@@ -338,3 +338,20 @@ def test_ensemble_nosync_matches_one_at_a_time(mf):
         res[conc] = [c.ctx.download(L.MF_BUF_U) for c in cs]
     for a, b in zip(res[True], res[False]):
         assert np.array_equal(a, b)
+
+
+def test_c_client(mf, tmp_path):
+    """A plain C99 host (tests/c_client/mfrbf_client.c) drives kNN, weights,
+    rhs! and graph-replayed SSPRK33 steps through the C ABI and checks each
+    against a closed form (linear exactness, quadratic reproduction, free
+    stream); no Python in its loop."""
+    import subprocess
+
+    from conftest import build_c_client
+
+    exe = build_c_client(tmp_path)
+    _say("C client")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    _say("  " + r.stdout.strip() + r.stderr.strip())
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+    assert "mfrbf C client ok" in r.stdout

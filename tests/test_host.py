@@ -83,3 +83,19 @@ def test_gaussian_noise_moments():
 
     g = gaussian_noise(1, 200000, 1e-3)
     assert abs(g.mean()) < 1e-5 and abs(g.std() - 1e-3) < 1e-5
+
+
+def test_c_client_builds_and_fails_loudly_without_device(tmp_path):
+    """The C ABI from plain C99: tests/c_client/mfrbf_client.c compiles with
+    warnings as errors against include/mfrbf.h and links libmfrbf.so; without
+    a GPU it reports the missing device (exit 2) instead of computing."""
+    import subprocess
+
+    from conftest import build_c_client, gpu_available
+
+    exe = build_c_client(tmp_path)
+    if gpu_available():
+        pytest.skip("a GPU is present: the GPU test runs the client")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2, (r.returncode, r.stdout, r.stderr)
+    assert "no device" in r.stderr

@@ -184,3 +184,55 @@ def test_grouped_set_refuses_global_reductions(mf):
     with pytest.raises(L.MfError) as e:
         ens.ctx.rhs(0.0, L.MF_PREC_F64)
     assert e.value.code == L.MF_ERR_UNSUPPORTED
+
+
+def test_grouped_set_equals_separate_contexts(mf):
+    """A grouped set is the clouds side by side, nothing more: with the
+    sources that act node by node (Flyer and Tominec hyperviscosity, the
+    upwind viscosity, with common coefficients) each cloud's rhs! in one
+    grouped context equals, bit for bit, the same cloud in a context of its
+    own, in the reference's operation order and in split-FP32."""
+    from mfrbf.clouds import cloud_square
+    from mfrbf.ensemble import Ensemble
+
+    specs = configs.c5(3)
+    for r, s in enumerate(specs):
+        s.side, s.hv = 24 + 6 * r, None
+    ens = Ensemble(specs)
+    sp = specs[0]
+    gam_f, gam_t, h = 1e-4 * min(ens.dx_min) ** 4, 1e-3 * min(ens.dx_min) ** 4.5, float(np.mean(ens.dx_avg))
+
+    def sources(ctx):
+        ctx.compute_operator(sp.p, sp.deg, 4, L.MF_OP_HYPERVISCOSITY, want=False)
+        ctx.set_source_hyperviscosity(gam_f)
+        ctx.compute_operator(sp.p, sp.deg, 2, L.MF_OP_LAPLACIAN, want=False)
+        ctx.set_source_hyperviscosity_tominec(gam_t)
+        ctx.set_source_viscosity(L.MF_VISC_UPWIND, 1.0, 1.0, h, 3, True)
+
+    sources(ens.ctx)
+    rng = np.random.default_rng(5)
+    u = ens.u0 * (1.0 + 0.01 * rng.standard_normal(ens.u0.shape))
+    for prec in (L.MF_PREC_F64_REF, L.MF_PREC_SPLIT32):
+        ug, dug = u.copy(), np.zeros_like(u)
+        ens.ctx.rhs_host(0.3, ug, dug, prec)
+        for r, spec in enumerate(specs):
+            lo, hi = ens.offsets[r], ens.offsets[r + 1]
+            x, y = ens.x[lo:hi], ens.y[lo:hi]
+            ctx = L.Context(0)
+            ctx.set_equations(L.MF_EQ_EULER_2D, spec.gamma)
+            ctx.set_points(x, y)
+            ctx.knn(spec.k)
+            ctx.compute_operator(spec.p, spec.deg, 1, L.MF_OP_FLUX, want=False)
+            tag = cloud_square(spec.side, spec.seed)[2]
+            inflow, outflow = np.nonzero(tag == 1)[0].astype(np.int32), np.nonzero(tag == 2)[0].astype(np.int32)
+            prm = spec.ic.params()
+            ctx.set_boundary([0, len(inflow), len(inflow) + len(outflow)], [L.MF_BC_DIRICHLET, L.MF_BC_DO_NOTHING],
+                             [spec.ic.code, spec.ic.code], np.concatenate([prm, prm]),
+                             np.concatenate([inflow, outflow]), np.zeros((len(inflow) + len(outflow), 2)))
+            sources(ctx)
+            us, dus = np.ascontiguousarray(u[:, lo:hi]), np.zeros((4, hi - lo))
+            ctx.rhs_host(0.3, us, dus, prec)
+            _say("  prec %d cloud %d (N=%d): max |du_grouped - du_alone| %.1e" % (
+                prec, r, hi - lo, np.abs(dug[:, lo:hi] - dus).max()))
+            assert np.array_equal(dug[:, lo:hi], dus)
+            assert np.array_equal(ug[:, lo:hi], us)

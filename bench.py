@@ -609,8 +609,8 @@ def profile_summary(kind, dirs):
         text = ("# Kernel trace: Euler + residual viscosity rhs!, 262,144 nodes, k = 30\n\n"
                 "Commands (from the repository root, on one MI355X):\n\n```\n%s\n```\n\n"
                 "Top kernels by total time.  The run includes the autotuner's sweep (every load-batch variant "
-                "inside whole rhs! calls) and the FP64, split-FP32 and reference-order timing loops, so each "
-                "kernel's average is over whole-rhs! sequences.\n\n%s\n" % (cmds, "\n".join(lines)))
+                "of each precision inside whole rhs! calls) and the headline (reference-order FP64) timing "
+                "loop, so each kernel's average is over whole-rhs! sequences.\n\n%s\n" % (cmds, "\n".join(lines)))
     else:  # weights
         lines, rows = _trace_table(dirs[0], top=8)
         cnt = _counter_medians(dirs[1])

@@ -123,7 +123,9 @@ int mf_set_points(mf_ctx* ctx, const double* x, const double* y, int64_t n_point
  * internal node order keeps each group contiguous (Morton order inside the
  * group), so every kernel launch of rhs! and of the SSPRK33 step covers all
  * clouds while no LDS tile mixes more than two of them.  Stencils must stay
- * inside their group (mf_set_stencil; mf_knn searches the whole set).  The
+ * inside their group: mf_set_stencil returns MF_ERR_ARGUMENT for one that
+ * crosses, and mf_knn / mf_knn_noalloc return MF_ERR_UNSUPPORTED on a
+ * grouped set (search each cloud on its own, as a set of its own).  The
  * residual viscosity and IGR sources, which reduce over all points, return
  * MF_ERR_UNSUPPORTED at rhs! on a grouped set.  No counterpart in the
  * reference (which runs one cloud per semidiscretization). */

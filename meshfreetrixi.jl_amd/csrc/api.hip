@@ -432,6 +432,10 @@ static int set_points(mf_ctx* c, const double* x, const double* y, int64_t n, in
     dfree(yu);
     dfree(gu);
     c->n_groups = group_ptr ? n_groups : 1;
+    if (group_ptr)
+        c->group_ptr.assign(group_ptr, group_ptr + n_groups + 1);
+    else
+        c->group_ptr = {0, n};
     return rc;
 }
 
@@ -461,6 +465,9 @@ int mf_knn(mf_ctx* c, int k, int32_t* idx_out, double* dist_out, double* dx_min,
     if (rc) return rc;
     if (c->N == 0) return fail(c, MF_ERR_STATE, "mf_knn: no points");
     if (k < 2 || k > 64 || k > c->N) return fail(c, MF_ERR_ARGUMENT, "mf_knn: need 2 <= k <= min(64, N)");
+    if (c->n_groups > 1)
+        return fail(c, MF_ERR_UNSUPPORTED,
+                    "mf_knn: the search would cross the clouds of a grouped set; search each cloud on its own");
     if ((rc = set_k(c, k))) return rc;
     double* d2 = nullptr;
     if ((rc = dalloc(c, &d2, (size_t)k * c->N))) return rc;
@@ -499,6 +506,9 @@ int mf_knn_noalloc(mf_ctx* c, int k, int32_t* d_idx_out, double* d_dist_out)
     if (c->N == 0) return fail(c, MF_ERR_STATE, "mf_knn_noalloc: no points");
     if (k < 2 || k > 64 || k > c->N) return fail(c, MF_ERR_ARGUMENT, "mf_knn_noalloc: need 2 <= k <= min(64, N)");
     if (!d_idx_out) return fail(c, MF_ERR_ARGUMENT, "mf_knn_noalloc: null output");
+    if (c->n_groups > 1)
+        return fail(c, MF_ERR_UNSUPPORTED,
+                    "mf_knn_noalloc: the search would cross the clouds of a grouped set; search each cloud on its own");
     if (c->N > knn_noalloc_max_nodes())
         return fail(c, MF_ERR_UNSUPPORTED, "mf_knn_noalloc: more than 262,144 points, the size of its static scratch");
     return launch_knn_noalloc(c, k, d_idx_out, d_dist_out);
@@ -511,6 +521,15 @@ int mf_set_stencil(mf_ctx* c, const int32_t* idx, int64_t n_points, int k)
     if (!idx || n_points != c->N || k < 1 || k > 64) return fail(c, MF_ERR_ARGUMENT, "mf_set_stencil: bad size");
     for (int64_t q = 0; q < n_points * k; ++q)
         if (idx[q] < 0 || idx[q] >= n_points) return fail(c, MF_ERR_ARGUMENT, "mf_set_stencil: index out of range");
+    // a grouped set's stencils stay inside their node's cloud
+    if (c->n_groups > 1)
+        for (int g = 0; g < c->n_groups; ++g)
+            for (int64_t i = c->group_ptr[g]; i < c->group_ptr[g + 1]; ++i)
+                for (int s = 0; s < k; ++s) {
+                    const int32_t j = idx[i * k + s];
+                    if (j < c->group_ptr[g] || j >= c->group_ptr[g + 1])
+                        return fail(c, MF_ERR_ARGUMENT, "mf_set_stencil: a stencil crosses the clouds of a grouped set");
+                }
     if ((rc = set_k(c, k))) return rc;
     int32_t* rows = nullptr;
     if ((rc = dalloc(c, &rows, (size_t)k * c->N))) return rc;

@@ -88,6 +88,7 @@ struct mf_ctx {
     std::vector<int32_t> h_inv;  // host copy of d_inv (boundary and stencil indices)
     int node_order = 0;          // MF_TUNE_NODE_ORDER value for the next mf_set_points
     int n_groups = 1;            // independent clouds in the point set (mf_set_points_grouped)
+    std::vector<int64_t> group_ptr;  // n_groups + 1: each cloud's node range (caller's numbering)
 
     // stencil (ELL, slot-major: entry s of node i at s*N + i)
     int k = 0;

@@ -172,13 +172,21 @@ def test_c5_grouped_ensemble_vs_oracle(mf):
 
 def test_grouped_set_refuses_global_reductions(mf):
     """The residual viscosity reduces over all points, which would couple the
-    clouds of a grouped set: rhs! refuses it there."""
+    clouds of a grouped set: rhs! refuses it there.  So do a kNN search over
+    the whole set and a stencil that reaches into another cloud."""
     from mfrbf.ensemble import Ensemble
 
     specs = configs.c5(2)
     for s in specs:
         s.side = 24
     ens = Ensemble(specs)
+    with pytest.raises(L.MfError) as e:
+        ens.ctx.knn(specs[0].k)
+    assert e.value.code == L.MF_ERR_UNSUPPORTED
+    bad = np.tile(np.arange(specs[0].k, dtype=np.int32), (ens.N, 1))  # every stencil in cloud 0
+    with pytest.raises(L.MfError) as e:
+        ens.ctx.set_stencil(bad)
+    assert e.value.code == L.MF_ERR_ARGUMENT
     ens.ctx.set_source_viscosity(L.MF_VISC_RESIDUAL, 1.0, 1.0, ens.dx_avg[0], 3, True)
     ens.ctx.upload(L.MF_BUF_U, ens.u0)
     with pytest.raises(L.MfError) as e:

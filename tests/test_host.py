@@ -99,3 +99,48 @@ def test_c_client_builds_and_fails_loudly_without_device(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2, (r.returncode, r.stdout, r.stderr)
     assert "no device" in r.stderr
+
+
+def _c_params(decl):
+    """Parameter types of a C prototype's argument list, const and names
+    dropped: 'const double* const* u' -> 'double**'."""
+    out = []
+    for p in decl.split(","):
+        p = p.replace("const", " ").strip()
+        if not p or p == "void":
+            continue
+        m = re.match(r"^(.*[\s*])(\w+)$", p)  # type, then the parameter name
+        out.append(m.group(1).replace(" ", ""))
+    return out
+
+
+_JULIA_C = {"Ptr{Cvoid}": "ptr", "Ref{Ptr{Cvoid}}": "mf_ctx**", "Ptr{Cdouble}": "double*",
+            "Ptr{Ptr{Cdouble}}": "double**", "Ptr{Int32}": "int32_t*", "Ptr{Int64}": "int64_t*",
+            "Ptr{Int8}": "int8_t*", "Int64": "int64_t", "Cint": "int", "Cdouble": "double", "Cstring": "char*"}
+
+
+def test_julia_ccalls_match_header():
+    """Every ccall of the Julia engine (julia/MeshfreeTrixiHIP.jl, which runs
+    only where Julia is installed) names a function of include/mfrbf.h with
+    the same number of arguments, each of the C type its Julia type passes,
+    and the same return type."""
+    with open(os.path.join(ROOT, "include", "mfrbf.h")) as f:
+        header = f.read()
+    protos = {m.group(2): (m.group(1).replace("const", "").replace(" ", ""), _c_params(m.group(3)))
+              for m in re.finditer(r"^(int|const char\*)\s+(mf_\w+)\(([^;]*?)\);", header, re.M | re.S)}
+    with open(os.path.join(ROOT, "meshfreetrixi.jl_amd", "julia", "MeshfreeTrixiHIP.jl")) as f:
+        jl = f.read()
+    calls = re.findall(r"ccall\(\(:(mf_\w+),\s*libmfrbf\),\s*(\w+),\s*\(([^()]*)\)", jl, re.S)
+    assert len(calls) >= 20
+    for name, ret, args in calls:
+        assert name in protos, name
+        cret, cparams = protos[name]
+        assert _JULIA_C[ret] == cret.replace("const", "") or (ret == "Cstring" and cret == "char*"), (name, ret, cret)
+        jtypes = [a.strip() for a in args.split(",") if a.strip()]
+        assert len(jtypes) == len(cparams), (name, jtypes, cparams)
+        for jt, ct in zip(jtypes, cparams):
+            jc = _JULIA_C[jt]
+            if jc == "ptr":
+                assert ct.endswith("*") and ct.count("*") == 1, (name, jt, ct)
+            else:
+                assert jc == ct, (name, jt, ct)

@@ -672,3 +672,38 @@ def test_state_copy(mf):
     c.ctx.upload(L.MF_BUF_U, c.u0.copy())
     c.ctx.copy(L.MF_BUF_AUX0, L.MF_BUF_U)
     assert np.array_equal(c.ctx.download(L.MF_BUF_AUX0), c.u0)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_rhs_random_configuration(mf, seed):
+    """rhs! on seeded random combinations against the oracle: cloud side 16
+    to 40, basis and stencil from (r^3, deg 2, 15), (r^5, deg 3, 30),
+    (r^3, deg 3, 20), (r^5, deg 2, 20); Flyer hyperviscosity or not; no,
+    upwind or residual viscosity (with a history); the Tominec source where
+    the keyword order allows it; Dirichlet, do-nothing or slip on each of the
+    two boundary tags.  Reference order within 1e-12 (bitwise where no
+    transcendental and no reduction enters), split-FP32 within 1e-6."""
+    rng = np.random.default_rng(7000 + seed)
+    spec = configs.c2()
+    spec.p, spec.deg, spec.k = [(3, 2, 15), (5, 3, 30), (3, 3, 20), (5, 2, 20)][int(rng.integers(4))]
+    side = int(rng.integers(16, 41))
+    hv = (2, 1e-4) if rng.random() < 0.5 else None
+    visc = [None, "upwind", "residual"][int(rng.integers(3))]
+    tom = 1e-3 if (visc != "residual" and rng.random() < 0.4) else None
+    bc = tuple(str(b) for b in rng.choice(["dirichlet", "nothing", "slip"], 2))
+    _say("random rhs: side %d, p %d deg %d k %d, hv %s, visc %s, tominec %s, bc %s" % (
+        side, spec.p, spec.deg, spec.k, hv is not None, visc, tom is not None, bc))
+    c = cases.build(spec, side=side, hv=hv, visc=visc, tom=tom, bc=bc)
+    if visc == "residual":
+        _rv_history(c, c.u0 * 1.0001)
+    t = 0.001 if visc == "residual" else 0.02
+    for prec, tol in ((L.MF_PREC_F64_REF, TOL64), (L.MF_PREC_SPLIT32, TOL32)):
+        u, du, uo, duo, eps, epsc = _rhs_pair(c, prec=prec, t=t)
+        _say("  prec %d: du rel err %.2e, u rel err %.2e" % (prec, rel_inf(du, duo), rel_inf(u, uo)))
+        assert rel_inf(du, duo) < tol
+        assert rel_inf(u, uo) < 1e-14
+        if visc is not None:
+            geps, gcode = c.ctx.viscosity()
+            assert rel_inf(geps, eps) < (1e-12 if prec == L.MF_PREC_F64_REF else 1e-6)
+            if prec == L.MF_PREC_F64_REF:
+                assert np.array_equal(gcode, epsc)

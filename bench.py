@@ -323,6 +323,12 @@ def _ensemble_runs(L, specs, steps):
                max_density_error_vs_exact=err_g,
                rhs_grouped={"ms": rhs_ms, "node_evals_per_s": total / (rhs_ms * 1e-3),
                             "kernels_ms": {"stencil": rhs_parts[0], "transpose": rhs_parts[1]}})
+    # the split-FP32 stencil kernel with Flyer hyperviscosity, each array once:
+    # u in, du out (FP64), 16-bit halo positions, (Dx, Dy) and H weights in FP32
+    k = specs[0].k
+    bpn = 4 * 8 + 4 * 8 + 2 * k + 8 * k + 4 * k
+    out["rhs_grouped"]["bytes_per_node"] = bpn
+    out["rhs_grouped"]["stencil_frac"] = bpn * total / (rhs_parts[0] * 1e-3) / 1e9 / HBM_PEAK_GBS
     del ens, g
     t0 = time.perf_counter()
     clouds = []

@@ -332,7 +332,10 @@ int mf_set_stage_limiter(mf_ctx* ctx, int n, const double* thresholds, const int
  * HistoryCallback runs at init (when resume == 0) and after every step.
  * Replaces solve(ode, SSPRK33(...), dt, callbacks) on the reference's
  * semidiscretization.  resume = 1 continues from the previous call's FSAL
- * and history state (same dt; MF_BUF_U as that call left it). */
+ * and history state (same dt; MF_BUF_U as that call left it).  With the
+ * residual viscosity and no stage limiter each stage is written with the
+ * next rhs!'s first boundary pass and sums already applied, so a resume
+ * that switches MF_STEP_LIMITER on or off there returns MF_ERR_STATE. */
 int mf_ssprk33(mf_ctx* ctx, double t0, double dt, int n_steps, int precision, int flags,
                int resume);
 

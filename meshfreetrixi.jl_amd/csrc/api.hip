@@ -177,9 +177,11 @@ static void free_state(mf_ctx* c)
     dfree(c->d_approx);
     dfree(c->d_hist);
     dfree(c->d_partials);
+    dfree(c->d_stage_psum);
     dfree(c->d_lim);
     dfree(c->d_adj);
     c->d_S = c->d_T = c->d_G = c->d_eps = c->d_approx = c->d_hist = c->d_partials = c->d_lim = nullptr;
+    c->d_stage_psum = nullptr;
     c->d_adj = nullptr;
     c->adj_len = 0;
     c->d_epsc = nullptr;
@@ -251,6 +253,8 @@ int ensure_state(mf_ctx* c)
             c->n_pmax = (int)std::min<int64_t>(kBlock, (c->N + 1023) / 1024);
             if ((rc = dalloc(c, &c->d_partials, 4 * (size_t)(c->n_psum + c->n_pmax)))) return rc;
         }
+        if (!c->d_stage_psum && (rc = dalloc(c, &c->d_stage_psum, 4 * (size_t)((c->N + kBlock - 1) / kBlock))))
+            return rc;
     }
     // limiter scratch: nvar N blended values, then N flag bytes
     if (!c->d_lim && (rc = dalloc(c, &c->d_lim, nv + (size_t)(c->N + 7) / 8))) return rc;
@@ -597,8 +601,8 @@ int mf_compute_operator(mf_ctx* c, int phs_power, int poly_degree, int deriv_ord
     if (!c->d_status && (rc = dalloc(c, &c->d_status, 1))) return rc;
     MF_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
     if (c->weights_solver == 2 && (phs_power & 1) && poly_degree <= 4 && c->k > np)
-        rc = launch_weights_ns(c, nullptr, c->d_idx, c->k, nullptr, c->N, c->k, phs_power, poly_degree, deriv_order,
-                               wx, wy, c->d_status);
+        rc = launch_weights_ns(c, nullptr, c->d_idx, c->k, nullptr, c->N, c->k, c->k, phs_power, poly_degree,
+                               deriv_order, wx, wy, c->d_status);
     else
         rc = launch_weights(c, nullptr, c->d_idx, c->k, c->N, c->k, phs_power, poly_degree, deriv_order, wx, wy,
                             c->d_status);
@@ -732,15 +736,16 @@ int mf_weights_ragged_run(mf_ctx* c, int phs_power, int poly_degree, int deriv_o
         int64_t lo = 0;
         for (int bound : {32, 48, 64}) {
             int64_t hi = lo;
-            int maxn = 0;
+            int maxn = 0, classes = 0;
             for (size_t q = 0; q < c->rclass_n.size(); ++q)
                 if (c->rclass_n[q] <= bound) {
                     hi = c->rclass_start[q + 1];
                     maxn = c->rclass_n[q];
+                    classes += c->rclass_start[q] >= lo;
                 }
             if (hi > lo && (rc = launch_weights_ns(c, c->d_rptr, c->d_ridx, 0, c->d_rorder + lo, hi - lo, maxn,
-                                                   phs_power, poly_degree, deriv_order, c->d_rwx, c->d_rwy,
-                                                   c->d_status)))
+                                                   classes == 1 ? maxn : 0, phs_power, poly_degree, deriv_order,
+                                                   c->d_rwx, c->d_rwy, c->d_status)))
                 return rc;
             lo = std::max(lo, hi);
         }
@@ -1239,8 +1244,21 @@ int mf_set_stage_limiter(mf_ctx* c, int n, const double* thresholds, const int32
     return MF_OK;
 }
 
+// Whether the stages of mf_ssprk33 hand each other the next rhs!'s first
+// boundary pass and sums (the residual viscosity's transposed kernel
+// writes each stage; no stage limiter runs between them).
+static bool stage_sums(const mf_ctx* c, bool limiter)
+{
+    return c->visc_mode == MF_VISC_RESIDUAL && !limiter && c->igr_alpha == 0.0 && c->d_stage_psum &&
+           !(c->have_bc && c->bc.multi) && !(c->tiles_b && c->n_tiles != (c->N + kBlock - 1) / kBlock);
+}
+
 // One SSPRK33 step on U with the FSAL stage S = U + dt f(U) already formed.
-static int ssprk_step(mf_ctx* c, double dt, int prec, bool limiter, bool history)
+// With sums, every rhs! of the step takes its input's boundary pass and
+// sums from the rhs! before it and leaves them for the next: T's and U's
+// at this step's stage times t + dt/2 and t + dt, S's at the next step's
+// t + dt (the time is advanced after the last rhs!).
+static int ssprk_step(mf_ctx* c, double dt, int prec, bool limiter, bool history, bool sums)
 {
     int rc;
     auto limit = [&](double* u) -> int {
@@ -1258,6 +1276,8 @@ static int ssprk_step(mf_ctx* c, double dt, int prec, bool limiter, bool history
     s2.dt = dt;
     s2.uprev = U;
     s2.out = c->d_T;
+    s2.sums_in = s2.sums_out = sums;
+    s2.out_dt_off = dt / 2.0;
     if ((rc = run_rhs(c, prec, c->d_S, c->d_buf[MF_BUF_DU], dt, s2))) return rc;
     if ((rc = limit(c->d_T))) return rc;
     StageOut s3;
@@ -1265,12 +1285,17 @@ static int ssprk_step(mf_ctx* c, double dt, int prec, bool limiter, bool history
     s3.dt = dt;
     s3.uprev = U;
     s3.out = U;
+    s3.sums_in = s3.sums_out = sums;
+    s3.out_dt_off = dt;
     if ((rc = run_rhs(c, prec, c->d_T, c->d_buf[MF_BUF_DU], dt / 2.0, s3))) return rc;
     if ((rc = limit(U))) return rc;
     StageOut s1;
     s1.kind = ST_FSAL;
     s1.dt = dt;
     s1.out = c->d_S;
+    s1.sums_in = s1.sums_out = sums;
+    s1.out_t_adv = dt;
+    s1.out_dt_off = dt;
     if ((rc = run_rhs(c, prec, U, c->d_buf[MF_BUF_DU], dt, s1))) return rc;
     if (history) {
         if ((rc = run_history(c, U, 0, dt))) return rc;  // also advances the time
@@ -1289,9 +1314,14 @@ int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int 
     const bool limiter = (flags & MF_STEP_LIMITER) && c->lim_n > 0;
     const int gflags = flags & ~MF_STEP_NOSYNC;  // what the captured step depends on
     const bool history = (flags & MF_STEP_HISTORY) && c->visc_mode == MF_VISC_RESIDUAL;
+    const bool sums = stage_sums(c, limiter);
     if (resume) {
         if (!c->fsal_valid || c->fsal_dt != dt)
             return fail(c, MF_ERR_STATE, "mf_ssprk33: resume needs the previous call's state and the same dt");
+        // with the residual viscosity the stored FSAL stage carries the
+        // next boundary pass and sums exactly when no stage limiter runs
+        if (c->fsal_sums != sums)
+            return fail(c, MF_ERR_STATE, "mf_ssprk33: resume needs the previous call's stage limiter setting");
     } else {
         if ((rc = run_set_time(c, t0, 0))) return rc;
         if (history && (rc = run_history(c, c->d_buf[MF_BUF_U], 1))) return rc;
@@ -1299,6 +1329,8 @@ int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int 
         s1.kind = ST_FSAL;
         s1.dt = dt;
         s1.out = c->d_S;
+        s1.sums_out = sums;  // for the first step's rhs! at t0 + dt
+        s1.out_dt_off = dt;
         if ((rc = run_rhs(c, precision, c->d_buf[MF_BUF_U], c->d_buf[MF_BUF_DU], 0.0, s1))) return rc;
         c->host_success_iter = 0;
     }
@@ -1308,7 +1340,7 @@ int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int 
         auto capture = [&](int steps, hipGraph_t* graph, hipGraphExec_t* exec) -> int {
             MF_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int r = MF_OK;
-            for (int s = 0; s < steps && r == MF_OK; ++s) r = ssprk_step(c, dt, precision, limiter, history);
+            for (int s = 0; s < steps && r == MF_OK; ++s) r = ssprk_step(c, dt, precision, limiter, history, sums);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(c->stream, &g);
             if (r) {
@@ -1320,13 +1352,15 @@ int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int 
             MF_HIP(c, hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
             return MF_OK;
         };
-        if (!c->step_exec || c->graph_prec != precision || c->graph_flags != gflags || c->graph_dt != dt) {
+        if (!c->step_exec || c->graph_prec != precision || c->graph_flags != gflags || c->graph_dt != dt ||
+            c->graph_sums != sums) {
             free_graph(c);
             MF_HIP(c, hipStreamSynchronize(c->stream));
             if ((rc = capture(1, &c->step_graph, &c->step_exec))) return rc;
             c->graph_prec = precision;
             c->graph_flags = gflags;
             c->graph_dt = dt;
+            c->graph_sums = sums;
         }
         if (n_steps >= kGraphSteps && !c->multi_exec &&
             (rc = capture(kGraphSteps, &c->multi_graph, &c->multi_exec)))
@@ -1336,9 +1370,10 @@ int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int 
         for (; n < n_steps; ++n) MF_HIP(c, hipGraphLaunch(c->step_exec, c->stream));
     } else {
         for (int n = 0; n < n_steps; ++n)
-            if ((rc = ssprk_step(c, dt, precision, limiter, history))) return rc;
+            if ((rc = ssprk_step(c, dt, precision, limiter, history, sums))) return rc;
     }
     c->fsal_dt = dt;
+    c->fsal_sums = sums;
     c->host_success_iter += n_steps;
     c->fsal_valid = true;
     if (!(flags & MF_STEP_NOSYNC)) MF_HIP(c, hipStreamSynchronize(c->stream));

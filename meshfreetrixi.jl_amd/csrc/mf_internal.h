@@ -159,11 +159,14 @@ struct mf_ctx {
     double* d_approx = nullptr;
     double* d_hist = nullptr;
     double* d_partials = nullptr;  // 4 n_psum sums, then 4 n_pmax lexicographic maxima
+    double* d_stage_psum = nullptr;  // 4 sums per block of kBlock nodes: a stage's output (stepping)
     double* d_flush = nullptr;     // kFlushDoubles, allocated by the first mf_time_rhs_cold
     int n_psum = 0, n_pmax = 0;
     mf::StepScalars* d_sc = nullptr;
     int host_success_iter = 0;
     bool fsal_valid = false;
+    bool fsal_sums = false;   // the stored FSAL stage carries its next boundary pass and sums
+    bool graph_sums = false;  // the captured step passes them between stages
 
     // stage limiter
     int lim_n = 0;
@@ -274,9 +277,12 @@ int launch_weights(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int el
 // the same weights by the null-space form on the matrix cores: one
 // workgroup per stencil, stencil d_sid[b] (or b when d_sid is null) for
 // workgroup b of n_launch; maxn, the largest of their sizes, picks the
-// variant (up to 32 or up to 64 nodes)
+// variant (up to 32, 48 or 64 nodes); uniform_n is the size of every
+// stencil of the launch when they all have the same one (else 0), which
+// selects the kernels compiled for that shape
 int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k, const int32_t* d_sid,
-                      int64_t n_launch, int maxn, int p, int deg, int kd, double* d_wx, double* d_wy, int* d_status);
+                      int64_t n_launch, int maxn, int uniform_n, int p, int deg, int kd, double* d_wx, double* d_wy,
+                      int* d_status);
 int launch_pack_ops(mf_ctx* c, int target, const double* d_wx, const double* d_wy);
 int launch_build_transpose(mf_ctx* c);
 int launch_dx_stats(mf_ctx* c, const double* d_dist2, double* dx_min, double* dx_avg);
@@ -311,6 +317,13 @@ struct StageOut {
     double dt = 0.0;
     const double* uprev = nullptr;
     double* out = nullptr;
+    // stepping with the residual viscosity: sums_out applies the next
+    // rhs!'s first boundary pass (at (t + out_t_adv) + out_dt_off) to the
+    // output and leaves the output's per-block sums in d_stage_psum;
+    // sums_in: this rhs!'s input came so (its first pass and sum pass are
+    // skipped)
+    bool sums_out = false, sums_in = false;
+    double out_t_adv = 0.0, out_dt_off = 0.0;
 };
 int run_rhs(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off, const StageOut& st);
 // parts of one rhs! (for timing single kernels)

@@ -17,6 +17,70 @@
 
 namespace mf {
 
+// Slip-wall projection of the momentum (PointCloudBCs.jl:15-21, :87-106).
+__device__ __forceinline__ void slip_project(const double* nrm2, double* s)
+{
+    const double n1 = nrm2[0], n2 = nrm2[1];
+    const double nn = sqrt(n1 * n1 + n2 * n2);
+    const double e1 = n1 / nn, e2 = n2 / nn;
+    const double m1 = s[1], m2 = s[2];
+    const double dot = m1 * e1 + m2 * e2;
+    s[1] = fma(-dot, e1, m1);
+    s[2] = fma(-dot, e2, m2);
+}
+
+// ---------------------------------------- boundary-pass arguments
+struct BcArgs {
+    int e0, e1;
+    const int32_t* bidx;
+    const int32_t* btag;
+    const double* bnormal;
+    const double* btable;
+    const double* px;
+    const double* py;
+    const uint8_t* flag;   // per node (fused use)
+    const int32_t* entry;  // per node (fused use)
+    double gamma;
+    const StepScalars* sc;
+    double dt_off;
+    double t_adv;  // the pass's time is (sc->t + t_adv) + dt_off (t_adv = 0: sc->t + dt_off)
+    int homog;
+    const int32_t* tag_kind;  // per tag, device tables
+    const int32_t* tag_fn;
+    const double* tag_params;  // 8 per tag
+};
+
+// The first-pass condition of boundary entry e on the state s of its node;
+// false if the entry leaves the state unchanged.
+template <int V>
+__device__ __forceinline__ bool bc_entry(const BcArgs& b, int e, int32_t node, double* s)
+{
+    const int tag = b.btag[e];
+    const int kind = b.tag_kind[tag];
+    if (kind == MF_BC_DIRICHLET) {
+        if (b.homog) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) s[v] = 0.0;
+        } else if (b.tag_fn[tag] == MF_FN_TABLE) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) s[v] = b.btable[(int64_t)e * V + v];
+        } else {
+            double t = b.sc->t;
+            if (b.t_adv != 0.0) t = t + b.t_adv;
+            t = t + b.dt_off;
+            bc_value<V>(b.tag_fn[tag], b.tag_params + 8 * tag, b.gamma, b.px[node], b.py[node], t, s);
+        }
+        return true;
+    }
+    if (kind == MF_BC_SLIP_WALL) {
+        if constexpr (V == 4) {
+            slip_project(b.bnormal + 2 * e, s);
+            return true;
+        }
+    }
+    return false;
+}
+
 // ----------------------------------------------------------- arguments
 struct RhsArgs {
     int64_t N;
@@ -94,19 +158,14 @@ struct RhsArgs {
     double dt;
     const double* __restrict__ uprev;
     double* __restrict__ out;
+    // stepping: the output is the next rhs!'s input.  With stage_psum set
+    // the transposed kernel applies that rhs!'s first boundary pass (obc)
+    // to the output and stores the output's per-block sums there (tile or
+    // node block t at 4 t), which that rhs!'s max pass reads in place of
+    // its own sum pass
+    double* __restrict__ stage_psum;
+    BcArgs obc;
 };
-
-// Slip-wall projection of the momentum (PointCloudBCs.jl:15-21, :87-106).
-__device__ __forceinline__ void slip_project(const double* nrm2, double* s)
-{
-    const double n1 = nrm2[0], n2 = nrm2[1];
-    const double nn = sqrt(n1 * n1 + n2 * n2);
-    const double e1 = n1 / nn, e2 = n2 / nn;
-    const double m1 = s[1], m2 = s[2];
-    const double dot = m1 * e1 + m2 * e2;
-    s[1] = fma(-dot, e1, m1);
-    s[2] = fma(-dot, e2, m2);
-}
 
 // FP32 node records (split-FP32 intermediates): V floats per record.
 template <int V>
@@ -161,13 +220,9 @@ __device__ __forceinline__ void mask_du(const RhsArgs& a, int64_t i, double* du)
 // u is this node's (boundary-imposed) input state; slip nodes see the
 // second boundary pass's projection first, as the reference's u does.
 template <int V>
-__device__ __forceinline__ void emit(const RhsArgs& a, int64_t i, const double* du)
+__device__ __forceinline__ void stage_value(const RhsArgs& a, int64_t i, const double* du, double* o)
 {
-    if (a.st_kind == ST_NONE) {
-        store_state<V>(a.du, i, du);
-        return;
-    }
-    double ui[V], o[V];
+    double ui[V];
     load_state<V>(a.u, i, ui);
     if constexpr (V == 4) {
         if (a.flag && (a.flag[i] & 2u)) slip_project(a.bnormal + 2 * a.entry[i], ui);
@@ -187,7 +242,28 @@ __device__ __forceinline__ void emit(const RhsArgs& a, int64_t i, const double* 
             for (int v = 0; v < V; ++v) o[v] = fma(2.0, ui[v], fma(dt2, du[v], up[v])) / 3.0;
         }
     }
+}
+
+template <int V>
+__device__ __forceinline__ void emit(const RhsArgs& a, int64_t i, const double* du)
+{
+    if (a.st_kind == ST_NONE) {
+        store_state<V>(a.du, i, du);
+        return;
+    }
+    double o[V];
+    stage_value<V>(a, i, du, o);
     store_state<V>(a.out, i, o);
+}
+
+// emit() of a stage whose output the next rhs! reads (a.stage_psum set):
+// that rhs!'s first boundary pass is applied to the output here, as its
+// sum pass would apply it, and o returns the stored state for the sums.
+__device__ __forceinline__ void emit_staged(const RhsArgs& a, int64_t i, const double* du, double* o)
+{
+    stage_value<4>(a, i, du, o);
+    if (a.obc.flag && a.obc.flag[i]) bc_entry<4>(a.obc, a.obc.entry[i], (int32_t)i, o);
+    store_state<4>(a.out, i, o);
 }
 
 // Residual/upwind viscosity at node i (hyperviscosity.jl:246-349): eps_uw
@@ -785,11 +861,14 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
     constexpr int RU = (HG ? 2 : 0) + (HL ? 1 : 0);  // V-value units per halo record
     constexpr int LU = HG ? 2 : 0;                   // unit of L in a record
     extern __shared__ double lds[];
-    int64_t i, off;
-    int lane, W;
+    int64_t i, off = 0;
+    int lane = 0, W = 0;
+    bool live;       // with a.stage_psum, threads past the last node stay for the block's sums
+    int64_t blk;     // tile, or block of kBlock nodes
     if constexpr (TL) {
         const int64_t t = remapped_block();
         if (t >= a.n_tiles) return;  // whole block
+        blk = t;
         const int64_t h0 = a.hb_off[t];
         const int nh = (int)(a.hb_off[t + 1] - h0);
         for (int q = threadIdx.x; q < nh; q += kBlock) {
@@ -824,18 +903,26 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
         __syncthreads();
         const int pos = threadIdx.x;
         i = t * kBlock + a.rowb[t * kBlock + pos];
-        if (i >= a.N) return;
-        lane = pos & 63;
-        const int64_t sl = t * (kBlock / kSellC) + (pos >> 6);
-        off = a.sb_off[sl];
-        W = a.sb_w[sl];
+        live = i < a.N;
+        if (!live && !a.stage_psum) return;
+        if (live) {
+            lane = pos & 63;
+            const int64_t sl = t * (kBlock / kSellC) + (pos >> 6);
+            off = a.sb_off[sl];
+            W = a.sb_w[sl];
+        }
     } else {
         i = remapped_node();
-        if (i >= a.N) return;
-        const int64_t slice = i >> 6;
-        lane = (int)(i & 63);
-        off = a.sell_off[slice];
-        W = a.sell_w[slice];
+        if (i - threadIdx.x >= a.N) return;  // whole block
+        blk = (i - threadIdx.x) / kBlock;
+        live = i < a.N;
+        if (!live && !a.stage_psum) return;
+        if (live) {
+            const int64_t slice = i >> 6;
+            lane = (int)(i & 63);
+            off = a.sell_off[slice];
+            W = a.sell_w[slice];
+        }
     }
     double tx[V], ty[V], th[V];
 #pragma unroll
@@ -928,46 +1015,64 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
             }
         }
     }
-    double du[V];
-    if constexpr (TM == 1)
-        load_state32<V>(a.du32, i, du);
-    else
-        load_state<V>(a.du, i, du);
-    if constexpr (TM == 2) {
-        // the phase-B sources in keyword order
-        for (int sq = a.seq_b; sq != 0; sq >>= 2) {
-            const int kind = sq & 3;
-            if (kind == kSeqViscosity) {
+    double o[4] = {0.0, 0.0, 0.0, 0.0};  // the stored output (staged stages)
+    if (live) {
+        double du[V];
+        if constexpr (TM == 1)
+            load_state32<V>(a.du32, i, du);
+        else
+            load_state<V>(a.du, i, du);
+        if constexpr (TM == 2) {
+            // the phase-B sources in keyword order
+            for (int sq = a.seq_b; sq != 0; sq >>= 2) {
+                const int kind = sq & 3;
+                if (kind == kSeqViscosity) {
 #pragma unroll
-                for (int v = 0; v < V; ++v) du[v] = (du[v] - tx[v]) - ty[v];
-            } else if (kind == kSeqTominec) {
-                const double ng = -a.tom_gamma;
+                    for (int v = 0; v < V; ++v) du[v] = (du[v] - tx[v]) - ty[v];
+                } else if (kind == kSeqTominec) {
+                    const double ng = -a.tom_gamma;
 #pragma unroll
-                for (int v = 0; v < V; ++v) du[v] = du[v] + th[v] * ng;
-            } else {
-                const double ng = -a.hv_gamma;
-                for (int s = 0; s < a.k; ++s) {
-                    const int64_t q = (int64_t)s * a.N + i;
-                    double uj[V];
-                    load_state<V>(a.u, a.idx_s[q], uj);
-                    const double h = a.hv_s[q];
+                    for (int v = 0; v < V; ++v) du[v] = du[v] + th[v] * ng;
+                } else {
+                    const double ng = -a.hv_gamma;
+                    for (int s = 0; s < a.k; ++s) {
+                        const int64_t q = (int64_t)s * a.N + i;
+                        double uj[V];
+                        load_state<V>(a.u, a.idx_s[q], uj);
+                        const double h = a.hv_s[q];
 #pragma unroll
-                    for (int v = 0; v < V; ++v) du[v] = du[v] + h * (uj[v] * ng);
+                        for (int v = 0; v < V; ++v) du[v] = du[v] + h * (uj[v] * ng);
+                    }
                 }
             }
-        }
-    } else {
-        if constexpr (HG) {
+        } else {
+            if constexpr (HG) {
 #pragma unroll
-            for (int v = 0; v < V; ++v) du[v] = (du[v] - tx[v]) - ty[v];
-        }
-        if constexpr (HL) {
+                for (int v = 0; v < V; ++v) du[v] = (du[v] - tx[v]) - ty[v];
+            }
+            if constexpr (HL) {
 #pragma unroll
-            for (int v = 0; v < V; ++v) du[v] = fma(-a.tom_gamma, th[v], du[v]);
+                for (int v = 0; v < V; ++v) du[v] = fma(-a.tom_gamma, th[v], du[v]);
+            }
+        }
+        mask_du<V>(a, i, du);
+        if constexpr (V == 4) {
+            if (a.stage_psum)
+                emit_staged(a, i, du, o);
+            else
+                emit<V>(a, i, du);
+        } else {
+            emit<V>(a, i, du);
         }
     }
-    mask_du<V>(a, i, du);
-    emit<V>(a, i, du);
+    if constexpr (V == 4) {
+        if (a.stage_psum) {  // the same for the whole block
+            block_sum4(o);
+            if (threadIdx.x == 0)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) a.stage_psum[4 * blk + v] = o[v];
+        }
+    }
 }
 
 // ----------------------------------------- boundary pass on the state
@@ -975,54 +1080,6 @@ __global__ __launch_bounds__(kBlock) void k_rhs_transpose(const RhsArgs a)
 // entries [e0, e1): Dirichlet u_b <- g(x_b, t), slip u_b <- projection.
 // With homog = 1 it is the adjoint of that map: Dirichlet rows zeroed,
 // slip momentum projected.
-struct BcArgs {
-    int e0, e1;
-    const int32_t* bidx;
-    const int32_t* btag;
-    const double* bnormal;
-    const double* btable;
-    const double* px;
-    const double* py;
-    const uint8_t* flag;   // per node (fused use)
-    const int32_t* entry;  // per node (fused use)
-    double gamma;
-    const StepScalars* sc;
-    double dt_off;
-    int homog;
-    const int32_t* tag_kind;  // per tag, device tables
-    const int32_t* tag_fn;
-    const double* tag_params;  // 8 per tag
-};
-
-// The first-pass condition of boundary entry e on the state s of its node;
-// false if the entry leaves the state unchanged.
-template <int V>
-__device__ __forceinline__ bool bc_entry(const BcArgs& b, int e, int32_t node, double* s)
-{
-    const int tag = b.btag[e];
-    const int kind = b.tag_kind[tag];
-    if (kind == MF_BC_DIRICHLET) {
-        if (b.homog) {
-#pragma unroll
-            for (int v = 0; v < V; ++v) s[v] = 0.0;
-        } else if (b.tag_fn[tag] == MF_FN_TABLE) {
-#pragma unroll
-            for (int v = 0; v < V; ++v) s[v] = b.btable[(int64_t)e * V + v];
-        } else {
-            const double t = b.sc->t + b.dt_off;
-            bc_value<V>(b.tag_fn[tag], b.tag_params + 8 * tag, b.gamma, b.px[node], b.py[node], t, s);
-        }
-        return true;
-    }
-    if (kind == MF_BC_SLIP_WALL) {
-        if constexpr (V == 4) {
-            slip_project(b.bnormal + 2 * e, s);
-            return true;
-        }
-    }
-    return false;
-}
-
 template <int V>
 __global__ __launch_bounds__(kBlock) void k_bc_state(const BcArgs b, double* u)
 {
@@ -2141,7 +2198,15 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
     // first boundary pass on u
     // with the residual viscosity the first boundary pass rides on the sum pass
     const bool fused_bc = rv && c->visc_mode == MF_VISC_RESIDUAL && c->have_bc && !c->bc.multi;
-    int rc = ((parts & RHS_BC) && !fused_bc) ? bc_state_pass(c, d_u, dt_off, 0, false) : MF_OK;
+    // a stage handed over by the previous rhs! (stepping) has had this
+    // rhs!'s first boundary pass and carries its sums
+    const int64_t nblk = (c->N + kBlock - 1) / kBlock;
+    if (st.sums_in || st.sums_out) {
+        if (c->visc_mode != MF_VISC_RESIDUAL || igr || !c->d_stage_psum || (c->have_bc && c->bc.multi) ||
+            (c->tiles_b && c->n_tiles != nblk))
+            return fail(c, MF_ERR_STATE, "rhs!: stage sums without the residual viscosity's transposed kernel");
+    }
+    int rc = ((parts & RHS_BC) && !fused_bc && !st.sums_in) ? bc_state_pass(c, d_u, dt_off, 0, false) : MF_OK;
     if (rc) return rc;
     // with the IGR source the other kernels leave du (masked) for its last
     // kernel, which writes the output
@@ -2150,22 +2215,33 @@ int run_rhs_parts(mf_ctx* c, int prec, double* d_u, double* d_du, double dt_off,
     out_args.dt = st.dt;
     out_args.uprev = st.uprev;
     out_args.out = st.out;
+    if (st.sums_out) {
+        out_args.stage_psum = c->d_stage_psum;
+        out_args.obc = make_bc(c, st.out_dt_off, 0);
+        out_args.obc.t_adv = st.out_t_adv;
+        if (!c->have_bc) out_args.obc.flag = nullptr;
+    }
     RhsArgs a = out_args;
     if (igr) a.st_kind = ST_NONE;
-    const int64_t nblk = (c->N + kBlock - 1) / kBlock;
     const unsigned grid = xcd_grid(nblk);
     if (rv || tom) {
         if (rv && c->visc_mode == MF_VISC_RESIDUAL && (parts & RHS_REDUCE)) {
             const int ns = c->n_psum, nm = c->n_pmax;
-            BcArgs bc = make_bc(c, dt_off, 0);
-            if (!((parts & RHS_BC) && fused_bc)) bc.flag = nullptr;
-            launch_k(c, 2, k_rv_sum, dim3(ns), dim3(kBlock), 0, c->N, (c->N + ns - 1) / ns, d_u, c->d_partials,
-                     bc);
-            launch_k(c, 3, k_rv_max, dim3(nm), dim3(kBlock), 0, c->N, (c->N + nm - 1) / nm, (const double*)d_u,
-                     (const double*)c->d_partials, ns, c->d_partials + 4 * ns, c->d_sc);
+            if (st.sums_in) {
+                launch_k(c, 3, k_rv_max, dim3(nm), dim3(kBlock), 0, c->N, (c->N + nm - 1) / nm, (const double*)d_u,
+                         (const double*)c->d_stage_psum, (int)nblk, c->d_partials + 4 * ns, c->d_sc);
+            } else {
+                BcArgs bc = make_bc(c, dt_off, 0);
+                if (!((parts & RHS_BC) && fused_bc)) bc.flag = nullptr;
+                launch_k(c, 2, k_rv_sum, dim3(ns), dim3(kBlock), 0, c->N, (c->N + ns - 1) / ns, d_u,
+                         c->d_partials, bc);
+                launch_k(c, 3, k_rv_max, dim3(nm), dim3(kBlock), 0, c->N, (c->N + nm - 1) / nm, (const double*)d_u,
+                         (const double*)c->d_partials, ns, c->d_partials + 4 * ns, c->d_sc);
+            }
         }
         RhsArgs pa = a;
         pa.st_kind = ST_NONE;
+        pa.stage_psum = nullptr;
         if (parts & RHS_FLUX) launch_flux_v<4>(c, prec, pa, hvk, rv, grid);
         if (parts & RHS_TRANSPOSE) {
             const int gt = (rv ? 1 : 0) | (tom ? 2 : 0);

@@ -130,7 +130,11 @@ __device__ __forceinline__ double phs_r(const double* co, int i, int j, int p)
     }
 }
 
-template <int MT, int P>
+// NF, NPF: the stencil size and monomial count when the launch is compiled
+// for one shape (every stencil of it has NF nodes), 0 when they are taken
+// at run time; a compiled shape folds the size guards and unrolls every
+// loop over nodes and monomials.
+template <int MT, int P, int NF, int NPF>
 __global__ __launch_bounds__(64, MT == 2 ? 4 : (MT == 3 ? 3 : 1)) void k_weights_ns(const NsArgs a)
 {
 #pragma clang fp contract(off)
@@ -138,17 +142,18 @@ __global__ __launch_bounds__(64, MT == 2 ? 4 : (MT == 3 ? 3 : 1)) void k_weights
     extern __shared__ double sm[];
     const int lane = threadIdx.x;
     const int q = lane >> 4, c = lane & 15;
-    const int LQ = a.lq, LZ = kNsLz;
+    const int LQ = NPF ? ns_lq(NPF) : a.lq, LZ = kNsLz;
+    const int rows = NF ? (NF < 16 ? 16 : NF) : a.rows;
     double* co = sm;                      // NL x (x, y)
     double* Qm = sm + 2 * NL;             // rows x LQ: P, then Q (columns < np)
-    double* Xm = Qm + a.rows * LQ;        // rows x LQ: X = R Q, then W (columns < np)
-    double* Zm = Xm + a.rows * LQ;        // 16 x LZ: G, L of G, Z, then K tiles
+    double* Xm = Qm + rows * LQ;          // rows x LQ: X = R Q, then W (columns < np)
+    double* Zm = Xm + rows * LQ;          // 16 x LZ: G, L of G, Z, then K tiles
     double* col = Zm + 16 * LZ;           // NL: broadcast column (16-byte aligned)
     double* hv = col + NL;                // 2 NL: h = b - X g (2 per node)
     double* gs = hv + 2 * NL;             // 32: g (2 per monomial)
     double* ss = gs + 32;                 // 32: s = Q'h (2 per monomial)
     double* dg = ss + 32;                 // 16: 1 / L_jj of the Cholesky factor of G
-    const int np = a.np;
+    const int np = NPF ? NPF : a.np;
 
     {
         const int64_t e = a.sid ? (int64_t)a.sid[blockIdx.x] : (int64_t)blockIdx.x;
@@ -156,10 +161,10 @@ __global__ __launch_bounds__(64, MT == 2 ? 4 : (MT == 3 ? 3 : 1)) void k_weights
         int n;
         if (a.ptr) {
             off = a.ptr[e];
-            n = (int)(a.ptr[e + 1] - off);
+            n = NF ? NF : (int)(a.ptr[e + 1] - off);
         } else {
             off = e;
-            n = a.ell_k;
+            n = NF ? NF : a.ell_k;
         }
         auto sidx = [&](int j) -> int32_t { return a.ptr ? a.ridx[off + j] : a.ell[(int64_t)j * a.N + e]; };
         const int T = (n + 15) >> 4;      // 16-row tiles
@@ -183,7 +188,7 @@ __global__ __launch_bounds__(64, MT == 2 ? 4 : (MT == 3 ? 3 : 1)) void k_weights
         }
 
         // ---- P row of this node (poly_block, :211-223), into LDS
-        if (lane < a.rows) {
+        if (lane < rows) {
 #pragma unroll
             for (int t = 0; t < 16; ++t) {
                 double v = 0.0;
@@ -553,7 +558,8 @@ static size_t ns_lds_doubles(int mt, int rows, int lq)
 }
 
 int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int ell_k, const int32_t* d_sid,
-                      int64_t n_launch, int maxn, int p, int deg, int kd, double* d_wx, double* d_wy, int* d_status)
+                      int64_t n_launch, int maxn, int uniform_n, int p, int deg, int kd, double* d_wx, double* d_wy,
+                      int* d_status)
 {
     if (n_launch == 0) return MF_OK;
     NsArgs a;
@@ -584,14 +590,26 @@ int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int
     a.lq = ns_lq(a.np);
     const int mt = maxn <= 32 ? 2 : (maxn <= 48 ? 3 : 4);
     const size_t lds = sizeof(double) * ns_lds_doubles(mt, a.rows, a.lq);
-    // the configurations' PHS powers at compile time, others at run time
+    // the configurations' PHS powers at compile time, others at run time;
+    // two stencil shapes compiled in full: 15 nodes with the 6 monomials of
+    // degree 2, and 30 nodes with the 10 of degree 3
     void (*kern)(NsArgs) = nullptr;
     auto pick = [&](auto k2, auto k3, auto k4) { kern = mt == 2 ? k2 : (mt == 3 ? k3 : k4); };
+    const int shape = uniform_n == 15 && a.np == 6 ? 1 : (uniform_n == 30 && a.np == 10 ? 2 : 0);
     switch (p) {
-        case 3: pick(k_weights_ns<2, 3>, k_weights_ns<3, 3>, k_weights_ns<4, 3>); break;
-        case 5: pick(k_weights_ns<2, 5>, k_weights_ns<3, 5>, k_weights_ns<4, 5>); break;
-        case 7: pick(k_weights_ns<2, 7>, k_weights_ns<3, 7>, k_weights_ns<4, 7>); break;
-        default: pick(k_weights_ns<2, 0>, k_weights_ns<3, 0>, k_weights_ns<4, 0>); break;
+        case 3:
+            pick(k_weights_ns<2, 3, 0, 0>, k_weights_ns<3, 3, 0, 0>, k_weights_ns<4, 3, 0, 0>);
+            if (shape) kern = shape == 1 ? k_weights_ns<2, 3, 15, 6> : k_weights_ns<2, 3, 30, 10>;
+            break;
+        case 5:
+            pick(k_weights_ns<2, 5, 0, 0>, k_weights_ns<3, 5, 0, 0>, k_weights_ns<4, 5, 0, 0>);
+            if (shape) kern = shape == 1 ? k_weights_ns<2, 5, 15, 6> : k_weights_ns<2, 5, 30, 10>;
+            break;
+        case 7:
+            pick(k_weights_ns<2, 7, 0, 0>, k_weights_ns<3, 7, 0, 0>, k_weights_ns<4, 7, 0, 0>);
+            if (shape) kern = shape == 1 ? k_weights_ns<2, 7, 15, 6> : k_weights_ns<2, 7, 30, 10>;
+            break;
+        default: pick(k_weights_ns<2, 0, 0, 0>, k_weights_ns<3, 0, 0, 0>, k_weights_ns<4, 0, 0, 0>); break;
     }
     hipLaunchKernelGGL(kern, dim3((unsigned)n_launch), dim3(64), lds, c->stream, a);
     MF_HIP(c, hipGetLastError());

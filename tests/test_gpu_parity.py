@@ -49,7 +49,9 @@ def test_knn_bit_exact_large(mf):
 
 
 # ------------------------------------------------------------- weights
-@pytest.mark.parametrize("p,deg,k,kd", [(3, 2, 15, 1), (5, 3, 30, 1), (5, 3, 30, 4), (3, 3, 20, 1)])
+# (15, degree 2) and (30, degree 3) run the kernels compiled for those shapes
+@pytest.mark.parametrize("p,deg,k,kd", [(3, 2, 15, 1), (5, 3, 30, 1), (5, 3, 30, 4), (3, 3, 20, 1),
+                                        (7, 2, 15, 2), (7, 3, 30, 1), (3, 3, 30, 2), (5, 2, 15, 4)])
 def test_weights_vs_oracle(mf, p, deg, k, kd):
     _say("weights p=%d deg=%d k=%d kd=%d" % (p, deg, k, kd))
     x, y, _, _, _ = O.cloud_square(40, 20261202)
@@ -334,6 +336,66 @@ def test_ssprk33_euler_rv_history(mf):
     uo = c.u0.copy()
     c.prob.ssprk33(uo, 0.0, c.dt, n, approx_order=3)
     assert rel_inf(ug, uo) < 1e-12
+
+
+def test_ssprk33_rv_slip_walls(mf):
+    """Residual viscosity, history and slip walls: each stage hands the
+    next rhs! its first boundary pass (slip projection, Dirichlet values at
+    the next stage's time) and sums; 8 steps over two calls (graph, then
+    eager resume) against the oracle."""
+    _say("ssprk33 euler + residual viscosity + slip walls")
+    c = cases.build(configs.c2(), side=48, visc="residual", bc=("dirichlet", "slip"))
+    u0 = c.u0.copy()
+    u0[1:3] += 0.02 * np.random.default_rng(11).standard_normal(u0[1:3].shape)
+    c.ctx.upload(L.MF_BUF_U, u0.copy())
+    c.ctx.ssprk33(0.0, c.dt, 5, flags=L.MF_STEP_HISTORY | L.MF_STEP_GRAPH)
+    c.ctx.ssprk33(0.0, c.dt, 3, flags=L.MF_STEP_HISTORY, resume=True)
+    ug = c.ctx.download(L.MF_BUF_U)
+    uo = u0.copy()
+    c.prob.ssprk33(uo, 0.0, c.dt, 8, approx_order=3)
+    assert rel_inf(ug, uo) < 1e-12
+
+
+def test_ssprk33_rv_with_stage_limiter(mf):
+    """With a stage limiter between the stages each rhs! does its own
+    boundary pass and sums: residual viscosity + history + the Zhang-Shu
+    limiter against the oracle; a resume that changes the limiter setting is
+    refused (the stored stage was formed for the other sequence)."""
+    _say("ssprk33 euler + residual viscosity + stage limiter")
+    c = cases.build(configs.c2(), side=40, visc="residual")
+    u0 = c.u0.copy()
+    u0[0, ::97] = 0.3
+    lim = [(0.6, O.VAR_DENSITY), (0.5, O.VAR_PRESSURE)]
+    c.ctx.set_stage_limiter([t for t, _ in lim], [L.MF_VAR_DENSITY, L.MF_VAR_PRESSURE])
+    c.ctx.upload(L.MF_BUF_U, u0.copy())
+    c.ctx.ssprk33(0.0, c.dt, 5, flags=L.MF_STEP_LIMITER | L.MF_STEP_HISTORY)
+    ug = c.ctx.download(L.MF_BUF_U)
+    uo = u0.copy()
+    c.prob.ssprk33(uo, 0.0, c.dt, 5, approx_order=3, limiter=lim, stencil=(O.ell_ptr(c.N, c.k), c.idx))
+    assert rel_inf(ug, uo) < 1e-12
+    # the limited call's stored stage does not carry the next boundary pass
+    with pytest.raises(L.MfError) as e:
+        c.ctx.ssprk33(0.0, c.dt, 1, flags=L.MF_STEP_HISTORY, resume=True)
+    assert e.value.code == L.MF_ERR_STATE
+
+
+def test_ssprk33_rv_stage_sums_match_separate_passes(mf):
+    """The stage hand-over changes only the order of the density, momentum
+    and energy sums (per tile of the transposed kernel instead of the sum
+    pass's blocks): the same steps with a stage limiter that never acts
+    (thresholds 0) run every rhs! with its own passes and agree to 1e-13."""
+    _say("stage sums vs separate passes")
+    out = []
+    for limited in (False, True):
+        c = cases.build(configs.c2(), side=48, visc="residual")
+        f = L.MF_STEP_HISTORY | L.MF_STEP_GRAPH
+        if limited:
+            c.ctx.set_stage_limiter([0.0], [L.MF_VAR_DENSITY])
+            f |= L.MF_STEP_LIMITER
+        c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+        c.ctx.ssprk33(0.0, c.dt, 6, flags=f)
+        out.append(c.ctx.download(L.MF_BUF_U))
+    assert rel_inf(out[0], out[1]) < 1e-13
 
 
 # ------------------------------------------------------------- adjoint

@@ -318,6 +318,7 @@ static int prepare_vjp(mf_ctx* c)
     if ((rc = ensure_transpose_any(c))) return rc;
     if (c->use_tiles && c->have_flux && (!c->tried_a || (c->tiles_a && !c->tried_b)))
         if ((rc = build_tiles(c, true, c->nvar))) return rc;
+    if ((rc = ensure_tile_ops(c))) return rc;
     return MF_OK;
 }
 

@@ -1118,6 +1118,11 @@ static int prepare_rhs(mf_ctx* c)
         free_graph(c);
         if ((rc = build_tiles(c, want_b, rec_b))) return rc;
     }
+    if (c->tiles_a && c->tm_dirty) {
+        free_graph(c);
+        if ((rc = ensure_tile_ops(c))) return rc;
+        MF_HIP(c, hipStreamSynchronize(c->stream));
+    }
     if (tom && c->tiles_b && !c->lap_tb) {
         free_graph(c);
         if ((!c->d_twl && (rc = dalloc(c, &c->d_twl, c->nb_entries))) ||

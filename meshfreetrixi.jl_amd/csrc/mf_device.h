@@ -12,6 +12,16 @@ namespace mf {
 
 constexpr int kBlock = 256;
 
+// Entry s of node i's stencil in the tile-major layout of the LDS-tile
+// kernels: the K slots of each tile of kBlock nodes back to back, kBlock
+// entries each.  A tile's entries of one array are then one contiguous run
+// (a few pages) instead of K runs N entries apart, which on large point
+// sets made the address translation of the slot-major layout the limit.
+__device__ __forceinline__ int64_t tm_pos(int s, int64_t i, int K)
+{
+    return ((i / kBlock) * K + s) * kBlock + (i % kBlock);
+}
+
 // Logical node of this thread under the XCD-aware block remap (grid size a
 // multiple of 8; workgroup b runs on XCD b % 8).
 __device__ __forceinline__ int64_t remapped_node()

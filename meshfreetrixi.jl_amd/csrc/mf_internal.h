@@ -214,8 +214,21 @@ struct mf_ctx {
     int64_t nb_entries = 0;              // phase-B entries with padding
     int64_t* d_ha_off = nullptr;         // n_tiles + 1
     int32_t* d_ha_ids = nullptr;         // halo node ids, sorted per tile
-    uint16_t* d_la = nullptr;            // slot-major halo positions (like d_idx)
+    uint16_t* d_la = nullptr;            // halo positions of the d_idx entries, tile-major
     uint16_t* d_la_s = nullptr;          // the same for the column-sorted stencil d_idx_s
+    // tile-major copies of the operators for the tile kernels (rebuilt from
+    // the slot-major arrays after every operator change: tm_dirty)
+    double2* d_wxy_t = nullptr;
+    float2* d_wxy32_t = nullptr;
+    double* d_hv_t = nullptr;
+    float* d_hv32_t = nullptr;
+    double* d_lap_t = nullptr;
+    float* d_lap32_t = nullptr;
+    double* d_wx_s_t = nullptr;
+    double* d_wy_s_t = nullptr;
+    double* d_hv_s_t = nullptr;
+    double* d_lap_s_t = nullptr;
+    bool tm_dirty = true;
     int64_t* d_hb_off = nullptr;
     int32_t* d_hb_ids = nullptr;
     int64_t* d_sb_off = nullptr;         // 4 slices (64 rows) per tile
@@ -309,6 +322,9 @@ int launch_sort_stencil(mf_ctx* c);
 int launch_pack_sorted(mf_ctx* c, int target, const double* d_wx, const double* d_wy);
 int launch_fill_lap_t(mf_ctx* c, int64_t n, const int64_t* d_src, double* d_w, float* d_w32);
 int build_tiles(mf_ctx* c, bool want_b, int rec_b);
+// tile-major copies of the operators for the tile kernels (after
+// build_tiles and after every operator change)
+int ensure_tile_ops(mf_ctx* c);
 
 // rhs
 enum StageKind { ST_NONE = 0, ST_FSAL = 1, ST_TWO = 2, ST_THREE = 3 };

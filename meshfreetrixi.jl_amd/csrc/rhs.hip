@@ -497,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_f64(const RhsArgs a)
         for (int b = 0; b < BT; ++b) {
             const int s = s0 + b;
             if ((NS && NS % BT == 0) || s < K) {
-                const int64_t q = (int64_t)s * N + i;
+                const int64_t q = TL ? tm_pos(s, i, K) : (int64_t)s * N + i;
                 if constexpr (TL)
                     j[b] = a.la[q];
                 else
@@ -610,7 +610,7 @@ __device__ __forceinline__ void ref_pass(const RhsArgs& a, int64_t i, int K, dou
         for (int b = 0; b < BT; ++b) {
             const int s = s0 + b;
             const bool in = (NS && NS % BT == 0) || s < K;
-            const int64_t q = (int64_t)(in ? s : s0) * N + i;
+            const int64_t q = TL ? tm_pos(in ? s : s0, i, K) : (int64_t)(in ? s : s0) * N + i;
             if constexpr (TL)
                 j[b] = a.la_s[q];
             else
@@ -737,7 +737,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_s32(const RhsArgs a)
         for (int b = 0; b < BT; ++b) {
             const int s = s0 + b;
             if ((NS && NS % BT == 0) || s < K) {
-                const int64_t q = (int64_t)s * N + i;
+                const int64_t q = TL ? tm_pos(s, i, K) : (int64_t)s * N + i;
                 if constexpr (TL)
                     jj[b] = a.la[q];
                 else if constexpr (D16)
@@ -1482,7 +1482,7 @@ __global__ __launch_bounds__(kBlock) void k_limiter_blend_tl(int64_t N, int K, c
     for (int s0 = 0; s0 < K; s0 += kLimBatch) {
         double val[kLimBatch];
 #pragma unroll
-        for (int b = 0; b < kLimBatch; ++b) val[b] = lvar[(s0 + b < K) ? la[(int64_t)(s0 + b) * N + i] : 0];
+        for (int b = 0; b < kLimBatch; ++b) val[b] = lvar[(s0 + b < K) ? la[tm_pos(s0 + b, i, K)] : 0];
 #pragma unroll
         for (int b = 0; b < kLimBatch; ++b)
             if (s0 + b < K) vmin = (val[b] < vmin) ? val[b] : vmin;
@@ -1611,8 +1611,9 @@ __device__ __forceinline__ double2 igr_grad_tl(const IgrArgs& a, int64_t i, cons
 #pragma unroll
         for (int q = 0; q < BT; ++q) {
             const int s = s0 + q < a.k ? s0 + q : s0;
-            pos[q] = a.la[(int64_t)s * a.N + i];
-            w[q] = a.wxy[(int64_t)s * a.N + i];
+            const int64_t tq = tm_pos(s, i, a.k);  // tile-major (la, and wxy the tile-major copy)
+            pos[q] = a.la[tq];
+            w[q] = a.wxy[tq];
         }
         double gx[BT], gy[BT];
 #pragma unroll
@@ -2052,11 +2053,30 @@ static void launch_ref(mf_ctx* c, const RhsArgs& a, int hvk, bool rv, const dim3
 }
 #undef MF_LAUNCH_HVK
 
+// The operators in the tile-major layout of the tile kernels.
+static RhsArgs tile_major_args(const mf_ctx* c, const RhsArgs& a)
+{
+    RhsArgs t = a;
+    t.wxy = c->d_wxy_t;
+    t.wxy32 = c->d_wxy32_t;
+    t.hv = c->d_hv_t;
+    t.hv32 = c->d_hv32_t;
+    t.lap = c->d_lap_t;
+    t.lap32 = c->d_lap32_t;
+    t.wx_s = c->d_wx_s_t;
+    t.wy_s = c->d_wy_s_t;
+    t.hv_s = c->d_hv_s_t;
+    t.lap_s = c->d_lap_s_t;
+    return t;
+}
+
 template <int V, int NS, int BT>
-static void launch_flux_bt(mf_ctx* c, int prec, const RhsArgs& a, int hvk, bool rv, unsigned grid)
+static void launch_flux_bt(mf_ctx* c, int prec, const RhsArgs& a_in, int hvk, bool rv, unsigned grid)
 {
     const dim3 G(grid);
-    const bool tl = c->tiles_a;
+    // the tile kernels read the tile-major operator copies
+    const bool tl = c->tiles_a && !c->tm_dirty;
+    const RhsArgs a = tl ? tile_major_args(c, a_in) : a_in;
     const size_t lds = tl ? (size_t)c->ha_max * rec_len<V>() * sizeof(double) : 0;  // halo records
     if (prec == MF_PREC_F64) {
         if (tl)
@@ -2141,13 +2161,15 @@ static int run_igr(mf_ctx* c, const double* u, const RhsArgs& a, unsigned grid)
     const dim3 G(grid), B(kBlock);
     launch_k(c, -1, k_igr_init, G, B, 0, g);
     // the stencil kernel's tiles, when built (one tile per block of nodes)
-    const bool tl = c->tiles_a && c->d_la && c->n_tiles == g.nblk;
+    const bool tl = c->tiles_a && c->d_la && c->n_tiles == g.nblk && !c->tm_dirty && c->d_wxy_t;
     const size_t lds1 = sizeof(double) * (size_t)c->ha_max, lds2 = 2 * lds1;
+    IgrArgs gt = g;  // the tile kernels read the tile-major copy of (Dx, Dy)
+    gt.wxy = c->d_wxy_t;
     for (int it = 0; it < c->igr_maxiter; ++it) {
         launch_k(c, -1, k_igr_p, G, B, 0, g, it);
         if (tl) {
-            launch_k(c, -1, k_igr_t_tl<5>, G, B, lds1, g);
-            launch_k(c, -1, k_igr_c_tl<5>, G, B, lds2, g);
+            launch_k(c, -1, k_igr_t_tl<5>, G, B, lds1, gt);
+            launch_k(c, -1, k_igr_c_tl<5>, G, B, lds2, gt);
         } else {
             launch_k(c, -1, k_igr_t<5>, G, B, 0, g);
             launch_k(c, -1, k_igr_c<5>, G, B, 0, g);

@@ -1042,6 +1042,7 @@ int launch_pack_ops(mf_ctx* c, int target, const double* d_wx, const double* d_w
     else
         hipLaunchKernelGGL(k_pack_hv, dim3(g), dim3(256), 0, c->stream, n, d_wx, d_wy, c->d_lap, c->d_lap32);
     MF_HIP(c, hipGetLastError());
+    c->tm_dirty = true;
     return MF_OK;
 }
 
@@ -1290,6 +1291,7 @@ int launch_pack_sorted(mf_ctx* c, int target, const double* d_wx, const double* 
                            target == MF_OP_HYPERVISCOSITY ? c->d_hv_s : c->d_lap_s);
     }
     MF_HIP(c, hipGetLastError());
+    c->tm_dirty = true;
     return MF_OK;
 }
 

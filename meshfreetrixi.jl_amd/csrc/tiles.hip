@@ -9,7 +9,12 @@
 // halo positions stored in place of the 32-bit node indices.
 //
 // Phase A (flux divergence, sources, viscosity phase A): halo = the union
-// of the tile's stencils; positions slot-major like the node indices.
+// of the tile's stencils; positions, and copies of the operators, in the
+// tile-major layout (tm_pos: a tile's K slots back to back), so that one
+// tile's entries of an array share a few pages: with the slot-major layout
+// (slot stride N) a tile touched K pages per array, and on multi-million
+// node sets the address translation (UTCL2 busy 91% of the time) limited
+// the stencil kernel.
 // Phase B (transposed gathers D^T G, lap^T L): rows of the tile sorted by their
 // transposed degree (long rows first) so that the 64 rows of a wavefront
 // have similar lengths, stored SELL-64 per wavefront; entries of a row in
@@ -19,6 +24,7 @@
 
 #include <algorithm>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "mf_device.h"
@@ -59,9 +65,29 @@ void free_tiles_b(mf_ctx* c)
     c->hb_max = 0;
 }
 
+static void free_tile_ops(mf_ctx* c)
+{
+    dfree(c->d_wxy_t);
+    dfree(c->d_wxy32_t);
+    dfree(c->d_hv_t);
+    dfree(c->d_hv32_t);
+    dfree(c->d_lap_t);
+    dfree(c->d_lap32_t);
+    dfree(c->d_wx_s_t);
+    dfree(c->d_wy_s_t);
+    dfree(c->d_hv_s_t);
+    dfree(c->d_lap_s_t);
+    c->d_wxy_t = nullptr;
+    c->d_wxy32_t = nullptr;
+    c->d_hv_t = c->d_lap_t = c->d_wx_s_t = c->d_wy_s_t = c->d_hv_s_t = c->d_lap_s_t = nullptr;
+    c->d_hv32_t = c->d_lap32_t = nullptr;
+    c->tm_dirty = true;
+}
+
 void free_tiles(mf_ctx* c)
 {
     free_tiles_b(c);
+    free_tile_ops(c);
     dfree(c->d_ha_off);
     dfree(c->d_ha_ids);
     dfree(c->d_la);
@@ -86,6 +112,45 @@ __global__ void k_fill_tile_weights(int64_t n, const int64_t* src, const double2
     const double2 w = q >= 0 ? wxy[q] : make_double2(0.0, 0.0);
     tw[e] = w;
     tw32[e] = make_float2((float)w.x, (float)w.y);
+}
+
+// Slot-major (s N + i) operator array to the tile-major layout (tm_pos);
+// the padding entries of the last tile are zero.
+template <class T>
+__global__ void k_to_tile_major(int64_t N, int K, int64_t n_out, const T* src, T* dst)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_out) return;
+    const int64_t r = q % kTile, ts = q / kTile;  // ts = t K + s
+    const int64_t t = ts / K, sl = ts % K;
+    const int64_t i = t * kTile + r;
+    dst[q] = i < N ? src[sl * N + i] : T{};
+}
+
+// The tile-major copies of every operator array present, after the tiles
+// are built and whenever an operator changed since the last copy.
+int ensure_tile_ops(mf_ctx* c)
+{
+    if (!c->tiles_a || !c->tm_dirty) return MF_OK;
+    const int64_t n_out = c->n_tiles * (int64_t)c->k * kTile;
+    int rc;
+    auto copy = [&](auto* src, auto** dst) -> int {
+        using T = std::remove_pointer_t<decltype(src)>;
+        if (!src) return MF_OK;
+        if (!*dst && (rc = dalloc(c, dst, (size_t)n_out))) return rc;
+        hipLaunchKernelGGL(k_to_tile_major<T>, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, c->stream,
+                           c->N, c->k, n_out, (const T*)src, *dst);
+        MF_HIP(c, hipGetLastError());
+        return MF_OK;
+    };
+    if ((rc = copy(c->d_wxy, &c->d_wxy_t)) || (rc = copy(c->d_wxy32, &c->d_wxy32_t)) ||
+        (rc = copy(c->d_hv, &c->d_hv_t)) || (rc = copy(c->d_hv32, &c->d_hv32_t)) ||
+        (rc = copy(c->d_lap, &c->d_lap_t)) || (rc = copy(c->d_lap32, &c->d_lap32_t)) ||
+        (rc = copy(c->d_wx_s, &c->d_wx_s_t)) || (rc = copy(c->d_wy_s, &c->d_wy_s_t)) ||
+        (rc = copy(c->d_hv_s, &c->d_hv_s_t)) || (rc = copy(c->d_lap_s, &c->d_lap_s_t)))
+        return rc;
+    c->tm_dirty = false;
+    return MF_OK;
 }
 
 template <class T>
@@ -123,7 +188,8 @@ int build_tiles(mf_ctx* c, bool want_b, int rec_b)
         std::vector<int64_t> off(nt + 1, 0);
         std::vector<int32_t> all;
         all.reserve((size_t)N * 2);
-        std::vector<uint16_t> la((size_t)N * K), la_s((size_t)N * K);
+        // tile-major (tm_pos): tile t's slot s at (t K + s) kTile
+        std::vector<uint16_t> la((size_t)nt * K * kTile, 0), la_s((size_t)nt * K * kTile, 0);
         int hmax = 0;
         bool fits = true;
         for (int64_t t = 0; t < nt && fits; ++t) {
@@ -144,8 +210,9 @@ int build_tiles(mf_ctx* c, bool want_b, int rec_b)
             for (size_t q = 0; q < ids.size(); ++q) mark[ids[q]] = (int32_t)q;
             for (int s = 0; s < K; ++s)
                 for (int64_t i = i0; i < i1; ++i) {
-                    la[(size_t)s * N + i] = (uint16_t)mark[idx[(size_t)s * N + i]];
-                    la_s[(size_t)s * N + i] = (uint16_t)mark[idx_s[(size_t)s * N + i]];
+                    const size_t tq = (size_t)(t * K + s) * kTile + (size_t)(i - i0);
+                    la[tq] = (uint16_t)mark[idx[(size_t)s * N + i]];
+                    la_s[tq] = (uint16_t)mark[idx_s[(size_t)s * N + i]];
                 }
             for (int32_t j : ids) mark[j] = -1;
             hmax = std::max<int>(hmax, (int)ids.size());

@@ -313,6 +313,36 @@ def test_rhs_gpu_weights_vs_oracle(mf):
     assert rel_inf(du, duo) < 1e-10
 
 
+@pytest.mark.parametrize("prec", [L.MF_PREC_F64, L.MF_PREC_SPLIT32, L.MF_PREC_F64_REF])
+def test_operator_change_after_rhs(mf, prec):
+    """The tile kernels read tile-major copies of the operators: after an
+    rhs! has built them, new flux and hyperviscosity operators must be what
+    the next rhs! uses, bit for bit as in a context that never saw the old
+    ones."""
+    _say("operator change after rhs! prec %d" % prec)
+    spec = configs.c2()
+    rng = np.random.default_rng(5)
+    out = []
+    for fresh in (False, True):
+        c = cases.build(spec, side=48, hv=(2, 1e-4))
+        wx2 = c.wx * (1.0 + 0.01 * rng.standard_normal(c.wx.shape))
+        wy2 = c.wy * (1.0 + 0.01 * rng.standard_normal(c.wy.shape))
+        hv2 = 2.0 * c.hv_w
+        rng = np.random.default_rng(5)  # the same new operators both times
+        if not fresh:
+            u = c.u0.copy()
+            du = np.zeros_like(u)
+            c.ctx.rhs_host(0.0, u, du, prec)  # builds the tiles and the copies
+        c.ctx.set_operator(L.MF_OP_FLUX, np.ascontiguousarray(wx2), np.ascontiguousarray(wy2))
+        c.ctx.set_operator(L.MF_OP_HYPERVISCOSITY, np.ascontiguousarray(hv2))
+        u = c.u0.copy()
+        du = np.zeros_like(u)
+        c.ctx.rhs_host(0.0, u, du, prec)
+        assert c.ctx.tile_info()["tiles_a"] == 1  # the tile kernels ran
+        out.append(du)
+    assert np.array_equal(out[0], out[1])
+
+
 # ------------------------------------------------------------- SSPRK3
 def test_ssprk33_advection(mf):
     _say("ssprk33 advection 20 steps")

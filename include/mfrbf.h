@@ -335,7 +335,8 @@ int mf_set_stage_limiter(mf_ctx* ctx, int n, const double* thresholds, const int
  * and history state (same dt; MF_BUF_U as that call left it).  With the
  * residual viscosity and no stage limiter each stage is written with the
  * next rhs!'s first boundary pass and sums already applied, so a resume
- * that switches MF_STEP_LIMITER on or off there returns MF_ERR_STATE. */
+ * that switches MF_STEP_LIMITER on or off there (or the residual viscosity
+ * on or off) returns MF_ERR_STATE. */
 int mf_ssprk33(mf_ctx* ctx, double t0, double dt, int n_steps, int precision, int flags,
                int resume);
 

@@ -1326,7 +1326,8 @@ int mf_ssprk33(mf_ctx* c, double t0, double dt, int n_steps, int precision, int 
         // with the residual viscosity the stored FSAL stage carries the
         // next boundary pass and sums exactly when no stage limiter runs
         if (c->fsal_sums != sums)
-            return fail(c, MF_ERR_STATE, "mf_ssprk33: resume needs the previous call's stage limiter setting");
+            return fail(c, MF_ERR_STATE,
+                        "mf_ssprk33: resume needs the previous call's stage limiter and viscosity settings");
     } else {
         if ((rc = run_set_time(c, t0, 0))) return rc;
         if (history && (rc = run_history(c, c->d_buf[MF_BUF_U], 1))) return rc;

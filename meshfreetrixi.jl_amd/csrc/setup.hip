@@ -2,6 +2,8 @@
 // transposed operator used by the viscosity sources.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <algorithm>
 #include <cfloat>
 #include <hipcub/hipcub.hpp>
@@ -520,44 +522,79 @@ int launch_knn_noalloc(mf_ctx* c, int k, int32_t* d_idx, double* d_dist)
 }
 
 // dx_min = min, dx_avg = mean of the nearest-neighbour distance
-// (geometry_primatives.jl:332-334).  One block, fixed reduction order.
-__global__ void k_dx_stats(int64_t N, const double* d2nn, double* out)
+// (geometry_primatives.jl:332-334).  The mean sums in the caller's node
+// order exactly as Julia's `sum` does (Base.mapreduce_impl: halve the
+// range until it has at most 1024 entries, add each such block left to
+// right, then add the halves back up), so dx_avg, which scales the upwind
+// and residual viscosities, has the reference's bits: the GPU forms every
+// block's sum (and minimum), the host adds the blocks along the same
+// recursion.
+__global__ void k_dx_blocks(int64_t nb, const int64_t* start, const double* d2nn, const int32_t* inv,
+                            double* bsum, double* bmin)
 {
-    __shared__ double smin[1024], ssum[1024];
-    double mn = INFINITY, sm = 0.0;
-    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
-        const double d = sqrt(d2nn[i]);
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int64_t lo = start[b], hi = start[b + 1];
+    double v = sqrt(d2nn[inv[lo]]), mn = v;
+    for (int64_t i = lo + 1; i < hi; ++i) {
+        const double d = sqrt(d2nn[inv[i]]);
+        v = v + d;
         mn = fmin(mn, d);
-        sm += d;
     }
-    smin[threadIdx.x] = mn;
-    ssum[threadIdx.x] = sm;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
-            ssum[threadIdx.x] += ssum[threadIdx.x + w];
-        }
-        __syncthreads();
+    bsum[b] = v;
+    bmin[b] = mn;
+}
+
+// The blocks of the pairwise recursion over [lo, hi], in order.
+static void pairwise_blocks(int64_t lo, int64_t hi, std::vector<int64_t>& start)
+{
+    if (hi - lo < 1024) {
+        start.push_back(lo);
+        return;
     }
-    if (threadIdx.x == 0) {
-        out[0] = smin[0];
-        out[1] = ssum[0] / (double)N;
-    }
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    pairwise_blocks(lo, mid, start);
+    pairwise_blocks(mid + 1, hi, start);
+}
+
+static double pairwise_combine(int64_t lo, int64_t hi, const std::vector<double>& bsum, size_t& next)
+{
+    if (hi - lo < 1024) return bsum[next++];
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    const double a = pairwise_combine(lo, mid, bsum, next);
+    return a + pairwise_combine(mid + 1, hi, bsum, next);
 }
 
 int launch_dx_stats(mf_ctx* c, const double* d_d2, double* dx_min, double* dx_avg)
 {
+    const int64_t N = c->N;
+    std::vector<int64_t> start;
+    pairwise_blocks(0, N - 1, start);
+    const int64_t nb = (int64_t)start.size();
+    start.push_back(N);
+    int64_t* d_start = nullptr;
     double* d_out = nullptr;
-    int rc = dalloc(c, &d_out, 2);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_dx_stats, dim3(1), dim3(1024), 0, c->stream, c->N, d_d2 + c->N, d_out);
-    double h[2];
-    MF_HIP(c, hipMemcpyAsync(h, d_out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    int rc;
+    if ((rc = dalloc(c, &d_start, start.size())) || (rc = dalloc(c, &d_out, 2 * (size_t)nb))) {
+        dfree(d_start);
+        return rc;
+    }
+    MF_HIP(c, hipMemcpyAsync(d_start, start.data(), sizeof(int64_t) * start.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    hipLaunchKernelGGL(k_dx_blocks, dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, c->stream, nb, d_start,
+                       d_d2 + N, c->d_inv, d_out, d_out + nb);
+    MF_HIP(c, hipGetLastError());
+    std::vector<double> h(2 * (size_t)nb);
+    MF_HIP(c, hipMemcpyAsync(h.data(), d_out, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
     MF_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(d_start);
     dfree(d_out);
-    *dx_min = h[0];
-    *dx_avg = h[1];
+    std::vector<double> bsum(h.begin(), h.begin() + nb);
+    double mn = h[nb];
+    for (int64_t q = 1; q < nb; ++q) mn = std::min(mn, h[nb + q]);
+    size_t next = 0;
+    *dx_min = mn;
+    *dx_avg = pairwise_combine(0, N - 1, bsum, next) / (double)N;
     return MF_OK;
 }
 

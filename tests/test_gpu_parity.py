@@ -35,7 +35,7 @@ def test_knn_bit_exact_vs_brute(mf, side, k, seed):
     assert np.array_equal(dist, np.sqrt(d2))
     omin, oavg = O.dx_stats(d2[:, 1])
     assert dmin == omin
-    assert abs(davg - oavg) <= 2e-15 * oavg  # mean: tree vs pairwise summation order
+    assert davg == oavg  # the mean in Julia's pairwise order, caller's node order
 
 
 def test_knn_bit_exact_large(mf):
@@ -689,12 +689,13 @@ def test_node_order_changes_no_result(mf):
         if order == 1:
             assert np.array_equal(perm, np.arange(len(x)))
         wx, wy = ctx.compute_operator(5, 3, 1, L.MF_OP_FLUX, want=True, k=30)
-        res[order] = dict(idx=idx, dist=dist, dmin=dmin, wx=wx, wy=wy, perm=perm)
+        res[order] = dict(idx=idx, dist=dist, dmin=dmin, davg=davg, wx=wx, wy=wy, perm=perm)
     a, b = res[0], res[1]
     assert not np.array_equal(a["perm"], b["perm"])
     for key in ("idx", "dist", "wx", "wy"):
         assert np.array_equal(a[key], b[key]), key
     assert a["dmin"] == b["dmin"]
+    assert a["davg"] == b["davg"]  # summed in the caller's order either way
     outs = {}
     for order in (0, 1):
         c = cases.build(configs.c2(), side=48, hv=(2, 1e-4), visc="upwind", bc=("dirichlet", "nothing"),

@@ -197,13 +197,14 @@ static void free_bc(mf_ctx* c)
     dfree(c->bc.btable);
     dfree(c->bc.flag);
     dfree(c->bc.entry);
+    dfree(c->bc.ecode);
     dfree(c->bc.d_kind);
     dfree(c->bc.d_kind_slip);
     dfree(c->bc.d_fn);
     dfree(c->bc.d_params);
     c->bc.d_kind = c->bc.d_kind_slip = c->bc.d_fn = nullptr;
     c->bc.d_params = nullptr;
-    c->bc.bidx = c->bc.btag = c->bc.entry = nullptr;
+    c->bc.bidx = c->bc.btag = c->bc.entry = c->bc.ecode = nullptr;
     c->bc.bnormal = c->bc.btable = nullptr;
     c->bc.flag = nullptr;
     c->have_bc = false;
@@ -861,6 +862,7 @@ int mf_set_boundary(mf_ctx* c, int n_tags, const int32_t* tag_ptr, const int32_t
     b.nb = nb;
     if ((rc = dalloc(c, &b.bidx, nb)) || (rc = dalloc(c, &b.btag, nb)) || (rc = dalloc(c, &b.bnormal, 2 * (size_t)nb)) ||
         (rc = dalloc(c, &b.btable, (size_t)nb * 4)) || (rc = dalloc(c, &b.flag, c->N)) || (rc = dalloc(c, &b.entry, c->N)) ||
+        (rc = dalloc(c, &b.ecode, std::max(nb, 1))) ||
         (rc = dalloc(c, &b.d_kind, n_tags)) || (rc = dalloc(c, &b.d_kind_slip, n_tags)) ||
         (rc = dalloc(c, &b.d_fn, n_tags)) || (rc = dalloc(c, &b.d_params, 8 * (size_t)n_tags)))
         return rc;
@@ -877,6 +879,13 @@ int mf_set_boundary(mf_ctx* c, int n_tags, const int32_t* tag_ptr, const int32_t
     if (normals) std::copy(normals, normals + 2 * (size_t)nb, nrm.begin());
     MF_HIP(c, hipMemcpyAsync(b.bidx, bnode.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream));
     MF_HIP(c, hipMemcpyAsync(b.btag, btag.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream));
+    // each entry's kind, value function and tag in one word: the fused
+    // first pass (sum pass, stage hand-over) reads them with one load
+    std::vector<int32_t> ecode(std::max(nb, 1), 0);
+    for (int e = 0; e < nb; ++e)
+        ecode[e] = b.tag_kind[btag[e]] | (b.tag_fn[btag[e]] << 4) | (btag[e] << 8);
+    MF_HIP(c, hipMemcpyAsync(b.ecode, ecode.data(), sizeof(int32_t) * ecode.size(), hipMemcpyHostToDevice,
+                             c->stream));
     MF_HIP(c, hipMemcpyAsync(b.bnormal, nrm.data(), sizeof(double) * 2 * nb, hipMemcpyHostToDevice, c->stream));
     MF_HIP(c, hipMemsetAsync(b.btable, 0, sizeof(double) * nb * 4, c->stream));
     MF_HIP(c, hipMemcpyAsync(b.flag, flag.data(), c->N, hipMemcpyHostToDevice, c->stream));

@@ -59,6 +59,7 @@ struct BcDev {
     double* btable = nullptr;    // nvar nb
     uint8_t* flag = nullptr;     // N: bit0 Dirichlet, bit1 slip
     int32_t* entry = nullptr;    // N: boundary entry of the node, -1 if none
+    int32_t* ecode = nullptr;    // nb: first-pass kind | function << 4 | tag << 8 of each entry
     std::vector<int32_t> tag_kind;
     std::vector<int32_t> tag_fn;
     std::vector<int32_t> tag_ptr;

@@ -1111,7 +1111,7 @@ static int prepare_rhs(mf_ctx* c)
     }
     if (c->igr_alpha != 0.0) {
         const size_t nblk = (size_t)((c->N + 255) / 256);
-        if ((!c->d_igr && (rc = dalloc(c, &c->d_igr, 8 * (size_t)c->N))) ||
+        if ((!c->d_igr && (rc = dalloc(c, &c->d_igr, 9 * (size_t)c->N))) ||
             (!c->d_igr_part && (rc = dalloc(c, &c->d_igr_part, 2 * nblk))) ||
             (!c->d_igr_sc && (rc = dalloc(c, &c->d_igr_sc, 1))))
             return rc;

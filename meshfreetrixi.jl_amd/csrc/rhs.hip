@@ -1558,7 +1558,8 @@ __global__ void k_fill(double* p, int64_t n, double v)
 // then du_m1 -= Dx sigma, du_m2 -= Dy sigma.  Scalar node fields; every
 // reduction is per block (fixed node ranges) into partials, which the next
 // kernel's blocks each sum in the same fixed order, so every block, and
-// every run, gets the same bits.  Four launches per CG iteration.
+// every run, gets the same bits.  Three launches per CG iteration over the
+// LDS tiles, four with global gathers.
 struct IgrArgs {
     int64_t N;
     int k;
@@ -1569,6 +1570,7 @@ struct IgrArgs {
     int maxiter;
     const double* __restrict__ u;
     double *rinv, *b, *x, *r, *p, *c, *t1, *t2;
+    double* p2;       // the tile path's second search direction (ping-pong with p)
     double* part_rr;  // per-block partial sums of r . r
     double* part_pc;  // and of p . c
     IgrScalars* sc;
@@ -1767,17 +1769,40 @@ __global__ __launch_bounds__(kBlock) void k_igr_t(const IgrArgs a)
     a.t2[i] = ri * g.y;
 }
 
-// k_igr_t over the LDS tiles.
+// k_igr_p and k_igr_t in one launch over the LDS tiles: every block forms
+// res, beta and the stop test from the partials as k_igr_p does (block 0
+// publishes them),
+// stages r + beta p_old of its halo nodes (the same expression as the new
+// direction of those nodes), writes its own nodes' new direction to p_new
+// (another buffer: neighbouring blocks still read p_old) and forms t.
 template <int BT>
-__global__ __launch_bounds__(kBlock) void k_igr_t_tl(const IgrArgs a)
+__global__ __launch_bounds__(kBlock) void k_igr_pt_tl(const IgrArgs a, const double* __restrict__ p_old,
+                                                      double* __restrict__ p_new, int it)
 {
     extern __shared__ double lds[];
-    if (a.sc->done) return;
+    const double res = sqrt(sum_partials(a.part_rr, a.nblk));
+    const double tol = it == 0 ? sqrt(DBL_EPSILON) * res : a.sc->tol;
+    const double prev = it == 0 ? 1.0 : a.sc->res[(it - 1) & 1];
+    const bool done = res <= tol;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.sc->res[it & 1] = res;
+        a.sc->tol = tol;
+        a.sc->done = done ? 1 : 0;
+    }
+    if (done) return;
     const int64_t t = remapped_block();
     if (t >= a.n_tiles) return;  // whole block
-    igr_stage<false>(a, t, a.p, nullptr, lds);
+    const double beta = res * res / (prev * prev);
+    const int64_t h0 = a.ha_off[t];
+    const int nh = (int)(a.ha_off[t + 1] - h0);
+    for (int q = threadIdx.x; q < nh; q += kBlock) {
+        const int32_t j = a.ha_ids[h0 + q];
+        lds[q] = a.r[j] + beta * p_old[j];
+    }
+    __syncthreads();
     const int64_t i = t * kBlock + threadIdx.x;
     if (i >= a.N) return;
+    p_new[i] = a.r[i] + beta * p_old[i];
     const double2 g = igr_grad_tl<BT, false>(a, i, lds);
     const double ri = a.rinv[i];
     a.t1[i] = ri * g.x;
@@ -1883,6 +1908,7 @@ static IgrArgs make_igr(mf_ctx* c, const double* u)
     g.c = f + 5 * N;
     g.t1 = f + 6 * N;
     g.t2 = f + 7 * N;
+    g.p2 = f + 8 * N;
     g.part_rr = c->d_igr_part;
     g.part_pc = c->d_igr_part + g.nblk;
     g.n_tiles = c->n_tiles;
@@ -2205,14 +2231,23 @@ static int run_igr(mf_ctx* c, const double* u, const RhsArgs& a, unsigned grid)
     IgrArgs gt = g;  // the tile kernels read the tile-major copy of (Dx, Dy)
     gt.wxy = c->d_wxy_t;
     for (int it = 0; it < c->igr_maxiter; ++it) {
-        launch_k(c, -1, k_igr_p, G, B, 0, g, it);
         if (tl) {
-            launch_k(c, -1, k_igr_t_tl<5>, G, B, lds1, gt);
-            launch_k(c, -1, k_igr_c_tl<5>, G, B, lds2, gt);
-        } else {
-            launch_k(c, -1, k_igr_t<5>, G, B, 0, g);
-            launch_k(c, -1, k_igr_c<5>, G, B, 0, g);
+            // three launches per iteration: the direction update rides on
+            // the t pass, p and p2 alternate (p is zero after k_igr_init)
+            double* p_old = (it & 1) ? g.p2 : g.p;
+            double* p_new = (it & 1) ? g.p : g.p2;
+            IgrArgs gi = gt;
+            gi.p = p_new;
+            IgrArgs gx = g;
+            gx.p = p_new;
+            launch_k(c, -1, k_igr_pt_tl<5>, G, B, lds1, gt, (const double*)p_old, p_new, it);
+            launch_k(c, -1, k_igr_c_tl<5>, G, B, lds2, gi);
+            launch_k(c, -1, k_igr_x, G, B, 0, gx, it);
+            continue;
         }
+        launch_k(c, -1, k_igr_p, G, B, 0, g, it);
+        launch_k(c, -1, k_igr_t<5>, G, B, 0, g);
+        launch_k(c, -1, k_igr_c<5>, G, B, 0, g);
         launch_k(c, -1, k_igr_x, G, B, 0, g, it);
     }
     launch_k(c, -1, k_igr_apply<5>, G, B, 0, g, a);

@@ -353,13 +353,33 @@ def _ensemble_runs(L, specs, steps):
     run()  # capture the graphs, build the LDS tiles
     wall_c = run()
     out["contexts"] = dict(setup_s=t_setup, ms_concurrent=1e3 * wall_c, node_evals_per_s=evals / wall_c)
+    del clouds
+    # lanes: grouped contexts of at most 131,072 nodes, one stream each, at
+    # the common time step (each cloud's state the grouped context's bits)
+    from mfrbf.ensemble import EnsembleLanes
+    t0 = time.perf_counter()
+    lanes = EnsembleLanes(specs)
+    t_setup_l = time.perf_counter() - t0
+
+    def run_lanes():
+        lanes.upload_initial()
+        lanes.synchronize()
+        w0 = time.perf_counter()
+        lanes.ssprk33(0.0, steps)
+        return time.perf_counter() - w0
+
+    run_lanes()  # capture the graphs, build the LDS tiles
+    wall_l = run_lanes()
+    out["lanes"] = dict(lane_nodes=131072, lanes=len(lanes.lanes), setup_s=t_setup_l, ms=1e3 * wall_l,
+                        node_evals_per_s=evals / wall_l)
     return out
 
 
 def ensemble(L, configs, steps):
     """C5: 64 Euler clouds (16k..262k nodes) with Flyer hyperviscosity,
-    split-FP32 rhs!, SSPRK33 steps, as one grouped context and as 64
-    contexts; and the same physics on 256 small clouds (48 x 48 nodes each),
+    split-FP32 rhs!, SSPRK33 steps, as one grouped context, as 64 contexts
+    and as lanes (grouped contexts of at most 131,072 nodes on their own
+    streams); and the same physics on 256 small clouds (48 x 48 nodes each),
     where the launches rather than the bytes set the time."""
     out = _ensemble_runs(L, configs.c5(), steps)
     small = configs.c5(256)

@@ -195,6 +195,25 @@ def test_grouped_set_refuses_global_reductions(mf):
     assert e.value.code == L.MF_ERR_UNSUPPORTED
 
 
+def test_c5_lanes_equal_one_grouped_context(mf):
+    """EnsembleLanes (the clouds packed into grouped contexts of at most
+    100,000 nodes, each on its own stream) against one grouped context of
+    the same clouds: 5 graph-replayed SSPRK33 steps at the common time
+    step, split-FP32 with Flyer hyperviscosity, bit for bit."""
+    from mfrbf.ensemble import Ensemble, EnsembleLanes
+    _say("C5 lanes vs one grouped context")
+    specs = configs.c5()[:6]
+    g = Ensemble(specs)
+    g.ctx.upload(L.MF_BUF_U, g.u0)
+    g.ctx.ssprk33(0.0, g.dt, 5, L.MF_PREC_SPLIT32, L.MF_STEP_GRAPH)
+    ug = g.ctx.download(L.MF_BUF_U)
+    lanes = EnsembleLanes(specs, lane_nodes=100000)
+    assert len(lanes.lanes) > 1 and lanes.dt == g.dt
+    lanes.upload_initial()
+    lanes.ssprk33(0.0, 5)
+    assert np.array_equal(lanes.download(), ug)
+
+
 def test_grouped_set_equals_separate_contexts(mf):
     """A grouped set is the clouds side by side, nothing more: with the
     sources that act node by node (Flyer and Tominec hyperviscosity, the

@@ -83,9 +83,10 @@ __device__ __forceinline__ bool bc_entry(const BcArgs& b, int e, int32_t node, d
 }
 
 // bc_entry's first pass for node `node` with boundary entry e, from the
-// entry's packed code: the code, the node's coordinates, the time and the
-// entry's table row and normal are loaded together, the tag's parameters
-// after (two dependent loads instead of bc_entry's four); same arithmetic.
+// entry's packed code: the code, the node's coordinates and the time load
+// together, then the tag's parameters (or the entry's table row or normal):
+// two dependent loads after the entry index instead of bc_entry's four;
+// same arithmetic.
 template <int V>
 __device__ __forceinline__ bool bc_node(const BcArgs& b, int e, int32_t node, double* s)
 {
@@ -1771,10 +1772,10 @@ __global__ __launch_bounds__(kBlock) void k_igr_t(const IgrArgs a)
 
 // k_igr_p and k_igr_t in one launch over the LDS tiles: every block forms
 // res, beta and the stop test from the partials as k_igr_p does (block 0
-// publishes them),
-// stages r + beta p_old of its halo nodes (the same expression as the new
-// direction of those nodes), writes its own nodes' new direction to p_new
-// (another buffer: neighbouring blocks still read p_old) and forms t.
+// publishes them), stages r + beta p_old of its halo nodes (the same
+// expression as the new direction of those nodes), writes its own nodes'
+// new direction to p_new (another buffer: neighbouring blocks still read
+// p_old) and forms t.
 template <int BT>
 __global__ __launch_bounds__(kBlock) void k_igr_pt_tl(const IgrArgs a, const double* __restrict__ p_old,
                                                       double* __restrict__ p_new, int it)

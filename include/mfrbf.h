@@ -187,8 +187,9 @@ int mf_set_operator(mf_ctx* ctx, int target, const double* wx, const double* wy)
  * size.  By default (MF_TUNE_WEIGHTS_SOLVER 0 or 2) the saddle systems are
  * solved in their null-space form on the FP64 matrix cores, one workgroup
  * per stencil in one launch per size variant (stencils of up to 32, of 33
- * to 48 and of 49 to 64 nodes; odd PHS power, polynomial degree <= 4, every stencil
- * larger than the monomial count); otherwise, or with
+ * to 48 and of 49 to 64 nodes; odd PHS power p, polynomial degree from
+ * ceil(p/2) - 1 (where r^p is conditionally positive definite) to 4, every
+ * stencil larger than the monomial count); otherwise, or with
  * MF_TUNE_WEIGHTS_SOLVER 1, by the Bunch-Kaufman restatement, one launch
  * per stencil size.  ptr/idx are uploaded once; mf_weights_ragged_run
  * recomputes on the device (timing loop), mf_weights_ragged_get copies

@@ -583,6 +583,15 @@ static void op_installed(mf_ctx* c, int target)
     }
 }
 
+// The null-space solve needs K = s Pi R Pi + QQ' positive definite: r^p
+// (p odd) is conditionally positive definite of order ceil(p/2), which the
+// monomials up to degree ceil(p/2) - 1 must span; other bases go to
+// Bunch-Kaufman, which does not need it (as the reference's solve).
+static bool ns_basis_ok(int p, int deg)
+{
+    return (p & 1) && deg <= 4 && deg >= (p + 1) / 2 - 1;
+}
+
 int mf_compute_operator(mf_ctx* c, int phs_power, int poly_degree, int deriv_order, int target,
                         double* wx_out, double* wy_out)
 {
@@ -601,7 +610,7 @@ int mf_compute_operator(mf_ctx* c, int phs_power, int poly_degree, int deriv_ord
     if ((rc = dalloc(c, &wx, n)) || (rc = dalloc(c, &wy, n))) return rc;
     if (!c->d_status && (rc = dalloc(c, &c->d_status, 1))) return rc;
     MF_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
-    if (c->weights_solver == 2 && (phs_power & 1) && poly_degree <= 4 && c->k > np)
+    if (c->weights_solver == 2 && ns_basis_ok(phs_power, poly_degree) && c->k > np)
         rc = launch_weights_ns(c, nullptr, c->d_idx, c->k, nullptr, c->N, c->k, c->k, phs_power, poly_degree,
                                deriv_order, wx, wy, c->d_status);
     else
@@ -731,7 +740,7 @@ int mf_weights_ragged_run(mf_ctx* c, int phs_power, int poly_degree, int deriv_o
         return fail(c, MF_ERR_ARGUMENT, "mf_weights_ragged_run: bad basis");
     MF_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
     const int np = (poly_degree + 1) * (poly_degree + 2) / 2;
-    if (c->weights_solver != 1 && (phs_power & 1) && poly_degree <= 4 && c->rminn > np) {
+    if (c->weights_solver != 1 && ns_basis_ok(phs_power, poly_degree) && c->rminn > np) {
         // the stencils in size order (d_rorder), one launch per kernel
         // variant: up to 32, 33 to 48 and 49 to 64 nodes
         int64_t lo = 0;

@@ -596,17 +596,23 @@ int launch_weights_ns(mf_ctx* c, const int64_t* d_ptr, const int32_t* d_idx, int
     void (*kern)(NsArgs) = nullptr;
     auto pick = [&](auto k2, auto k3, auto k4) { kern = mt == 2 ? k2 : (mt == 3 ? k3 : k4); };
     const int shape = uniform_n == 15 && a.np == 6 ? 1 : (uniform_n == 30 && a.np == 10 ? 2 : 0);
+    // the larger variants with the 10 monomials of degree 3 compiled in (the
+    // stencil size at run time)
+    const bool np10 = a.np == 10 && mt > 2;
     switch (p) {
         case 3:
             pick(k_weights_ns<2, 3, 0, 0>, k_weights_ns<3, 3, 0, 0>, k_weights_ns<4, 3, 0, 0>);
+            if (np10) kern = mt == 3 ? k_weights_ns<3, 3, 0, 10> : k_weights_ns<4, 3, 0, 10>;
             if (shape) kern = shape == 1 ? k_weights_ns<2, 3, 15, 6> : k_weights_ns<2, 3, 30, 10>;
             break;
         case 5:
             pick(k_weights_ns<2, 5, 0, 0>, k_weights_ns<3, 5, 0, 0>, k_weights_ns<4, 5, 0, 0>);
+            if (np10) kern = mt == 3 ? k_weights_ns<3, 5, 0, 10> : k_weights_ns<4, 5, 0, 10>;
             if (shape) kern = shape == 1 ? k_weights_ns<2, 5, 15, 6> : k_weights_ns<2, 5, 30, 10>;
             break;
         case 7:
             pick(k_weights_ns<2, 7, 0, 0>, k_weights_ns<3, 7, 0, 0>, k_weights_ns<4, 7, 0, 0>);
+            if (np10) kern = mt == 3 ? k_weights_ns<3, 7, 0, 10> : k_weights_ns<4, 7, 0, 10>;
             if (shape) kern = shape == 1 ? k_weights_ns<2, 7, 15, 6> : k_weights_ns<2, 7, 30, 10>;
             break;
         default: pick(k_weights_ns<2, 0, 0, 0>, k_weights_ns<3, 0, 0, 0>, k_weights_ns<4, 0, 0, 0>); break;

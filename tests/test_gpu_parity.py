@@ -49,7 +49,6 @@ def test_knn_bit_exact_large(mf):
 
 
 # ------------------------------------------------------------- weights
-# (15, degree 2) and (30, degree 3) run the kernels compiled for those shapes
 @pytest.mark.parametrize("p,deg,k,kd", [(3, 2, 15, 1), (5, 3, 30, 1), (5, 3, 30, 4), (3, 3, 20, 1),
                                         (7, 2, 15, 2), (7, 3, 30, 1), (3, 3, 30, 2), (5, 2, 15, 4)])
 def test_weights_vs_oracle(mf, p, deg, k, kd):

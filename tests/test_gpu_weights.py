@@ -32,11 +32,16 @@ def _per_stencil_err(w, wo, ptr):
 
 
 @pytest.mark.parametrize("p,deg,k,kd", [(3, 2, 15, 1), (5, 3, 30, 1), (5, 3, 30, 4), (3, 3, 20, 1),
-                                        (5, 3, 30, 2), (5, 3, 50, 1), (7, 4, 40, 1)])
+                                        (5, 3, 30, 2), (5, 3, 50, 1), (7, 4, 40, 1),
+                                        (7, 3, 30, 1), (3, 3, 30, 2), (5, 2, 15, 4),
+                                        (3, 3, 40, 1), (5, 3, 44, 1), (3, 3, 50, 2)])
 def test_ns_operator_vs_oracle(mf, p, deg, k, kd):
     """mf_compute_operator with the null-space solver, every basis the
     configurations and the hyperviscosity sources use (and a degree-4 one):
-    per-stencil agreement with the oracle and polynomial reproduction."""
+    per-stencil agreement with the oracle and polynomial reproduction.  The
+    kernels compiled for 15 nodes of degree 2 and 30 of degree 3 (PHS powers
+    3, 5, 7), and the 33..64-node variants with the 10 monomials of degree 3
+    compiled in, are among them."""
     _say("null-space weights p=%d deg=%d k=%d kd=%d" % (p, deg, k, kd))
     x, y, _, _, _ = O.cloud_square(40, 20261202)
     N = len(x)
@@ -62,6 +67,24 @@ def test_ns_operator_vs_oracle(mf, p, deg, k, kd):
             dfy = np.prod(np.arange(b - kd + 1, b + 1)) * x ** a * y ** (b - kd) if b >= kd else 0 * x
             assert np.abs((wx * f[idx]).sum(1) - dfx).max() < tol_x
             assert np.abs((wy * f[idx]).sum(1) - dfy).max() < tol_y
+
+
+def test_ns_basis_below_cpd_order_uses_bunch_kaufman(mf):
+    """r^7 with degree-2 monomials is below the order (4) of conditional
+    positive definiteness, where the null-space K need not be positive
+    definite: the null-space setting then solves by Bunch-Kaufman, and the
+    weights equal those of MF_TUNE_WEIGHTS_SOLVER 1."""
+    _say("null-space setting below the CPD order")
+    x, y, _, _, _ = O.cloud_square(40, 20261202)
+    idx, _ = O.knn_grid(x, y, 15)
+    out = []
+    for solver in (2, 1):
+        ctx = L.Context(0)
+        ctx.set_tuning(L.MF_TUNE_WEIGHTS_SOLVER, solver)
+        ctx.set_points(x, y)
+        ctx.set_stencil(idx)
+        out.append(ctx.compute_operator(7, 2, 2, L.MF_OP_FLUX, want=True, k=15))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
 def test_ns_ragged_c3_sample_vs_oracle(mf):
@@ -137,3 +160,51 @@ def test_ns_closer_to_exact_than_tolerance(mf):
         e_bk = np.abs(ox[ptr[s]:ptr[s + 1]] - exact).max() / scale
         _say("  stencil %d (n=%d): null-space %.2e, Bunch-Kaufman %.2e vs 40 digits" % (e, n, e_ns, e_bk))
         assert e_ns < 1e-11
+
+
+def test_ns_r7_large_stencils_vs_40_digits(mf):
+    """r^7 with degree 3 on stencils of 44 and 50 nodes (the larger
+    variants with the 10 monomials compiled in), where the saddle systems
+    are ill-conditioned enough that the null-space and Bunch-Kaufman weights
+    differ by several 1e-10: against a 40-digit solve both are about
+    equally close, and the null-space error stays below 1e-9."""
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 40
+    side = 64
+    x, y, _, _, _ = O.cloud_square(side, configs.SEED0 + 3)
+    N = len(x)
+    big, _ = O.knn_grid(x, y, 50)
+    picks = [(side * side // 2 + side // 2, 44), (side * 10 + 20, 50), (side * 7, 44), (N - 1, 50)]
+    ptr = np.concatenate([[0], np.cumsum([n for _, n in picks])]).astype(np.int64)
+    idx = np.concatenate([big[e, :n] for e, n in picks]).astype(np.int32)
+    ctx = L.Context(0)
+    ctx.set_points(x, y)
+    ctx.weights_ragged_set(ptr, idx)
+    ctx.weights_ragged_run(7, 3, 1)
+    wx, _ = ctx.weights_ragged_get()
+    ox, _, _ = O.weights(x, y, ptr, idx, 7, 3, 1)
+    mons = [(d - t, t) for d in range(4) for t in range(d + 1)]
+    for s, (e, n) in enumerate(picks):
+        nodes = idx[ptr[s]:ptr[s + 1]]
+        X = [(mp.mpf(x[q]) - mp.mpf(x[nodes[0]]), mp.mpf(y[q]) - mp.mpf(y[nodes[0]])) for q in nodes]
+        sx = 1 / max(abs(v[0]) for v in X)
+        sy = 1 / max(abs(v[1]) for v in X)
+        Xs = [(v[0] * sx, v[1] * sy) for v in X]
+        m = n + len(mons)
+        M = mp.zeros(m, m)
+        rhs = mp.zeros(m, 1)
+        for i in range(n):
+            for j in range(n):
+                M[i, j] = mp.sqrt((Xs[i][0] - Xs[j][0]) ** 2 + (Xs[i][1] - Xs[j][1]) ** 2) ** 7
+            for t, (ax, ay) in enumerate(mons):
+                M[i, n + t] = M[n + t, i] = Xs[i][0] ** ax * Xs[i][1] ** ay
+            px, py = (-Xs[i][0], -Xs[i][1]) if i else (mp.mpf(2) ** -52, mp.mpf(2) ** -52)
+            rhs[i] = 7 * px * mp.sqrt(px * px + py * py) ** 5
+        rhs[n + 1] = 1  # d/dx of the monomial x at the centre
+        w = mp.lu_solve(M, rhs)
+        exact = np.array([float(w[i] * sx) for i in range(n)])
+        scale = np.abs(exact).max()
+        e_ns = np.abs(wx[ptr[s]:ptr[s + 1]] - exact).max() / scale
+        e_bk = np.abs(ox[ptr[s]:ptr[s + 1]] - exact).max() / scale
+        _say("  stencil %d (n=%d): null-space %.2e, Bunch-Kaufman %.2e vs 40 digits" % (e, n, e_ns, e_bk))
+        assert e_ns < 1e-9 and e_ns < 10 * e_bk + 1e-12

@@ -427,6 +427,36 @@ def test_ssprk33_rv_stage_sums_match_separate_passes(mf):
     assert rel_inf(out[0], out[1]) < 1e-13
 
 
+@pytest.mark.parametrize("prec,tol,tol_sums", [(L.MF_PREC_F64_REF, 1e-12, 1e-13), (L.MF_PREC_SPLIT32, TOL32, 1e-9)])
+def test_ssprk33_stage_sums_every_source(mf, prec, tol, tol_sums):
+    """The stage hand-over with every source the transposed kernel applies
+    (Flyer hyperviscosity, the residual viscosity with the history callback,
+    then Tominec hyperviscosity), slip walls and Dirichlet inflow, in reference order
+    and split-FP32: 6 graph-replayed steps against the oracle, and against
+    the same steps with every rhs! running its own boundary and sum passes
+    (a stage limiter that never acts)."""
+    _say("stage sums, flyer + tominec + residual viscosity, prec=%d" % prec)
+    out = []
+    for limited in (False, True):
+        c = cases.build(configs.c2(), side=45, visc="residual", hv=(2, 1e-4), tom=1.0, bc=("dirichlet", "slip"),
+                        order=("flyer", "visc", "tominec"))
+        u0 = c.u0.copy()
+        u0[1:3] += 0.02 * np.random.default_rng(5).standard_normal(u0[1:3].shape)
+        f = L.MF_STEP_HISTORY | L.MF_STEP_GRAPH
+        if limited:
+            c.ctx.set_stage_limiter([0.0], [L.MF_VAR_DENSITY])
+            f |= L.MF_STEP_LIMITER
+        c.ctx.upload(L.MF_BUF_U, u0.copy())
+        c.ctx.ssprk33(0.0, c.dt, 6, prec, f)
+        out.append(c.ctx.download(L.MF_BUF_U))
+    uo = u0.copy()
+    c.prob.ssprk33(uo, 0.0, c.dt, 6, approx_order=3)
+    e_or, e_sums = rel_inf(out[0], uo), rel_inf(out[0], out[1])
+    _say("  vs oracle %.2e, vs separate passes %.2e" % (e_or, e_sums))
+    assert np.abs(out[0] - u0).max() > 1e-6  # the steps moved the state
+    assert e_or < tol and e_sums < tol_sums
+
+
 # ------------------------------------------------------------- adjoint
 def test_rhs_vjp_vs_oracle(mf):
     _say("vjp euler")

@@ -91,6 +91,24 @@ class Ensemble:
         return u[:, self.offsets[r]:self.offsets[r + 1]]
 
 
+def pack_lanes(sizes, lane_nodes):
+    """Consecutive clouds (node counts `sizes`) packed into lanes of at most
+    `lane_nodes` nodes; a cloud larger than that is a lane of its own.
+    Returns the clouds of each lane as lists of indices, in order."""
+    if lane_nodes < 1:
+        raise ValueError("pack_lanes: lane_nodes must be positive")
+    lanes, cur, n = [], [], 0
+    for r, m in enumerate(sizes):
+        if cur and n + m > lane_nodes:
+            lanes.append(cur)
+            cur, n = [], 0
+        cur.append(r)
+        n += m
+    if cur:
+        lanes.append(cur)
+    return lanes
+
+
 class EnsembleLanes:
     """The same ensemble as lanes: consecutive clouds packed into grouped
     contexts of at most `lane_nodes` nodes (a larger cloud is a lane of its
@@ -109,17 +127,8 @@ class EnsembleLanes:
 
     def __init__(self, specs, lane_nodes=131072, device=0):
         self.specs = list(specs)
-        groups, cur, n = [], [], 0
-        for spec in self.specs:
-            m = spec.side * spec.side
-            if cur and n + m > lane_nodes:
-                groups.append(cur)
-                cur, n = [], 0
-            cur.append(spec)
-            n += m
-        if cur:
-            groups.append(cur)
-        self.lanes = [Ensemble(g, device) for g in groups]
+        groups = pack_lanes([spec.side * spec.side for spec in self.specs], lane_nodes)
+        self.lanes = [Ensemble([self.specs[r] for r in g], device) for g in groups]
         self.dt = min(lane.dt for lane in self.lanes)
         self.N = sum(lane.N for lane in self.lanes)
 

@@ -78,6 +78,30 @@ def test_configs_sizes():
     assert 0.06 < frac < 0.10 and n.min() == 30 and set(np.unique(n[n > 30])) <= set(range(40, 51))
 
 
+def test_ensemble_lane_packing():
+    """Lanes hold consecutive clouds up to the node budget; a cloud above the
+    budget is a lane of its own; every cloud lands in exactly one lane, in
+    order.  C5's 64 clouds at 131,072 nodes per lane."""
+    from mfrbf import configs
+    from mfrbf.ensemble import pack_lanes
+
+    assert pack_lanes([5, 5, 5], 10) == [[0, 1], [2]]
+    assert pack_lanes([3, 20, 3, 3], 10) == [[0], [1], [2, 3]]
+    assert pack_lanes([10, 10], 10) == [[0], [1]]
+    assert pack_lanes([], 10) == []
+    with pytest.raises(ValueError):
+        pack_lanes([1], 0)
+    sides, _ = configs.c5_sizes()
+    sizes = [int(s) * int(s) for s in sides]
+    lanes = pack_lanes(sizes, 131072)
+    assert [r for g in lanes for r in g] == list(range(64))
+    for g in lanes:
+        total = sum(sizes[r] for r in g)
+        assert total <= 131072 or len(g) == 1
+    for a, b in zip(lanes[:-1], lanes[1:]):  # a lane closes only when the next cloud would overflow it
+        assert sum(sizes[r] for r in a) + sizes[b[0]] > 131072
+
+
 def test_gaussian_noise_moments():
     from mfrbf.clouds import gaussian_noise
 

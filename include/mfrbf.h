@@ -188,10 +188,10 @@ int mf_set_operator(mf_ctx* ctx, int target, const double* wx, const double* wy)
  * solved in their null-space form on the FP64 matrix cores, one workgroup
  * per stencil in one launch per size variant (stencils of up to 32, of 33
  * to 48 and of 49 to 64 nodes; odd PHS power p, polynomial degree from
- * ceil(p/2) - 1 (where r^p is conditionally positive definite) to 4, every
- * stencil larger than the monomial count); otherwise, or with
- * MF_TUNE_WEIGHTS_SOLVER 1, by the Bunch-Kaufman restatement, one launch
- * per stencil size.  ptr/idx are uploaded once; mf_weights_ragged_run
+ * ceil(p/2) - 1 (where r^p is conditionally positive definite) to 4);
+ * stencils of fewer than twice the monomial count, every stencil of another
+ * basis, and all of them with MF_TUNE_WEIGHTS_SOLVER 1, by the
+ * Bunch-Kaufman restatement, one launch per stencil size.  ptr/idx are uploaded once; mf_weights_ragged_run
  * recomputes on the device (timing loop), mf_weights_ragged_get copies
  * the result (layout like idx). */
 int mf_weights_ragged_set(mf_ctx* ctx, const int64_t* ptr, const int32_t* idx,
@@ -365,7 +365,9 @@ int mf_ssprk33_vjp(mf_ctx* ctx, double t0, double dt, int n_steps, int checkpoin
                                      null-space form on the matrix cores for ragged batches
                                      and Bunch-Kaufman (the reference's factorization) for
                                      mf_compute_operator; 1 Bunch-Kaufman for both; 2 the
-                                     null-space form for both; precision is ignored */
+                                     null-space form for both (where the basis allows it
+                                     and stencils have at least twice as many nodes as
+                                     monomials); precision is ignored */
 /* Selects a compiled kernel variant for one precision (MF_PREC_*, or -1 for
  * all).  A batch size that does not divide the stencil size falls back to
  * the default.  Results do not depend on it (the per-node operation order is

@@ -592,6 +592,16 @@ static bool ns_basis_ok(int p, int deg)
     return (p & 1) && deg <= 4 && deg >= (p + 1) / 2 - 1;
 }
 
+// The null-space solve forms Q by the Cholesky QR of P (n x np), which
+// squares P's condition number; P is well conditioned only with enough
+// nodes per monomial.  Stencils of fewer than 2 np nodes go to
+// Bunch-Kaufman: at 11 nodes with the 10 cubic monomials (cond P up to
+// 2e5) the null-space weights are off by 3e-7, at 20 nodes by 6e-13.
+static bool ns_size_ok(int n, int np)
+{
+    return n >= 2 * np;
+}
+
 int mf_compute_operator(mf_ctx* c, int phs_power, int poly_degree, int deriv_order, int target,
                         double* wx_out, double* wy_out)
 {
@@ -610,7 +620,7 @@ int mf_compute_operator(mf_ctx* c, int phs_power, int poly_degree, int deriv_ord
     if ((rc = dalloc(c, &wx, n)) || (rc = dalloc(c, &wy, n))) return rc;
     if (!c->d_status && (rc = dalloc(c, &c->d_status, 1))) return rc;
     MF_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
-    if (c->weights_solver == 2 && ns_basis_ok(phs_power, poly_degree) && c->k > np)
+    if (c->weights_solver == 2 && ns_basis_ok(phs_power, poly_degree) && ns_size_ok(c->k, np))
         rc = launch_weights_ns(c, nullptr, c->d_idx, c->k, nullptr, c->N, c->k, c->k, phs_power, poly_degree,
                                deriv_order, wx, wy, c->d_status);
     else
@@ -740,14 +750,23 @@ int mf_weights_ragged_run(mf_ctx* c, int phs_power, int poly_degree, int deriv_o
         return fail(c, MF_ERR_ARGUMENT, "mf_weights_ragged_run: bad basis");
     MF_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
     const int np = (poly_degree + 1) * (poly_degree + 2) / 2;
-    if (c->weights_solver != 1 && ns_basis_ok(phs_power, poly_degree) && c->rminn > np) {
-        // the stencils in size order (d_rorder), one launch per kernel
-        // variant: up to 32, 33 to 48 and 49 to 64 nodes
-        int64_t lo = 0;
+    if (c->weights_solver != 1 && ns_basis_ok(phs_power, poly_degree)) {
+        // the stencils in size order (d_rorder): the size classes below the
+        // null-space size (they come first) by Bunch-Kaufman, one launch
+        // each, then one launch per null-space variant: up to 32, 33 to 48
+        // and 49 to 64 nodes
+        size_t q0 = 0;
+        for (; q0 < c->rclass_n.size() && !ns_size_ok(c->rclass_n[q0], np); ++q0) {
+            const int64_t b0 = c->rclass_start[q0], b1 = c->rclass_start[q0 + 1];
+            if ((rc = launch_weights(c, c->d_rptr, c->d_ridx, 0, b1 - b0, c->rclass_n[q0], phs_power, poly_degree,
+                                     deriv_order, c->d_rwx, c->d_rwy, c->d_status, c->d_rorder, b0)))
+                return rc;
+        }
+        int64_t lo = c->rclass_start[q0];
         for (int bound : {32, 48, 64}) {
             int64_t hi = lo;
             int maxn = 0, classes = 0;
-            for (size_t q = 0; q < c->rclass_n.size(); ++q)
+            for (size_t q = q0; q < c->rclass_n.size(); ++q)
                 if (c->rclass_n[q] <= bound) {
                     hi = c->rclass_start[q + 1];
                     maxn = c->rclass_n[q];

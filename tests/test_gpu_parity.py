@@ -23,7 +23,7 @@ def _say(msg):
 
 
 # ----------------------------------------------------------------- kNN
-@pytest.mark.parametrize("side,k,seed", [(40, 15, 20261202), (48, 30, 20261203), (33, 50, 5)])
+@pytest.mark.parametrize("side,k,seed", [(40, 15, 20261202), (48, 30, 20261203), (33, 50, 5), (30, 64, 20261209)])
 def test_knn_bit_exact_vs_brute(mf, side, k, seed):
     _say("knn brute side=%d k=%d" % (side, k))
     x, y, _, _, _ = O.cloud_square(side, seed)

@@ -87,6 +87,24 @@ def test_ns_basis_below_cpd_order_uses_bunch_kaufman(mf):
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
+def test_ns_small_stencils_use_bunch_kaufman(mf):
+    """A stencil of fewer nodes than twice the monomial count (15 nodes with
+    the 10 of degree 3) is solved by Bunch-Kaufman under the null-space
+    setting: the operator equals that of MF_TUNE_WEIGHTS_SOLVER 1 bit for
+    bit."""
+    _say("null-space setting, 15-node stencils of degree 3")
+    x, y, _, _, _ = O.cloud_square(40, 20261202)
+    idx, _ = O.knn_grid(x, y, 15)
+    out = []
+    for solver in (2, 1):
+        ctx = L.Context(0)
+        ctx.set_tuning(L.MF_TUNE_WEIGHTS_SOLVER, solver)
+        ctx.set_points(x, y)
+        ctx.set_stencil(idx)
+        out.append(ctx.compute_operator(5, 3, 1, L.MF_OP_FLUX, want=True, k=15))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_ns_ragged_c3_sample_vs_oracle(mf):
     """C3's stencil mix (n = 30, 8 % with n in 40..50; here 20 % so that
     every size occurs often) on a 96 x 96 cloud, all sizes in one launch:
@@ -113,6 +131,42 @@ def test_ns_ragged_c3_sample_vs_oracle(mf):
     ctx.weights_ragged_run(5, 3, 1)
     bx, by = ctx.weights_ragged_get()
     assert rel_inf(bx, ox) < 1e-12 and rel_inf(by, oy) < 1e-12
+
+
+def test_ns_ragged_extreme_sizes(mf):
+    """Ragged stencils at the edges of the supported range and of each
+    path, interleaved: 11 nodes (one more than the 10 monomials of degree
+    3), 12 and 19 (below twice the monomial count: Bunch-Kaufman, since the
+    Cholesky QR of a nearly square P loses digits, 3e-7 at 11 nodes), 20
+    (the smallest null-space size), 31, 32 (the largest of the two-tile
+    variant), 33, 48, 49 and 64 (the largest stencil): per stencil within
+    1e-10 of the oracle.  (An LU solve of the same systems differs from the
+    oracle's by up to 2.5e-11 per stencil, largest at 11 and 64 nodes.)"""
+    _say("null-space ragged weights, extreme sizes")
+    x, y, _, _, _ = O.cloud_square(40, 20261208)
+    N = len(x)
+    choices = np.array([11, 12, 19, 20, 31, 32, 33, 48, 49, 64])
+    sizes = choices[np.arange(N) % len(choices)]
+    big, _ = O.knn_grid(x, y, 64)
+    ptr = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    idx = np.concatenate([big[e, :sizes[e]] for e in range(N)]).astype(np.int32)
+    ox, oy, rc = O.weights(x, y, ptr, idx, 5, 3, 1)
+    assert rc == 0
+    ctx = L.Context(0)
+    ctx.set_points(x, y)
+    ctx.weights_ragged_set(ptr, idx)
+    ctx.weights_ragged_run(5, 3, 1)
+    wx, wy = ctx.weights_ragged_get()
+    for n in choices:
+        sel = np.nonzero(sizes == n)[0]
+        sp = np.concatenate([[0], np.cumsum(sizes[sel])])
+        gx = np.concatenate([wx[ptr[e]:ptr[e + 1]] for e in sel])
+        gox = np.concatenate([ox[ptr[e]:ptr[e + 1]] for e in sel])
+        gy = np.concatenate([wy[ptr[e]:ptr[e + 1]] for e in sel])
+        goy = np.concatenate([oy[ptr[e]:ptr[e + 1]] for e in sel])
+        ex, ey = _per_stencil_err(gx, gox, sp), _per_stencil_err(gy, goy, sp)
+        _say("  n=%d: per-stencil rel diff %.2e %.2e" % (n, ex, ey))
+        assert ex < 1e-10 and ey < 1e-10, n
 
 
 def test_ns_closer_to_exact_than_tolerance(mf):

@@ -699,6 +699,36 @@ def test_rhs_generic_stencil_size(mf, k):
         assert rel_inf(du, duo) < tol, prec
 
 
+@pytest.mark.parametrize("side,k", [(12, 30), (24, 64)])
+def test_rhs_small_cloud_and_largest_stencil(mf, side, k):
+    """A cloud smaller than one 256-node tile (12 x 12 = 144 nodes, 30-point
+    stencils, with the Flyer hyperviscosity) and the largest stencil (64
+    nodes, r^3 with degree 2, on 24 x 24): rhs! with the residual viscosity
+    and a history in the three precisions against the oracle, then 4
+    graph-replayed SSPRK33 steps in reference order."""
+    _say("rhs small cloud / largest stencil side=%d k=%d" % (side, k))
+    spec = configs.c2()
+    spec.k = k
+    hv = (2, 1e-4)
+    if k == 64:
+        spec.p, spec.deg = 3, 2
+        hv = None  # the 4th derivatives of r^3 are singular at the centre
+    c = _rhs_residual_case(spec, side, hv=hv)
+    ad = c.ctx.approx_du()
+    c.prob.approx_du[:] = ad.ravel()
+    c.prob.cfg.success_iter = 1
+    for prec, tol in ((L.MF_PREC_F64, TOL64), (L.MF_PREC_F64_REF, TOL64), (L.MF_PREC_SPLIT32, TOL32)):
+        u, du, uo, duo, _, _ = _rhs_pair(c, prec=prec, t=0.001)
+        _say("  prec %d: rel err %.2e" % (prec, rel_inf(du, duo)))
+        assert rel_inf(du, duo) < tol, prec
+    c2 = cases.build(spec, side=side, visc="residual", hv=hv)
+    c2.ctx.upload(L.MF_BUF_U, c2.u0.copy())
+    c2.ctx.ssprk33(0.0, c2.dt, 4, L.MF_PREC_F64_REF, L.MF_STEP_HISTORY | L.MF_STEP_GRAPH)
+    uo = c2.u0.copy()
+    c2.prob.ssprk33(uo, 0.0, c2.dt, 4, approx_order=3)
+    assert rel_inf(c2.ctx.download(L.MF_BUF_U), uo) < TOL64
+
+
 # ------------------------------------------------------------ node order
 def test_node_order_changes_no_result(mf):
     """The context stores nodes in Morton order; every result in the

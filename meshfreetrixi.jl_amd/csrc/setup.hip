@@ -236,6 +236,27 @@ __global__ void k_cell_finish(int64_t ncell, const int32_t* start, const int32_t
     }
 }
 
+// The search grid over the bounding box: square cells of about two points
+// each, their side at least the longer extent / N, so that an elongated or
+// collinear cloud gets at most N + 1 cells along its length instead of a
+// count that overflows; then W H / cs^2 <= N / 2 and the grid has at most
+// 2.5 N + 1 cells.  The search is exact on any grid; only its speed
+// depends on the cells.
+static Grid knn_grid(const double box[4], int64_t N)
+{
+    Grid g;
+    g.x0 = box[0];
+    g.y0 = box[2];
+    const double W = fmax(box[1] - box[0], 0.0), H = fmax(box[3] - box[2], 0.0);
+    double cs = sqrt(2.0 * W * H / (double)N);
+    cs = fmax(cs, fmax(W, H) / (double)N);
+    if (!(cs > 0)) cs = 1.0;  // every point at one place: one cell
+    g.cs = cs;
+    g.ncx = (int64_t)fmin(floor(W / cs), (double)N) + 1;
+    g.ncy = (int64_t)fmin(floor(H / cs), (double)N) + 1;
+    return g;
+}
+
 __device__ __forceinline__ bool knn_before(double da, int32_t ia, double db, int32_t ib)
 {
     return (da < db) || (da == db && ia < ib);
@@ -338,18 +359,9 @@ int launch_knn(mf_ctx* c, int k, double* d_d2)
     MF_HIP(c, hipMemcpyAsync(box, d_box, sizeof box, hipMemcpyDeviceToHost, c->stream));
     MF_HIP(c, hipStreamSynchronize(c->stream));
     dfree(d_box);
-    Grid g;
-    g.x0 = box[0];
-    g.y0 = box[2];
-    double W = box[1] - box[0], H = box[3] - box[2];
-    if (W <= 0) W = 1e-300;
-    if (H <= 0) H = 1e-300;
-    g.cs = sqrt(2.0 * W * H / (double)N);
-    if (!(g.cs > 0)) g.cs = fmax(W, H);
-    g.ncx = (int64_t)floor(W / g.cs) + 1;
-    g.ncy = (int64_t)floor(H / g.cs) + 1;
+    const Grid g = knn_grid(box, N);
     const int64_t ncell = g.ncx * g.ncy;
-    if (ncell > (int64_t)1 << 30) return fail(c, MF_ERR_ARGUMENT, "kNN: degenerate point cloud extent");
+    if (ncell > (int64_t)1 << 30) return fail(c, MF_ERR_ARGUMENT, "kNN: search grid too large");
     int32_t *cell = nullptr, *count = nullptr, *start = nullptr, *cursor = nullptr, *order = nullptr;
     double2* xys = nullptr;
     if ((rc = dalloc(c, &cell, N)) || (rc = dalloc(c, &count, ncell)) || (rc = dalloc(c, &start, ncell)) ||
@@ -479,16 +491,7 @@ int launch_knn_noalloc(mf_ctx* c, int k, int32_t* d_idx, double* d_dist)
     double box[4];
     MF_HIP(c, hipMemcpyAsync(box, d_box, sizeof box, hipMemcpyDeviceToHost, c->stream));
     MF_HIP(c, hipStreamSynchronize(c->stream));
-    Grid g;
-    g.x0 = box[0];
-    g.y0 = box[2];
-    double W = box[1] - box[0], H = box[3] - box[2];
-    if (W <= 0) W = 1e-300;
-    if (H <= 0) H = 1e-300;
-    g.cs = sqrt(2.0 * W * H / (double)N);
-    if (!(g.cs > 0)) g.cs = fmax(W, H);
-    g.ncx = (int64_t)floor(W / g.cs) + 1;
-    g.ncy = (int64_t)floor(H / g.cs) + 1;
+    const Grid g = knn_grid(box, N);
     const int64_t ncell = g.ncx * g.ncy;
     if (ncell > kKnnScratchCells)
         return fail(c, MF_ERR_UNSUPPORTED, "mf_knn_noalloc: the cloud's search grid needs more cells than its "

@@ -48,6 +48,35 @@ def test_knn_bit_exact_large(mf):
     assert np.array_equal(idx, ig)
 
 
+@pytest.mark.parametrize("shape", ["horizontal", "vertical", "nearly_collinear", "elongated"])
+def test_knn_degenerate_extents(mf, shape):
+    """Clouds whose bounding box is flat (collinear points, along either
+    axis), nearly flat, or very elongated: the search grid stays small (at
+    most N + 1 cells along the long side; a flat box made the cell count
+    overflow before) and the stencils and distances equal brute force, ties
+    broken by index."""
+    _say("knn degenerate extent: %s" % shape)
+    rng = np.random.default_rng(20261210)
+    n = 600
+    t = rng.random(n)
+    if shape == "horizontal":
+        x, y = t, np.full(n, 0.25)
+    elif shape == "vertical":
+        x, y = np.full(n, 0.7), t
+    elif shape == "nearly_collinear":
+        x, y = t, 0.5 + 1e-13 * rng.standard_normal(n)
+    else:
+        x, y = 1000.0 * t, rng.random(n)
+    ctx = L.Context(0)
+    ctx.set_points(x, y)
+    idx, dist, dmin, davg = ctx.knn(15, want_dist=True)
+    ib, d2 = O.knn_brute(x, y, 15)
+    assert np.array_equal(idx, ib)
+    assert np.array_equal(dist, np.sqrt(d2))
+    omin, oavg = O.dx_stats(d2[:, 1])
+    assert dmin == omin and davg == oavg
+
+
 # ------------------------------------------------------------- weights
 @pytest.mark.parametrize("p,deg,k,kd", [(3, 2, 15, 1), (5, 3, 30, 1), (5, 3, 30, 4), (3, 3, 20, 1),
                                         (7, 2, 15, 2), (7, 3, 30, 1), (3, 3, 30, 2), (5, 2, 15, 4)])

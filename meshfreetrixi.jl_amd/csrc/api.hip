@@ -852,19 +852,24 @@ int mf_set_boundary(mf_ctx* c, int n_tags, const int32_t* tag_ptr, const int32_t
     b.tag_params.assign(8 * (size_t)n_tags, 0.0);
     if (!tag_ptr || !tag_kind || !node_idx) return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: null arrays");
     if (tag_ptr[0] != 0) return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: tag_ptr[0] must be 0");
+    // the whole of tag_ptr before any entry is read: a decreasing step
+    // would otherwise index past the arrays sized by its last element
+    for (int g = 0; g < n_tags; ++g)
+        if (tag_ptr[g + 1] < tag_ptr[g]) return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: tag_ptr not monotone");
     const int nb = tag_ptr[n_tags];
     std::vector<int32_t> btag(std::max(nb, 1));
     std::vector<int32_t> bnode(std::max(nb, 1));
     std::vector<uint8_t> flag(c->N, 0);
     std::vector<int32_t> entry(c->N, -1);
     for (int g = 0; g < n_tags; ++g) {
-        if (tag_ptr[g + 1] < tag_ptr[g]) return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: tag_ptr not monotone");
         const int kind = tag_kind[g];
         if (kind < MF_BC_DIRICHLET || kind > MF_BC_DO_NOTHING)
             return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: unknown boundary kind");
         if (kind == MF_BC_SLIP_WALL && c->nvar != 4)
             return fail(c, MF_ERR_UNSUPPORTED, "mf_set_boundary: slip walls need the Euler equations");
-        const int fn = tag_fn ? tag_fn[g] : MF_FN_CONST;
+        // the value function matters for Dirichlet tags only; the others
+        // keep 0, so that it fits its 4 bits of the packed entry code
+        const int fn = tag_fn && kind == MF_BC_DIRICHLET ? tag_fn[g] : MF_FN_CONST;
         if (kind == MF_BC_DIRICHLET && (fn < MF_FN_CONST || fn > MF_FN_TABLE))
             return fail(c, MF_ERR_ARGUMENT, "mf_set_boundary: unknown value function");
         if (kind == MF_BC_DIRICHLET && (fn == MF_FN_WAVE || fn == MF_FN_GRADIENT) && c->nvar != 4)

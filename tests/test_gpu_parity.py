@@ -705,6 +705,21 @@ def test_api_rejects_bad_input(mf):
     assert code(lambda: ctx.ssprk33(0.0, -0.01, 2)) == L.MF_ERR_ARGUMENT
     assert code(lambda: ctx.compute_operator(5, 9, 1, L.MF_OP_FLUX, want=False)) in (L.MF_ERR_ARGUMENT,
                                                                                     L.MF_ERR_UNSUPPORTED)
+    # a decreasing tag_ptr is refused before any entry is read (its last
+    # element sizes the arrays that the earlier tags' entries fill)
+    nodes = np.arange(6, dtype=np.int32)
+    kinds = [L.MF_BC_DIRICHLET, L.MF_BC_DO_NOTHING]
+    assert code(lambda: ctx.set_boundary([0, 5, 1], kinds, [0, 0], np.zeros(16), nodes, np.zeros((6, 2)))) \
+        == L.MF_ERR_ARGUMENT
+    # a value-function code on a tag that is not Dirichlet is ignored (it
+    # would not fit the packed entry code): the same du as with code 0
+    dus = []
+    for fn_other in (0, 100):
+        ctx.set_boundary([0, 3, 6], kinds, [L.MF_FN_CONST, fn_other], np.ones(16), nodes, np.zeros((6, 2)))
+        ctx.upload(L.MF_BUF_U, u)
+        ctx.rhs(0.0)
+        dus.append(ctx.download(L.MF_BUF_DU))
+    assert np.array_equal(dus[0], dus[1])
     # the context still works after the errors
     ctx.upload(L.MF_BUF_U, u)
     ctx.rhs(0.0)

@@ -355,3 +355,37 @@ def test_c_client(mf, tmp_path):
     _say("  " + r.stdout.strip() + r.stderr.strip())
     assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
     assert "mfrbf C client ok" in r.stdout
+
+
+def test_small_cloud_igr_limiter_adjoint(mf):
+    """A cloud smaller than one 256-node tile (12 x 12 = 144 nodes, 30-point
+    stencils): the IGR source (alpha = dx_min^2), the Zhang-Shu limiter on
+    density and pressure, and the VJP of rhs!, each against the oracle."""
+    _say("small cloud: IGR, limiter, VJP")
+    spec = configs.c2()
+    c0 = cases.build(spec, side=12, use_gpu=False)
+    c = cases.build(spec, side=12, igr=c0.dx_min ** 2)
+    u = c.u0.copy()
+    du = np.zeros_like(u)
+    c.ctx.rhs_host(0.01, u, du, L.MF_PREC_F64_REF)
+    duo, _, _ = c.prob.rhs(c.u0.copy(), 0.01)
+    assert rel_inf(c.ctx.igr_sigma(), c.prob.igr_sigma) < 1e-10
+    assert rel_inf(du, duo) < 1e-10
+    ub = c.u0.copy()
+    rng = np.random.default_rng(4)
+    bad = rng.choice(c.N, 12, replace=False)
+    ub[0, bad[:6]] = 0.05
+    ub[3, bad[6:]] = 0.2 * ub[3, bad[6:]]
+    ptr = O.ell_ptr(c.N, c.k)
+    for thr, var in ((0.5, L.MF_VAR_DENSITY), (0.5, L.MF_VAR_PRESSURE)):
+        uo = O.limiter_zhang_shu(ub.copy(), ptr, c.idx, thr, var)
+        assert np.any(uo != ub)
+        c.ctx.upload(L.MF_BUF_U, ub.copy())
+        c.ctx.limiter_zhang_shu(thr, var)
+        assert rel_inf(c.ctx.download(L.MF_BUF_U), uo) < 1e-14
+    cv = cases.build(spec, side=12)
+    lam = rng.standard_normal(cv.u0.shape)
+    cv.ctx.upload(L.MF_BUF_U, cv.u0.copy())
+    cv.ctx.upload(L.MF_BUF_AUX0, lam)
+    cv.ctx.rhs_vjp(0.0)
+    assert rel_inf(cv.ctx.download(L.MF_BUF_AUX1), cv.prob.vjp(cv.u0.copy(), lam, 0.0)) < 1e-12

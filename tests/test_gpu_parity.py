@@ -572,9 +572,12 @@ def test_rhs_reference_order_hv_and_rv(mf):
 
 
 def test_rhs_full_size_c2(mf):
-    """262,144 nodes (the headline configuration): reference-order FP64
-    within 1e-12 of the oracle, fused-multiply-add FP64 and split-FP32 within
-    their stated tolerances."""
+    """262,144 nodes (the headline configuration): reference-order FP64, the
+    parity mode, within 1e-12 of the oracle; split-FP32 within 1e-6.  The
+    fused-multiply-add FP64 mode rounds each product-sum once instead of
+    twice, a different operation order from the reference's, and measures
+    about 1.0e-12 here (the cancellation in the flux divergence scales it), so
+    its bar is 5e-12."""
     _say("rhs full size 512x512 (kNN + weights on the GPU, oracle on the CPU)")
     from mfrbf.clouds import cloud_square
     spec = configs.c2()

@@ -160,12 +160,13 @@ int or_knn_grid(const double* x, const double* y, int64_t N, int k, int32_t* idx
         if (y[i] < y0) y0 = y[i];
         if (y[i] > y1) y1 = y[i];
     }
-    double W = x1 - x0, H = y1 - y0;
-    if (W <= 0) W = 1e-300;
-    if (H <= 0) H = 1e-300;
-    double cs = sqrt(2.0 * W * H / (double)N);
-    if (!(cs > 0)) cs = fmax(W, H);
-    int64_t ncx = (int64_t)floor(W / cs) + 1, ncy = (int64_t)floor(H / cs) + 1;
+    /* cells of about two points; their side at least the longer extent / N,
+     * so that a flat or elongated cloud gets at most N + 1 cells along its
+     * length (the search is exact on any grid) */
+    const double W = fmax(x1 - x0, 0.0), H = fmax(y1 - y0, 0.0);
+    double cs = fmax(sqrt(2.0 * W * H / (double)N), fmax(W, H) / (double)N);
+    if (!(cs > 0)) cs = 1.0;
+    int64_t ncx = (int64_t)fmin(floor(W / cs), (double)N) + 1, ncy = (int64_t)fmin(floor(H / cs), (double)N) + 1;
     int64_t ncell = ncx * ncy;
     int64_t* start = (int64_t*)calloc((size_t)ncell + 1, sizeof(int64_t));
     int64_t* cell = (int64_t*)malloc(sizeof(int64_t) * N);

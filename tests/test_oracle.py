@@ -65,6 +65,20 @@ def test_knn_tie_order_by_index():
     assert np.array_equal(ib, ig)
 
 
+def test_knn_grid_degenerate_extents():
+    """The grid search on flat (collinear along either axis), nearly flat
+    and elongated clouds equals brute force: its grid stays at most N + 1
+    cells along the long side instead of a cell count that overflows."""
+    rng = np.random.default_rng(20261210)
+    n = 600
+    t = rng.random(n)
+    for x, y in ((t, np.full(n, 0.25)), (np.full(n, 0.7), t), (t, 0.5 + 1e-13 * rng.standard_normal(n)),
+                 (1000.0 * t, rng.random(n))):
+        ib, d2 = O.knn_brute(x, y, 15)
+        ig, dg = O.knn_grid(x, y, 15)
+        assert np.array_equal(ib, ig) and np.array_equal(d2, dg)
+
+
 def test_knn_grid_large_vs_ckdtree():
     x, y, _, _, _ = O.cloud_square(256, 20261205)
     ig, dg = O.knn_grid(x, y, 30)

@@ -191,9 +191,9 @@ int mf_set_operator(mf_ctx* ctx, int target, const double* wx, const double* wy)
  * ceil(p/2) - 1 (where r^p is conditionally positive definite) to 4);
  * stencils of fewer than twice the monomial count, every stencil of another
  * basis, and all of them with MF_TUNE_WEIGHTS_SOLVER 1, by the
- * Bunch-Kaufman restatement, one launch per stencil size.  ptr/idx are uploaded once; mf_weights_ragged_run
- * recomputes on the device (timing loop), mf_weights_ragged_get copies
- * the result (layout like idx). */
+ * Bunch-Kaufman restatement, one launch per stencil size.  ptr/idx are
+ * uploaded once; mf_weights_ragged_run recomputes on the device (timing
+ * loop), mf_weights_ragged_get copies the result (layout like idx). */
 int mf_weights_ragged_set(mf_ctx* ctx, const int64_t* ptr, const int32_t* idx,
                           int64_t n_stencils);
 int mf_weights_ragged_run(mf_ctx* ctx, int phs_power, int poly_degree, int deriv_order);

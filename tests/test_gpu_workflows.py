@@ -389,3 +389,55 @@ def test_small_cloud_igr_limiter_adjoint(mf):
     cv.ctx.upload(L.MF_BUF_AUX0, lam)
     cv.ctx.rhs_vjp(0.0)
     assert rel_inf(cv.ctx.download(L.MF_BUF_AUX1), cv.prob.vjp(cv.u0.copy(), lam, 0.0)) < 1e-12
+
+
+@pytest.mark.parametrize("order", [1, 2, 5, 7])
+def test_ssprk33_rv_history_orders(mf, order):
+    """HistoryCallback(approx_order) for orders other than the
+    configurations' 3, up to the largest history kept (8 states): 10
+    graph-replayed SSPRK33 steps with the residual viscosity, reference
+    order, against the oracle (state 1e-12; approx_du, a time difference
+    with weights of order 1/dt, 1e-8)."""
+    _say("ssprk33 + residual viscosity, approx_order=%d" % order)
+    spec = configs.c2()
+    spec.approx_order = order
+    c = cases.build(spec, side=40, visc="residual")
+    c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+    c.ctx.ssprk33(0.0, c.dt, 10, L.MF_PREC_F64_REF, L.MF_STEP_HISTORY | L.MF_STEP_GRAPH)
+    uo = c.u0.copy()
+    c.prob.ssprk33(uo, 0.0, c.dt, 10, approx_order=order)
+    e_u = rel_inf(c.ctx.download(L.MF_BUF_U), uo)
+    e_ad = rel_inf(c.ctx.approx_du().ravel(), c.prob.approx_du)
+    _say("  state %.2e, approx_du %.2e" % (e_u, e_ad))
+    assert np.abs(c.prob.approx_du).max() > 0.0
+    assert e_u < 1e-12 and e_ad < 1e-8
+
+
+def test_rhs_igr_zero_iterations(mf):
+    """The IGR source with no CG iteration: sigma keeps its starting value
+    and du its flux terms, as the oracle's."""
+    _say("IGR with maxiter = 0")
+    spec = configs.c2()
+    c0 = cases.build(spec, side=32, use_gpu=False)
+    c = cases.build(spec, side=32, igr=c0.dx_min ** 2, igr_maxiter=0)
+    u = c.u0.copy()
+    du = np.zeros_like(u)
+    c.ctx.rhs_host(0.01, u, du, L.MF_PREC_F64_REF)
+    duo, _, _ = c.prob.rhs(c.u0.copy(), 0.01)
+    assert np.array_equal(c.ctx.igr_sigma(), c.prob.igr_sigma)
+    assert rel_inf(du, duo) < 1e-12
+
+
+def test_limiter_every_node_below_threshold(mf):
+    """A density threshold above every node's density: every row is blended
+    with its stencil mean, as the oracle does."""
+    _say("limiter with every node below the threshold")
+    c = cases.build(configs.c2(), side=24)
+    u = c.u0.copy()
+    ptr = O.ell_ptr(c.N, c.k)
+    thr = 1.5 * u[0].max()
+    uo = O.limiter_zhang_shu(u.copy(), ptr, c.idx, thr, O.VAR_DENSITY)
+    assert np.all(np.any(uo != u, axis=0))
+    c.ctx.upload(L.MF_BUF_U, u.copy())
+    c.ctx.limiter_zhang_shu(thr, L.MF_VAR_DENSITY)
+    assert rel_inf(c.ctx.download(L.MF_BUF_U), uo) < 1e-14

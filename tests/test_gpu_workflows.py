@@ -441,3 +441,42 @@ def test_limiter_every_node_below_threshold(mf):
     c.ctx.upload(L.MF_BUF_U, u.copy())
     c.ctx.limiter_zhang_shu(thr, L.MF_VAR_DENSITY)
     assert rel_inf(c.ctx.download(L.MF_BUF_U), uo) < 1e-14
+
+
+def test_boundary_node_in_two_tags(mf):
+    """Nodes in two boundary tags (the bottom edge: in the Dirichlet inflow
+    tag, then in a slip-wall tag) take both, in tag order, as the
+    reference's per-tag loop does (pointcloudsolver/rbfsolver.jl:277-286):
+    rhs! in the three precisions, and 4 graph-replayed SSPRK33 steps with
+    the residual viscosity and the history callback (each rhs! then runs its
+    own boundary and sum passes), against the oracle."""
+    _say("boundary nodes in two tags")
+    spec = configs.c2()
+    c = cases.build(spec, side=32, visc="residual")
+    inflow = np.nonzero(c.tag == 1)[0].astype(np.int32)
+    outflow = np.nonzero(c.tag == 2)[0].astype(np.int32)
+    bottom = inflow[c.y[inflow] == 0.0]
+    assert len(bottom) > 3
+    nrm = np.tile([0.0, -1.0], (len(bottom), 1))
+    fn, prm = spec.ic.code, spec.ic.params()
+    ptr = [0, len(inflow), len(inflow) + len(bottom), len(inflow) + len(bottom) + len(outflow)]
+    c.ctx.set_boundary(ptr, [L.MF_BC_DIRICHLET, L.MF_BC_SLIP_WALL, L.MF_BC_DO_NOTHING], [fn, 0, 0],
+                       np.concatenate([prm, prm, prm]), np.concatenate([inflow, bottom, outflow]),
+                       np.concatenate([np.zeros((len(inflow), 2)), nrm, np.zeros((len(outflow), 2))]))
+    prob = O.Problem(c.x, c.y, c.ptr, c.idx, c.wx.ravel(), c.wy.ravel(), c.eq, c.nvar, gamma=spec.gamma,
+                     boundary=[(O.BC_DIRICHLET, inflow, None, fn, prm, None), (O.BC_SLIP, bottom, nrm, fn, prm, None),
+                               (O.BC_NOTHING, outflow, None, fn, prm, None)],
+                     rv_mode=O.RV_RESIDUAL, dx_avg=c.dx_avg)
+    u0 = c.u0.copy()
+    u0[1:3] += 0.02 * np.random.default_rng(12).standard_normal(u0[1:3].shape)
+    duo, _, _ = prob.rhs(u0.copy(), 0.02)
+    for prec, tol in ((L.MF_PREC_F64, 1e-12), (L.MF_PREC_F64_REF, 1e-12), (L.MF_PREC_SPLIT32, 1e-6)):
+        u = u0.copy()
+        du = np.zeros_like(u)
+        c.ctx.rhs_host(0.02, u, du, prec)
+        assert rel_inf(du, duo) < tol, prec
+    c.ctx.upload(L.MF_BUF_U, u0.copy())
+    c.ctx.ssprk33(0.0, c.dt, 4, L.MF_PREC_F64_REF, L.MF_STEP_HISTORY | L.MF_STEP_GRAPH)
+    uo = u0.copy()
+    prob.ssprk33(uo, 0.0, c.dt, 4, approx_order=3)
+    assert rel_inf(c.ctx.download(L.MF_BUF_U), uo) < 1e-12

@@ -616,35 +616,43 @@ int mf_compute_operator(mf_ctx* c, int phs_power, int poly_degree, int deriv_ord
     if (c->k + np > kMaxStencilM) return fail(c, MF_ERR_UNSUPPORTED, "mf_compute_operator: stencil + monomials > 64");
     if (np > c->k) return fail(c, MF_ERR_ARGUMENT, "mf_compute_operator: fewer stencil nodes than monomials");
     const size_t n = (size_t)c->N * c->k;
-    double *wx = nullptr, *wy = nullptr;
-    if ((rc = dalloc(c, &wx, n)) || (rc = dalloc(c, &wy, n))) return rc;
-    if (!c->d_status && (rc = dalloc(c, &c->d_status, 1))) return rc;
-    MF_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
-    if (c->weights_solver == 2 && ns_basis_ok(phs_power, poly_degree) && ns_size_ok(c->k, np))
-        rc = launch_weights_ns(c, nullptr, c->d_idx, c->k, nullptr, c->N, c->k, c->k, phs_power, poly_degree,
-                               deriv_order, wx, wy, c->d_status);
-    else
-        rc = launch_weights(c, nullptr, c->d_idx, c->k, c->N, c->k, phs_power, poly_degree, deriv_order, wx, wy,
-                            c->d_status);
-    if (rc) return rc;
-    if ((rc = alloc_op(c, target))) return rc;
-    if ((rc = launch_pack_ops(c, target, wx, wy))) return rc;
-    if ((rc = launch_pack_sorted(c, target, wx, wy))) return rc;
+    double *wx = nullptr, *wy = nullptr, *rows = nullptr;
     int st = 0;
-    MF_HIP(c, hipMemcpyAsync(&st, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    for (int w = 0; w < 2; ++w) {
-        double* out = w == 0 ? wx_out : wy_out;
-        if (!out) continue;
-        double* rows = nullptr;
-        if ((rc = dalloc(c, &rows, n))) return rc;
-        if ((rc = launch_ell_to_rows_d(c, w == 0 ? wx : wy, rows, c->k))) return rc;
-        MF_HIP(c, hipMemcpyAsync(out, rows, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    // the work buffers are freed on every return (after the stream has
+    // drained on the success path)
+    auto body = [&]() -> int {
+        int r;
+        if ((r = dalloc(c, &wx, n)) || (r = dalloc(c, &wy, n))) return r;
+        if (!c->d_status && (r = dalloc(c, &c->d_status, 1))) return r;
+        MF_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
+        if (c->weights_solver == 2 && ns_basis_ok(phs_power, poly_degree) && ns_size_ok(c->k, np))
+            r = launch_weights_ns(c, nullptr, c->d_idx, c->k, nullptr, c->N, c->k, c->k, phs_power, poly_degree,
+                                  deriv_order, wx, wy, c->d_status);
+        else
+            r = launch_weights(c, nullptr, c->d_idx, c->k, c->N, c->k, phs_power, poly_degree, deriv_order, wx, wy,
+                               c->d_status);
+        if (r) return r;
+        if ((r = alloc_op(c, target))) return r;
+        if ((r = launch_pack_ops(c, target, wx, wy))) return r;
+        if ((r = launch_pack_sorted(c, target, wx, wy))) return r;
+        MF_HIP(c, hipMemcpyAsync(&st, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        for (int w = 0; w < 2; ++w) {
+            double* out = w == 0 ? wx_out : wy_out;
+            if (!out) continue;
+            if (!rows && (r = dalloc(c, &rows, n))) return r;
+            if ((r = launch_ell_to_rows_d(c, w == 0 ? wx : wy, rows, c->k))) return r;
+            MF_HIP(c, hipMemcpyAsync(out, rows, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+            MF_HIP(c, hipStreamSynchronize(c->stream));
+        }
         MF_HIP(c, hipStreamSynchronize(c->stream));
-        dfree(rows);
-    }
-    MF_HIP(c, hipStreamSynchronize(c->stream));
+        return MF_OK;
+    };
+    rc = body();
+    if (rc) (void)hipStreamSynchronize(c->stream);  // nothing in flight reads them
     dfree(wx);
     dfree(wy);
+    dfree(rows);
+    if (rc) return rc;
     op_installed(c, target);
     if (st) return fail(c, MF_ERR_SINGULAR, "mf_compute_operator: singular stencil system");
     return MF_OK;
@@ -660,22 +668,31 @@ int mf_set_operator(mf_ctx* c, int target, const double* wx, const double* wy)
         return fail(c, MF_ERR_ARGUMENT, "mf_set_operator: bad arguments");
     const size_t n = (size_t)c->N * c->k;
     double *rx = nullptr, *ex = nullptr, *ey = nullptr;
-    if ((rc = dalloc(c, &rx, n)) || (rc = dalloc(c, &ex, n))) return rc;
-    if (wy && (rc = dalloc(c, &ey, n))) return rc;
-    MF_HIP(c, hipMemcpyAsync(rx, wx, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    if ((rc = launch_rows_to_ell_d(c, rx, ex, c->k))) return rc;
-    if (wy) {
+    // the staging buffers are freed on every return (after the stream has
+    // drained on the success path)
+    auto body = [&]() -> int {
+        int r;
+        if ((r = dalloc(c, &rx, n)) || (r = dalloc(c, &ex, n))) return r;
+        if (wy && (r = dalloc(c, &ey, n))) return r;
+        MF_HIP(c, hipMemcpyAsync(rx, wx, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        if ((r = launch_rows_to_ell_d(c, rx, ex, c->k))) return r;
+        if (wy) {
+            MF_HIP(c, hipStreamSynchronize(c->stream));
+            MF_HIP(c, hipMemcpyAsync(rx, wy, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+            if ((r = launch_rows_to_ell_d(c, rx, ey, c->k))) return r;
+        }
+        if ((r = alloc_op(c, target))) return r;
+        if ((r = launch_pack_ops(c, target, ex, ey))) return r;
+        if ((r = launch_pack_sorted(c, target, ex, ey))) return r;
         MF_HIP(c, hipStreamSynchronize(c->stream));
-        MF_HIP(c, hipMemcpyAsync(rx, wy, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-        if ((rc = launch_rows_to_ell_d(c, rx, ey, c->k))) return rc;
-    }
-    if ((rc = alloc_op(c, target))) return rc;
-    if ((rc = launch_pack_ops(c, target, ex, ey))) return rc;
-    if ((rc = launch_pack_sorted(c, target, ex, ey))) return rc;
-    MF_HIP(c, hipStreamSynchronize(c->stream));
+        return MF_OK;
+    };
+    rc = body();
+    if (rc) (void)hipStreamSynchronize(c->stream);  // nothing in flight reads them
     dfree(rx);
     dfree(ex);
     dfree(ey);
+    if (rc) return rc;
     op_installed(c, target);
     return MF_OK;
 }

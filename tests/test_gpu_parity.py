@@ -906,3 +906,20 @@ def test_rhs_random_configuration(mf, seed):
             assert rel_inf(geps, eps) < (1e-12 if prec == L.MF_PREC_F64_REF else 1e-6)
             if prec == L.MF_PREC_F64_REF:
                 assert np.array_equal(gcode, epsc)
+
+
+def test_advection_with_flyer_hyperviscosity(mf):
+    """The Flyer source on the scalar advection equation (the reference's
+    source acts on any equations; k = 1 here, second derivatives of r^3):
+    rhs! in the three precisions and 10 graph-replayed SSPRK33 steps in
+    reference order against the oracle."""
+    _say("advection + Flyer hyperviscosity")
+    c = cases.build(configs.c1(), side=40, hv=(1, 0.01))
+    for prec, tol in ((L.MF_PREC_F64, TOL64), (L.MF_PREC_F64_REF, TOL64), (L.MF_PREC_SPLIT32, TOL32)):
+        u, du, uo, duo, _, _ = _rhs_pair(c, prec=prec, t=0.05)
+        assert rel_inf(du, duo) < tol, prec
+    c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+    c.ctx.ssprk33(0.0, c.dt, 10, L.MF_PREC_F64_REF, L.MF_STEP_GRAPH)
+    uo = c.u0.copy()
+    c.prob.ssprk33(uo, 0.0, c.dt, 10)
+    assert rel_inf(c.ctx.download(L.MF_BUF_U), uo) < TOL64

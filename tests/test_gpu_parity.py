@@ -923,3 +923,35 @@ def test_advection_with_flyer_hyperviscosity(mf):
     uo = c.u0.copy()
     c.prob.ssprk33(uo, 0.0, c.dt, 10)
     assert rel_inf(c.ctx.download(L.MF_BUF_U), uo) < TOL64
+
+
+@pytest.mark.parametrize("visc", ["upwind", "residual"])
+def test_rhs_viscosity_nonphysical_states(mf, visc):
+    """Nodes with negative pressure and nodes with negative density: the
+    upwind viscosity takes the sound speed as 0 there
+    (hyperviscosity.jl:246-285), and the residual branch compares with it;
+    du, eps and its codes in reference order against the oracle."""
+    _say("viscosity on non-physical states: %s" % visc)
+    c = cases.build(configs.c2(), side=32, visc=visc)
+    if visc == "residual":
+        u1 = c.u0 * 1.0001
+        c.ctx.upload(L.MF_BUF_U, c.u0.copy())
+        c.ctx.history_update(0.0, 0)
+        c.ctx.upload(L.MF_BUF_U, np.ascontiguousarray(u1))
+        c.ctx.history_update(0.001, 1)
+        c.prob.approx_du[:] = c.ctx.approx_du().ravel()
+        c.prob.cfg.success_iter = 1
+    u = c.u0.copy()
+    rng = np.random.default_rng(21)
+    low_p = rng.choice(c.N, 20, replace=False)
+    u[3, low_p] = 0.1 * (u[1, low_p] ** 2 + u[2, low_p] ** 2) / u[0, low_p]  # E below the kinetic energy: p < 0
+    neg_rho = rng.choice(np.setdiff1d(np.arange(c.N), low_p), 10, replace=False)
+    u[0, neg_rho] = -0.5
+    c.u0 = u
+    ug, du, uo, duo, eps_o, code_o = _rhs_pair(c, prec=L.MF_PREC_F64_REF, t=0.0)
+    eps, code = c.ctx.viscosity()
+    if visc == "residual":
+        assert np.any(code_o == 0) and np.any(code_o == 1)  # both branches decide somewhere
+    assert rel_inf(du, duo) < TOL64
+    assert rel_inf(eps, eps_o) < 1e-12
+    assert np.array_equal(code, code_o)

@@ -480,3 +480,24 @@ def test_boundary_node_in_two_tags(mf):
     uo = u0.copy()
     prob.ssprk33(uo, 0.0, c.dt, 4, approx_order=3)
     assert rel_inf(c.ctx.download(L.MF_BUF_U), uo) < 1e-12
+
+
+@pytest.mark.parametrize("bc", [("nothing", "nothing"), ("dirichlet", "slip")])
+def test_adjoint_checkpoint_intervals_agree(mf, bc):
+    """The reverse sweep recomputes each segment from its checkpoint with
+    the forward's own kernels, so the gradient does not depend on the
+    checkpoint interval: 7 SSPRK33 steps with checkpoints every 7 (one
+    segment, the tape of the forward pass), 3 (a partial last segment), 2
+    and 1 step give the same loss and dL/du0 bit for bit."""
+    _say("adjoint checkpoint intervals, bc=%s" % (bc,))
+    c = cases.build(configs.c2(), side=24, bc=bc)
+    u0 = c.u0.copy()
+    u0[1:3] += 0.02 * np.random.default_rng(8).standard_normal(u0[1:3].shape)
+    outs = []
+    for every in (7, 3, 2, 1):
+        c.ctx.upload(L.MF_BUF_U, u0.copy())
+        loss = c.ctx.ssprk33_vjp(0.0, c.dt, 7, every)
+        outs.append((loss, c.ctx.download(L.MF_BUF_AUX1)))
+    assert np.abs(outs[0][1]).max() > 0.0
+    for loss, g in outs[1:]:
+        assert loss == outs[0][0] and np.array_equal(g, outs[0][1])

@@ -105,6 +105,28 @@ def test_ns_small_stencils_use_bunch_kaufman(mf):
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
+def test_ns_ragged_all_below_null_space_size(mf):
+    """A ragged batch whose every stencil is below twice the monomial count
+    (12 to 19 nodes for 10 monomials) runs only the Bunch-Kaufman launches:
+    its weights equal those of MF_TUNE_WEIGHTS_SOLVER 1 bit for bit."""
+    _say("ragged weights, every stencil below the null-space size")
+    x, y, _, _, _ = O.cloud_square(32, 20261211)
+    N = len(x)
+    sizes = 12 + np.arange(N) % 8
+    big, _ = O.knn_grid(x, y, 19)
+    ptr = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    idx = np.concatenate([big[e, :sizes[e]] for e in range(N)]).astype(np.int32)
+    out = []
+    for solver in (0, 1):
+        ctx = L.Context(0)
+        ctx.set_tuning(L.MF_TUNE_WEIGHTS_SOLVER, solver)
+        ctx.set_points(x, y)
+        ctx.weights_ragged_set(ptr, idx)
+        ctx.weights_ragged_run(5, 3, 1)
+        out.append(ctx.weights_ragged_get())
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_ns_ragged_c3_sample_vs_oracle(mf):
     """C3's stencil mix (n = 30, 8 % with n in 40..50; here 20 % so that
     every size occurs often) on a 96 x 96 cloud, all sizes in one launch:
